@@ -1,0 +1,298 @@
+// Oracle: the ICGenerator field pipeline restated (genetIC/src/ic.hpp).
+//   grids and zooms     ic.hpp:365-614, multilevelgrid.hpp:97-140
+//   seeding / drawing   ic.hpp:620-666, randomfieldgenerator.hpp:122-260
+//   zero_level          ic.hpp:787-802
+//   power spectrum      ic.hpp:825-843, multilevelfield.hpp:410-454, field.hpp:394-412
+//   Zel'dovich + levels zeldovich.hpp:152-253, multilevelgenerator.hpp:78-128,
+//                       field.hpp:483-617,804-849 (FILTER_ON_COARSE_GRID branch)
+//   particles           zeldovich.hpp:87-124, generator.hpp:49-53
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.hpp).
+#include "oracle.hpp"
+#include <cmath>
+#include <stdexcept>
+#include <algorithm>
+
+namespace orc {
+
+void IC::initBaseGrid(double box, int n) {
+  if (!grids.empty()) throw std::runtime_error("Cannot re-initialize the base grid");
+  if (drawn) throw std::runtime_error("Trying to initialize a grid after the random field was already drawn");
+  auto g = std::make_unique<Grid>();
+  g->periodic = box;
+  g->n = n;
+  g->dx = box / n;
+  g->size = g->dx * n;
+  grids.push_back(std::move(g));
+  weights.push_back(1.0);
+}
+
+void IC::initZoomGridWithLowLeftCornerAt(int x0, int y0, int z0, int zoomfac, int n) {
+  if (drawn) throw std::runtime_error("Trying to initialize a grid after the random field was already drawn");
+  if (grids.empty()) throw std::runtime_error("Cannot initialise a zoom grid before initialising the base grid");
+  const Grid &above = *grids.back();
+  int nAbove = above.n;
+  if (x0 < 0 || y0 < 0 || z0 < 0)
+    throw std::runtime_error("The newly initialised zoom level is not fully contained within the previous level");
+  int nc = nAbove / zoomfac;
+  if (!above.coversFull()) {
+    if (x0 + nc > nAbove || y0 + nc > nAbove || z0 + nc > nAbove)
+      throw std::runtime_error("The newly initialised zoom level is not fully contained within the previous level");
+  }
+  double size = above.size / zoomfac;
+  auto g = std::make_unique<Grid>();
+  g->periodic = grids[0]->periodic;
+  g->n = n;
+  g->dx = size / n;
+  g->size = g->dx * n;
+  g->ox = above.ox + x0 * above.dx;
+  g->oy = above.oy + y0 * above.dx;
+  g->oz = above.oz + z0 * above.dx;
+  weights.push_back(pow(g->dx / grids[0]->dx, 3.0));
+  grids.push_back(std::move(g));
+}
+
+void IC::setSeed(unsigned long s, SeedMode m) {
+  if (seeded) throw std::runtime_error("The random number generator has already been seeded");
+  seed = s;
+  seedMode = m;
+  seeded = true;
+}
+
+static void drawOneFourierMode(Field &f, int k1, int k2, int k3, double norm, Ranlxs &r, bool reverse) {
+  double a = norm * gaussian_ziggurat(r);
+  double b = norm * gaussian_ziggurat(r);
+  f.setK(k1, k2, k3, reverse ? cplx(b, a) : cplx(a, b));
+  int nyq = f.g->n / 2;
+  if (k1 == 0 || k1 == nyq || k2 == 0 || k2 == nyq || k3 == 0 || k3 == nyq)
+    f.setK(-k1, -k2, -k3, reverse ? cplx(b, -a) : cplx(a, -b));
+}
+
+void IC::draw() {
+  if (!seeded) throw std::runtime_error("The random number generator has not been seeded");
+  if (drawn) throw std::runtime_error("Trying to re-draw the random field after it was already initialised");
+  noise.clear();
+  for (auto &g : grids) noise.emplace_back(*g, false);
+  Ranlxs main(seed);
+  for (size_t L = 0; L < grids.size(); ++L) {
+    Field &f = noise[L];
+    const Grid &g = *grids[L];
+    if (seedMode == SeedMode::Real) {
+      f.fourier = false;
+      for (size_t i = 0; i < g.n3(); ++i) f.d[i] = gaussian_ziggurat(main);
+      f.toFourier();
+      f.d[0] = 0;
+    } else {
+      f.fourier = true;
+      std::fill(f.d.begin(), f.d.end(), 0.0);
+      double sigma = 1.0 / sqrt(2.0);
+      bool rev = seedMode == SeedMode::FourierReverse;
+      for (int ks = 0; ks < g.n / 2; ks++) {
+        std::unique_ptr<Ranlxs> local;
+        Ranlxs *rr = &main;
+        if (seedMode == SeedMode::FourierParallel) {
+          local = std::make_unique<Ranlxs>(seed + ks);
+          rr = local.get();
+        }
+        for (int k1 = -ks; k1 < ks; k1++)
+          for (int k2 = -ks; k2 < ks; k2++) {
+            drawOneFourierMode(f, ks, k1, k2, sigma, *rr, rev);
+            drawOneFourierMode(f, k1, ks, k2, sigma, *rr, rev);
+            drawOneFourierMode(f, k1, k2, ks, sigma, *rr, rev);
+          }
+      }
+    }
+  }
+  drawn = true;
+}
+
+void IC::zeroLevel(int level) {
+  if (level < 0 || level >= (int)grids.size()) throw std::runtime_error("Specified level does not exist.");
+  if (!drawn) draw();
+  std::fill(noise[level].d.begin(), noise[level].d.end(), 0.0);
+}
+
+void IC::applyPowerSpec() {
+  if (!drawn) draw();
+  if (!camb) throw std::runtime_error("No power spectrum has been set");
+  for (size_t L = 0; L < grids.size(); ++L) {
+    Field P;
+    powerSpectrumField(*camb, *grids[L], P, true);
+    Field &f = noise[L];
+    f.toFourier();
+    f.forEachK([&](cplx v, int kx, int ky, int kz) {
+      double spec = P.getK(kx, ky, kz).real();
+      if (spec != 0.0) spec = pow(spec, 0.5);
+      return v * spec;
+    });
+  }
+  powerApplied = true;
+}
+
+double IC::kcutForLevel(int level) const {
+  const Grid &g = *grids[level];
+  double kpix = double(g.n) * g.kmin();
+  return 0.3 * kpix;
+}
+
+// --- tricubic interpolation --------------------------------------------------
+static inline double cr_w(int i, double t) {
+  // 1D Catmull-Rom weights; the reference's 64-coefficient polynomial
+  // (tricubic.hpp:80-449) is their tensor product.
+  switch (i) {
+    case 0: return -0.5 * t * (t - 1) * (t - 1);
+    case 1: return (2 - 5 * t * t + 3 * t * t * t) / 2;
+    case 2: return -t * (-1 - 4 * t + 3 * t * t) / 2;
+    default: return (t - 1) * t * t / 2;
+  }
+}
+
+double evaluateInterpolated(const Field &f, double px, double py, double pz) {
+  const Grid &g = *f.g;
+  double loc[3] = {px - g.ox, py - g.oy, pz - g.oz};
+  int p0[3];
+  double t[3];
+  for (int a = 0; a < 3; ++a) {
+    double v = fmod(loc[a], g.periodic);
+    if (v < 0) v += g.periodic;
+    double u = v / g.dx - 0.5;
+    p0[a] = (int)floor(u);
+    t[a] = u - p0[a];
+  }
+  bool wrap = g.n == g.simEquivalentSize();
+  int sim = g.simEquivalentSize();
+  double w[3][4];
+  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < 4; ++i) w[a][i] = cr_w(i, t[a]);
+  double acc = 0;
+  for (int i = 0; i < 4; ++i) {
+    for (int j = 0; j < 4; ++j) {
+      for (int k = 0; k < 4; ++k) {
+        int c[3] = {p0[0] + i - 1, p0[1] + j - 1, p0[2] + k - 1};
+        for (int a = 0; a < 3; ++a) {
+          if (wrap) {
+            if (c[a] > sim - 1) c[a] -= sim;
+            if (c[a] < 0) c[a] += sim;
+          } else {
+            if (c[a] > g.n - 1) c[a] = g.n - 1;
+            if (c[a] < 0) c[a] = 0;
+          }
+        }
+        acc += w[0][i] * w[1][j] * w[2][k] * f.d[(size_t(c[0]) * g.n + c[1]) * g.n + c[2]];
+      }
+    }
+  }
+  return acc;
+}
+
+void addFieldFromDifferentGrid(Field &fine, const Field &coarse) {
+  // makeProxyGridToMatch -> SectionOfGrid(SuperSampleGrid(coarse)) ; addTo
+  const Grid &gf = *fine.g, &gc = *coarse.g;
+  int ratio = (int)llround(gc.dx / gf.dx);
+  double dxs = gc.dx / ratio;      // supersampled cell size
+  int off[3] = {(int)llround((gf.ox - gc.ox) / dxs), (int)llround((gf.oy - gc.oy) / dxs),
+                (int)llround((gf.oz - gc.oz) / dxs)};
+  int sim = (int)llround(gf.periodic / dxs);
+  int nsup = gc.n * ratio;
+  int n = gf.n;
+#pragma omp parallel for collapse(2)
+  for (int x = 0; x < n; ++x)
+    for (int y = 0; y < n; ++y)
+      for (int z = 0; z < n; ++z) {
+        int c[3] = {x + off[0], y + off[1], z + off[2]};
+        bool inside = true;
+        for (int a = 0; a < 3; ++a) {
+          if (c[a] > sim - 1) c[a] -= sim;
+          if (c[a] < 0) c[a] += sim;
+          if (c[a] < 0 || c[a] >= nsup) inside = false;
+        }
+        if (!inside) continue;
+        double p[3] = {c[0] * dxs + gc.ox + dxs / 2, c[1] * dxs + gc.oy + dxs / 2, c[2] * dxs + gc.oz + dxs / 2};
+        fine.d[(size_t(x) * n + y) * n + z] += evaluateInterpolated(coarse, p[0], p[1], p[2]);
+      }
+}
+
+// --- Zel'dovich + level combination -----------------------------------------
+static void zeldovichOneLevel(const Field &delta, std::vector<Field> &out) {
+  const Grid &g = *delta.g;
+  double kMin = g.kmin();
+  double nyquist = (g.n / 2) * kMin;   // getNyquistModeThatMustBeReal(grid) * kMin
+  out.assign(3, Field(g, true));
+  Field &dk = const_cast<Field &>(delta);
+  dk.toFourier();
+  // generateNewFourierFields: iterate the half space of delta, write three new fields
+  Field tmp = delta;
+  tmp.forEachK([&](cplx in, int ix, int iy, int iz) {
+    double kx = ix * kMin, ky = iy * kMin, kz = iz * kMin;
+    double kfft = kx * kx + ky * ky + kz * kz;
+    cplx rx(-in.imag() / kfft, in.real() / kfft);
+    cplx ry = rx, rz = rx;
+    rx *= kx;
+    ry *= ky;
+    rz *= kz;
+    if (kx == nyquist || kfft == 0) rx = 0;
+    if (ky == nyquist || kfft == 0) ry = 0;
+    if (kz == nyquist || kfft == 0) rz = 0;
+    out[0].setK(ix, iy, iz, rx);
+    out[1].setK(ix, iy, iz, ry);
+    out[2].setK(ix, iy, iz, rz);
+    return in;
+  });
+  for (auto &f : out) f.toReal();
+}
+
+static void addWithFilterCoarse(Field &fine, const Field &source, double kcut) {
+  Field temp = source;
+  temp.applyFilterFn([kcut](double k) { return lowPassFermi(k, kcut); });
+  temp.toReal();
+  fine.toReal();
+  addFieldFromDifferentGrid(fine, temp);
+}
+
+void IC::zeldovich() {
+  if (!powerApplied) applyPowerSpec();
+  size_t nl = grids.size();
+  for (auto &f : noise) f.toFourier();
+  psi.assign(nl, {});
+  for (size_t L = 0; L < nl; ++L) zeldovichOneLevel(noise[L], psi[L]);
+  if (nl >= 2) {
+    for (size_t L = 1; L < nl; ++L) {
+      double kcHP = kcutForLevel(L - 1);
+      auto hp = [kcHP](double k) { return highPassCov(k, kcHP); };
+      noise[L].applyFilterFn(hp);
+      for (auto &f : psi[L]) f.applyFilterFn(hp);
+      double kcLP = kcutForLevel(L - 1);
+      addWithFilterCoarse(noise[L], noise[L - 1], kcLP);
+      for (int c = 0; c < 3; ++c) addWithFilterCoarse(psi[L][c], psi[L - 1][c], kcLP);
+    }
+    combined = true;
+    for (auto &lv : psi)
+      for (auto &f : lv) f.toReal();
+  }
+}
+
+double IC::particleMass(int level) const {
+  const Grid &g = *grids[level];
+  double boxMass = 27.744948 * cosmo.OmegaM0 * pow(g.periodic, 3.0);
+  return boxMass * pow(g.dx / g.periodic, 3.0);
+}
+
+double IC::particleEps(int level) const { return grids[level]->dx * 1.0 * 0.01075; }
+
+void IC::particle(int level, size_t id, double pos[3], double vel[3]) const {
+  const Grid &g = *grids[level];
+  double ratio = zeldovichVelocityToOffsetRatio(cosmo);
+  int n = g.n;
+  size_t x = id / (size_t(n) * n), y = (id / n) % n, z = id % n;
+  double c[3] = {x * g.dx + g.ox + g.dx / 2, y * g.dx + g.oy + g.dx / 2, z * g.dx + g.oz + g.dx / 2};
+  for (int a = 0; a < 3; ++a) {
+    double off = psi[level][a].d[id];
+    vel[a] = off * ratio;
+    double p = off + c[a];
+    p = fmod(p, g.periodic);
+    if (p < 0) p += g.periodic;
+    pos[a] = p;
+  }
+}
+
+}  // namespace orc

@@ -1,0 +1,192 @@
+// C entry points of the CPU oracle, for ctypes use by tests/ and bench.py's
+// cpu_baseline leg.  TEST INFRASTRUCTURE ONLY (see oracle.hpp).
+#include "oracle.hpp"
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <chrono>
+
+using namespace orc;
+
+static thread_local std::string g_err;
+
+#define ORC_TRY(body)                \
+  try {                              \
+    body;                            \
+    return 0;                        \
+  } catch (std::exception & e) {     \
+    g_err = e.what();                \
+    return 1;                        \
+  }
+
+extern "C" {
+
+const char *orc_last_error() { return g_err.c_str(); }
+
+int orc_ranlxs_raw(unsigned long seed, size_t n, uint32_t *out) {
+  ORC_TRY({
+    Ranlxs r(seed);
+    for (size_t i = 0; i < n; ++i) out[i] = r.get();
+  })
+}
+
+int orc_gaussians(unsigned long seed, size_t n, double *out, size_t *raw_used) {
+  ORC_TRY({
+    Ranlxs r(seed);
+    for (size_t i = 0; i < n; ++i) out[i] = gaussian_ziggurat(r);
+    if (raw_used) *raw_used = (r.nblocks) * 24 - (24 - r.pos);
+  })
+}
+
+int orc_zig_tables(double *ytab, double *wtab, uint32_t *ktab) {
+  ORC_TRY({
+    const ZigTables &t = ZigTables::get();
+    memcpy(ytab, t.ytab, sizeof t.ytab);
+    memcpy(wtab, t.wtab, sizeof t.wtab);
+    memcpy(ktab, t.ktab, sizeof t.ktab);
+  })
+}
+
+// in-place transforms on a padded buffer of 2 N^2 (N/2+1) doubles
+int orc_fft_r2c(int n, double *buf) {
+  ORC_TRY({
+    std::vector<double> d(buf, buf + 2 * size_t(n) * n * (n / 2 + 1));
+    fft_r2c_inplace(d, n);
+    memcpy(buf, d.data(), d.size() * sizeof(double));
+  })
+}
+int orc_fft_c2r(int n, double *buf) {
+  ORC_TRY({
+    std::vector<double> d(buf, buf + 2 * size_t(n) * n * (n / 2 + 1));
+    fft_c2r_inplace(d, n);
+    memcpy(buf, d.data(), d.size() * sizeof(double));
+  })
+}
+
+int orc_growth_factor(double Om, double Ol, double a, double *out) {
+  ORC_TRY({
+    Cosmology c;
+    c.OmegaM0 = Om;
+    c.OmegaLambda0 = Ol;
+    c.scalefactor = a;
+    *out = growthFactor(c);
+  })
+}
+
+// --- generator handle ---------------------------------------------------------
+void *orc_ic_create() { return new IC(); }
+void orc_ic_destroy(void *h) { delete (IC *)h; }
+
+int orc_ic_set_cosmology(void *h, double Om, double Ol, double Ob, double hubble, double z, double s8, double ns) {
+  ORC_TRY({
+    IC &ic = *(IC *)h;
+    ic.cosmo.OmegaM0 = Om;
+    ic.cosmo.OmegaLambda0 = Ol;
+    ic.cosmo.OmegaBaryons0 = Ob;
+    ic.cosmo.hubble = hubble;
+    ic.cosmo.redshift = z;
+    ic.cosmo.scalefactor = 1. / (z + 1.);
+    ic.cosmo.sigma8 = s8;
+    ic.cosmo.ns = ns;
+  })
+}
+
+int orc_ic_set_camb(void *h, const double *table, size_t nvalues, int ncols) {
+  ORC_TRY({
+    IC &ic = *(IC *)h;
+    ic.camb = std::make_unique<CAMB>();
+    ic.camb->init(ic.cosmo, std::vector<double>(table, table + nvalues), ncols);
+  })
+}
+
+int orc_ic_camb_info(void *h, double *amplitude, double *sigma8pre) {
+  ORC_TRY({
+    IC &ic = *(IC *)h;
+    *amplitude = ic.camb->amplitude;
+    *sigma8pre = ic.camb->varianceInSphere(8.);
+  })
+}
+
+int orc_ic_pk(void *h, int n_k, const double *k, double *out) {
+  ORC_TRY({
+    IC &ic = *(IC *)h;
+    for (int i = 0; i < n_k; ++i) out[i] = ic.camb->P(k[i], true);
+  })
+}
+
+int orc_ic_base_grid(void *h, double box, int n) { ORC_TRY(((IC *)h)->initBaseGrid(box, n)) }
+int orc_ic_zoom_at(void *h, int x0, int y0, int z0, int fac, int n) {
+  ORC_TRY(((IC *)h)->initZoomGridWithLowLeftCornerAt(x0, y0, z0, fac, n))
+}
+int orc_ic_seed(void *h, unsigned long seed, int mode) { ORC_TRY(((IC *)h)->setSeed(seed, (SeedMode)mode)) }
+int orc_ic_draw(void *h) { ORC_TRY(((IC *)h)->draw()) }
+int orc_ic_zero_level(void *h, int level) { ORC_TRY(((IC *)h)->zeroLevel(level)) }
+int orc_ic_apply_ps(void *h) { ORC_TRY(((IC *)h)->applyPowerSpec()) }
+int orc_ic_zeldovich(void *h) { ORC_TRY(((IC *)h)->zeldovich()) }
+
+// which: 0 = noise/delta field, 1..3 = psi x,y,z.  fourier: 0 -> real space
+// N^3 array, 1 -> padded FFTW layout (2 N^2 (N/2+1) doubles).
+int orc_ic_get_field(void *h, int which, int level, int fourier, double *out) {
+  ORC_TRY({
+    IC &ic = *(IC *)h;
+    Field f = which == 0 ? ic.noise.at(level) : ic.psi.at(level).at(which - 1);
+    if (fourier) {
+      f.toFourier();
+      memcpy(out, f.d.data(), f.d.size() * sizeof(double));
+    } else {
+      f.toReal();
+      memcpy(out, f.d.data(), f.g->n3() * sizeof(double));
+    }
+  })
+}
+
+static void particles_level(IC &ic, int level, double *pos, double *vel) {
+  size_t n3 = ic.grids.at(level)->n3();
+#pragma omp parallel for
+  for (size_t i = 0; i < n3; ++i) ic.particle(level, i, pos + 3 * i, vel + 3 * i);
+}
+
+int orc_ic_particles(void *h, int level, double *pos, double *vel, double *mass, double *eps) {
+  ORC_TRY({
+    IC &ic = *(IC *)h;
+    particles_level(ic, level, pos, vel);
+    *mass = ic.particleMass(level);
+    *eps = ic.particleEps(level);
+  })
+}
+
+int orc_ic_velocity_ratio(void *h, double *out) { ORC_TRY(*out = zeldovichVelocityToOffsetRatio(((IC *)h)->cosmo)) }
+
+// Times the reference stage order seed -> delta, psi on one level at size n
+// (serial real-space stream, r2c, sqrt P, 3 c2r, c2r of delta).  Used by
+// bench.py's cpu_baseline leg.
+int orc_time_pipeline(double box, int n, unsigned long seed, const double *table, size_t nvalues, int ncols,
+                      double Om, double Ol, double s8, double ns, double z, double *seconds_total,
+                      double *seconds_rng) {
+  ORC_TRY({
+    auto t0 = std::chrono::steady_clock::now();
+    IC ic;
+    ic.cosmo.OmegaM0 = Om;
+    ic.cosmo.OmegaLambda0 = Ol;
+    ic.cosmo.sigma8 = s8;
+    ic.cosmo.ns = ns;
+    ic.cosmo.redshift = z;
+    ic.cosmo.scalefactor = 1. / (z + 1.);
+    ic.camb = std::make_unique<CAMB>();
+    ic.camb->init(ic.cosmo, std::vector<double>(table, table + nvalues), ncols);
+    ic.initBaseGrid(box, n);
+    ic.setSeed(seed, SeedMode::Real);
+    auto t1 = std::chrono::steady_clock::now();
+    ic.draw();
+    auto t2 = std::chrono::steady_clock::now();
+    ic.applyPowerSpec();
+    ic.zeldovich();
+    ic.noise[0].toReal();
+    auto t3 = std::chrono::steady_clock::now();
+    *seconds_total = std::chrono::duration<double>(t3 - t1).count();
+    *seconds_rng = std::chrono::duration<double>(t2 - t1).count();
+    (void)t0;
+  })
+}
+
+}  // extern "C"
