@@ -1,0 +1,865 @@
+// C-ABI implementation: the ICGenerator field path on one GPU.
+//
+// The orchestration follows the reference's ICGenerator (genetIC/src/ic.hpp)
+// and MultiLevelParticleGenerator (particles/multilevelgenerator.hpp:78-128);
+// the arithmetic runs in the kernels of fft.hip, rng.hip and kspace.hip.
+#include "../../include/gic.h"
+#include "gic_common.h"
+#include "fft.h"
+#include "rng.h"
+#include "kspace.h"
+#include <vector>
+#include <string>
+#include <map>
+#include <memory>
+#include <cmath>
+#include <cstring>
+#include <algorithm>
+#include <limits>
+
+using namespace gic;
+
+namespace {
+
+// ---------------------------------------------------------------- host maths --
+struct Cosmo {
+  double Om = 0.3, Ol = 0, Ob = 0, h = 0.701, s8 = 0, ns = 0.96, z = 0, a = 1;
+};
+
+// growthFactor, cosmology/parameters.hpp:55-70 (a^3 through single-precision powf there)
+double growth_factor(double a, double Om, double Ol) {
+  double Hsq = Om / (double)powf((float)a, 3.0f) + (1. - Om - Ol) / a / a + Ol;
+  return 2.5 * a * Om / pow(a, 3.0) / Hsq /
+         (pow(Om / Hsq / a / a / a, (double)(4.f / 7.f)) - Ol / Hsq + (1. + 0.5 * Om / pow(a, 3.0) / Hsq) * (1. + 1. / 70. * Ol / Hsq));
+}
+
+// Natural cubic spline (the gsl_interp_cspline construction): second-derivative
+// coefficients from a symmetric tridiagonal system with zero end conditions.
+struct Spline {
+  std::vector<double> x, y, c;
+  void init(const std::vector<double> &xs, const std::vector<double> &ys) {
+    x = xs;
+    y = ys;
+    size_t n = x.size();
+    c.assign(n, 0.0);
+    if (n < 3) return;
+    size_t m = n - 2;
+    std::vector<double> dg(m), of(m), g(m);
+    for (size_t i = 0; i < m; ++i) {
+      double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
+      of[i] = h1;
+      dg[i] = 2.0 * (h0 + h1);
+      g[i] = 3.0 * ((y[i + 2] - y[i + 1]) * (h1 != 0 ? 1.0 / h1 : 0.0) - (y[i + 1] - y[i]) * (h0 != 0 ? 1.0 / h0 : 0.0));
+    }
+    if (m == 1) {
+      c[1] = g[0] / dg[0];
+      return;
+    }
+    std::vector<double> gam(m), al(m), zz(m);
+    al[0] = dg[0];
+    gam[0] = of[0] / al[0];
+    for (size_t i = 1; i + 1 < m; ++i) {
+      al[i] = dg[i] - of[i - 1] * gam[i - 1];
+      gam[i] = of[i] / al[i];
+    }
+    al[m - 1] = dg[m - 1] - of[m - 2] * gam[m - 2];
+    zz[0] = g[0];
+    for (size_t i = 1; i < m; ++i) zz[i] = g[i] - gam[i - 1] * zz[i - 1];
+    c[m] = zz[m - 1] / al[m - 1];
+    for (long i = long(m) - 2; i >= 0; --i) c[i + 1] = zz[i] / al[i] - gam[i] * c[i + 2];
+  }
+  double eval(double xv) const {
+    size_t n = x.size();
+    if (xv < x[0] || xv > x[n - 1]) return 0;
+    size_t lo = 0, hi = n - 1;
+    while (hi > lo + 1) {
+      size_t i = (hi + lo) / 2;
+      if (x[i] > xv) hi = i;
+      else lo = i;
+    }
+    double dx = x[lo + 1] - x[lo];
+    if (!(dx > 0)) return y[lo];
+    double dy = y[lo + 1] - y[lo], dl = xv - x[lo];
+    double bi = dy / dx - dx * (c[lo + 1] + 2.0 * c[lo]) / 3.0, di = (c[lo + 1] - c[lo]) / (3.0 * dx);
+    return y[lo] + dl * (bi + dl * (c[lo] + dl * di));
+  }
+};
+
+struct LogSpline {
+  Spline s;
+  double minx = 0, maxx = 0;
+  void init(const std::vector<double> &xv, const std::vector<double> &yv) {
+    std::vector<double> lx, ly;
+    for (size_t i = 0; i < xv.size(); ++i) {
+      double a = std::log(xv[i]), b = std::log(yv[i]);
+      if (std::isfinite(a) && std::isfinite(b)) {
+        lx.push_back(a);
+        ly.push_back(b);
+      }
+    }
+    if (lx.size() < 2)
+      throw std::range_error(
+          "Could not find enough valid values in the transfer function. This may mean that one of the transfer "
+          "function columns is full of zeros.");
+    s.init(lx, ly);
+    minx = lx.front();
+    maxx = lx.back();
+  }
+  double operator()(double k) const {
+    double l = std::log(k);
+    if (l < minx || l > maxx) return 0;
+    return std::exp(s.eval(l));
+  }
+};
+
+// ---------------------------------------------------------------- timing -----
+struct EventTimer : TimingHook {
+  struct Rec {
+    std::string name;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, long>> totals;
+  hipEvent_t get() {
+    if (!pool.empty()) {
+      hipEvent_t e = pool.back();
+      pool.pop_back();
+      return e;
+    }
+    hipEvent_t e;
+    GIC_HIP(hipEventCreate(&e));
+    return e;
+  }
+  void begin(const char *name, hipStream_t st) override {
+    Rec r{name, get(), nullptr};
+    GIC_HIP(hipEventRecord(r.a, st));
+    pending.push_back(r);
+  }
+  void end(const char *name, hipStream_t st) override {
+    for (auto it = pending.rbegin(); it != pending.rend(); ++it)
+      if (it->name == name && it->b == nullptr) {
+        it->b = get();
+        GIC_HIP(hipEventRecord(it->b, st));
+        break;
+      }
+  }
+  void collect() {
+    for (auto &r : pending) {
+      if (!r.b) continue;
+      GIC_HIP(hipEventSynchronize(r.b));
+      float ms = 0;
+      GIC_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      auto &t = totals[r.name];
+      t.first += ms;
+      t.second += 1;
+      pool.push_back(r.a);
+      pool.push_back(r.b);
+    }
+    pending.clear();
+  }
+  ~EventTimer() override {
+    for (auto &r : pending) {
+      (void)hipEventDestroy(r.a);
+      if (r.b) (void)hipEventDestroy(r.b);
+    }
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  void ensure(size_t b) {
+    if (b <= bytes) return;
+    release();
+    GIC_HIP(hipMalloc(&p, b));
+    bytes = b;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  ~DevBuf() { release(); }
+};
+
+struct Level {
+  LevelGeom g;
+  DevBuf noise;            // white noise, then the overdensity
+  DevBuf psi[3];
+  DevBuf sqrtp;            // sqrt P by |k|^2, element type T
+  bool noise_fourier = false;
+  bool psi_fourier = false;
+  bool have_psi = false;
+};
+
+}  // namespace
+
+struct gic_context {
+  int device = 0;
+  int prec = 64;
+  hipStream_t st = nullptr;
+  std::string err;
+  Cosmo cosmo;
+  // frozen at the camb command
+  bool have_camb = false;
+  LogSpline tf_all;
+  std::vector<double> kpts;
+  double amplitude = 0, ns_frozen = 0;
+  SplineHost spline_dev;
+  DevBuf spl_x, spl_y, spl_c;
+  std::vector<std::unique_ptr<Level>> levels;
+  unsigned long seed = 0;
+  int seed_mode = 0;
+  bool seeded = false, drawn = false, power_applied = false, zeldovich_done = false;
+  DevBuf work, stage, tab_a, tab_b;
+  std::unique_ptr<EventTimer> timer;
+  TimingHook *hook() { return timer.get(); }
+  size_t tsize() const { return prec == 32 ? 4 : 8; }
+};
+
+namespace {
+
+size_t total_bytes(gic_context *c) {
+  size_t b = c->work.bytes + c->stage.bytes + c->tab_a.bytes + c->tab_b.bytes + c->spl_x.bytes * 3;
+  for (auto &l : c->levels) {
+    b += l->noise.bytes + l->sqrtp.bytes;
+    for (auto &p : l->psi) b += p.bytes;
+  }
+  return b;
+}
+
+Level &level_at(gic_context *c, int level) {
+  if (level < 0 || level >= (int)c->levels.size()) throw std::runtime_error("Specified level does not exist.");
+  return *c->levels[level];
+}
+
+template <typename T> void to_fourier(gic_context *c, DevBuf &b, int n, bool &flag) {
+  if (flag) return;
+  fft3d<T>((T *)b.p, n, -1, c->st, c->hook());
+  flag = true;
+}
+template <typename T> void to_real(gic_context *c, DevBuf &b, int n, bool &flag) {
+  if (!flag) return;
+  fft3d<T>((T *)b.p, n, +1, c->st, c->hook());
+  flag = false;
+}
+
+void alloc_field(gic_context *c, DevBuf &b, const LevelGeom &g) {
+  b.ensure(g.padded_reals() * c->tsize());
+}
+
+// camb.hpp:300-328: sigma(R) by a 50000-step rectangle rule with accumulated k
+double variance_in_sphere(gic_context *c, double radius) {
+  double s = 0., k, t;
+  double amp = 9. / 2. / M_PI / M_PI;
+  double kmax = std::min(c->kpts.back(), 200.0 / radius) * 0.999999;
+  double kmin = c->kpts[0] * 1.000001;
+  double dk = (kmax - kmin) / 50000.;
+  for (k = kmin; k < kmax; k += dk) {
+    t = c->tf_all(k);
+    double kr = k * radius;
+    double w = (sin(kr) - kr * cos(kr)) / (kr * kr * kr);
+    s += pow(k, c->ns_frozen + 2.) * w * w * t * t;
+  }
+  return sqrt(s * amp * dk);
+}
+
+template <typename T> void build_sqrtp(gic_context *c, Level &L) {
+  if (!c->have_camb) throw std::runtime_error("No power spectrum has been set (use the camb command)");
+  long h = L.g.n / 2;
+  long nent = 3 * h * h + 1;
+  L.sqrtp.ensure(nent * sizeof(T));
+  double kw = 2. * M_PI / L.g.box;
+  double norm = kw * kw * kw / pow(2.f * M_PI, 3.f) * double(L.g.n3());
+  sqrtp_table<T>(c->spline_dev, c->amplitude, c->ns_frozen, L.g.kmin(), norm, nent, (T *)L.sqrtp.p, c->st);
+}
+
+template <typename T> void do_draw(gic_context *c) {
+  if (!c->seeded) throw std::runtime_error("The random number generator has not been seeded");
+  if (c->drawn) throw std::runtime_error("Trying to re-draw the random field after it was already initialised");
+  if (c->levels.empty()) throw std::runtime_error("Cannot draw a random field before the base grid is initialised");
+  for (auto &L : c->levels) alloc_field(c, L->noise, L->g);
+  if (c->seed_mode == GIC_SEED_REAL_SPACE) {
+    std::vector<RngLevelOut> outs;
+    size_t total = 0;
+    for (auto &L : c->levels) {
+      outs.push_back({L->noise.p, L->g.n, L->g.n3()});
+      total += L->g.n3();
+    }
+    RngPlan plan;
+    rng_plan(c->seed, total, plan);
+    c->work.ensure(rng_workspace_bytes(plan));
+    rng_draw_real<T>(plan, outs, c->work.p, c->work.bytes, c->st, c->hook());
+    for (auto &L : c->levels) {
+      L->noise_fourier = false;
+      to_fourier<T>(c, L->noise, L->g.n, L->noise_fourier);
+      // tools::set_zero(fieldData[0]): the real part of the k=0 mode
+      fill<T>((T *)L->noise.p, 1, T(0), c->st);
+    }
+  } else {
+    throw std::runtime_error("Fourier-space seeding is not implemented on the GPU path yet");
+  }
+  c->drawn = true;
+}
+
+template <typename T> void do_apply_ps(gic_context *c) {
+  if (!c->drawn) do_draw<T>(c);
+  for (auto &Lp : c->levels) {
+    Level &L = *Lp;
+    build_sqrtp<T>(c, L);
+    to_fourier<T>(c, L.noise, L.g.n, L.noise_fourier);
+    if (c->hook()) c->hook()->begin("apply_sqrtp", c->st);
+    mul_table<T>((T *)L.noise.p, L.g.n, (const T *)L.sqrtp.p, c->st);
+    if (c->hook()) c->hook()->end("apply_sqrtp", c->st);
+  }
+  c->power_applied = true;
+}
+
+double kcut_for_level(gic_context *c, int level) {
+  const LevelGeom &g = c->levels[level]->g;
+  return 0.3 * (double(g.n) * g.kmin());   // FRACTIONAL_K_SPLIT * k_pixel (filterfamily.hpp:110-118)
+}
+
+InterpGeom interp_geom(const LevelGeom &gf, const LevelGeom &gc) {
+  InterpGeom ig;
+  int ratio = (int)llround(gc.dx / gf.dx);
+  double dxs = gc.dx / ratio;
+  ig.off[0] = (int)llround((gf.ox - gc.ox) / dxs);
+  ig.off[1] = (int)llround((gf.oy - gc.oy) / dxs);
+  ig.off[2] = (int)llround((gf.oz - gc.oz) / dxs);
+  ig.sim = (int)llround(gf.periodic / dxs);
+  ig.nsup = gc.n * ratio;
+  ig.dxs = dxs;
+  ig.dxc = gc.dx;
+  ig.co[0] = gc.ox;
+  ig.co[1] = gc.oy;
+  ig.co[2] = gc.oz;
+  ig.periodic = gc.periodic;
+  ig.nc = gc.n;
+  ig.csim = (int)llround(gc.periodic / gc.dx);
+  ig.wrap = gc.n == ig.csim ? 1 : 0;
+  return ig;
+}
+
+// temp <- LP-filtered real-space copy of a coarse field.  `src` is either a
+// spectrum (fourier=true) or a real field, as in
+// addFieldFromDifferentGridWithFilter's FILTER_ON_COARSE_GRID branch
+// (field.hpp:816-830): copy, filter in Fourier space, transform to real.
+template <typename T>
+void lowpass_copy(gic_context *c, const DevBuf &src, bool fourier, const LevelGeom &g, const T *lp) {
+  size_t bytes = g.padded_reals() * sizeof(T);
+  c->stage.ensure(bytes);
+  GIC_HIP(hipMemcpyAsync(c->stage.p, src.p, bytes, hipMemcpyDeviceToDevice, c->st));
+  bool f = fourier;
+  to_fourier<T>(c, c->stage, g.n, f);
+  mul_table<T>((T *)c->stage.p, g.n, lp, c->st);
+  to_real<T>(c, c->stage, g.n, f);
+}
+
+template <typename T> void do_zeldovich(gic_context *c) {
+  if (!c->power_applied) do_apply_ps<T>(c);
+  size_t nl = c->levels.size();
+  // ZA per level; zoom levels get their high-pass filter (cut of the level
+  // above) on delta and psi while still in Fourier space.
+  for (size_t l = 0; l < nl; ++l) {
+    Level &L = *c->levels[l];
+    to_fourier<T>(c, L.noise, L.g.n, L.noise_fourier);
+    for (auto &p : L.psi) alloc_field(c, p, L.g);
+    const T *hp = nullptr;
+    if (l >= 1) {
+      long h = L.g.n / 2, nent = 3 * h * h + 1;
+      c->tab_a.ensure(nent * sizeof(T));
+      filter_table<T>(1, kcut_for_level(c, l - 1), L.g.kmin(), nent, (T *)c->tab_a.p, c->st);
+      hp = (const T *)c->tab_a.p;
+    }
+    if (c->hook()) c->hook()->begin("zeldovich", c->st);
+    zeldovich<T>((const T *)L.noise.p, L.g.n, L.g.kmin(), (T *)L.psi[0].p, (T *)L.psi[1].p, (T *)L.psi[2].p, hp,
+                 c->st);
+    if (c->hook()) c->hook()->end("zeldovich", c->st);
+    if (hp) mul_table<T>((T *)L.noise.p, L.g.n, hp, c->st);
+    L.psi_fourier = true;
+    L.have_psi = true;
+  }
+  // combine: fine <- HP(fine) + interpolate(LP(coarse)), coarse already combined
+  for (size_t l = 1; l < nl; ++l) {
+    Level &F = *c->levels[l];
+    Level &C = *c->levels[l - 1];
+    long h = C.g.n / 2, nent = 3 * h * h + 1;
+    c->tab_b.ensure(nent * sizeof(T));
+    filter_table<T>(0, kcut_for_level(c, l - 1), C.g.kmin(), nent, (T *)c->tab_b.p, c->st);
+    const T *lp = (const T *)c->tab_b.p;
+    InterpGeom ig = interp_geom(F.g, C.g);
+    // overdensity
+    lowpass_copy<T>(c, C.noise, C.noise_fourier, C.g, lp);
+    to_real<T>(c, F.noise, F.g.n, F.noise_fourier);
+    if (c->hook()) c->hook()->begin("interp_add", c->st);
+    interp_add<T>((T *)F.noise.p, F.g.n, (const T *)c->stage.p, ig, c->st);
+    if (c->hook()) c->hook()->end("interp_add", c->st);
+    for (int a = 0; a < 3; ++a) {
+      lowpass_copy<T>(c, C.psi[a], C.psi_fourier, C.g, lp);
+      bool f = F.psi_fourier;
+      to_real<T>(c, F.psi[a], F.g.n, f);
+      if (c->hook()) c->hook()->begin("interp_add", c->st);
+      interp_add<T>((T *)F.psi[a].p, F.g.n, (const T *)c->stage.p, ig, c->st);
+      if (c->hook()) c->hook()->end("interp_add", c->st);
+    }
+    F.psi_fourier = false;
+    // the coarse psi, if still spectra, are no longer needed in Fourier space
+    if (C.psi_fourier) {
+      for (int a = 0; a < 3; ++a) {
+        bool f = true;
+        to_real<T>(c, C.psi[a], C.g.n, f);
+      }
+      C.psi_fourier = false;
+    }
+  }
+  for (size_t l = 0; l < nl; ++l) {
+    Level &L = *c->levels[l];
+    if (L.psi_fourier) {
+      for (int a = 0; a < 3; ++a) {
+        bool f = true;
+        to_real<T>(c, L.psi[a], L.g.n, f);
+      }
+      L.psi_fourier = false;
+    }
+  }
+  c->zeldovich_done = true;
+}
+
+template <typename T> void do_copy_field(gic_context *c, int field, int level, int space, void *host, int dtype) {
+  Level &L = level_at(c, level);
+  DevBuf *b;
+  bool *flag;
+  if (field == GIC_FIELD_OVERDENSITY) {
+    if (!c->drawn) do_draw<T>(c);
+    b = &L.noise;
+    flag = &L.noise_fourier;
+  } else if (field >= 1 && field <= 3) {
+    if (!L.have_psi) throw std::runtime_error("Attempt to generate particles before they have been calculated");
+    b = &L.psi[field - 1];
+    flag = &L.psi_fourier;
+  } else {
+    throw std::runtime_error("unknown field");
+  }
+  int n = L.g.n;
+  if (space == GIC_REAL_SPACE) {
+    if (field == GIC_FIELD_OVERDENSITY) to_real<T>(c, *b, n, *flag);
+    else if (*flag) {
+      for (int a = 0; a < 3; ++a) {
+        bool f = true;
+        to_real<T>(c, L.psi[a], n, f);
+      }
+      *flag = false;
+    }
+    size_t n3 = L.g.n3();
+    size_t ob = n3 * (dtype == GIC_DTYPE_F32 ? 4 : 8);
+    c->stage.ensure(ob);
+    if (dtype == GIC_DTYPE_F32) unpad<T, float>((const T *)b->p, n, (float *)c->stage.p, c->st);
+    else unpad<T, double>((const T *)b->p, n, (double *)c->stage.p, c->st);
+    GIC_HIP(hipMemcpyAsync(host, c->stage.p, ob, hipMemcpyDeviceToHost, c->st));
+  } else {
+    if (field == GIC_FIELD_OVERDENSITY) to_fourier<T>(c, *b, n, *flag);
+    else if (!*flag) {
+      for (int a = 0; a < 3; ++a) {
+        bool f = false;
+        to_fourier<T>(c, L.psi[a], n, f);
+      }
+      *flag = true;
+    }
+    size_t nreal = L.g.padded_reals();
+    size_t ob = nreal * (dtype == GIC_DTYPE_F32 ? 4 : 8);
+    c->stage.ensure(ob);
+    if (dtype == GIC_DTYPE_F32) convert<T, float>((const T *)b->p, nreal, (float *)c->stage.p, c->st);
+    else convert<T, double>((const T *)b->p, nreal, (double *)c->stage.p, c->st);
+    GIC_HIP(hipMemcpyAsync(host, c->stage.p, ob, hipMemcpyDeviceToHost, c->st));
+  }
+  GIC_HIP(hipStreamSynchronize(c->st));
+}
+
+template <typename T> void do_set_field(gic_context *c, int field, int level, int space, const void *host, int dtype) {
+  Level &L = level_at(c, level);
+  if (field != GIC_FIELD_OVERDENSITY) throw std::runtime_error("only the overdensity / noise field can be set");
+  alloc_field(c, L.noise, L.g);
+  int n = L.g.n;
+  size_t es = dtype == GIC_DTYPE_F32 ? 4 : 8;
+  if (space == GIC_REAL_SPACE) {
+    size_t ob = L.g.n3() * es;
+    c->stage.ensure(ob);
+    GIC_HIP(hipMemcpyAsync(c->stage.p, host, ob, hipMemcpyHostToDevice, c->st));
+    if (dtype == GIC_DTYPE_F32) pad<float, T>((const float *)c->stage.p, n, (T *)L.noise.p, c->st);
+    else pad<double, T>((const double *)c->stage.p, n, (T *)L.noise.p, c->st);
+    L.noise_fourier = false;
+  } else {
+    size_t nreal = L.g.padded_reals();
+    c->stage.ensure(nreal * es);
+    GIC_HIP(hipMemcpyAsync(c->stage.p, host, nreal * es, hipMemcpyHostToDevice, c->st));
+    if (dtype == GIC_DTYPE_F32) convert<float, T>((const float *)c->stage.p, nreal, (T *)L.noise.p, c->st);
+    else convert<double, T>((const double *)c->stage.p, nreal, (T *)L.noise.p, c->st);
+    L.noise_fourier = true;
+  }
+  GIC_HIP(hipStreamSynchronize(c->st));
+}
+
+double velocity_ratio(const Cosmo &c) {
+  double f = 1.0;
+  return f * 100. * sqrt(c.Om / (c.a * c.a * c.a) + c.Ol) * sqrt(c.a);
+}
+
+template <typename T>
+void do_particles(gic_context *c, int level, const uint64_t *cells, size_t start, size_t count, float *pos,
+                  float *vel) {
+  Level &L = level_at(c, level);
+  if (!L.have_psi) throw std::runtime_error("Attempt to generate particles before they have been calculated");
+  if (L.psi_fourier) {
+    for (int a = 0; a < 3; ++a) {
+      bool f = true;
+      to_real<T>(c, L.psi[a], L.g.n, f);
+    }
+    L.psi_fourier = false;
+  }
+  size_t n3 = L.g.n3();
+  if (cells) {
+    for (size_t i = 0; i < count; ++i)
+      if (cells[i] >= n3) throw std::out_of_range("particle cell index out of range for the level");
+  } else if (start > n3 || count > n3 - start) {
+    throw std::out_of_range("particle range exceeds the level's cells");
+  }
+  ParticleGeom pg;
+  pg.dx = L.g.dx;
+  pg.o[0] = L.g.ox;
+  pg.o[1] = L.g.oy;
+  pg.o[2] = L.g.oz;
+  pg.periodic = L.g.periodic;
+  pg.vratio = velocity_ratio(c->cosmo);
+  // device staging: cell list + outputs, processed in slices
+  const size_t slice = size_t(1) << 24;
+  for (size_t s0 = 0; s0 < count; s0 += slice) {
+    size_t m = std::min(slice, count - s0);
+    size_t need = m * 24 + (cells ? m * 8 : 0);
+    c->stage.ensure(need);
+    float *dpos = (float *)c->stage.p;
+    float *dvel = dpos + 3 * m;
+    uint64_t *dcells = nullptr;
+    if (cells) {
+      dcells = (uint64_t *)(dvel + 3 * m);
+      GIC_HIP(hipMemcpyAsync(dcells, cells + s0, m * 8, hipMemcpyHostToDevice, c->st));
+    }
+    particles<T>((const T *)L.psi[0].p, (const T *)L.psi[1].p, (const T *)L.psi[2].p, L.g.n, pg, dcells, start + s0, m,
+                 dpos, dvel, c->st);
+    GIC_HIP(hipMemcpyAsync(pos + 3 * s0, dpos, m * 12, hipMemcpyDeviceToHost, c->st));
+    GIC_HIP(hipMemcpyAsync(vel + 3 * s0, dvel, m * 12, hipMemcpyDeviceToHost, c->st));
+    GIC_HIP(hipStreamSynchronize(c->st));
+  }
+}
+
+template <typename F> int guarded(gic_context *c, F &&f) {
+  if (!c) return 2;
+  try {
+    GIC_HIP(hipSetDevice(c->device));
+    f();
+    return 0;
+  } catch (std::exception &e) {
+    c->err = e.what();
+    return 1;
+  }
+}
+
+#define DISPATCH(c, fn, ...)                  \
+  do {                                        \
+    if ((c)->prec == 32) fn<float>(__VA_ARGS__); \
+    else fn<double>(__VA_ARGS__);             \
+  } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int gic_create(gic_context **out, int device, int precision) {
+  if (!out) return 2;
+  *out = nullptr;
+  if (precision != 32 && precision != 64) return 3;
+  auto *c = new gic_context();
+  c->device = device;
+  c->prec = precision;
+  try {
+    int ndev = 0;
+    GIC_HIP(hipGetDeviceCount(&ndev));
+    if (device < 0 || device >= ndev) throw std::runtime_error("no such HIP device");
+    GIC_HIP(hipSetDevice(device));
+    GIC_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+  } catch (std::exception &e) {
+    delete c;
+    return 4;
+  }
+  *out = c;
+  return 0;
+}
+
+void gic_destroy(gic_context *c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->st) (void)hipStreamSynchronize(c->st);
+  c->timer.reset();
+  c->levels.clear();
+  if (c->st) (void)hipStreamDestroy(c->st);
+  delete c;
+}
+
+const char *gic_last_error(const gic_context *c) { return c ? c->err.c_str() : "null context"; }
+
+int gic_set_cosmology(gic_context *c, double Om, double Ol, double Ob, double hubble, double s8, double ns, double z) {
+  return guarded(c, [&] {
+    c->cosmo.Om = Om;
+    c->cosmo.Ol = Ol;
+    c->cosmo.Ob = Ob;
+    c->cosmo.h = hubble;
+    c->cosmo.s8 = s8;
+    c->cosmo.ns = ns;
+    c->cosmo.z = z;
+    c->cosmo.a = 1. / (z + 1.);
+  });
+}
+
+int gic_set_camb_table(gic_context *c, const double *v, size_t nv, int ncols) {
+  return guarded(c, [&] {
+    if (ncols != 7 && ncols != 13) throw std::runtime_error("CAMB transfer file doesn't have a sensible number of columns");
+    if (nv < size_t(ncols) * 2) throw std::runtime_error("CAMB transfer file is too short");
+    double tnorm = v[1];
+    std::vector<double> kp, tall;
+    for (size_t j = 0; j < nv / ncols; ++j) {
+      if (v[ncols * j] > 0) {
+        kp.push_back(v[ncols * j]);
+        tall.push_back(v[ncols * j + 6] / tnorm);
+      }
+    }
+    c->kpts = kp;
+    c->tf_all = LogSpline();
+    c->tf_all.init(kp, tall);
+    c->ns_frozen = c->cosmo.ns;
+    double gf = growth_factor(c->cosmo.a, c->cosmo.Om, c->cosmo.Ol) / growth_factor(1.0, c->cosmo.Om, c->cosmo.Ol);
+    double s8pre = variance_in_sphere(c, 8.);
+    double lr = (c->cosmo.s8 / s8pre) * gf;
+    c->amplitude = lr * lr;
+    const Spline &s = c->tf_all.s;
+    int n = (int)s.x.size();
+    c->spl_x.ensure(n * 8);
+    c->spl_y.ensure(n * 8);
+    c->spl_c.ensure(n * 8);
+    GIC_HIP(hipMemcpy(c->spl_x.p, s.x.data(), n * 8, hipMemcpyHostToDevice));
+    GIC_HIP(hipMemcpy(c->spl_y.p, s.y.data(), n * 8, hipMemcpyHostToDevice));
+    GIC_HIP(hipMemcpy(c->spl_c.p, s.c.data(), n * 8, hipMemcpyHostToDevice));
+    c->spline_dev = SplineHost{(double *)c->spl_x.p, (double *)c->spl_y.p, (double *)c->spl_c.p, n, c->tf_all.minx,
+                               c->tf_all.maxx};
+    c->have_camb = true;
+  });
+}
+
+int gic_init_base_grid(gic_context *c, double box, int n) {
+  return guarded(c, [&] {
+    if (!c->levels.empty()) throw std::runtime_error("Cannot re-initialize the base grid");
+    if (c->drawn) throw std::runtime_error("Trying to initialize a grid after the random field was already drawn");
+    if (box <= 0) throw std::runtime_error("Box size must be positive");
+    if (n < 4 || (n & (n - 1))) throw std::runtime_error("Grid size must be a power of two >= 4");
+    auto L = std::make_unique<Level>();
+    L->g.n = n;
+    L->g.periodic = box;
+    L->g.dx = box / n;
+    L->g.box = L->g.dx * n;
+    c->levels.push_back(std::move(L));
+  });
+}
+
+int gic_init_zoom_grid_with_origin_at(gic_context *c, int x0, int y0, int z0, int zoomfac, int n) {
+  return guarded(c, [&] {
+    if (c->drawn) throw std::runtime_error("Trying to initialize a grid after the random field was already drawn");
+    if (c->levels.empty()) throw std::runtime_error("Cannot initialise a zoom grid before initialising the base grid");
+    if (c->levels.size() >= 4) throw std::runtime_error("At most 4 levels are supported");
+    if (n < 4 || (n & (n - 1))) throw std::runtime_error("Grid size must be a power of two >= 4");
+    if (zoomfac < 1) throw std::runtime_error("Zoom factor must be positive");
+    const LevelGeom &A = c->levels.back()->g;
+    if (x0 < 0 || y0 < 0 || z0 < 0)
+      throw std::runtime_error("The newly initialised zoom level is not fully contained within the previous level");
+    int nc = A.n / zoomfac;
+    if (!A.covers_full() && (x0 + nc > A.n || y0 + nc > A.n || z0 + nc > A.n))
+      throw std::runtime_error("The newly initialised zoom level is not fully contained within the previous level");
+    auto L = std::make_unique<Level>();
+    double size = A.box / zoomfac;
+    L->g.n = n;
+    L->g.periodic = A.periodic;
+    L->g.dx = size / n;
+    L->g.box = L->g.dx * n;
+    L->g.ox = A.ox + x0 * A.dx;
+    L->g.oy = A.oy + y0 * A.dx;
+    L->g.oz = A.oz + z0 * A.dx;
+    double r = A.dx / L->g.dx;
+    if (std::fabs(r - std::round(r)) > 1e-6 || std::round(r) < 1)
+      throw std::runtime_error("Zoom cell size must divide the parent cell size");
+    c->levels.push_back(std::move(L));
+  });
+}
+
+int gic_num_levels(gic_context *c, int *out) {
+  return guarded(c, [&] { *out = (int)c->levels.size(); });
+}
+
+int gic_level_geometry(gic_context *c, int level, int *n, double *box, double *off, double *dx) {
+  return guarded(c, [&] {
+    const LevelGeom &g = level_at(c, level).g;
+    if (n) *n = g.n;
+    if (box) *box = g.box;
+    if (off) {
+      off[0] = g.ox;
+      off[1] = g.oy;
+      off[2] = g.oz;
+    }
+    if (dx) *dx = g.dx;
+  });
+}
+
+int gic_seed(gic_context *c, unsigned long seed, int mode) {
+  return guarded(c, [&] {
+    if (c->seeded) throw std::runtime_error("The random number generator has already been seeded");
+    if (mode < 0 || mode > 3) throw std::runtime_error("unknown seeding mode");
+    c->seed = seed;
+    c->seed_mode = mode;
+    c->seeded = true;
+  });
+}
+
+int gic_draw(gic_context *c) {
+  return guarded(c, [&] { DISPATCH(c, do_draw, c); });
+}
+
+int gic_zero_level(gic_context *c, int level) {
+  return guarded(c, [&] {
+    Level &L = level_at(c, level);
+    if (!c->drawn) DISPATCH(c, do_draw, c);
+    if (c->prec == 32) fill<float>((float *)L.noise.p, L.g.padded_reals(), 0.f, c->st);
+    else fill<double>((double *)L.noise.p, L.g.padded_reals(), 0.0, c->st);
+    GIC_HIP(hipStreamSynchronize(c->st));
+  });
+}
+
+int gic_apply_power_spectrum(gic_context *c) {
+  return guarded(c, [&] {
+    DISPATCH(c, do_apply_ps, c);
+    GIC_HIP(hipStreamSynchronize(c->st));
+  });
+}
+
+int gic_zeldovich(gic_context *c) {
+  return guarded(c, [&] {
+    DISPATCH(c, do_zeldovich, c);
+    GIC_HIP(hipStreamSynchronize(c->st));
+  });
+}
+
+int gic_copy_field(gic_context *c, int field, int level, int space, void *host, int dtype) {
+  return guarded(c, [&] { DISPATCH(c, do_copy_field, c, field, level, space, host, dtype); });
+}
+
+int gic_set_field(gic_context *c, int field, int level, int space, const void *host, int dtype) {
+  return guarded(c, [&] {
+    DISPATCH(c, do_set_field, c, field, level, space, host, dtype);
+    c->drawn = true;
+  });
+}
+
+int gic_particles(gic_context *c, int level, const uint64_t *cells, size_t start, size_t count, float *pos,
+                  float *vel) {
+  return guarded(c, [&] { DISPATCH(c, do_particles, c, level, cells, start, count, pos, vel); });
+}
+
+int gic_particle_mass_eps(gic_context *c, int level, double *mass, double *eps) {
+  return guarded(c, [&] {
+    const LevelGeom &g = level_at(c, level).g;
+    double boxMass = 27.744948 * c->cosmo.Om * pow(g.periodic, 3.0);
+    if (mass) *mass = boxMass * pow(g.dx / g.periodic, 3.0);
+    if (eps) *eps = g.dx * 1.0 * 0.01075;
+  });
+}
+
+int gic_velocity_ratio(gic_context *c, double *out) {
+  return guarded(c, [&] { *out = velocity_ratio(c->cosmo); });
+}
+
+int gic_fft_inplace(gic_context *c, void *dev, int n, int direction) {
+  return guarded(c, [&] {
+    if (direction != -1 && direction != 1) throw std::runtime_error("direction must be -1 or +1");
+    if (c->prec == 32) fft3d<float>((float *)dev, n, direction, c->st, c->hook());
+    else fft3d<double>((double *)dev, n, direction, c->st, c->hook());
+    GIC_HIP(hipStreamSynchronize(c->st));
+  });
+}
+
+int gic_raw_stream(gic_context *c, unsigned long seed, uint32_t *host_out, size_t n) {
+  return guarded(c, [&] {
+    RngPlan plan;
+    rng_plan(seed, n / 2 + 1, plan);
+    c->work.ensure(rng_workspace_bytes(plan));
+    DevBuf out;
+    out.ensure(n * 4);
+    rng_raw(plan, (uint32_t *)out.p, n, c->work.p, c->work.bytes, c->st);
+    GIC_HIP(hipMemcpyAsync(host_out, out.p, n * 4, hipMemcpyDeviceToHost, c->st));
+    GIC_HIP(hipStreamSynchronize(c->st));
+  });
+}
+
+int gic_gaussian_stream(gic_context *c, unsigned long seed, double *host_out, size_t n) {
+  return guarded(c, [&] {
+    // draw into a dense pseudo-level: n values as rows of a padded field
+    // of width 1 are not possible, so use an (n x 1) layout through a
+    // temporary 1-level target of size m^3 >= n and copy the first n.
+    int m = 4;
+    while (size_t(m) * m * m < n) m *= 2;
+    RngPlan plan;
+    rng_plan(seed, size_t(m) * m * m, plan);
+    c->work.ensure(rng_workspace_bytes(plan));
+    DevBuf f, dense;
+    f.ensure(size_t(m) * m * (m + 2) * 8);
+    dense.ensure(size_t(m) * m * m * 8);
+    std::vector<RngLevelOut> outs{{f.p, m, size_t(m) * m * m}};
+    rng_draw_real<double>(plan, outs, c->work.p, c->work.bytes, c->st, c->hook());
+    unpad<double, double>((const double *)f.p, m, (double *)dense.p, c->st);
+    GIC_HIP(hipMemcpyAsync(host_out, dense.p, n * 8, hipMemcpyDeviceToHost, c->st));
+    GIC_HIP(hipStreamSynchronize(c->st));
+  });
+}
+
+int gic_synchronize(gic_context *c) {
+  return guarded(c, [&] { GIC_HIP(hipStreamSynchronize(c->st)); });
+}
+
+int gic_set_event_timing(gic_context *c, int enable) {
+  return guarded(c, [&] {
+    if (enable && !c->timer) c->timer = std::make_unique<EventTimer>();
+    if (!enable) c->timer.reset();
+  });
+}
+
+int gic_get_event_timing(gic_context *c, const char *kernel, double *total_ms, long *count) {
+  return guarded(c, [&] {
+    if (!c->timer) throw std::runtime_error("event timing is not enabled");
+    c->timer->collect();
+    auto it = c->timer->totals.find(kernel);
+    *total_ms = it == c->timer->totals.end() ? 0.0 : it->second.first;
+    *count = it == c->timer->totals.end() ? 0 : it->second.second;
+  });
+}
+
+int gic_reset_event_timing(gic_context *c) {
+  return guarded(c, [&] {
+    if (c->timer) {
+      c->timer->collect();
+      c->timer->totals.clear();
+    }
+  });
+}
+
+int gic_device_bytes(gic_context *c, size_t *out) {
+  return guarded(c, [&] { *out = total_bytes(c); });
+}
+
+}  // extern "C"

@@ -1,0 +1,414 @@
+// 3D real<->complex FFT for the padded FFTW layout, written for gfx950.
+//
+// Replaces the FFTW calls behind FieldFourierManager::performTransform
+// (genetIC/src/tools/numerics/fourier.hpp:479-563): in-place r2c / c2r over
+// [N][N][N+2] reals <-> [N][N][N/2+1] complex, forward sign e^{-2 pi i k.x/N}.
+// The reference's pad/unpad copies (fourier.hpp:368-405) disappear because
+// the real-space field stays padded in HBM.
+//
+// One kernel launch per axis:
+//   z  : contiguous rows, N/2-point complex Stockham FFT of the packed
+//        (even, odd) samples plus the r2c post-/c2r pre-twiddle;
+//   y,x: strided lines, a tile of C adjacent kz columns is staged in LDS so
+//        every global access is a C*sizeof(complex)-byte contiguous segment.
+// Stages are radix-8 (plus one radix-2/4 stage) Stockham autosort steps;
+// the first stage reads HBM, the last writes HBM, the rest go through LDS.
+// Twiddles come from an N-entry table generated in FP64 on the host.
+#include "gic_common.h"
+#include "fft.h"
+#include <vector>
+#include <cmath>
+#include <map>
+#include <mutex>
+
+namespace gic {
+
+constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
+
+template <int L> struct RadixPlan {
+  static constexpr int lg = ilog2(L);
+  static constexpr int rem = lg % 3;
+  static constexpr int nstages = lg / 3 + (rem ? 1 : 0);
+  static constexpr int radix(int s) { return (rem && s == 0) ? (1 << rem) : 8; }
+  static constexpr int pbefore(int s) {
+    int p = 1;
+    for (int i = 0; i < s; ++i) p *= radix(i);
+    return p;
+  }
+};
+
+// multiply by i*DIR
+template <int DIR, typename C> __device__ __forceinline__ C mul_i(C a) {
+  return DIR > 0 ? C{-a.y, a.x} : C{a.y, -a.x};
+}
+
+template <int R, int DIR, typename T> struct Dft;
+
+template <int DIR, typename T> struct Dft<1, DIR, T> {
+  __device__ __forceinline__ static void run(cx<T> *) {}
+};
+template <int DIR, typename T> struct Dft<2, DIR, T> {
+  __device__ __forceinline__ static void run(cx<T> *x) {
+    cx<T> a = x[0];
+    x[0] = cadd(a, x[1]);
+    x[1] = csub(a, x[1]);
+  }
+};
+template <int DIR, typename T> struct Dft<4, DIR, T> {
+  __device__ __forceinline__ static void run(cx<T> *x) {
+    cx<T> a0 = cadd(x[0], x[2]), a1 = csub(x[0], x[2]);
+    cx<T> a2 = cadd(x[1], x[3]), a3 = mul_i<DIR>(csub(x[1], x[3]));
+    x[0] = cadd(a0, a2);
+    x[2] = csub(a0, a2);
+    x[1] = cadd(a1, a3);
+    x[3] = csub(a1, a3);
+  }
+};
+template <int DIR, typename T> struct Dft<8, DIR, T> {
+  __device__ __forceinline__ static void run(cx<T> *x) {
+    cx<T> e[4] = {x[0], x[2], x[4], x[6]};
+    cx<T> o[4] = {x[1], x[3], x[5], x[7]};
+    Dft<4, DIR, T>::run(e);
+    Dft<4, DIR, T>::run(o);
+    const T h = T(0.70710678118654752440084436210484903928);
+    // w8^1 = (1 + DIR i)/sqrt2, w8^2 = DIR i, w8^3 = (-1 + DIR i)/sqrt2
+    cx<T> o1 = o[1], o3 = o[3];
+    cx<T> t1 = cx<T>{h * (o1.x - DIR * o1.y), h * (o1.y + DIR * o1.x)};
+    cx<T> t2 = mul_i<DIR>(o[2]);
+    cx<T> t3 = cx<T>{h * (-o3.x - DIR * o3.y), h * (-o3.y + DIR * o3.x)};
+    x[0] = cadd(e[0], o[0]);
+    x[4] = csub(e[0], o[0]);
+    x[1] = cadd(e[1], t1);
+    x[5] = csub(e[1], t1);
+    x[2] = cadd(e[2], t2);
+    x[6] = csub(e[2], t2);
+    x[3] = cadd(e[3], t3);
+    x[7] = csub(e[3], t3);
+  }
+};
+
+// XCD-aware tile order: consecutive tiles land on the same XCD (the
+// dispatcher places workgroup b on XCD b % 8), so tiles that share 128-byte
+// lines are served by one L2.
+__device__ __forceinline__ int xcd_tile(int b, int nblk) {
+  if ((nblk & 7) != 0) return b;
+  return (b & 7) * (nblk >> 3) + (b >> 3);
+}
+
+// One Stockham stage.  Butterfly q is mapped to (line, i): LINE_FAST puts
+// adjacent q on adjacent lines (strided passes), otherwise on adjacent points.
+template <typename T, int L, int NL, int THREADS, int S, int DIR, bool LINE_FAST, class Ld, class St>
+__device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, Ld ld, St st, bool sync_between) {
+  using P = RadixPlan<L>;
+  constexpr int R = P::radix(S);
+  constexpr int p = P::pbefore(S);
+  constexpr int NB = NL * (L / R);
+  constexpr int PER = (NB + THREADS - 1) / THREADS;
+  const int tid = threadIdx.x;
+  cx<T> v[PER][R];
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    int q = tid + b * THREADS;
+    if (NB % THREADS == 0 || q < NB) {
+      int line = LINE_FAST ? q % NL : q / (L / R);
+      int i = LINE_FAST ? q / NL : q % (L / R);
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[b][r] = ld(line, i + r * (L / R));
+    }
+  }
+  if (sync_between) __syncthreads();
+#pragma unroll
+  for (int b = 0; b < PER; ++b) {
+    int q = tid + b * THREADS;
+    if (NB % THREADS == 0 || q < NB) {
+      int line = LINE_FAST ? q % NL : q / (L / R);
+      int i = LINE_FAST ? q / NL : q % (L / R);
+      int k = i % p;
+      if (p > 1) {
+#pragma unroll
+        for (int r = 1; r < R; ++r) {
+          cx<T> w = tw[(r * k * (L / (p * R))) * twmul];
+          if (DIR > 0) w.y = -w.y;
+          v[b][r] = cmul(v[b][r], w);
+        }
+      }
+      Dft<R, DIR, T>::run(v[b]);
+      int j = (i - k) * R + k;
+#pragma unroll
+      for (int s = 0; s < R; ++s) st(line, j + s * p, v[b][s]);
+    }
+  }
+}
+
+// Run all stages: stage 0 loads with gld, the last stage stores with gst, the
+// others go through LDS laid out by lidx(line, point).  GLD_LDS / GST_LDS say
+// whether gld / gst themselves touch LDS, in which case the stage must put a
+// barrier between its loads and its stores.
+template <typename T, int L, int NL, int THREADS, int DIR, bool LINE_FAST, bool GLD_LDS, bool GST_LDS, int S,
+          class GLd, class GSt, class LIdx>
+__device__ __forceinline__ void run_stages(cx<T> *lds, const cx<T> *tw, int twmul, GLd gld, GSt gst, LIdx lidx) {
+  using P = RadixPlan<L>;
+  constexpr int NS = P::nstages;
+  auto lds_ld = [&](int line, int pt) { return lds[lidx(line, pt)]; };
+  auto lds_st = [&](int line, int pt, cx<T> v) { lds[lidx(line, pt)] = v; };
+  if constexpr (NS == 1) {
+    stage<T, L, NL, THREADS, 0, DIR, LINE_FAST>(tw, twmul, gld, gst, GLD_LDS && GST_LDS);
+  } else if constexpr (S == 0) {
+    stage<T, L, NL, THREADS, 0, DIR, LINE_FAST>(tw, twmul, gld, lds_st, GLD_LDS);
+    __syncthreads();
+    run_stages<T, L, NL, THREADS, DIR, LINE_FAST, GLD_LDS, GST_LDS, 1>(lds, tw, twmul, gld, gst, lidx);
+  } else if constexpr (S == NS - 1) {
+    stage<T, L, NL, THREADS, S, DIR, LINE_FAST>(tw, twmul, lds_ld, gst, GST_LDS);
+  } else {
+    stage<T, L, NL, THREADS, S, DIR, LINE_FAST>(tw, twmul, lds_ld, lds_st, true);
+    __syncthreads();
+    run_stages<T, L, NL, THREADS, DIR, LINE_FAST, GLD_LDS, GST_LDS, S + 1>(lds, tw, twmul, gld, gst, lidx);
+  }
+}
+
+// ---- strided c2c pass (y or x) ------------------------------------------------
+// Lines of length L with element stride `ls` (complex units); a tile holds C
+// adjacent columns.  Tile t: outer = t / nkzb, column block = t % nkzb.
+template <typename T, int L, int C, int DIR, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_fft_strided(cx<T> *__restrict__ data, long ls, long os, int ncol, int nkzb,
+                                                         T scale, const cx<T> *__restrict__ tw) {
+  extern __shared__ char smem_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  int tile = xcd_tile(blockIdx.x, gridDim.x);
+  int outer = tile / nkzb, kzb = tile - outer * nkzb;
+  int col0 = kzb * C;
+  int nc = min(C, ncol - col0);
+  cx<T> *base = data + outer * os + col0;
+  auto gld = [&](int line, int pt) -> cx<T> {
+    if (line < nc) return base[pt * ls + line];
+    return cx<T>{0, 0};
+  };
+  auto gst = [&](int line, int pt, cx<T> v) {
+    if (line < nc) base[pt * ls + line] = cx<T>{v.x * scale, v.y * scale};
+  };
+  auto lidx = [](int line, int pt) { return pt * C + line; };
+  run_stages<T, L, C, THREADS, DIR, true, false, false, 0>(lds, tw, 1, gld, gst, lidx);
+}
+
+// ---- z pass, r2c --------------------------------------------------------------
+// ROWS rows of N reals (row stride N+2) per block.  Complex FFT of length
+// M=N/2 on z_j = x_{2j} + i x_{2j+1}, then X_k = F_k + W^k G_k.
+template <typename T, int N, int ROWS, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_fft_z_r2c(T *__restrict__ data, long nrows, T scale,
+                                                       const cx<T> *__restrict__ twN) {
+  constexpr int M = N / 2;
+  extern __shared__ char smem_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  long row0 = long(blockIdx.x) * ROWS;
+  cx<T> *rows = reinterpret_cast<cx<T> *>(data) + row0 * (M + 1);
+  auto gld = [&](int line, int pt) -> cx<T> { return rows[long(line) * (M + 1) + pt]; };
+  auto lidx = [](int line, int pt) { return line * (M + 1) + pt; };
+  auto lst = [&](int line, int pt, cx<T> v) { lds[lidx(line, pt)] = v; };
+  if constexpr (M == 1) {
+    if (threadIdx.x < ROWS) lds[lidx(threadIdx.x, 0)] = gld(threadIdx.x, 0);
+  } else {
+    run_stages<T, M, ROWS, THREADS, -1, false, false, true, 0>(lds, twN, 2, gld, lst, lidx);
+  }
+  __syncthreads();
+  constexpr int H = M / 2 + 1;
+  for (int q = threadIdx.x; q < ROWS * H; q += THREADS) {
+    int line = q / H, k = q - line * H;
+    cx<T> zk = lds[lidx(line, k % M)];
+    cx<T> zm = lds[lidx(line, (M - k) % M)];
+    cx<T> F{(zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5)};
+    // G = -i (Zk - conj Zm)/2
+    cx<T> D{(zk.x - zm.x) * T(0.5), (zk.y + zm.y) * T(0.5)};
+    cx<T> G{D.y, -D.x};
+    cx<T> w = twN[k];
+    cx<T> wg = cmul(w, G);
+    cx<T> Xk{(F.x + wg.x) * scale, (F.y + wg.y) * scale};
+    cx<T> Xm{(F.x - wg.x) * scale, -(F.y - wg.y) * scale};
+    cx<T> *out = rows + long(line) * (M + 1);
+    out[k] = Xk;
+    if (M - k != k) out[M - k] = Xm;
+  }
+}
+
+// ---- z pass, c2r --------------------------------------------------------------
+template <typename T, int N, int ROWS, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_fft_z_c2r(T *__restrict__ data, long nrows, T scale,
+                                                       const cx<T> *__restrict__ twN) {
+  constexpr int M = N / 2;
+  extern __shared__ char smem_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  long row0 = long(blockIdx.x) * ROWS;
+  cx<T> *rows = reinterpret_cast<cx<T> *>(data) + row0 * (M + 1);
+  auto lidx = [](int line, int pt) { return line * (M + 1) + pt; };
+  constexpr int H = M / 2 + 1;
+  for (int q = threadIdx.x; q < ROWS * H; q += THREADS) {
+    int line = q / H, k = q - line * H;
+    const cx<T> *in = rows + long(line) * (M + 1);
+    cx<T> xk = in[k], xm = in[M - k];
+    if (k == 0) {
+      // c2r ignores the imaginary parts of the kz=0 and kz=N/2 entries
+      xk.y = 0;
+      xm.y = 0;
+    }
+    cx<T> E{xk.x + xm.x, xk.y - xm.y};
+    cx<T> D{xk.x - xm.x, xk.y + xm.y};
+    cx<T> w = twN[k];
+    w.y = -w.y;                       // e^{+2 pi i k/N}
+    cx<T> O = cmul(D, w);
+    cx<T> Zk{E.x - O.y, E.y + O.x};   // E + i O
+    cx<T> Zm{E.x + O.y, -E.y + O.x};  // conj(E) + i conj(O)
+    if (k == 0) {
+      lds[lidx(line, 0)] = Zk;
+    } else {
+      lds[lidx(line, k)] = Zk;
+      if (M - k != k) lds[lidx(line, M - k)] = Zm;
+    }
+  }
+  __syncthreads();
+  auto lld = [&](int line, int pt) -> cx<T> { return lds[lidx(line, pt)]; };
+  auto gst = [&](int line, int pt, cx<T> v) { rows[long(line) * (M + 1) + pt] = cx<T>{v.x * scale, v.y * scale}; };
+  if constexpr (M == 1) {
+    if (threadIdx.x < ROWS) gst(threadIdx.x, 0, lld(threadIdx.x, 0));
+  } else {
+    run_stages<T, M, ROWS, THREADS, +1, false, true, false, 0>(lds, twN, 2, lld, gst, lidx);
+  }
+}
+
+// ---- Hermitian mirror of the kz=0 and kz=N/2 planes ------------------------------
+// ensureFourierModesAreMirrored (fourier.hpp:331-357): for kx in [0,N/2] and
+// ky from N/2 down to -N/2+1 the conjugate of mode (kx,ky) is copied into
+// (-kx,-ky).  Equivalent closed form: a destination (kx',ky') with kx' > N/2,
+// or kx' in {0,N/2} and ky' > N/2 receives conj(mode(-kx',-ky')); the
+// self-conjugate modes get their imaginary part negated.
+template <typename T>
+__global__ void k_mirror_planes(cx<T> *data, int n) {
+  int nz = n / 2 + 1;
+  long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= long(n) * n) return;
+  int kx = int(t / n), ky = int(t % n);
+  int h = n / 2;
+  int sx = (n - kx) % n, sy = (n - ky) % n;
+  bool xedge = (kx == 0 || kx == h);
+  bool dest = kx > h || (xedge && ky > h);
+  bool self = xedge && (ky == 0 || ky == h);
+  for (int kz : {0, h}) {
+    if (dest) {
+      cx<T> s = data[(long(sx) * n + sy) * nz + kz];
+      data[(long(kx) * n + ky) * nz + kz] = cx<T>{s.x, -s.y};
+    } else if (self) {
+      cx<T> &v = data[(long(kx) * n + ky) * nz + kz];
+      v.y = -v.y;
+    }
+  }
+}
+
+// ----------------------------------------------------------------------------------
+template <typename T> struct TwTable {
+  std::map<int, cx<T> *> tabs;
+  std::mutex mu;
+  const cx<T> *get(int n) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = tabs.find(n);
+    if (it != tabs.end()) return it->second;
+    std::vector<cx<T>> h(n);
+    for (int k = 0; k < n; ++k) {
+      // FP64-generated twiddles, rounded to T
+      double a = -2.0 * M_PI * double(k) / double(n);
+      h[k] = cx<T>{T(cos(a)), T(sin(a))};
+    }
+    cx<T> *d = nullptr;
+    GIC_HIP(hipMalloc(&d, sizeof(cx<T>) * n));
+    GIC_HIP(hipMemcpy(d, h.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice));
+    tabs[n] = d;
+    return d;
+  }
+};
+template <typename T> TwTable<T> &twtable() {
+  static TwTable<T> t;
+  return t;
+}
+
+template <typename T> constexpr int strided_cols() { return sizeof(T) == 4 ? 8 : 4; }
+
+template <typename T, int N> struct FftN {
+  static constexpr int THREADS = 256;
+  static constexpr int C = strided_cols<T>();
+  static constexpr int M = N / 2;
+  static constexpr int ROWS0 = (8192 / N) < 1 ? 1 : (8192 / N);
+  static constexpr int ROWS = ROWS0 > N * N ? N * N : ROWS0;
+
+  static void strided(cx<T> *d, long ls, long os, int nouter, T scale, int dir, hipStream_t st, TimingHook *hook,
+                      const char *name) {
+    int ncol = N / 2 + 1;
+    int nkzb = (ncol + C - 1) / C;
+    int nblk = nouter * nkzb;
+    size_t shm = sizeof(cx<T>) * C * N;
+    const cx<T> *tw = twtable<T>().get(N);
+    auto kern = dir < 0 ? k_fft_strided<T, N, C, -1, THREADS> : k_fft_strided<T, N, C, +1, THREADS>;
+    if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    if (hook) hook->begin(name, st);
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, d, ls, os, ncol, nkzb, scale, tw);
+    GIC_HIP(hipGetLastError());
+    if (hook) hook->end(name, st);
+  }
+  static void zpass(T *d, T scale, int dir, hipStream_t st, TimingHook *hook, const char *name) {
+    long nrows = long(N) * N;
+    int nblk = int(nrows / ROWS);
+    size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
+    const cx<T> *tw = twtable<T>().get(N);
+    auto kern = dir < 0 ? k_fft_z_r2c<T, N, ROWS, THREADS> : k_fft_z_c2r<T, N, ROWS, THREADS>;
+    if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    if (hook) hook->begin(name, st);
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, d, nrows, scale, tw);
+    GIC_HIP(hipGetLastError());
+    if (hook) hook->end(name, st);
+  }
+  static void r2c(T *d, hipStream_t st, TimingHook *hook) {
+    cx<T> *c = reinterpret_cast<cx<T> *>(d);
+    long nz = N / 2 + 1;
+    T scale = T(1.0 / pow(double(N), 1.5));
+    zpass(d, T(1), -1, st, hook, "fft_z_r2c");
+    strided(c, nz, long(N) * nz, N, T(1), -1, st, hook, "fft_y");
+    strided(c, long(N) * nz, nz, N, scale, -1, st, hook, "fft_x");
+  }
+  static void c2r(T *d, hipStream_t st, TimingHook *hook) {
+    cx<T> *c = reinterpret_cast<cx<T> *>(d);
+    long nz = N / 2 + 1;
+    T scale = T(1.0 / pow(double(N), 1.5));
+    int nt = 256;
+    long nn = long(N) * N;
+    if (hook) hook->begin("fft_mirror", st);
+    hipLaunchKernelGGL(k_mirror_planes<T>, dim3((nn + nt - 1) / nt), dim3(nt), 0, st, c, N);
+    GIC_HIP(hipGetLastError());
+    if (hook) hook->end("fft_mirror", st);
+    strided(c, long(N) * nz, nz, N, T(1), +1, st, hook, "fft_x");
+    strided(c, nz, long(N) * nz, N, T(1), +1, st, hook, "fft_y");
+    zpass(d, scale, +1, st, hook, "fft_z_c2r");
+  }
+};
+
+template <typename T> void fft3d(T *d, int n, int dir, hipStream_t st, TimingHook *hook) {
+#define GIC_FFT_CASE(NN)                                   \
+  case NN:                                                 \
+    if (dir < 0) FftN<T, NN>::r2c(d, st, hook);            \
+    else FftN<T, NN>::c2r(d, st, hook);                    \
+    return;
+  switch (n) {
+    GIC_FFT_CASE(4)
+    GIC_FFT_CASE(8)
+    GIC_FFT_CASE(16)
+    GIC_FFT_CASE(32)
+    GIC_FFT_CASE(64)
+    GIC_FFT_CASE(128)
+    GIC_FFT_CASE(256)
+    GIC_FFT_CASE(512)
+    GIC_FFT_CASE(1024)
+    default:
+      throw std::runtime_error("FFT size " + std::to_string(n) + " is not supported (powers of two 4..1024)");
+  }
+#undef GIC_FFT_CASE
+}
+
+template void fft3d<float>(float *, int, int, hipStream_t, TimingHook *);
+template void fft3d<double>(double *, int, int, hipStream_t, TimingHook *);
+
+}  // namespace gic

@@ -1,0 +1,50 @@
+// Fourier-space, level-combination and particle kernels.
+#pragma once
+#include "gic_common.h"
+
+namespace gic {
+
+// Natural cubic spline of log T(k) against log k, coefficients on the device.
+struct SplineHost {
+  double *d_x = nullptr, *d_y = nullptr, *d_c = nullptr;
+  int n = 0;
+  double minx = 0, maxx = 0;
+};
+
+struct InterpGeom {
+  int off[3];        // fine-grid origin in supersampled coarse cells
+  int sim;           // box size in supersampled cells (wrap period)
+  int nsup;          // coarse grid size in supersampled cells
+  double dxs;        // supersampled cell size
+  double dxc;        // coarse cell size
+  double co[3];      // coarse grid offsetLower
+  double periodic;   // box size
+  int nc;            // coarse grid cells per side
+  int csim;          // box size in coarse cells
+  int wrap;          // coarse grid covers the box: wrap, else clamp
+};
+
+struct ParticleGeom {
+  double dx;
+  double o[3];
+  double periodic;
+  double vratio;
+};
+
+template <typename T>
+void sqrtp_table(const SplineHost &sp, double amplitude, double ns, double kmin, double norm, long nent, T *out,
+                 hipStream_t st);
+template <typename T> void filter_table(int kind, double kc, double kmin, long nent, T *out, hipStream_t st);
+template <typename T> void mul_table(T *field, int n, const T *tab, hipStream_t st);
+template <typename T>
+void zeldovich(const T *delta, int n, double kmin, T *px, T *py, T *pz, const T *post, hipStream_t st);
+template <typename T> void interp_add(T *fine, int nf, const T *coarse, const InterpGeom &g, hipStream_t st);
+template <typename T>
+void particles(const T *px, const T *py, const T *pz, int n, const ParticleGeom &g, const uint64_t *cells,
+               size_t start, size_t count, float *pos, float *vel, hipStream_t st);
+template <typename T> void fill(T *p, size_t n, T v, hipStream_t st);
+template <typename S, typename D> void unpad(const S *src, int n, D *dst, hipStream_t st);
+template <typename S, typename D> void pad(const S *src, int n, D *dst, hipStream_t st);
+template <typename S, typename D> void convert(const S *a, size_t n, D *b, hipStream_t st);
+
+}  // namespace gic

@@ -1,0 +1,323 @@
+// Fourier-space operators, level combination and particle evaluation.
+//
+//   sqrt P(k) table      CAMB::operator() + getPowerSpectrumForGridUncached
+//                        (genetIC/src/cosmology/camb.hpp:101-119,204-216) and the
+//                        log-log cubic spline (tools/numerics/interpolation.hpp:81-110).
+//                        Every mode's |k|^2 is an integer multiple of kMin^2,
+//                        so one FP64 table indexed by kx^2+ky^2+kz^2 replaces the
+//                        reference's per-mode spline calls and its covariance field.
+//   apply transfer       Field::applyTransferFunction (field/field.hpp:394-412)
+//   Zel'dovich           ZeldovichParticleGenerator::calculateOffsetFields
+//                        (particles/zeldovich.hpp:155-181)
+//   filters              LowPassFermiFilter / ComplementaryCovarianceFilterAdaptor
+//                        (filters/filter.hpp:208-275), applied per mode
+//   level combination    Field::addFieldFromDifferentGrid + evaluateInterpolated
+//                        (field/field.hpp:483-617,804-812, evaluator.hpp:31-57)
+//   particles            ZeldovichParticleEvaluator::getParticleNoWrap + wrapPoint
+//                        (particles/zeldovich.hpp:87-109, generator.hpp:49-53)
+#include "kspace.h"
+
+namespace gic {
+
+// mode index -> signed integer wavenumbers, with the stored N/2 read as +N/2
+__device__ __forceinline__ void mode_ints(size_t idx, int n, int &ix, int &iy, int &iz) {
+  int nz = n / 2 + 1;
+  iz = int(idx % nz);
+  size_t r = idx / nz;
+  iy = int(r % n);
+  ix = int(r / n);
+  int h = n / 2;
+  if (ix > h) ix -= n;
+  if (iy > h) iy -= n;
+}
+
+// ---- spline-evaluated sqrt(P) table ----------------------------------------------
+struct SplineDev {
+  const double *x, *y, *c;
+  int n;
+  double minx, maxx;
+};
+
+__device__ double spline_eval(const SplineDev &s, double xv) {
+  if (xv < s.x[0] || xv > s.x[s.n - 1]) return 0;
+  int lo = 0, hi = s.n - 1;
+  while (hi > lo + 1) {
+    int i = (hi + lo) / 2;
+    if (s.x[i] > xv) hi = i;
+    else lo = i;
+  }
+  double xl = s.x[lo], xh = s.x[lo + 1], dx = xh - xl;
+  if (!(dx > 0)) return s.y[lo];
+  double dy = s.y[lo + 1] - s.y[lo], delx = xv - xl;
+  double ci = s.c[lo], cip = s.c[lo + 1];
+  double bi = dy / dx - dx * (cip + 2.0 * ci) / 3.0;
+  double di = (cip - ci) / (3.0 * dx);
+  return s.y[lo] + delx * (bi + delx * (ci + delx * di));
+}
+
+template <typename T>
+__global__ void k_sqrtp_table(SplineDev sp, double amplitude, double ns, double kmin, double norm, long nent,
+                              T *__restrict__ out) {
+  long s = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= nent) return;
+  double k = sqrt(double(s)) * kmin;
+  double tf = 0;
+  if (k != 0) {
+    double lx = log(k);
+    if (!(lx < sp.minx || lx > sp.maxx)) tf = exp(spline_eval(sp, lx));
+  }
+  double P = amplitude * pow(k, ns) * tf * tf * norm;
+  out[s] = T(P != 0.0 ? pow(P, 0.5) : 0.0);
+}
+
+template <typename T>
+void sqrtp_table(const SplineHost &sp, double amplitude, double ns, double kmin, double norm, long nent, T *out,
+                 hipStream_t st) {
+  SplineDev d{sp.d_x, sp.d_y, sp.d_c, sp.n, sp.minx, sp.maxx};
+  hipLaunchKernelGGL(k_sqrtp_table<T>, dim3((nent + 255) / 256), dim3(256), 0, st, d, amplitude, ns, kmin, norm, nent,
+                     out);
+  GIC_HIP(hipGetLastError());
+}
+
+// ---- filter tables by |k|^2 -------------------------------------------------------
+// kind 0: low pass 1/(1+exp((k-kc)/(0.1 kc)))
+// kind 1: high pass sqrt(1 - lp^2)
+template <typename T>
+__global__ void k_filter_table(int kind, double kc, double kmin, long nent, T *__restrict__ out) {
+  long s = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= nent) return;
+  double k = sqrt(double(s)) * kmin;
+  double lp = 1. / (1. + exp((k - kc) / (kc * 0.1)));
+  out[s] = T(kind == 0 ? lp : sqrt(1. - lp * lp));
+}
+
+template <typename T> void filter_table(int kind, double kc, double kmin, long nent, T *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_filter_table<T>, dim3((nent + 255) / 256), dim3(256), 0, st, kind, kc, kmin, nent, out);
+  GIC_HIP(hipGetLastError());
+}
+
+// ---- multiply a spectrum by a table indexed by |k|^2 ---------------------------------
+template <typename T>
+__global__ void k_mul_table(cx<T> *__restrict__ f, int n, const T *__restrict__ tab) {
+  size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t nm = size_t(n) * n * (n / 2 + 1);
+  if (idx >= nm) return;
+  int ix, iy, iz;
+  mode_ints(idx, n, ix, iy, iz);
+  T v = tab[ix * ix + iy * iy + iz * iz];
+  cx<T> a = f[idx];
+  f[idx] = cx<T>{a.x * v, a.y * v};
+}
+
+template <typename T> void mul_table(T *field, int n, const T *tab, hipStream_t st) {
+  size_t nm = size_t(n) * n * (n / 2 + 1);
+  hipLaunchKernelGGL(k_mul_table<T>, dim3((nm + 255) / 256), dim3(256), 0, st, reinterpret_cast<cx<T> *>(field), n,
+                     tab);
+  GIC_HIP(hipGetLastError());
+}
+
+// ---- Zel'dovich: psi_j(k) = i k_j delta(k) / k^2 -----------------------------------------
+// Computed in T with k_j = int * kMin as the reference does; the component
+// at k_j = +Nyquist and the k = 0 mode are zero.  An optional table factor
+// (the high-pass filter of a zoom level) is applied to the three outputs.
+template <typename T>
+__global__ void k_zeldovich(const cx<T> *__restrict__ d, int n, T kmin, cx<T> *__restrict__ px,
+                            cx<T> *__restrict__ py, cx<T> *__restrict__ pz, const T *__restrict__ post) {
+  size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t nm = size_t(n) * n * (n / 2 + 1);
+  if (idx >= nm) return;
+  int ix, iy, iz;
+  mode_ints(idx, n, ix, iy, iz);
+  T kx = T(ix) * kmin, ky = T(iy) * kmin, kz = T(iz) * kmin;
+  T nyq = T(n / 2) * kmin;
+  T kfft = kx * kx + ky * ky + kz * kz;
+  cx<T> v = d[idx];
+  cx<T> r{-v.y / kfft, v.x / kfft};
+  T f = post ? post[ix * ix + iy * iy + iz * iz] : T(1);
+  cx<T> rx{r.x * kx * f, r.y * kx * f}, ry{r.x * ky * f, r.y * ky * f}, rz{r.x * kz * f, r.y * kz * f};
+  if (kx == nyq || kfft == 0) rx = cx<T>{0, 0};
+  if (ky == nyq || kfft == 0) ry = cx<T>{0, 0};
+  if (kz == nyq || kfft == 0) rz = cx<T>{0, 0};
+  px[idx] = rx;
+  py[idx] = ry;
+  pz[idx] = rz;
+}
+
+template <typename T>
+void zeldovich(const T *delta, int n, double kmin, T *px, T *py, T *pz, const T *post, hipStream_t st) {
+  size_t nm = size_t(n) * n * (n / 2 + 1);
+  hipLaunchKernelGGL(k_zeldovich<T>, dim3((nm + 255) / 256), dim3(256), 0, st, reinterpret_cast<const cx<T> *>(delta),
+                     n, T(kmin), reinterpret_cast<cx<T> *>(px), reinterpret_cast<cx<T> *>(py),
+                     reinterpret_cast<cx<T> *>(pz), post);
+  GIC_HIP(hipGetLastError());
+}
+
+// ---- tricubic (Catmull-Rom tensor) coarse -> fine interpolate-add -----------------------
+__device__ __forceinline__ void cr_weights(double t, double *w) {
+  w[0] = -0.5 * t * (t - 1) * (t - 1);
+  w[1] = (2 - 5 * t * t + 3 * t * t * t) / 2;
+  w[2] = -t * (-1 - 4 * t + 3 * t * t) / 2;
+  w[3] = (t - 1) * t * t / 2;
+}
+
+template <typename T>
+__global__ void k_interp_add(T *__restrict__ fine, int nf, const T *__restrict__ coarse, InterpGeom g) {
+  size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t nf3 = size_t(nf) * nf * nf;
+  if (idx >= nf3) return;
+  int x = int(idx / (size_t(nf) * nf)), y = int((idx / nf) % nf), z = int(idx % nf);
+  int c[3] = {x + g.off[0], y + g.off[1], z + g.off[2]};
+  int p0[3];
+  double t[3];
+  for (int a = 0; a < 3; ++a) {
+    if (c[a] > g.sim - 1) c[a] -= g.sim;
+    if (c[a] < 0) c[a] += g.sim;
+    if (c[a] < 0 || c[a] >= g.nsup) return;   // outside the coarse grid: nothing added
+    double cen = c[a] * g.dxs + g.co[a] + g.dxs / 2;   // supersampled centroid
+    double loc = cen - g.co[a];
+    loc = fmod(loc, g.periodic);
+    if (loc < 0) loc += g.periodic;
+    double u = loc / g.dxc - 0.5;
+    p0[a] = int(floor(u));
+    t[a] = u - p0[a];
+  }
+  double w[3][4];
+  for (int a = 0; a < 3; ++a) cr_weights(t[a], w[a]);
+  int nc = g.nc;
+  int ci[3][4];
+  for (int a = 0; a < 3; ++a)
+    for (int i = 0; i < 4; ++i) {
+      int v = p0[a] + i - 1;
+      if (g.wrap) {
+        if (v > g.csim - 1) v -= g.csim;
+        if (v < 0) v += g.csim;
+      } else {
+        if (v > nc - 1) v = nc - 1;
+        if (v < 0) v = 0;
+      }
+      ci[a][i] = v;
+    }
+  T acc = 0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      T wij = T(w[0][i] * w[1][j]);
+      const T *row = coarse + (size_t(ci[0][i]) * nc + ci[1][j]) * (nc + 2);
+      for (int k = 0; k < 4; ++k) acc += wij * T(w[2][k]) * row[ci[2][k]];
+    }
+  fine[(size_t(x) * nf + y) * (nf + 2) + z] += acc;
+}
+
+template <typename T> void interp_add(T *fine, int nf, const T *coarse, const InterpGeom &g, hipStream_t st) {
+  size_t nf3 = size_t(nf) * nf * nf;
+  hipLaunchKernelGGL(k_interp_add<T>, dim3((nf3 + 255) / 256), dim3(256), 0, st, fine, nf, coarse, g);
+  GIC_HIP(hipGetLastError());
+}
+
+// ---- particles --------------------------------------------------------------------------
+template <typename T>
+__global__ void k_particles(const T *__restrict__ px, const T *__restrict__ py, const T *__restrict__ pz, int n,
+                            ParticleGeom g, const uint64_t *__restrict__ cells, size_t start, size_t count,
+                            float *__restrict__ pos, float *__restrict__ vel) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  size_t id = cells ? cells[i] : start + i;
+  size_t nn = size_t(n) * n;
+  size_t x = id / nn, r = id - x * nn, y = r / n, z = r - y * n;
+  size_t p = (x * n + y) * (n + 2) + z;
+  double off[3] = {double(px[p]), double(py[p]), double(pz[p])};
+  double cen[3] = {double(x) * g.dx + g.o[0] + g.dx / 2, double(y) * g.dx + g.o[1] + g.dx / 2,
+                   double(z) * g.dx + g.o[2] + g.dx / 2};
+  for (int a = 0; a < 3; ++a) {
+    T o = T(off[a]);
+    T v = o * T(g.vratio);
+    T q = T(off[a]) + T(cen[a]);
+    double qd = double(q);
+    qd = fmod(qd, g.periodic);
+    if (qd < 0) qd += g.periodic;
+    pos[3 * i + a] = float(qd);
+    vel[3 * i + a] = float(v);
+  }
+}
+
+template <typename T>
+void particles(const T *px, const T *py, const T *pz, int n, const ParticleGeom &g, const uint64_t *cells,
+               size_t start, size_t count, float *pos, float *vel, hipStream_t st) {
+  if (count == 0) return;
+  hipLaunchKernelGGL(k_particles<T>, dim3((count + 255) / 256), dim3(256), 0, st, px, py, pz, n, g, cells, start,
+                     count, pos, vel);
+  GIC_HIP(hipGetLastError());
+}
+
+// ---- small utilities -------------------------------------------------------------------
+template <typename T> __global__ void k_fill(T *p, size_t n, T v) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+template <typename T> void fill(T *p, size_t n, T v, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill<T>, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
+  GIC_HIP(hipGetLastError());
+}
+
+// copy a padded real field to / from a dense N^3 array of another type
+template <typename S, typename D>
+__global__ void k_unpad(const S *__restrict__ src, int n, D *__restrict__ dst) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t n3 = size_t(n) * n * n;
+  if (i >= n3) return;
+  size_t row = i / n, z = i - row * n;
+  dst[i] = D(src[row * (n + 2) + z]);
+}
+template <typename S, typename D>
+__global__ void k_pad(const S *__restrict__ src, int n, D *__restrict__ dst) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t n3 = size_t(n) * n * n;
+  if (i >= n3) return;
+  size_t row = i / n, z = i - row * n;
+  dst[row * (n + 2) + z] = D(src[i]);
+}
+template <typename S, typename D> void unpad(const S *src, int n, D *dst, hipStream_t st) {
+  size_t n3 = size_t(n) * n * n;
+  hipLaunchKernelGGL((k_unpad<S, D>), dim3((n3 + 255) / 256), dim3(256), 0, st, src, n, dst);
+  GIC_HIP(hipGetLastError());
+}
+template <typename S, typename D> void pad(const S *src, int n, D *dst, hipStream_t st) {
+  size_t n3 = size_t(n) * n * n;
+  hipLaunchKernelGGL((k_pad<S, D>), dim3((n3 + 255) / 256), dim3(256), 0, st, src, n, dst);
+  GIC_HIP(hipGetLastError());
+}
+template <typename S, typename D> __global__ void k_convert(const S *__restrict__ a, size_t n, D *__restrict__ b) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) b[i] = D(a[i]);
+}
+template <typename S, typename D> void convert(const S *a, size_t n, D *b, hipStream_t st) {
+  if (n == 0) return;
+  hipLaunchKernelGGL((k_convert<S, D>), dim3((n + 255) / 256), dim3(256), 0, st, a, n, b);
+  GIC_HIP(hipGetLastError());
+}
+
+#define GIC_INST(T)                                                                                            \
+  template void sqrtp_table<T>(const SplineHost &, double, double, double, double, long, T *, hipStream_t);   \
+  template void filter_table<T>(int, double, double, long, T *, hipStream_t);                                 \
+  template void mul_table<T>(T *, int, const T *, hipStream_t);                                               \
+  template void zeldovich<T>(const T *, int, double, T *, T *, T *, const T *, hipStream_t);                  \
+  template void interp_add<T>(T *, int, const T *, const InterpGeom &, hipStream_t);                          \
+  template void particles<T>(const T *, const T *, const T *, int, const ParticleGeom &, const uint64_t *,    \
+                             size_t, size_t, float *, float *, hipStream_t);                                  \
+  template void fill<T>(T *, size_t, T, hipStream_t);
+GIC_INST(float)
+GIC_INST(double)
+template void unpad<float, float>(const float *, int, float *, hipStream_t);
+template void unpad<float, double>(const float *, int, double *, hipStream_t);
+template void unpad<double, double>(const double *, int, double *, hipStream_t);
+template void unpad<double, float>(const double *, int, float *, hipStream_t);
+template void pad<float, float>(const float *, int, float *, hipStream_t);
+template void pad<double, float>(const double *, int, float *, hipStream_t);
+template void pad<double, double>(const double *, int, double *, hipStream_t);
+template void pad<float, double>(const float *, int, double *, hipStream_t);
+template void convert<float, double>(const float *, size_t, double *, hipStream_t);
+template void convert<double, float>(const double *, size_t, float *, hipStream_t);
+template void convert<double, double>(const double *, size_t, double *, hipStream_t);
+template void convert<float, float>(const float *, size_t, float *, hipStream_t);
+
+}  // namespace gic

@@ -1,0 +1,45 @@
+// Bit-exact parallel ranlxs2 stream and ziggurat Gaussians.
+#pragma once
+#include "gic_common.h"
+#include <vector>
+
+namespace gic {
+
+// 576-bit numbers as 9 little-endian 64-bit limbs.
+struct U576 {
+  uint64_t w[9];
+};
+
+// Host-side set-up of one real-space draw: the seeded RANLUX state and the
+// table of jump multipliers b^{-397*KC*2^i} mod m (b = 2^48, m = b^12-b^5+1).
+struct RngPlan {
+  static constexpr int KC = 64;            // ranlxs2 blocks (24 numbers) per chunk
+  static constexpr int CH = KC * 24;       // raw numbers per chunk
+  static constexpr int MAXBITS = 32;
+  uint64_t seed_words[12];                 // the 12 seeded 48-bit words, oldest first
+  U576 y0;                                 // LCG state 11 steps past the seeded state, times ... (see rng.hip)
+  U576 jump[MAXBITS];
+  long nchunks = 0;
+};
+
+void rng_plan(unsigned long seed, size_t n_gaussians, RngPlan &plan);
+
+// Draw n_gaussians of the stream into the listed level fields (padded real
+// layout), in level order, as the reference's serial loop does.
+struct RngLevelOut {
+  void *field;       // T* padded real field
+  int n;
+  size_t count;      // n^3
+};
+template <typename T>
+void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, void *workspace, size_t ws_bytes,
+                   hipStream_t st, struct TimingHook *hook);
+size_t rng_workspace_bytes(const RngPlan &plan);
+
+// Raw 24-bit stream (testing / bit-exactness checks): the first n numbers.
+void rng_raw(const RngPlan &plan, uint32_t *d_out, size_t n, void *workspace, size_t ws_bytes, hipStream_t st);
+
+// Host reference pieces used by tests of the jump arithmetic.
+void u576_mulmod_host(const U576 &a, const U576 &b, U576 &r);
+
+}  // namespace gic

@@ -1,0 +1,690 @@
+// Parallel, bit-exact ranlxs2 + ziggurat Gaussians on gfx950.
+//
+// Replaces the serial draw loop of RandomFieldGenerator::drawRandomForSpecifiedGrid
+// (genetIC/src/simulation/field/randomfieldgenerator.hpp:181-197), which calls
+// gsl_ran_gaussian_ziggurat(ranlxs2, 1) once per cell in index order and
+// continues the same stream across levels.
+//
+// ranlxs2 (Luescher's RANLUX at luxury p=397, GSL's 48-bit implementation)
+// runs the subtract-with-borrow recurrence x_n = x_{n-5} - x_{n-12} - c_{n-1}
+// (mod b = 2^48) and emits the 12 newest words after every 397 steps as 24
+// 24-bit numbers (low half first, oldest word first).  The recurrence is an
+// LCG in disguise: Z_n = sum_j a_j x_{n-12+j} + b c_{n-1} (mod m, m =
+// b^12 - b^5 + 1, a_j = b^{j+1} - [j>=7] b^{j-6}) obeys Z_{n+1} = b^{-1} Z_n.
+// So chunk t of the stream starts from a 576-bit modular product, is decoded
+// to the 12-word state (the carry ambiguity of the decode disappears after
+// 12 recurrence steps), and is then generated independently.
+//
+// gsl_ran_gaussian_ziggurat consumes 2 numbers per accepted iteration and
+// 3 or 4 on the rare edge/tail tests, so the Gaussian index of a chunk is
+// data dependent.  Pass 1 parses every chunk from the 4 possible entry
+// offsets (the parses merge after a few iterations), pass 2 scans the
+// entry->(exit, count) maps, pass 3 re-parses each chunk from its true entry
+// and writes the Gaussians to their cells.
+#include "rng.h"
+#include "fft.h"
+#include <cstring>
+#include <cmath>
+
+namespace gic {
+
+#include "zig_tables.inc"
+
+__constant__ double c_ytab[128];
+__constant__ double c_wtab[128];
+__constant__ unsigned int c_ktab[128];
+
+typedef unsigned __int128 u128;
+static constexpr uint64_t MASK48 = (uint64_t(1) << 48) - 1;
+
+// ---- 576-bit arithmetic mod m = 2^576 - 2^240 + 1 -------------------------
+__host__ __device__ inline bool u576_geq_m(const uint64_t *a) {
+  // m limbs: w0 = 1, w3 = 0xFFFF000000000000 (bits 240..255 clear... ) see below
+  // m = 2^576 - 2^240 + 1: limbs 0..2: [1, 0, 0], limb 3: 2^64 - 2^48 (bits 48..63 set),
+  // limbs 4..8: all ones.
+  const uint64_t m[9] = {1, 0, 0, 0xFFFF000000000000ull, ~0ull, ~0ull, ~0ull, ~0ull, ~0ull};
+  for (int i = 8; i >= 0; --i) {
+    if (a[i] != m[i]) return a[i] > m[i];
+  }
+  return true;
+}
+__host__ __device__ inline void u576_sub_m(uint64_t *a) {
+  const uint64_t m[9] = {1, 0, 0, 0xFFFF000000000000ull, ~0ull, ~0ull, ~0ull, ~0ull, ~0ull};
+  uint64_t borrow = 0;
+  for (int i = 0; i < 9; ++i) {
+    u128 d = (u128)a[i] - m[i] - borrow;
+    a[i] = (uint64_t)d;
+    borrow = (uint64_t)(d >> 64) & 1;
+  }
+}
+
+// s (nlimbs) <- L + H * 2^240 - H  with H given as hn limbs; result limbs
+// written to out (length 14).  Requires H*2^240 >= H, true for any H >= 0.
+__host__ __device__ inline void fold(const uint64_t *L, const uint64_t *H, int hn, uint64_t *out) {
+  uint64_t t[14];
+  for (int i = 0; i < 14; ++i) t[i] = 0;
+  for (int i = 0; i < 9; ++i) t[i] = L[i];
+  // add H << 240 (= 3 limbs + 48 bits)
+  u128 carry = 0;
+  for (int i = 0; i < 11; ++i) {
+    uint64_t sh = 0;
+    if (i < hn) sh |= H[i] << 48;
+    if (i >= 1 && i - 1 < hn) sh |= H[i - 1] >> 16;
+    u128 s = (u128)t[i + 3] + sh + carry;
+    t[i + 3] = (uint64_t)s;
+    carry = s >> 64;
+  }
+  // subtract H
+  uint64_t borrow = 0;
+  for (int i = 0; i < 14; ++i) {
+    uint64_t hv = i < hn ? H[i] : 0;
+    u128 d = (u128)t[i] - hv - borrow;
+    t[i] = (uint64_t)d;
+    borrow = (uint64_t)(d >> 64) & 1;
+  }
+  for (int i = 0; i < 14; ++i) out[i] = t[i];
+}
+
+__host__ __device__ inline void u576_mulmod(const uint64_t *a, const uint64_t *b, uint64_t *r) {
+  uint64_t p[18];
+  for (int i = 0; i < 18; ++i) p[i] = 0;
+  for (int i = 0; i < 9; ++i) {
+    uint64_t carry = 0;
+    for (int j = 0; j < 9; ++j) {
+      u128 cur = (u128)a[i] * b[j] + p[i + j] + carry;
+      p[i + j] = (uint64_t)cur;
+      carry = (uint64_t)(cur >> 64);
+    }
+    p[i + 9] = carry;
+  }
+  uint64_t s[14];
+  fold(p, p + 9, 9, s);          // < 2^576 + 2^816
+  uint64_t s2[14];
+  fold(s, s + 9, 5, s2);         // < 2^576 + 2^561
+  for (int i = 0; i < 9; ++i) r[i] = s2[i];
+  // s2 fits in 9 limbs plus a possible carry bit in limb 9
+  if (s2[9]) {
+    // value >= 2^576 > m: subtract m once (2^576 - m = 2^240 - 1 < m)
+    u576_sub_m(r);  // wraps limb 9 away
+  }
+  while (u576_geq_m(r)) u576_sub_m(r);
+}
+
+void u576_mulmod_host(const U576 &a, const U576 &b, U576 &r) { u576_mulmod(a.w, b.w, r.w); }
+
+// 48-bit field at bit offset off of a little-endian limb array
+__host__ __device__ inline uint64_t get48(const uint64_t *v, int off) {
+  int li = off >> 6, bi = off & 63;
+  uint64_t x = v[li] >> bi;
+  if (bi > 16) x |= v[li + 1] << (64 - bi);
+  return x & MASK48;
+}
+__host__ __device__ inline void put48(uint64_t *v, int off, uint64_t x) {
+  int li = off >> 6, bi = off & 63;
+  v[li] |= x << bi;
+  if (bi > 16) v[li + 1] |= x >> (64 - bi);
+}
+
+// Decode V (the LCG state one step past the state S_n) into the 12 words of
+// S_n with carry 0: V = A_low + (b^7 - 1) B + c, A_low the 7 oldest words,
+// B the 5 newest.
+__host__ __device__ inline void decode_state(const uint64_t *V, uint64_t *x) {
+  // v0 = V mod 2^336, v1 = V >> 336
+  uint64_t v0[7], v1[7];
+  for (int i = 0; i < 7; ++i) v0[i] = v1[i] = 0;
+  for (int i = 0; i < 5; ++i) v0[i] = V[i];
+  v0[5] = V[5] & 0xFFFF;   // bits 320..335
+  for (int i = 0; i < 4; ++i) v1[i] = (V[5 + i] >> 16) | (i + 6 < 9 ? V[6 + i] << 48 : 0);
+  // v1 has at most 240 bits (4 limbs, top limb < 2^48)
+  uint64_t s[7];
+  u128 carry = 0;
+  for (int i = 0; i < 7; ++i) {
+    u128 t = (u128)v0[i] + v1[i] + carry;
+    s[i] = (uint64_t)t;
+    carry = t >> 64;
+  }
+  // compare s with D = 2^336 - 1 (limbs 0..4 all ones, limb 5 = 0xFFFF)
+  bool ge = s[6] > 0 || s[5] > 0xFFFF ||
+            (s[5] == 0xFFFF && s[4] == ~0ull && s[3] == ~0ull && s[2] == ~0ull && s[1] == ~0ull && s[0] == ~0ull);
+  uint64_t B[7], R[7];
+  for (int i = 0; i < 7; ++i) {
+    B[i] = v1[i];
+    R[i] = s[i];
+  }
+  if (ge) {
+    // B += 1 ; R -= 2^336 - 1  (i.e. R = R - 2^336 + 1)
+    u128 c = 1;
+    for (int i = 0; i < 7; ++i) {
+      u128 t = (u128)B[i] + c;
+      B[i] = (uint64_t)t;
+      c = t >> 64;
+    }
+    // R - 2^336 + 1: add 1 then clear bit 336 (R >= 2^336 - 1 so R+1 >= 2^336)
+    c = 1;
+    for (int i = 0; i < 7; ++i) {
+      u128 t = (u128)R[i] + c;
+      R[i] = (uint64_t)t;
+      c = t >> 64;
+    }
+    // subtract 2^336 = limb 5 bit 16
+    u128 d = (u128)R[5] - (uint64_t(1) << 16);
+    R[5] = (uint64_t)d;
+    if ((uint64_t)(d >> 64) & 1) R[6] -= 1;
+  }
+  // B >= 2^240 (only for V = m - 1): move one unit back into R
+  if (B[3] >> 48) {
+    uint64_t borrow = 1;
+    for (int i = 0; i < 7; ++i) {
+      u128 t = (u128)B[i] - borrow;
+      B[i] = (uint64_t)t;
+      borrow = (uint64_t)(t >> 64) & 1;
+    }
+    // R += 2^336 - 1
+    u128 c = 0;
+    const uint64_t D[7] = {~0ull, ~0ull, ~0ull, ~0ull, ~0ull, 0xFFFF, 0};
+    for (int i = 0; i < 7; ++i) {
+      u128 t = (u128)R[i] + D[i] + c;
+      R[i] = (uint64_t)t;
+      c = t >> 64;
+    }
+  }
+  for (int j = 0; j < 7; ++j) x[j] = get48(R, 48 * j);
+  for (int j = 0; j < 5; ++j) x[7 + j] = get48(B, 48 * j);
+}
+
+// one subtract-with-borrow step on the ring: x[i] <- x[j] - x[i] - c
+#define SWB(i, j)                              \
+  {                                            \
+    int64_t d = (int64_t)x[j] - (int64_t)x[i] - c; \
+    c = (int64_t)((uint64_t)d >> 63);          \
+    x[i] = (uint64_t)d & MASK48;               \
+  }
+
+// 397 recurrence steps (33 rounds of 12 plus one), keeping x[0] as the
+// oldest word afterwards by rotating the registers once.
+__device__ __forceinline__ void ranlux_block(uint64_t *x, int64_t &c) {
+  for (int r = 0; r < 33; ++r) {
+    SWB(0, 7) SWB(1, 8) SWB(2, 9) SWB(3, 10) SWB(4, 11) SWB(5, 0)
+    SWB(6, 1) SWB(7, 2) SWB(8, 3) SWB(9, 4) SWB(10, 5) SWB(11, 6)
+  }
+  SWB(0, 7)
+  uint64_t t0 = x[0];
+#pragma unroll
+  for (int i = 0; i < 11; ++i) x[i] = x[i + 1];
+  x[11] = t0;
+}
+
+// raw buffer layout: chunk t, element e -> ((t/64)*CH + e)*64 + t%64
+__device__ __forceinline__ size_t raw_at(long t, int e) {
+  return (size_t(t >> 6) * RngPlan::CH + e) * 64 + (t & 63);
+}
+
+struct RngDev {
+  uint64_t seed_words[12];
+  U576 y0;
+  U576 jump[RngPlan::MAXBITS];
+};
+
+__device__ void chunk_start_state(const RngDev &p, long t, uint64_t *x, int64_t &c) {
+  c = 0;
+  if (t == 0) {
+    for (int i = 0; i < 12; ++i) x[i] = p.seed_words[i];
+    return;
+  }
+  uint64_t acc[9];
+  for (int i = 0; i < 9; ++i) acc[i] = p.y0.w[i];
+  for (int b = 0; b < RngPlan::MAXBITS; ++b) {
+    if ((t >> b) & 1) {
+      uint64_t r[9];
+      u576_mulmod(acc, p.jump[b].w, r);
+      for (int i = 0; i < 9; ++i) acc[i] = r[i];
+    }
+  }
+  decode_state(acc, x);
+  // 12 steps resolve the carry ambiguity and land on the chunk's start
+  SWB(0, 7) SWB(1, 8) SWB(2, 9) SWB(3, 10) SWB(4, 11) SWB(5, 0)
+  SWB(6, 1) SWB(7, 2) SWB(8, 3) SWB(9, 4) SWB(10, 5) SWB(11, 6)
+}
+
+// ---- ziggurat iteration ---------------------------------------------------------
+// One loop iteration of gsl_ran_gaussian_ziggurat starting at stream position
+// pos: returns the number of numbers consumed (2, 3 or 4) and whether it
+// accepted; *val receives the Gaussian when accepted.
+template <class Get>
+__device__ __forceinline__ int zig_iter(Get get, long pos, bool &accept, double &val) {
+  const double PARAM_R = 3.44428647676;
+  uint32_t k1 = get(pos), k2 = get(pos + 1);
+  uint32_t i = k1 & 0xFF, j = k2 & 0xFFFFFF;
+  double sign = (i & 0x80) ? 1.0 : -1.0;
+  i &= 0x7f;
+  double x = j * c_wtab[i];
+  if (j < c_ktab[i]) {
+    accept = true;
+    val = sign * x;
+    return 2;
+  }
+  double y;
+  int used;
+  if (i < 127) {
+    double y0 = c_ytab[i], y1 = c_ytab[i + 1];
+    double U1 = get(pos + 2) / 16777216.0;
+    y = y1 + (y0 - y1) * U1;
+    used = 3;
+  } else {
+    double U1 = 1.0 - get(pos + 2) / 16777216.0;
+    double U2 = get(pos + 3) / 16777216.0;
+    x = PARAM_R - log(U1) / PARAM_R;
+    y = exp(-PARAM_R * (x - 0.5 * PARAM_R)) * U2;
+    used = 4;
+  }
+  accept = y < exp(-0.5 * x * x);
+  val = sign * x;
+  return used;
+}
+
+// Pass 1: generate chunk t, store it, parse from the four entry offsets.
+__global__ __launch_bounds__(256) void k_rng_generate(RngDev p, long nchunks, uint32_t *__restrict__ raw,
+                                                     uint32_t *__restrict__ ovh, uint32_t *__restrict__ summary) {
+  long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nchunks) return;
+  uint64_t x[12];
+  int64_t c;
+  chunk_start_state(p, t, x, c);
+  for (int q = 0; q < RngPlan::KC; ++q) {
+    ranlux_block(x, c);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      raw[raw_at(t, q * 24 + 2 * k)] = uint32_t(x[k] & 0xFFFFFF);
+      raw[raw_at(t, q * 24 + 2 * k + 1)] = uint32_t(x[k] >> 24);
+    }
+  }
+  // the next chunk's first block, for iterations that straddle the boundary
+  ranlux_block(x, c);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    ovh[t * 24 + 2 * k] = uint32_t(x[k] & 0xFFFFFF);
+    ovh[t * 24 + 2 * k + 1] = uint32_t(x[k] >> 24);
+  }
+  auto get = [&](long pos) -> uint32_t {
+    return pos < RngPlan::CH ? raw[raw_at(t, int(pos))] : ovh[t * 24 + (pos - RngPlan::CH)];
+  };
+  // four cursors; advance the lowest until all coincide or leave the chunk
+  long cur[4] = {0, 1, 2, 3};
+  uint32_t cnt[4] = {0, 0, 0, 0};
+  bool merged = false;
+  while (true) {
+    int lo = -1;
+    for (int e = 0; e < 4; ++e)
+      if (cur[e] < RngPlan::CH && (lo < 0 || cur[e] < cur[lo])) lo = e;
+    if (lo < 0) break;
+    // cursors at the same position share the rest of the parse
+    bool all_same = true;
+    for (int e = 1; e < 4; ++e) all_same &= (cur[e] == cur[0]);
+    if (all_same) {
+      merged = true;
+      break;
+    }
+    bool acc;
+    double v;
+    long pos = cur[lo];
+    int used = zig_iter(get, pos, acc, v);
+    for (int e = 0; e < 4; ++e)
+      if (cur[e] == pos) {
+        cur[e] += used;
+        cnt[e] += acc;
+      }
+  }
+  if (merged) {
+    long pos = cur[0];
+    uint32_t extra = 0;
+    while (pos < RngPlan::CH) {
+      bool acc;
+      double v;
+      pos += zig_iter(get, pos, acc, v);
+      extra += acc;
+    }
+    for (int e = 0; e < 4; ++e) {
+      cur[e] = pos;
+      cnt[e] += extra;
+    }
+  }
+  // summary: exits packed in 8 bits (2 bits per entry), then four counts
+  uint32_t ex = 0;
+  for (int e = 0; e < 4; ++e) ex |= uint32_t(cur[e] - RngPlan::CH) << (2 * e);
+  summary[t * 5] = ex;
+  for (int e = 0; e < 4; ++e) summary[t * 5 + 1 + e] = cnt[e];
+}
+
+// Pass 2: single-workgroup scan of the chunk maps.  Writes per chunk its true
+// entry offset and the global index of its first Gaussian.
+struct Map4 {
+  uint8_t ex[4];
+  uint32_t cnt[4];
+};
+__device__ __forceinline__ Map4 compose(const Map4 &a, const Map4 &b) {
+  Map4 r;
+  for (int e = 0; e < 4; ++e) {
+    r.ex[e] = b.ex[a.ex[e]];
+    r.cnt[e] = a.cnt[e] + b.cnt[a.ex[e]];
+  }
+  return r;
+}
+__device__ __forceinline__ Map4 load_map(const uint32_t *s, long t) {
+  Map4 m;
+  uint32_t ex = s[t * 5];
+  for (int e = 0; e < 4; ++e) {
+    m.ex[e] = (ex >> (2 * e)) & 3;
+    m.cnt[e] = s[t * 5 + 1 + e];
+  }
+  return m;
+}
+
+__global__ __launch_bounds__(1024) void k_rng_scan(const uint32_t *__restrict__ summary, long nchunks,
+                                                  uint64_t *__restrict__ start) {
+  __shared__ Map4 sm[1024];
+  int tid = threadIdx.x;
+  long per = (nchunks + 1023) / 1024;
+  long b0 = tid * per, b1 = min(nchunks, b0 + per);
+  Map4 acc;
+  for (int e = 0; e < 4; ++e) {
+    acc.ex[e] = e;
+    acc.cnt[e] = 0;
+  }
+  for (long t = b0; t < b1; ++t) acc = compose(acc, load_map(summary, t));
+  sm[tid] = acc;
+  __syncthreads();
+  // inclusive Hillis-Steele scan of maps
+  for (int off = 1; off < 1024; off <<= 1) {
+    Map4 v = sm[tid];
+    if (tid >= off) v = compose(sm[tid - off], v);
+    __syncthreads();
+    sm[tid] = v;
+    __syncthreads();
+  }
+  // exclusive prefix for this thread: entry offset 0 at the stream start
+  int e = 0;
+  uint64_t base = 0;
+  if (tid > 0) {
+    e = sm[tid - 1].ex[0];
+    base = sm[tid - 1].cnt[0];
+  }
+  for (long t = b0; t < b1; ++t) {
+    start[t] = (base << 2) | uint64_t(e);
+    Map4 m = load_map(summary, t);
+    base += m.cnt[e];
+    e = m.ex[e];
+  }
+}
+
+// Pass 3: re-parse each chunk from its true entry and write its Gaussians.
+template <typename T>
+__global__ __launch_bounds__(256) void k_rng_emit(const uint32_t *__restrict__ raw, const uint64_t *__restrict__ start,
+                                                 long nchunks, T *f0, int n0, T *f1, int n1, T *f2, int n2, T *f3,
+                                                 int n3, size_t c1, size_t c2, size_t c3, size_t total) {
+  long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nchunks) return;
+  uint64_t s = start[t];
+  uint64_t g = s >> 2;
+  long pos = long(s & 3);
+  if (g >= total) return;
+  auto get = [&](long q) -> uint32_t {
+    return q < RngPlan::CH ? raw[raw_at(t, int(q))] : raw[raw_at(t + 1, int(q - RngPlan::CH))];
+  };
+  while (pos < RngPlan::CH && g < total) {
+    bool acc;
+    double v;
+    pos += zig_iter(get, pos, acc, v);
+    if (acc) {
+      T *f;
+      int n;
+      size_t i;
+      if (g < c1) { f = f0; n = n0; i = g; }
+      else if (g < c2) { f = f1; n = n1; i = g - c1; }
+      else if (g < c3) { f = f2; n = n2; i = g - c2; }
+      else { f = f3; n = n3; i = g - c3; }
+      size_t nn = size_t(n) * n;
+      size_t x = i / nn, r = i - x * nn, y = r / n, z = r - y * n;
+      f[(x * n + y) * (n + 2) + z] = T(v);
+      ++g;
+    }
+  }
+}
+
+__global__ void k_rng_raw_copy(const uint32_t *__restrict__ raw, long nchunks, uint32_t *out, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  long t = long(i / RngPlan::CH);
+  int e = int(i - size_t(t) * RngPlan::CH);
+  out[i] = raw[raw_at(t, e)];
+}
+
+// ---- host set-up ------------------------------------------------------------------
+static void seed_words(unsigned long s, uint64_t *xw) {
+  // GSL ranlxs_set_lux / Luescher rlxs_init: bits of the seed drive a
+  // 31-bit shift register clocked 48 times per word.
+  if (s == 0) s = 1;
+  long seed = (long)s;
+  int i = (int)(seed & 0xFFFFFFFFUL);
+  int xbit[31];
+  for (int k = 0; k < 31; ++k) {
+    xbit[k] = i % 2;
+    i /= 2;
+  }
+  int ibit = 0, jbit = 18;
+  for (int k = 0; k < 12; ++k) {
+    int64_t xv = 0;
+    for (int m = 1; m <= 48; ++m) {
+      int64_t y = xbit[ibit];
+      xv += xv + y;
+      xbit[ibit] = (xbit[ibit] + xbit[jbit]) % 2;
+      ibit = (ibit + 1) % 31;
+      jbit = (jbit + 1) % 31;
+    }
+    xw[k] = (uint64_t)xv & MASK48;
+  }
+}
+
+static void u576_from_words(const uint64_t *x, int nw, uint64_t *out) {
+  for (int i = 0; i < 9; ++i) out[i] = 0;
+  for (int j = 0; j < nw; ++j) put48(out, 48 * j, x[j]);
+}
+
+static void u576_powmod(const uint64_t *base, u128 e_lo, uint64_t *r) {
+  uint64_t acc[9] = {1, 0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t b[9];
+  for (int i = 0; i < 9; ++i) b[i] = base[i];
+  while (e_lo) {
+    if (e_lo & 1) {
+      uint64_t t[9];
+      u576_mulmod(acc, b, t);
+      memcpy(acc, t, sizeof acc);
+    }
+    uint64_t t[9];
+    u576_mulmod(b, b, t);
+    memcpy(b, t, sizeof b);
+    e_lo >>= 1;
+  }
+  memcpy(r, acc, sizeof acc);
+}
+
+void rng_plan(unsigned long seed, size_t n_gaussians, RngPlan &plan) {
+  seed_words(seed, plan.seed_words);
+  // A = sum_j x_j b^j (12 words), B = sum_j x_{7+j} b^j (5 words)
+  uint64_t A[9], B[9];
+  u576_from_words(plan.seed_words, 12, A);
+  u576_from_words(plan.seed_words + 7, 5, B);
+  // reduce A mod m (A < 2^576 but may exceed m)
+  while (u576_geq_m(A)) u576_sub_m(A);
+  // V_0 = A - B (mod m) is the LCG state one step past the seed (carry 0)
+  uint64_t V[9];
+  {
+    uint64_t borrow = 0;
+    for (int i = 0; i < 9; ++i) {
+      u128 d = (u128)A[i] - B[i] - borrow;
+      V[i] = (uint64_t)d;
+      borrow = (uint64_t)(d >> 64) & 1;
+    }
+    if (borrow) {
+      // add m back
+      const uint64_t m[9] = {1, 0, 0, 0xFFFF000000000000ull, ~0ull, ~0ull, ~0ull, ~0ull, ~0ull};
+      u128 c = 0;
+      for (int i = 0; i < 9; ++i) {
+        u128 t = (u128)V[i] + m[i] + c;
+        V[i] = (uint64_t)t;
+        c = t >> 64;
+      }
+    }
+  }
+  // b^{-1} = m - b^11 + b^4 ; b^12 = b^5 - 1 (mod m)
+  uint64_t binv[9];
+  {
+    const uint64_t m[9] = {1, 0, 0, 0xFFFF000000000000ull, ~0ull, ~0ull, ~0ull, ~0ull, ~0ull};
+    uint64_t b11[9] = {0}, b4[9] = {0};
+    put48(b11, 48 * 11, 1);
+    put48(b4, 48 * 4, 1);
+    uint64_t borrow = 0;
+    for (int i = 0; i < 9; ++i) {
+      u128 d = (u128)m[i] - b11[i] - borrow;
+      binv[i] = (uint64_t)d;
+      borrow = (uint64_t)(d >> 64) & 1;
+    }
+    u128 c = 0;
+    for (int i = 0; i < 9; ++i) {
+      u128 t = (u128)binv[i] + b4[i] + c;
+      binv[i] = (uint64_t)t;
+      c = t >> 64;
+    }
+  }
+  // The chunk-t start state is step T_t = 397*KC*t; the device decodes the
+  // LCG state V_{T-12} = Z_{T-11} = b^{-(T-12)} V_0 and runs 12 steps.
+  // y0 = b^{12} V_0 ;  jump[i] = b^{-397*KC*2^i}.
+  uint64_t b12[9];
+  {
+    uint64_t one_b[9] = {0};
+    put48(one_b, 48, 1);        // b
+    u576_powmod(one_b, 12, b12);
+  }
+  u576_mulmod(b12, V, plan.y0.w);
+  uint64_t step[9];
+  u576_powmod(binv, (u128)397 * RngPlan::KC, step);
+  for (int i = 0; i < RngPlan::MAXBITS; ++i) {
+    memcpy(plan.jump[i].w, step, sizeof step);
+    uint64_t t[9];
+    u576_mulmod(step, step, t);
+    memcpy(step, t, sizeof t);
+  }
+  // raw numbers needed: 2 per Gaussian plus rejections (~1.2% of iterations
+  // take 3-4 numbers and may repeat); 2.2x plus slack is never exceeded in
+  // practice and is checked after the scan.
+  double need = 2.2 * double(n_gaussians) + 4096.0;
+  plan.nchunks = long(need / RngPlan::CH) + 2;
+}
+
+size_t rng_workspace_bytes(const RngPlan &plan) {
+  long nc = plan.nchunks + 64;   // one extra wave of chunks so raw_at(t+1, .) stays in bounds
+  size_t raw = size_t((nc + 63) / 64) * 64 * RngPlan::CH * 4;
+  size_t ovh = size_t(nc) * 24 * 4;
+  size_t summ = size_t(nc) * 5 * 4;
+  size_t start = size_t(nc) * 8;
+  return raw + ovh + summ + start + 4 * 256;
+}
+
+struct RngBufs {
+  uint32_t *raw, *ovh, *summary;
+  uint64_t *start;
+};
+static RngBufs carve(const RngPlan &plan, void *ws, size_t bytes) {
+  long nc = plan.nchunks + 64;
+  char *p = (char *)ws;
+  auto take = [&](size_t b) {
+    char *r = p;
+    p += (b + 255) & ~size_t(255);
+    return r;
+  };
+  RngBufs r;
+  r.raw = (uint32_t *)take(size_t((nc + 63) / 64) * 64 * RngPlan::CH * 4);
+  r.ovh = (uint32_t *)take(size_t(nc) * 24 * 4);
+  r.summary = (uint32_t *)take(size_t(nc) * 5 * 4);
+  r.start = (uint64_t *)take(size_t(nc) * 8);
+  if ((size_t)(p - (char *)ws) > bytes) throw std::runtime_error("RNG workspace too small");
+  return r;
+}
+
+static void upload_tables() {
+  static bool done = false;
+  if (done) return;
+  GIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_ytab), ZIG_YTAB, sizeof ZIG_YTAB));
+  GIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_wtab), ZIG_WTAB, sizeof ZIG_WTAB));
+  GIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_ktab), ZIG_KTAB, sizeof ZIG_KTAB));
+  done = true;
+}
+
+static RngDev to_dev(const RngPlan &plan) {
+  RngDev d;
+  memcpy(d.seed_words, plan.seed_words, sizeof d.seed_words);
+  d.y0 = plan.y0;
+  memcpy(d.jump, plan.jump, sizeof d.jump);
+  return d;
+}
+
+static void generate(const RngPlan &plan, const RngBufs &b, hipStream_t st, TimingHook *hook) {
+  upload_tables();
+  RngDev d = to_dev(plan);
+  long nc = plan.nchunks;
+  if (hook) hook->begin("rng_generate", st);
+  hipLaunchKernelGGL(k_rng_generate, dim3((nc + 255) / 256), dim3(256), 0, st, d, nc, b.raw, b.ovh, b.summary);
+  GIC_HIP(hipGetLastError());
+  if (hook) hook->end("rng_generate", st);
+}
+
+template <typename T>
+void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, void *ws, size_t ws_bytes,
+                   hipStream_t st, TimingHook *hook) {
+  if (levels.empty() || levels.size() > 4) throw std::runtime_error("real-space draw supports 1 to 4 levels");
+  RngBufs b = carve(plan, ws, ws_bytes);
+  generate(plan, b, st, hook);
+  long nc = plan.nchunks;
+  if (hook) hook->begin("rng_scan", st);
+  hipLaunchKernelGGL(k_rng_scan, dim3(1), dim3(1024), 0, st, b.summary, nc, b.start);
+  GIC_HIP(hipGetLastError());
+  if (hook) hook->end("rng_scan", st);
+  // check coverage: the last chunk's start must be past the requested count
+  uint64_t last = 0;
+  GIC_HIP(hipMemcpyAsync(&last, b.start + (nc - 1), 8, hipMemcpyDeviceToHost, st));
+  GIC_HIP(hipStreamSynchronize(st));
+  size_t total = 0;
+  for (auto &l : levels) total += l.count;
+  if ((last >> 2) < total) throw std::runtime_error("internal error: RNG plan produced too few Gaussians");
+  T *f[4] = {nullptr, nullptr, nullptr, nullptr};
+  int n[4] = {0, 0, 0, 0};
+  size_t cum[4] = {0, 0, 0, 0};
+  size_t acc = 0;
+  for (size_t i = 0; i < 4; ++i) {
+    if (i < levels.size()) {
+      f[i] = (T *)levels[i].field;
+      n[i] = levels[i].n;
+      acc += levels[i].count;
+    }
+    cum[i] = acc;
+  }
+  if (hook) hook->begin("rng_emit", st);
+  hipLaunchKernelGGL(k_rng_emit<T>, dim3((nc + 255) / 256), dim3(256), 0, st, b.raw, b.start, nc, f[0], n[0], f[1],
+                     n[1], f[2], n[2], f[3], n[3], cum[0], cum[1], cum[2], total);
+  GIC_HIP(hipGetLastError());
+  if (hook) hook->end("rng_emit", st);
+}
+
+template void rng_draw_real<float>(const RngPlan &, const std::vector<RngLevelOut> &, void *, size_t, hipStream_t,
+                                   TimingHook *);
+template void rng_draw_real<double>(const RngPlan &, const std::vector<RngLevelOut> &, void *, size_t, hipStream_t,
+                                    TimingHook *);
+
+void rng_raw(const RngPlan &plan, uint32_t *d_out, size_t n, void *ws, size_t ws_bytes, hipStream_t st) {
+  RngBufs b = carve(plan, ws, ws_bytes);
+  if (n > size_t(plan.nchunks) * RngPlan::CH) throw std::runtime_error("rng_raw: plan too small");
+  generate(plan, b, st, nullptr);
+  hipLaunchKernelGGL(k_rng_raw_copy, dim3((n + 255) / 256), dim3(256), 0, st, b.raw, plan.nchunks, d_out, n);
+  GIC_HIP(hipGetLastError());
+}
+
+}  // namespace gic

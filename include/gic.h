@@ -1,0 +1,121 @@
+/*
+ * C-ABI of libgic_hip.so: genetIC's field-generation path on an MI355X.
+ *
+ * The reference (pynbody/genetIC) has no FFI; its hot path is reached
+ * through the paramfile command dispatcher (genetIC/src/main.cpp:22-172,
+ * genetIC/src/tools/parser.hpp:159-211) which calls ICGenerator methods
+ * (genetIC/src/ic.hpp).  Each entry point below replaces one such method,
+ * with plain pointers and sizes, so a paramfile front-end (C++ or ctypes)
+ * can drive it.  Conventions:
+ *   - every call returns 0 on success, non-zero on error; the message is
+ *     available from gic_last_error() (the front-end rethrows it, keeping the
+ *     reference's 'Error "<what>" on line N' contract of parser.hpp:202-207);
+ *   - the context owns all device memory; host pointers are borrowed for the
+ *     duration of the call; calls are stream-ordered and synchronous on
+ *     return;
+ *   - one host thread per context.
+ */
+#ifndef GIC_H
+#define GIC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gic_context gic_context;
+
+enum { GIC_FLOAT32 = 32, GIC_FLOAT64 = 64 };
+enum {
+  GIC_SEED_REAL_SPACE = 0,       /* random_seed_real_space, seed */
+  GIC_SEED_FOURIER_SERIAL = 1,   /* random_seed_serial, seedfourier */
+  GIC_SEED_FOURIER_PARALLEL = 2, /* random_seed, seedfourier_parallel */
+  GIC_SEED_FOURIER_REVERSE = 3   /* seedfourier_reverse */
+};
+enum { GIC_FIELD_OVERDENSITY = 0, GIC_FIELD_PSI_X = 1, GIC_FIELD_PSI_Y = 2, GIC_FIELD_PSI_Z = 3 };
+enum { GIC_REAL_SPACE = 0, GIC_FOURIER_SPACE = 1 };
+enum { GIC_DTYPE_F32 = 0, GIC_DTYPE_F64 = 1 };
+
+/* Context life cycle.  precision is GIC_FLOAT32 (the reference's -f) or
+ * GIC_FLOAT64 (its default).  Replaces ICGenerator construction,
+ * genetIC/src/main.cpp:200-241 and ic.hpp:178-220. */
+int gic_create(gic_context **ctx, int device, int precision);
+void gic_destroy(gic_context *ctx);
+const char *gic_last_error(const gic_context *ctx);
+
+/* Cosmology commands Om, Ol, Ob, hubble, s8, ns, zin (ic.hpp:229-340). */
+int gic_set_cosmology(gic_context *ctx, double omega_m, double omega_lambda, double omega_b, double hubble,
+                      double sigma8, double ns, double redshift);
+
+/* camb <file> (ic.hpp:725-728, cosmology/camb.hpp:173-332): the file's
+ * numbers flattened row by row, and its column count (7 or 13).  Freezes
+ * ns, sigma8 and the growth factor at the current cosmology. */
+int gic_set_camb_table(gic_context *ctx, const double *values, size_t nvalues, int ncols);
+
+/* base_grid <box> <n> (ic.hpp:365-386). */
+int gic_init_base_grid(gic_context *ctx, double box, int n);
+/* zoom_grid_with_origin_at <x0> <y0> <z0> <zoomfac> <n> (ic.hpp:501-601). */
+int gic_init_zoom_grid_with_origin_at(gic_context *ctx, int x0, int y0, int z0, int zoomfac, int n);
+int gic_num_levels(gic_context *ctx, int *out);
+/* n, box size (thisGridSize), offsetLower[3] and cell size of a level. */
+int gic_level_geometry(gic_context *ctx, int level, int *n, double *box, double *offset3, double *dx);
+
+/* random_seed* / seedfourier* (ic.hpp:620-666). */
+int gic_seed(gic_context *ctx, unsigned long seed, int mode);
+/* initialiseAllRandomComponents (ic.hpp:1334-1352): draws the white noise of
+ * every level; the real-space stream continues across levels. */
+int gic_draw(gic_context *ctx);
+/* zero_level <level> (ic.hpp:787-802). */
+int gic_zero_level(gic_context *ctx, int level);
+/* applyPowerSpec (ic.hpp:825-843): delta_k = w_k sqrt(P(k)) on every level. */
+int gic_apply_power_spectrum(gic_context *ctx);
+/* Zel'dovich displacements of every level and the combination of levels
+ * (multilevelgenerator.hpp:78-128, zeldovich.hpp:152-253). */
+int gic_zeldovich(gic_context *ctx);
+
+/* Copy a field of a level to host memory (dump_grid, ic.hpp:852-918).
+ * space GIC_REAL_SPACE: n^3 values in index order (x*n+y)*n+z;
+ * GIC_FOURIER_SPACE: FFTW layout, n*n*(n/2+1) complex values.  The field is
+ * transformed in place first if needed, as Field::toReal/toFourier do. */
+int gic_copy_field(gic_context *ctx, int field, int level, int space, void *host, int dtype);
+/* Overwrite a field (import_level, ic.hpp:809-821, and tests). */
+int gic_set_field(gic_context *ctx, int field, int level, int space, const void *host, int dtype);
+
+/* Particle evaluation (ZeldovichParticleEvaluator::getParticle,
+ * zeldovich.hpp:87-109, generator.hpp:49-53): for cells[i] (or start+i when
+ * cells is NULL) of a level, pos = centroid + psi wrapped into the box
+ * (Mpc/h) and vel = psi * H(a) sqrt(a) (gadget units), as float32. */
+int gic_particles(gic_context *ctx, int level, const uint64_t *cells, size_t start, size_t count, float *pos,
+                  float *vel);
+/* Particle mass (1e10 Msol/h) and softening (Mpc/h) of a level (zeldovich.hpp:53-119). */
+int gic_particle_mass_eps(gic_context *ctx, int level, double *mass, double *eps);
+/* zeldovichVelocityToOffsetRatio (cosmology/parameters.hpp:83-106). */
+int gic_velocity_ratio(gic_context *ctx, double *out);
+
+/* In-place 3D transform of a caller-owned device buffer holding a padded
+ * n*n*(n+2) field of the context's precision: FieldFourierManager::
+ * performTransform (tools/numerics/fourier.hpp:479-518).  direction -1:
+ * real -> Fourier, +1: Fourier -> real; both scale by n^-1.5. */
+int gic_fft_inplace(gic_context *ctx, void *device_field, int n, int direction);
+
+/* First n 24-bit outputs of gsl_rng_ranlxs2 seeded with seed (the integers
+ * gsl_rng_get returns; randomfieldgenerator.hpp:43-45,115). */
+int gic_raw_stream(gic_context *ctx, unsigned long seed, uint32_t *host_out, size_t n);
+/* First n gsl_ran_gaussian_ziggurat(ranlxs2, 1) draws (randomfieldgenerator.hpp:191). */
+int gic_gaussian_stream(gic_context *ctx, unsigned long seed, double *host_out, size_t n);
+
+/* Stream control and per-kernel event timing (hipEvents on the context's
+ * stream around each launch). */
+int gic_synchronize(gic_context *ctx);
+int gic_set_event_timing(gic_context *ctx, int enable);
+int gic_get_event_timing(gic_context *ctx, const char *kernel, double *total_ms, long *count);
+int gic_reset_event_timing(gic_context *ctx);
+/* Device bytes currently held by the context. */
+int gic_device_bytes(gic_context *ctx, size_t *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GIC_H */

@@ -1,0 +1,56 @@
+"""Copies the reference's own test fixtures used by this repository's parity
+tests into tests/golden/ (data only: inputs, paramfiles and expected
+outputs).  Run in a container where the reference is mounted at
+/root/reference; the copies are committed so the GPU box does not need it.
+
+Provenance: pynbody/genetIC, genetIC/tests/<test>/...
+"""
+import os
+import shutil
+
+SRC = "/root/reference/genetIC/tests"
+DST = os.path.dirname(os.path.abspath(__file__))
+
+FILES = [
+    "camb_transfer_kmax40_z0.dat",
+    "camb_transfer_kmax40_z0_post2015.dat",
+    "camb_transfer_kmax40_z500.dat",
+    "test_19_rounding_errors/paramfile.txt",
+    "test_19_rounding_errors/reference_grid/grid-0.npy",
+    "test_01c_zoom_grid/paramfile.txt",
+    "test_01c_zoom_grid/reference_grid/grid-0.npy",
+    "test_01c_zoom_grid/reference_grid/grid-1.npy",
+    "test_01c_zoom_grid/reference_grid/grid-info-0.txt",
+    "test_01c_zoom_grid/reference_grid/grid-info-1.txt",
+    "test_01c_zoom_grid/reference_grid/test_2_0.ps",
+    "test_01c_zoom_grid/reference_grid/test_2_1.ps",
+    "test_10a_gadget/paramfile.txt",
+    "test_10a_gadget/reference_output",
+    "test_02_simple_modif/paramfile.txt",
+    "test_02_simple_modif/reference_output",
+    "test_02_simple_modif/reference_output.overdensity",
+    "test_13_parallel_seed/paramfile.txt",
+    "test_13_parallel_seed/reference_output",
+    "test_05a_seed_old/paramfile.txt",
+    "test_05a_seed_old/reference_output",
+    "test_02k_chisquare_onelevel/paramfile.txt",
+    "test_02k_chisquare_onelevel/reference.txt",
+    "test_07a_CAMB_new_interface/paramfile.txt",
+    "test_07a_CAMB_new_interface/reference_output",
+]
+
+
+def main():
+    for rel in FILES:
+        src = os.path.join(SRC, rel)
+        dst = os.path.join(DST, rel)
+        if not os.path.exists(src):
+            print("missing", rel)
+            continue
+        os.makedirs(os.path.dirname(dst), exist_ok=True)
+        shutil.copyfile(src, dst)
+    print("copied", len(FILES), "files")
+
+
+if __name__ == "__main__":
+    main()
