@@ -728,6 +728,21 @@ int gic_seed(gic_context *c, unsigned long seed, int mode) {
   });
 }
 
+int gic_reseed(gic_context *c, unsigned long seed, int mode) {
+  return guarded(c, [&] {
+    if (mode < 0 || mode > 3) throw std::runtime_error("unknown seeding mode");
+    c->seed = seed;
+    c->seed_mode = mode;
+    c->seeded = true;
+    c->drawn = c->power_applied = c->zeldovich_done = false;
+    for (auto &L : c->levels) {
+      L->noise_fourier = false;
+      L->psi_fourier = false;
+      L->have_psi = false;
+    }
+  });
+}
+
 int gic_draw(gic_context *c) {
   return guarded(c, [&] { DISPATCH(c, do_draw, c); });
 }
@@ -854,6 +869,25 @@ int gic_reset_event_timing(gic_context *c) {
     if (c->timer) {
       c->timer->collect();
       c->timer->totals.clear();
+    }
+  });
+}
+
+int gic_list_event_timing(gic_context *c, char *buf, size_t buflen, size_t *needed) {
+  return guarded(c, [&] {
+    if (!c->timer) throw std::runtime_error("event timing is not enabled");
+    c->timer->collect();
+    std::string s;
+    char line[256];
+    for (auto &kv : c->timer->totals) {
+      snprintf(line, sizeof line, "%s\t%.6f\t%ld\n", kv.first.c_str(), kv.second.first, kv.second.second);
+      s += line;
+    }
+    if (needed) *needed = s.size() + 1;
+    if (buf && buflen) {
+      size_t m = std::min(buflen - 1, s.size());
+      memcpy(buf, s.data(), m);
+      buf[m] = 0;
     }
   });
 }

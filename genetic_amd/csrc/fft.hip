@@ -20,6 +20,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <string>
 
 namespace gic {
 
@@ -345,10 +346,11 @@ template <typename T, int N> struct FftN {
     const cx<T> *tw = twtable<T>().get(N);
     auto kern = dir < 0 ? k_fft_strided<T, N, C, -1, THREADS> : k_fft_strided<T, N, C, +1, THREADS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    if (hook) hook->begin(name, st);
+    std::string tag = std::string(name) + "." + std::to_string(N);
+    if (hook) hook->begin(tag.c_str(), st);
     hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, d, ls, os, ncol, nkzb, scale, tw);
     GIC_HIP(hipGetLastError());
-    if (hook) hook->end(name, st);
+    if (hook) hook->end(tag.c_str(), st);
   }
   static void zpass(T *d, T scale, int dir, hipStream_t st, TimingHook *hook, const char *name) {
     long nrows = long(N) * N;
@@ -357,10 +359,11 @@ template <typename T, int N> struct FftN {
     const cx<T> *tw = twtable<T>().get(N);
     auto kern = dir < 0 ? k_fft_z_r2c<T, N, ROWS, THREADS> : k_fft_z_c2r<T, N, ROWS, THREADS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    if (hook) hook->begin(name, st);
+    std::string tag = std::string(name) + "." + std::to_string(N);
+    if (hook) hook->begin(tag.c_str(), st);
     hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, d, nrows, scale, tw);
     GIC_HIP(hipGetLastError());
-    if (hook) hook->end(name, st);
+    if (hook) hook->end(tag.c_str(), st);
   }
   static void r2c(T *d, hipStream_t st, TimingHook *hook) {
     cx<T> *c = reinterpret_cast<cx<T> *>(d);
@@ -376,10 +379,11 @@ template <typename T, int N> struct FftN {
     T scale = T(1.0 / pow(double(N), 1.5));
     int nt = 256;
     long nn = long(N) * N;
-    if (hook) hook->begin("fft_mirror", st);
+    std::string tag = "fft_mirror." + std::to_string(N);
+    if (hook) hook->begin(tag.c_str(), st);
     hipLaunchKernelGGL(k_mirror_planes<T>, dim3((nn + nt - 1) / nt), dim3(nt), 0, st, c, N);
     GIC_HIP(hipGetLastError());
-    if (hook) hook->end("fft_mirror", st);
+    if (hook) hook->end(tag.c_str(), st);
     strided(c, long(N) * nz, nz, N, T(1), +1, st, hook, "fft_x");
     strided(c, nz, long(N) * nz, N, T(1), +1, st, hook, "fft_y");
     zpass(d, scale, +1, st, hook, "fft_z_c2r");

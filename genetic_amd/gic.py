@@ -35,6 +35,7 @@ SIGNATURES = {
     "gic_num_levels": ([_v, _P(_i)], _i),
     "gic_level_geometry": ([_v, _i, _P(_i), _P(_d), _v, _P(_d)], _i),
     "gic_seed": ([_v, _u, _i], _i),
+    "gic_reseed": ([_v, _u, _i], _i),
     "gic_draw": ([_v], _i),
     "gic_zero_level": ([_v, _i], _i),
     "gic_apply_power_spectrum": ([_v], _i),
@@ -51,6 +52,7 @@ SIGNATURES = {
     "gic_set_event_timing": ([_v, _i], _i),
     "gic_get_event_timing": ([_v, ctypes.c_char_p, _P(_d), _P(ctypes.c_long)], _i),
     "gic_reset_event_timing": ([_v], _i),
+    "gic_list_event_timing": ([_v, _v, _s, _P(_s)], _i),
     "gic_device_bytes": ([_v, _P(_s)], _i),
 }
 
@@ -173,6 +175,9 @@ class ICGenerator:
     def set_seed(self, seed, mode=SEED_REAL_SPACE):
         self._check(lib().gic_seed(self._h, int(seed), int(mode)))
 
+    def reseed(self, seed, mode=SEED_REAL_SPACE):
+        self._check(lib().gic_reseed(self._h, int(seed), int(mode)))
+
     def draw(self):
         self._check(lib().gic_draw(self._h))
 
@@ -255,6 +260,18 @@ class ICGenerator:
         c = ctypes.c_long()
         self._check(lib().gic_get_event_timing(self._h, kernel.encode(), ctypes.byref(t), ctypes.byref(c)))
         return t.value, c.value
+
+    def event_timings(self):
+        """{kernel: (total_ms, launches)} for every timed kernel."""
+        need = ctypes.c_size_t()
+        self._check(lib().gic_list_event_timing(self._h, None, 0, ctypes.byref(need)))
+        buf = ctypes.create_string_buffer(need.value + 16)
+        self._check(lib().gic_list_event_timing(self._h, buf, len(buf), ctypes.byref(need)))
+        out = {}
+        for ln in buf.value.decode().splitlines():
+            name, ms, cnt = ln.split("\t")
+            out[name] = (float(ms), int(cnt))
+        return out
 
     def reset_event_timing(self):
         self._check(lib().gic_reset_event_timing(self._h))

@@ -64,6 +64,11 @@ int gic_level_geometry(gic_context *ctx, int level, int *n, double *box, double 
 
 /* random_seed* / seedfourier* (ic.hpp:620-666). */
 int gic_seed(gic_context *ctx, unsigned long seed, int mode);
+/* Start a new realisation on the same grids with a new seed: forgets the
+ * drawn noise and everything derived from it but keeps the device memory.
+ * No reference counterpart (the reference seeds once per process); used to
+ * run many seeds through one context (batches, benchmarks). */
+int gic_reseed(gic_context *ctx, unsigned long seed, int mode);
 /* initialiseAllRandomComponents (ic.hpp:1334-1352): draws the white noise of
  * every level; the real-space stream continues across levels. */
 int gic_draw(gic_context *ctx);
@@ -112,6 +117,9 @@ int gic_synchronize(gic_context *ctx);
 int gic_set_event_timing(gic_context *ctx, int enable);
 int gic_get_event_timing(gic_context *ctx, const char *kernel, double *total_ms, long *count);
 int gic_reset_event_timing(gic_context *ctx);
+/* All timed kernels as text lines "name<TAB>total_ms<TAB>count"; writes at
+ * most buflen bytes (NUL-terminated) and the full length to *needed. */
+int gic_list_event_timing(gic_context *ctx, char *buf, size_t buflen, size_t *needed);
 /* Device bytes currently held by the context. */
 int gic_device_bytes(gic_context *ctx, size_t *out);
 
