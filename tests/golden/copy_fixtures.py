@@ -24,6 +24,14 @@ FILES = [
     "test_01c_zoom_grid/reference_grid/grid-info-1.txt",
     "test_01c_zoom_grid/reference_grid/test_2_0.ps",
     "test_01c_zoom_grid/reference_grid/test_2_1.ps",
+    "test_01d_two_zooms/paramfile.txt",
+    "test_01d_two_zooms/reference_grid/grid-2.npy",
+    "test_01d_two_zooms/reference_grid/grid-info-0.txt",
+    "test_01d_two_zooms/reference_grid/grid-info-1.txt",
+    "test_01d_two_zooms/reference_grid/grid-info-2.txt",
+    "test_01d_two_zooms/reference_ps/test_01d_0.ps",
+    "test_01d_two_zooms/reference_ps/test_01d_1.ps",
+    "test_01d_two_zooms/reference_ps/test_01d_2.ps",
     "test_10a_gadget/paramfile.txt",
     "test_10a_gadget/reference_output",
     "test_02_simple_modif/paramfile.txt",
