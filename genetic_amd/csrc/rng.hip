@@ -17,10 +17,11 @@
 //
 // gsl_ran_gaussian_ziggurat consumes 2 numbers per accepted iteration and
 // 3 or 4 on the rare edge/tail tests, so the Gaussian index of a chunk is
-// data dependent.  Pass 1 parses every chunk from the 4 possible entry
-// offsets (the parses merge after a few iterations), pass 2 scans the
-// entry->(exit, count) maps, pass 3 re-parses each chunk from its true entry
-// and writes the Gaussians to their cells.
+// data dependent.  Pass 1 generates every chunk and parses it from the 4
+// possible entry offsets (the parses merge after a few iterations), pass 2
+// scans the entry->(exit, count) maps, pass 3 regenerates each chunk, parses
+// it from its true entry and writes the Gaussians to their cells.  No raw
+// numbers go to HBM: the parse reads them from a per-thread LDS ring.
 #include "rng.h"
 #include "fft.h"
 #include <cstring>
@@ -214,11 +215,6 @@ __device__ __forceinline__ void ranlux_block(uint64_t *x, int64_t &c) {
   x[11] = t0;
 }
 
-// raw buffer layout: chunk t, element e -> ((t/64)*CH + e)*64 + t%64
-__device__ __forceinline__ size_t raw_at(long t, int e) {
-  return (size_t(t >> 6) * RngPlan::CH + e) * 64 + (t & 63);
-}
-
 struct RngDev {
   uint64_t seed_words[12];
   U576 y0;
@@ -282,81 +278,86 @@ __device__ __forceinline__ int zig_iter(Get get, long pos, bool &accept, double 
   return used;
 }
 
-// Pass 1: generate chunk t, store it, parse from the four entry offsets.
-__global__ __launch_bounds__(256) void k_rng_generate(RngDev p, long nchunks, uint32_t *__restrict__ raw,
-                                                     uint32_t *__restrict__ ovh, uint32_t *__restrict__ summary) {
+// Chunks are parsed straight out of the generator: each thread keeps the
+// newest 48 numbers of its chunk in an LDS ring (slot-major, so a wave's
+// threads touch consecutive banks) and advances its ziggurat cursors while
+// an iteration is guaranteed to fit in the numbers generated so far.
+constexpr int RING = 48;
+
+struct Ring {
+  uint32_t *base;
+  int stride;
+  __device__ __forceinline__ uint32_t get(long pos) const { return base[int(pos % RING) * stride]; }
+  __device__ __forceinline__ void put(long pos, uint32_t v) const { base[int(pos % RING) * stride] = v; }
+};
+
+__device__ __forceinline__ void push_block(const Ring &r, long at, const uint64_t *x) {
+#pragma unroll
+  for (int k = 0; k < 12; ++k) {
+    r.put(at + 2 * k, uint32_t(x[k] & 0xFFFFFF));
+    r.put(at + 2 * k + 1, uint32_t(x[k] >> 24));
+  }
+}
+
+// Pass 1: per chunk, the exit offset and Gaussian count for each of the four
+// possible entry offsets.  Once the four cursors sit on the same position
+// they share the rest of the parse, so only one is advanced and its later
+// acceptances are added to all four.
+__global__ __launch_bounds__(256) void k_rng_summary(RngDev p, long nchunks, uint32_t *__restrict__ summary) {
+  extern __shared__ uint32_t ring_mem[];
   long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= nchunks) return;
+  Ring ring{ring_mem + threadIdx.x, int(blockDim.x)};
   uint64_t x[12];
   int64_t c;
   chunk_start_state(p, t, x, c);
-  for (int q = 0; q < RngPlan::KC; ++q) {
-    ranlux_block(x, c);
-#pragma unroll
-    for (int k = 0; k < 12; ++k) {
-      raw[raw_at(t, q * 24 + 2 * k)] = uint32_t(x[k] & 0xFFFFFF);
-      raw[raw_at(t, q * 24 + 2 * k + 1)] = uint32_t(x[k] >> 24);
-    }
-  }
-  // the next chunk's first block, for iterations that straddle the boundary
-  ranlux_block(x, c);
-#pragma unroll
-  for (int k = 0; k < 12; ++k) {
-    ovh[t * 24 + 2 * k] = uint32_t(x[k] & 0xFFFFFF);
-    ovh[t * 24 + 2 * k + 1] = uint32_t(x[k] >> 24);
-  }
-  auto get = [&](long pos) -> uint32_t {
-    return pos < RngPlan::CH ? raw[raw_at(t, int(pos))] : ovh[t * 24 + (pos - RngPlan::CH)];
-  };
-  // four cursors; advance the lowest until all coincide or leave the chunk
+  auto get = [&](long pos) { return ring.get(pos); };
   long cur[4] = {0, 1, 2, 3};
   uint32_t cnt[4] = {0, 0, 0, 0};
+  uint32_t shared_cnt = 0;   // acceptances after the cursors merged
   bool merged = false;
-  while (true) {
-    int lo = -1;
-    for (int e = 0; e < 4; ++e)
-      if (cur[e] < RngPlan::CH && (lo < 0 || cur[e] < cur[lo])) lo = e;
-    if (lo < 0) break;
-    // cursors at the same position share the rest of the parse
-    bool all_same = true;
-    for (int e = 1; e < 4; ++e) all_same &= (cur[e] == cur[0]);
-    if (all_same) {
-      merged = true;
-      break;
-    }
-    bool acc;
-    double v;
-    long pos = cur[lo];
-    int used = zig_iter(get, pos, acc, v);
-    for (int e = 0; e < 4; ++e)
-      if (cur[e] == pos) {
-        cur[e] += used;
-        cnt[e] += acc;
+  long avail = 0;
+  for (int q = 0; q <= RngPlan::KC; ++q) {
+    ranlux_block(x, c);
+    push_block(ring, avail, x);
+    avail += 24;
+    if (merged) {
+      long pos = cur[0];
+      while (pos < RngPlan::CH && pos + 4 <= avail) {
+        bool acc;
+        double v;
+        pos += zig_iter(get, pos, acc, v);
+        shared_cnt += acc;
       }
-  }
-  if (merged) {
-    long pos = cur[0];
-    uint32_t extra = 0;
-    while (pos < RngPlan::CH) {
-      bool acc;
-      double v;
-      pos += zig_iter(get, pos, acc, v);
-      extra += acc;
+      cur[0] = pos;
+      if (pos >= RngPlan::CH) break;
+    } else {
+      bool done = true;
+      for (int e = 0; e < 4; ++e) {
+        long pos = cur[e];
+        while (pos < RngPlan::CH && pos + 4 <= avail) {
+          bool acc;
+          double v;
+          pos += zig_iter(get, pos, acc, v);
+          cnt[e] += acc;
+        }
+        cur[e] = pos;
+        done &= pos >= RngPlan::CH;
+      }
+      merged = cur[0] == cur[1] && cur[0] == cur[2] && cur[0] == cur[3];
+      if (done) break;
     }
-    for (int e = 0; e < 4; ++e) {
-      cur[e] = pos;
-      cnt[e] += extra;
-    }
   }
-  // summary: exits packed in 8 bits (2 bits per entry), then four counts
   uint32_t ex = 0;
-  for (int e = 0; e < 4; ++e) ex |= uint32_t(cur[e] - RngPlan::CH) << (2 * e);
+  for (int e = 0; e < 4; ++e) {
+    long end = merged ? cur[0] : cur[e];
+    ex |= uint32_t(end - RngPlan::CH) << (2 * e);
+    summary[t * 5 + 1 + e] = cnt[e] + (merged ? shared_cnt : 0);
+  }
   summary[t * 5] = ex;
-  for (int e = 0; e < 4; ++e) summary[t * 5 + 1 + e] = cnt[e];
 }
 
-// Pass 2: single-workgroup scan of the chunk maps.  Writes per chunk its true
-// entry offset and the global index of its first Gaussian.
+// Pass 2a: compose the maps of SEG consecutive chunks.
 struct Map4 {
   uint8_t ex[4];
   uint32_t cnt[4];
@@ -378,22 +379,42 @@ __device__ __forceinline__ Map4 load_map(const uint32_t *s, long t) {
   }
   return m;
 }
+__device__ __forceinline__ Map4 identity_map() {
+  Map4 m;
+  for (int e = 0; e < 4; ++e) {
+    m.ex[e] = e;
+    m.cnt[e] = 0;
+  }
+  return m;
+}
+constexpr int SEG = 64;
 
-__global__ __launch_bounds__(1024) void k_rng_scan(const uint32_t *__restrict__ summary, long nchunks,
-                                                  uint64_t *__restrict__ start) {
+__global__ __launch_bounds__(256) void k_rng_seg_compose(const uint32_t *__restrict__ summary, long nchunks, long nseg,
+                                                         uint32_t *__restrict__ segmaps) {
+  long s = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (s >= nseg) return;
+  Map4 acc = identity_map();
+  long b0 = s * SEG, b1 = min(nchunks, b0 + SEG);
+  for (long t = b0; t < b1; ++t) acc = compose(acc, load_map(summary, t));
+  uint32_t ex = 0;
+  for (int e = 0; e < 4; ++e) {
+    ex |= uint32_t(acc.ex[e]) << (2 * e);
+    segmaps[s * 5 + 1 + e] = acc.cnt[e];
+  }
+  segmaps[s * 5] = ex;
+}
+
+// Pass 2b: one workgroup scans the segment maps; segstart[s] = (base << 2) | entry.
+__global__ __launch_bounds__(1024) void k_rng_seg_scan(const uint32_t *__restrict__ segmaps, long nseg,
+                                                      uint64_t *__restrict__ segstart) {
   __shared__ Map4 sm[1024];
   int tid = threadIdx.x;
-  long per = (nchunks + 1023) / 1024;
-  long b0 = tid * per, b1 = min(nchunks, b0 + per);
-  Map4 acc;
-  for (int e = 0; e < 4; ++e) {
-    acc.ex[e] = e;
-    acc.cnt[e] = 0;
-  }
-  for (long t = b0; t < b1; ++t) acc = compose(acc, load_map(summary, t));
+  long per = (nseg + 1023) / 1024;
+  long b0 = tid * per, b1 = min(nseg, b0 + per);
+  Map4 acc = identity_map();
+  for (long t = b0; t < b1; ++t) acc = compose(acc, load_map(segmaps, t));
   sm[tid] = acc;
   __syncthreads();
-  // inclusive Hillis-Steele scan of maps
   for (int off = 1; off < 1024; off <<= 1) {
     Map4 v = sm[tid];
     if (tid >= off) v = compose(sm[tid - off], v);
@@ -401,7 +422,6 @@ __global__ __launch_bounds__(1024) void k_rng_scan(const uint32_t *__restrict__ 
     sm[tid] = v;
     __syncthreads();
   }
-  // exclusive prefix for this thread: entry offset 0 at the stream start
   int e = 0;
   uint64_t base = 0;
   if (tid > 0) {
@@ -409,53 +429,82 @@ __global__ __launch_bounds__(1024) void k_rng_scan(const uint32_t *__restrict__ 
     base = sm[tid - 1].cnt[0];
   }
   for (long t = b0; t < b1; ++t) {
-    start[t] = (base << 2) | uint64_t(e);
-    Map4 m = load_map(summary, t);
+    segstart[t] = (base << 2) | uint64_t(e);
+    Map4 m = load_map(segmaps, t);
     base += m.cnt[e];
     e = m.ex[e];
   }
 }
 
-// Pass 3: re-parse each chunk from its true entry and write its Gaussians.
+// Pass 3: regenerate each chunk, parse from its true entry, write Gaussians.
+struct LevelTargets {
+  void *f[4];
+  int n[4];
+  size_t cum[4];   // cumulative Gaussian counts at the end of each level
+};
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_rng_emit(const uint32_t *__restrict__ raw, const uint64_t *__restrict__ start,
-                                                 long nchunks, T *f0, int n0, T *f1, int n1, T *f2, int n2, T *f3,
-                                                 int n3, size_t c1, size_t c2, size_t c3, size_t total) {
+__global__ __launch_bounds__(256) void k_rng_emit(RngDev p, long nchunks, const uint32_t *__restrict__ summary,
+                                                 const uint64_t *__restrict__ segstart, LevelTargets lt,
+                                                 size_t total) {
+  extern __shared__ uint32_t ring_mem[];
   long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= nchunks) return;
-  uint64_t s = start[t];
-  uint64_t g = s >> 2;
-  long pos = long(s & 3);
+  // entry offset and first Gaussian index of this chunk
+  long s = t / SEG;
+  uint64_t ss = segstart[s];
+  uint64_t g = ss >> 2;
+  int e = int(ss & 3);
+  for (long u = s * SEG; u < t; ++u) {
+    Map4 m = load_map(summary, u);
+    g += m.cnt[e];
+    e = m.ex[e];
+  }
   if (g >= total) return;
-  auto get = [&](long q) -> uint32_t {
-    return q < RngPlan::CH ? raw[raw_at(t, int(q))] : raw[raw_at(t + 1, int(q - RngPlan::CH))];
-  };
-  while (pos < RngPlan::CH && g < total) {
-    bool acc;
-    double v;
-    pos += zig_iter(get, pos, acc, v);
-    if (acc) {
-      T *f;
-      int n;
-      size_t i;
-      if (g < c1) { f = f0; n = n0; i = g; }
-      else if (g < c2) { f = f1; n = n1; i = g - c1; }
-      else if (g < c3) { f = f2; n = n2; i = g - c2; }
-      else { f = f3; n = n3; i = g - c3; }
-      size_t nn = size_t(n) * n;
-      size_t x = i / nn, r = i - x * nn, y = r / n, z = r - y * n;
-      f[(x * n + y) * (n + 2) + z] = T(v);
-      ++g;
+  Ring ring{ring_mem + threadIdx.x, int(blockDim.x)};
+  uint64_t x[12];
+  int64_t c;
+  chunk_start_state(p, t, x, c);
+  auto get = [&](long pos) { return ring.get(pos); };
+  long pos = e;
+  long avail = 0;
+  for (int q = 0; q <= RngPlan::KC && pos < RngPlan::CH && g < total; ++q) {
+    ranlux_block(x, c);
+    push_block(ring, avail, x);
+    avail += 24;
+    while (pos < RngPlan::CH && pos + 4 <= avail && g < total) {
+      bool acc;
+      double v;
+      pos += zig_iter(get, pos, acc, v);
+      if (acc) {
+        int L = g < lt.cum[0] ? 0 : g < lt.cum[1] ? 1 : g < lt.cum[2] ? 2 : 3;
+        size_t i = g - (L ? lt.cum[L - 1] : 0);
+        int n = lt.n[L];
+        size_t nn = size_t(n) * n;
+        size_t xx = i / nn, r = i - xx * nn, y = r / n, z = r - y * n;
+        reinterpret_cast<T *>(lt.f[L])[(xx * n + y) * (n + 2) + z] = T(v);
+        ++g;
+      }
     }
   }
 }
 
-__global__ void k_rng_raw_copy(const uint32_t *__restrict__ raw, long nchunks, uint32_t *out, size_t n) {
-  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  long t = long(i / RngPlan::CH);
-  int e = int(i - size_t(t) * RngPlan::CH);
-  out[i] = raw[raw_at(t, e)];
+// Raw 24-bit numbers in stream order (for bit-exactness checks).
+__global__ __launch_bounds__(256) void k_rng_raw(RngDev p, long nchunks, uint32_t *__restrict__ out, size_t n) {
+  long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= nchunks) return;
+  uint64_t x[12];
+  int64_t c;
+  chunk_start_state(p, t, x, c);
+  size_t base = size_t(t) * RngPlan::CH;
+  for (int q = 0; q < RngPlan::KC; ++q) {
+    ranlux_block(x, c);
+    for (int k = 0; k < 12; ++k) {
+      size_t i = base + q * 24 + 2 * k;
+      if (i < n) out[i] = uint32_t(x[k] & 0xFFFFFF);
+      if (i + 1 < n) out[i + 1] = uint32_t(x[k] >> 24);
+    }
+  }
 }
 
 // ---- host set-up ------------------------------------------------------------------
@@ -581,20 +630,19 @@ void rng_plan(unsigned long seed, size_t n_gaussians, RngPlan &plan) {
 }
 
 size_t rng_workspace_bytes(const RngPlan &plan) {
-  long nc = plan.nchunks + 64;   // one extra wave of chunks so raw_at(t+1, .) stays in bounds
-  size_t raw = size_t((nc + 63) / 64) * 64 * RngPlan::CH * 4;
-  size_t ovh = size_t(nc) * 24 * 4;
-  size_t summ = size_t(nc) * 5 * 4;
-  size_t start = size_t(nc) * 8;
-  return raw + ovh + summ + start + 4 * 256;
+  long nc = plan.nchunks;
+  long nseg = (nc + SEG - 1) / SEG;
+  return (size_t(nc) * 5 * 4 + 255) / 256 * 256 + (size_t(nseg) * 5 * 4 + 255) / 256 * 256 + size_t(nseg) * 8 + 256;
 }
 
 struct RngBufs {
-  uint32_t *raw, *ovh, *summary;
-  uint64_t *start;
+  uint32_t *summary, *segmaps;
+  uint64_t *segstart;
+  long nseg;
 };
 static RngBufs carve(const RngPlan &plan, void *ws, size_t bytes) {
-  long nc = plan.nchunks + 64;
+  long nc = plan.nchunks;
+  long nseg = (nc + SEG - 1) / SEG;
   char *p = (char *)ws;
   auto take = [&](size_t b) {
     char *r = p;
@@ -602,10 +650,10 @@ static RngBufs carve(const RngPlan &plan, void *ws, size_t bytes) {
     return r;
   };
   RngBufs r;
-  r.raw = (uint32_t *)take(size_t((nc + 63) / 64) * 64 * RngPlan::CH * 4);
-  r.ovh = (uint32_t *)take(size_t(nc) * 24 * 4);
   r.summary = (uint32_t *)take(size_t(nc) * 5 * 4);
-  r.start = (uint64_t *)take(size_t(nc) * 8);
+  r.segmaps = (uint32_t *)take(size_t(nseg) * 5 * 4);
+  r.segstart = (uint64_t *)take(size_t(nseg) * 8);
+  r.nseg = nseg;
   if ((size_t)(p - (char *)ws) > bytes) throw std::runtime_error("RNG workspace too small");
   return r;
 }
@@ -627,49 +675,47 @@ static RngDev to_dev(const RngPlan &plan) {
   return d;
 }
 
-static void generate(const RngPlan &plan, const RngBufs &b, hipStream_t st, TimingHook *hook) {
-  upload_tables();
-  RngDev d = to_dev(plan);
-  long nc = plan.nchunks;
-  if (hook) hook->begin("rng_generate", st);
-  hipLaunchKernelGGL(k_rng_generate, dim3((nc + 255) / 256), dim3(256), 0, st, d, nc, b.raw, b.ovh, b.summary);
-  GIC_HIP(hipGetLastError());
-  if (hook) hook->end("rng_generate", st);
-}
-
 template <typename T>
 void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, void *ws, size_t ws_bytes,
                    hipStream_t st, TimingHook *hook) {
   if (levels.empty() || levels.size() > 4) throw std::runtime_error("real-space draw supports 1 to 4 levels");
+  upload_tables();
   RngBufs b = carve(plan, ws, ws_bytes);
-  generate(plan, b, st, hook);
+  RngDev d = to_dev(plan);
   long nc = plan.nchunks;
+  const int TB = 256;
+  size_t ring_bytes = size_t(RING) * TB * 4;
+  if (hook) hook->begin("rng_summary", st);
+  hipLaunchKernelGGL(k_rng_summary, dim3((nc + TB - 1) / TB), dim3(TB), ring_bytes, st, d, nc, b.summary);
+  GIC_HIP(hipGetLastError());
+  if (hook) hook->end("rng_summary", st);
   if (hook) hook->begin("rng_scan", st);
-  hipLaunchKernelGGL(k_rng_scan, dim3(1), dim3(1024), 0, st, b.summary, nc, b.start);
+  hipLaunchKernelGGL(k_rng_seg_compose, dim3((b.nseg + 255) / 256), dim3(256), 0, st, b.summary, nc, b.nseg,
+                     b.segmaps);
+  GIC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_rng_seg_scan, dim3(1), dim3(1024), 0, st, b.segmaps, b.nseg, b.segstart);
   GIC_HIP(hipGetLastError());
   if (hook) hook->end("rng_scan", st);
-  // check coverage: the last chunk's start must be past the requested count
+  // coverage: the stream must hold at least `total` Gaussians
   uint64_t last = 0;
-  GIC_HIP(hipMemcpyAsync(&last, b.start + (nc - 1), 8, hipMemcpyDeviceToHost, st));
+  uint32_t lastmap[5];
+  GIC_HIP(hipMemcpyAsync(&last, b.segstart + (b.nseg - 1), 8, hipMemcpyDeviceToHost, st));
+  GIC_HIP(hipMemcpyAsync(lastmap, b.segmaps + 5 * (b.nseg - 1), 20, hipMemcpyDeviceToHost, st));
   GIC_HIP(hipStreamSynchronize(st));
-  size_t total = 0;
-  for (auto &l : levels) total += l.count;
-  if ((last >> 2) < total) throw std::runtime_error("internal error: RNG plan produced too few Gaussians");
-  T *f[4] = {nullptr, nullptr, nullptr, nullptr};
-  int n[4] = {0, 0, 0, 0};
-  size_t cum[4] = {0, 0, 0, 0};
+  uint64_t produced = (last >> 2) + lastmap[1 + (last & 3)];
+  LevelTargets lt;
   size_t acc = 0;
   for (size_t i = 0; i < 4; ++i) {
-    if (i < levels.size()) {
-      f[i] = (T *)levels[i].field;
-      n[i] = levels[i].n;
-      acc += levels[i].count;
-    }
-    cum[i] = acc;
+    lt.f[i] = i < levels.size() ? levels[i].field : nullptr;
+    lt.n[i] = i < levels.size() ? levels[i].n : 0;
+    if (i < levels.size()) acc += levels[i].count;
+    lt.cum[i] = acc;
   }
+  size_t total = acc;
+  if (produced < total) throw std::runtime_error("internal error: RNG plan produced too few Gaussians");
   if (hook) hook->begin("rng_emit", st);
-  hipLaunchKernelGGL(k_rng_emit<T>, dim3((nc + 255) / 256), dim3(256), 0, st, b.raw, b.start, nc, f[0], n[0], f[1],
-                     n[1], f[2], n[2], f[3], n[3], cum[0], cum[1], cum[2], total);
+  hipLaunchKernelGGL(k_rng_emit<T>, dim3((nc + TB - 1) / TB), dim3(TB), ring_bytes, st, d, nc, b.summary, b.segstart,
+                     lt, total);
   GIC_HIP(hipGetLastError());
   if (hook) hook->end("rng_emit", st);
 }
@@ -679,11 +725,11 @@ template void rng_draw_real<float>(const RngPlan &, const std::vector<RngLevelOu
 template void rng_draw_real<double>(const RngPlan &, const std::vector<RngLevelOut> &, void *, size_t, hipStream_t,
                                     TimingHook *);
 
-void rng_raw(const RngPlan &plan, uint32_t *d_out, size_t n, void *ws, size_t ws_bytes, hipStream_t st) {
-  RngBufs b = carve(plan, ws, ws_bytes);
+void rng_raw(const RngPlan &plan, uint32_t *d_out, size_t n, void *, size_t, hipStream_t st) {
   if (n > size_t(plan.nchunks) * RngPlan::CH) throw std::runtime_error("rng_raw: plan too small");
-  generate(plan, b, st, nullptr);
-  hipLaunchKernelGGL(k_rng_raw_copy, dim3((n + 255) / 256), dim3(256), 0, st, b.raw, plan.nchunks, d_out, n);
+  RngDev d = to_dev(plan);
+  long nc = long((n + RngPlan::CH - 1) / RngPlan::CH);
+  hipLaunchKernelGGL(k_rng_raw, dim3((nc + 255) / 256), dim3(256), 0, st, d, nc, d_out, n);
   GIC_HIP(hipGetLastError());
 }
 
