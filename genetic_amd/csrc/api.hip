@@ -190,6 +190,7 @@ struct Level {
   DevBuf psi[3];
   DevBuf sqrtp;            // sqrt P by |k|^2, element type T
   bool noise_fourier = false;
+  bool dc_pending = false;   // Re(w_0) still to be zeroed after the first r2c
   bool psi_fourier = false;
   bool have_psi = false;
 };
@@ -213,7 +214,7 @@ struct gic_context {
   unsigned long seed = 0;
   int seed_mode = 0;
   bool seeded = false, drawn = false, power_applied = false, zeldovich_done = false;
-  DevBuf work, stage, tab_a, tab_b;
+  DevBuf work, stage, tab_a, tab_b, tab_c;
   std::unique_ptr<EventTimer> timer;
   TimingHook *hook() { return timer.get(); }
   size_t tsize() const { return prec == 32 ? 4 : 8; }
@@ -222,7 +223,7 @@ struct gic_context {
 namespace {
 
 size_t total_bytes(gic_context *c) {
-  size_t b = c->work.bytes + c->stage.bytes + c->tab_a.bytes + c->tab_b.bytes + c->spl_x.bytes * 3;
+  size_t b = c->work.bytes + c->stage.bytes + c->tab_a.bytes + c->tab_b.bytes + c->tab_c.bytes + c->spl_x.bytes * 3;
   for (auto &l : c->levels) {
     b += l->noise.bytes + l->sqrtp.bytes;
     for (auto &p : l->psi) b += p.bytes;
@@ -276,6 +277,23 @@ template <typename T> void build_sqrtp(gic_context *c, Level &L) {
   sqrtp_table<T>(c->spline_dev, c->amplitude, c->ns_frozen, L.g.kmin(), norm, nent, (T *)L.sqrtp.p, c->st);
 }
 
+// The white noise stays in real space after the draw; the first transform
+// to Fourier space zeroes Re(w_0) (randomfieldgenerator.hpp:194-195).  When
+// the power spectrum is applied to noise that is still real, the r2c and the
+// sqrt P multiply share one pass.
+template <typename T> void noise_to_fourier(gic_context *c, Level &L, const T *post) {
+  if (L.noise_fourier) {
+    if (post) mul_table<T>((T *)L.noise.p, L.g.n, post, c->st);
+    return;
+  }
+  fft_r2c_post<T>((T *)L.noise.p, L.g.n, post, c->st, c->hook());
+  L.noise_fourier = true;
+  if (L.dc_pending) {
+    if (!post) fill<T>((T *)L.noise.p, 1, T(0), c->st);   // with sqrt P applied the k=0 mode is already 0
+    L.dc_pending = false;
+  }
+}
+
 template <typename T> void do_draw(gic_context *c) {
   if (!c->seeded) throw std::runtime_error("The random number generator has not been seeded");
   if (c->drawn) throw std::runtime_error("Trying to re-draw the random field after it was already initialised");
@@ -294,9 +312,7 @@ template <typename T> void do_draw(gic_context *c) {
     rng_draw_real<T>(plan, outs, c->work.p, c->work.bytes, c->st, c->hook());
     for (auto &L : c->levels) {
       L->noise_fourier = false;
-      to_fourier<T>(c, L->noise, L->g.n, L->noise_fourier);
-      // tools::set_zero(fieldData[0]): the real part of the k=0 mode
-      fill<T>((T *)L->noise.p, 1, T(0), c->st);
+      L->dc_pending = true;
     }
   } else {
     throw std::runtime_error("Fourier-space seeding is not implemented on the GPU path yet");
@@ -304,14 +320,15 @@ template <typename T> void do_draw(gic_context *c) {
   c->drawn = true;
 }
 
+template <typename T> void ensure_noise_fourier(gic_context *c, Level &L) { noise_to_fourier<T>(c, L, nullptr); }
+
 template <typename T> void do_apply_ps(gic_context *c) {
   if (!c->drawn) do_draw<T>(c);
   for (auto &Lp : c->levels) {
     Level &L = *Lp;
     build_sqrtp<T>(c, L);
-    to_fourier<T>(c, L.noise, L.g.n, L.noise_fourier);
     if (c->hook()) c->hook()->begin("apply_sqrtp", c->st);
-    mul_table<T>((T *)L.noise.p, L.g.n, (const T *)L.sqrtp.p, c->st);
+    noise_to_fourier<T>(c, L, (const T *)L.sqrtp.p);
     if (c->hook()) c->hook()->end("apply_sqrtp", c->st);
   }
   c->power_applied = true;
@@ -343,86 +360,120 @@ InterpGeom interp_geom(const LevelGeom &gf, const LevelGeom &gc) {
   return ig;
 }
 
-// temp <- LP-filtered real-space copy of a coarse field.  `src` is either a
-// spectrum (fourier=true) or a real field, as in
-// addFieldFromDifferentGridWithFilter's FILTER_ON_COARSE_GRID branch
-// (field.hpp:816-830): copy, filter in Fourier space, transform to real.
-template <typename T>
-void lowpass_copy(gic_context *c, const DevBuf &src, bool fourier, const LevelGeom &g, const T *lp) {
+// Low-passed real-space copy of a coarse level for the interpolation onto the
+// next level (addFieldFromDifferentGridWithFilter, FILTER_ON_COARSE_GRID
+// branch, field.hpp:816-830): copy the field, filter in Fourier space,
+// transform back.  For a zoom level the coarse fields are combined real
+// fields; the copy is transformed forward with the filter fused into its
+// last pass.
+template <typename T> CoarseView lowpass_full(gic_context *c, const DevBuf &src, const LevelGeom &g, const T *lp) {
   size_t bytes = g.padded_reals() * sizeof(T);
   c->stage.ensure(bytes);
   GIC_HIP(hipMemcpyAsync(c->stage.p, src.p, bytes, hipMemcpyDeviceToDevice, c->st));
-  bool f = fourier;
-  to_fourier<T>(c, c->stage, g.n, f);
-  mul_table<T>((T *)c->stage.p, g.n, lp, c->st);
-  to_real<T>(c, c->stage, g.n, f);
+  fft_r2c_post<T>((T *)c->stage.p, g.n, lp, c->st, c->hook());
+  fft3d<T>((T *)c->stage.p, g.n, +1, c->st, c->hook());
+  CoarseView v;
+  v.p = c->stage.p;
+  for (int a = 0; a < 3; ++a) {
+    v.org[a] = 0;
+    v.len[a] = g.n;
+  }
+  v.sx = long(g.n) * (g.n + 2);
+  v.sy = g.n + 2;
+  return v;
+}
+
+// Coarse cells the tricubic stencils of a fine level touch: the fine grid
+// spans coarse cells [o, o + nf/r); the stencil reaches 2 cells below and
+// 2 above, plus one cell of slack for rounding in floor().
+void coarse_window(const LevelGeom &gf, const LevelGeom &gc, int *w0, int *wl) {
+  int r = (int)llround(gc.dx / gf.dx);
+  double off[3] = {gf.ox - gc.ox, gf.oy - gc.oy, gf.oz - gc.oz};
+  for (int a = 0; a < 3; ++a) {
+    int o = (int)llround(off[a] / gc.dx);
+    int start = o - 3;
+    int len = gf.n / r + 6;
+    start %= gc.n;
+    if (start < 0) start += gc.n;
+    w0[a] = start;
+    wl[a] = len;
+  }
 }
 
 template <typename T> void do_zeldovich(gic_context *c) {
   if (!c->power_applied) do_apply_ps<T>(c);
   size_t nl = c->levels.size();
-  // ZA per level; zoom levels get their high-pass filter (cut of the level
-  // above) on delta and psi while still in Fourier space.
+  // Zel'dovich per level, formed inside the first inverse pass from the
+  // overdensity spectrum; zoom levels carry their high-pass filter (the cut
+  // of the level above) on delta and psi.
   for (size_t l = 0; l < nl; ++l) {
     Level &L = *c->levels[l];
-    to_fourier<T>(c, L.noise, L.g.n, L.noise_fourier);
+    ensure_noise_fourier<T>(c, L);
     for (auto &p : L.psi) alloc_field(c, p, L.g);
-    const T *hp = nullptr;
+    long h = L.g.n / 2, nent = 3 * h * h + 1;
+    c->tab_a.ensure(nent * sizeof(T));
+    c->tab_c.ensure(nent * sizeof(T));
+    double kc = l >= 1 ? kcut_for_level(c, l - 1) : 0.0;
+    filter_table<T>(l >= 1 ? 1 : 2, kc, L.g.kmin(), nent, (T *)c->tab_c.p, c->st, true);
+    for (int a = 0; a < 3; ++a)
+      fft_c2r_from<T>((const T *)L.noise.p, (T *)L.psi[a].p, L.g.n, FFT_PRE_ZA, (const T *)c->tab_c.p, a,
+                      L.g.kmin(), c->st, c->hook());
     if (l >= 1) {
-      long h = L.g.n / 2, nent = 3 * h * h + 1;
-      c->tab_a.ensure(nent * sizeof(T));
-      filter_table<T>(1, kcut_for_level(c, l - 1), L.g.kmin(), nent, (T *)c->tab_a.p, c->st);
-      hp = (const T *)c->tab_a.p;
+      filter_table<T>(1, kc, L.g.kmin(), nent, (T *)c->tab_a.p, c->st);
+      mul_table<T>((T *)L.noise.p, L.g.n, (const T *)c->tab_a.p, c->st);
+      to_real<T>(c, L.noise, L.g.n, L.noise_fourier);
     }
-    if (c->hook()) c->hook()->begin("zeldovich", c->st);
-    zeldovich<T>((const T *)L.noise.p, L.g.n, L.g.kmin(), (T *)L.psi[0].p, (T *)L.psi[1].p, (T *)L.psi[2].p, hp,
-                 c->st);
-    if (c->hook()) c->hook()->end("zeldovich", c->st);
-    if (hp) mul_table<T>((T *)L.noise.p, L.g.n, hp, c->st);
-    L.psi_fourier = true;
+    L.psi_fourier = false;
     L.have_psi = true;
   }
-  // combine: fine <- HP(fine) + interpolate(LP(coarse)), coarse already combined
+  // fine <- HP(fine) + interpolate(LP(coarse)), coarse already combined
   for (size_t l = 1; l < nl; ++l) {
     Level &F = *c->levels[l];
     Level &C = *c->levels[l - 1];
     long h = C.g.n / 2, nent = 3 * h * h + 1;
+    double kc = kcut_for_level(c, l - 1);
     c->tab_b.ensure(nent * sizeof(T));
-    filter_table<T>(0, kcut_for_level(c, l - 1), C.g.kmin(), nent, (T *)c->tab_b.p, c->st);
+    c->tab_c.ensure(nent * sizeof(T));
+    filter_table<T>(0, kc, C.g.kmin(), nent, (T *)c->tab_b.p, c->st);
+    filter_table<T>(0, kc, C.g.kmin(), nent, (T *)c->tab_c.p, c->st, true);
     const T *lp = (const T *)c->tab_b.p;
+    const T *lp_k2 = (const T *)c->tab_c.p;
     InterpGeom ig = interp_geom(F.g, C.g);
-    // overdensity
-    lowpass_copy<T>(c, C.noise, C.noise_fourier, C.g, lp);
-    to_real<T>(c, F.noise, F.g.n, F.noise_fourier);
-    if (c->hook()) c->hook()->begin("interp_add", c->st);
-    interp_add<T>((T *)F.noise.p, F.g.n, (const T *)c->stage.p, ig, c->st);
-    if (c->hook()) c->hook()->end("interp_add", c->st);
-    for (int a = 0; a < 3; ++a) {
-      lowpass_copy<T>(c, C.psi[a], C.psi_fourier, C.g, lp);
-      bool f = F.psi_fourier;
-      to_real<T>(c, F.psi[a], F.g.n, f);
+    int w0[3], wl[3];
+    coarse_window(F.g, C.g, w0, wl);
+    bool pruned = l == 1 && C.noise_fourier && C.g.covers_full() && wl[0] < C.g.n && wl[1] < C.g.n && wl[2] < C.g.n;
+    DevBuf *targets[4] = {&F.noise, &F.psi[0], &F.psi[1], &F.psi[2]};
+    for (int f = 0; f < 4; ++f) {
+      CoarseView v;
+      if (pruned) {
+        // the base level's delta spectrum is intact: low-pass it (and form
+        // psi from it) straight into the coarse window only
+        int nz = C.g.n / 2 + 1;
+        size_t a_bytes = size_t(wl[0]) * C.g.n * nz * 2 * sizeof(T);
+        size_t b_bytes = size_t(wl[0]) * wl[1] * nz * 2 * sizeof(T);
+        size_t o_bytes = size_t(wl[0]) * wl[1] * wl[2] * sizeof(T);
+        c->stage.ensure(a_bytes + b_bytes + o_bytes);
+        T *bufA = (T *)c->stage.p;
+        T *bufB = (T *)((char *)c->stage.p + a_bytes);
+        T *out = (T *)((char *)c->stage.p + a_bytes + b_bytes);
+        fft_c2r_window<T>((const T *)C.noise.p, C.g.n, f == 0 ? FFT_PRE_TABLE : FFT_PRE_ZA, f == 0 ? lp : lp_k2,
+                          f - 1, C.g.kmin(), w0, wl, bufA, bufB, out, c->st, c->hook());
+        v.p = out;
+        for (int a = 0; a < 3; ++a) {
+          v.org[a] = w0[a];
+          v.len[a] = wl[a];
+        }
+        v.sx = long(wl[1]) * wl[2];
+        v.sy = wl[2];
+      } else {
+        const DevBuf &src = f == 0 ? C.noise : C.psi[f - 1];
+        if (f == 0) to_real<T>(c, C.noise, C.g.n, C.noise_fourier);
+        v = lowpass_full<T>(c, src, C.g, lp);
+      }
+      if (f == 0) to_real<T>(c, F.noise, F.g.n, F.noise_fourier);
       if (c->hook()) c->hook()->begin("interp_add", c->st);
-      interp_add<T>((T *)F.psi[a].p, F.g.n, (const T *)c->stage.p, ig, c->st);
+      interp_add<T>((T *)targets[f]->p, F.g.n, v, ig, c->st);
       if (c->hook()) c->hook()->end("interp_add", c->st);
-    }
-    F.psi_fourier = false;
-    // the coarse psi, if still spectra, are no longer needed in Fourier space
-    if (C.psi_fourier) {
-      for (int a = 0; a < 3; ++a) {
-        bool f = true;
-        to_real<T>(c, C.psi[a], C.g.n, f);
-      }
-      C.psi_fourier = false;
-    }
-  }
-  for (size_t l = 0; l < nl; ++l) {
-    Level &L = *c->levels[l];
-    if (L.psi_fourier) {
-      for (int a = 0; a < 3; ++a) {
-        bool f = true;
-        to_real<T>(c, L.psi[a], L.g.n, f);
-      }
-      L.psi_fourier = false;
     }
   }
   c->zeldovich_done = true;
@@ -445,7 +496,10 @@ template <typename T> void do_copy_field(gic_context *c, int field, int level, i
   }
   int n = L.g.n;
   if (space == GIC_REAL_SPACE) {
-    if (field == GIC_FIELD_OVERDENSITY) to_real<T>(c, *b, n, *flag);
+    if (field == GIC_FIELD_OVERDENSITY) {
+      if (L.dc_pending) ensure_noise_fourier<T>(c, L);
+      to_real<T>(c, *b, n, *flag);
+    }
     else if (*flag) {
       for (int a = 0; a < 3; ++a) {
         bool f = true;
@@ -460,7 +514,7 @@ template <typename T> void do_copy_field(gic_context *c, int field, int level, i
     else unpad<T, double>((const T *)b->p, n, (double *)c->stage.p, c->st);
     GIC_HIP(hipMemcpyAsync(host, c->stage.p, ob, hipMemcpyDeviceToHost, c->st));
   } else {
-    if (field == GIC_FIELD_OVERDENSITY) to_fourier<T>(c, *b, n, *flag);
+    if (field == GIC_FIELD_OVERDENSITY) ensure_noise_fourier<T>(c, L);
     else if (!*flag) {
       for (int a = 0; a < 3; ++a) {
         bool f = false;
@@ -491,6 +545,7 @@ template <typename T> void do_set_field(gic_context *c, int field, int level, in
     if (dtype == GIC_DTYPE_F32) pad<float, T>((const float *)c->stage.p, n, (T *)L.noise.p, c->st);
     else pad<double, T>((const double *)c->stage.p, n, (T *)L.noise.p, c->st);
     L.noise_fourier = false;
+    L.dc_pending = false;
   } else {
     size_t nreal = L.g.padded_reals();
     c->stage.ensure(nreal * es);
@@ -498,6 +553,7 @@ template <typename T> void do_set_field(gic_context *c, int field, int level, in
     if (dtype == GIC_DTYPE_F32) convert<float, T>((const float *)c->stage.p, nreal, (T *)L.noise.p, c->st);
     else convert<double, T>((const double *)c->stage.p, nreal, (T *)L.noise.p, c->st);
     L.noise_fourier = true;
+    L.dc_pending = false;
   }
   GIC_HIP(hipStreamSynchronize(c->st));
 }

@@ -18,4 +18,23 @@ struct TimingHook {
 // FieldFourierManager::performTransform does.
 template <typename T> void fft3d(T *d, int n, int dir, hipStream_t st, TimingHook *hook);
 
+// k-space operators applied inside the x pass
+enum { FFT_PRE_NONE = 0, FFT_PRE_TABLE = 1, FFT_PRE_ZA = 2 };
+
+// r2c whose last pass multiplies each mode by post[kx^2+ky^2+kz^2] (if non-null).
+template <typename T> void fft_r2c_post(T *d, int n, const T *post, hipStream_t st, TimingHook *hook);
+// out <- c2r(op(spec)), out-of-place from a spectrum that is left untouched.
+// op: FFT_PRE_TABLE multiplies by tab[|k|^2]; FFT_PRE_ZA forms the Zel'dovich
+// component comp, i k_c tab[|k|^2], with tab = filter(k)/k^2 (0 at k=0) and
+// zero at k_c = +Nyquist.
+template <typename T>
+void fft_c2r_from(const T *spec, T *out, int n, int pre, const T *tab, int comp, double kmin, hipStream_t st,
+                  TimingHook *hook);
+// Only the real-space window [w0, w0+wl) (per axis, wrapping mod n) of
+// c2r(op(spec)), via pruned passes: bufA holds wl[0]*n*(n/2+1) complex,
+// bufB wl[0]*wl[1]*(n/2+1) complex, out wl[0]*wl[1]*wl[2] reals.
+template <typename T>
+void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, double kmin, const int *w0, const int *wl,
+                    T *bufA, T *bufB, T *out, hipStream_t st, TimingHook *hook);
+
 }  // namespace gic

@@ -21,6 +21,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 
 namespace gic {
 
@@ -98,8 +99,13 @@ __device__ __forceinline__ int xcd_tile(int b, int nblk) {
 
 // One Stockham stage.  Butterfly q is mapped to (line, i): LINE_FAST puts
 // adjacent q on adjacent lines (strided passes), otherwise on adjacent points.
-template <typename T, int L, int NL, int THREADS, int S, int DIR, bool LINE_FAST, class Ld, class St>
-__device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, Ld ld, St st, bool sync_between) {
+struct NoOp {
+  template <typename C> __device__ __forceinline__ C operator()(int, int, C v) const { return v; }
+};
+
+template <typename T, int L, int NL, int THREADS, int S, int DIR, bool LINE_FAST, class Ld, class St, class Op = NoOp>
+__device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, Ld ld, St st, bool sync_between,
+                                      Op op = Op()) {
   using P = RadixPlan<L>;
   constexpr int R = P::radix(S);
   constexpr int p = P::pbefore(S);
@@ -115,6 +121,19 @@ __device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, L
       int i = LINE_FAST ? q / NL : q % (L / R);
 #pragma unroll
       for (int r = 0; r < R; ++r) v[b][r] = ld(line, i + r * (L / R));
+    }
+  }
+  // operators on the loaded values run after every load has been issued
+  if constexpr (!std::is_same<Op, NoOp>::value) {
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+      int q = tid + b * THREADS;
+      if (NB % THREADS == 0 || q < NB) {
+        int line = LINE_FAST ? q % NL : q / (L / R);
+        int i = LINE_FAST ? q / NL : q % (L / R);
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = op(line, i + r * (L / R), v[b][r]);
+      }
     }
   }
   if (sync_between) __syncthreads();
@@ -146,16 +165,17 @@ __device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, L
 // whether gld / gst themselves touch LDS, in which case the stage must put a
 // barrier between its loads and its stores.
 template <typename T, int L, int NL, int THREADS, int DIR, bool LINE_FAST, bool GLD_LDS, bool GST_LDS, int S,
-          class GLd, class GSt, class LIdx>
-__device__ __forceinline__ void run_stages(cx<T> *lds, const cx<T> *tw, int twmul, GLd gld, GSt gst, LIdx lidx) {
+          class GLd, class GSt, class LIdx, class Op = NoOp>
+__device__ __forceinline__ void run_stages(cx<T> *lds, const cx<T> *tw, int twmul, GLd gld, GSt gst, LIdx lidx,
+                                           Op op = Op()) {
   using P = RadixPlan<L>;
   constexpr int NS = P::nstages;
   auto lds_ld = [&](int line, int pt) { return lds[lidx(line, pt)]; };
   auto lds_st = [&](int line, int pt, cx<T> v) { lds[lidx(line, pt)] = v; };
   if constexpr (NS == 1) {
-    stage<T, L, NL, THREADS, 0, DIR, LINE_FAST>(tw, twmul, gld, gst, GLD_LDS && GST_LDS);
+    stage<T, L, NL, THREADS, 0, DIR, LINE_FAST>(tw, twmul, gld, gst, GLD_LDS && GST_LDS, op);
   } else if constexpr (S == 0) {
-    stage<T, L, NL, THREADS, 0, DIR, LINE_FAST>(tw, twmul, gld, lds_st, GLD_LDS);
+    stage<T, L, NL, THREADS, 0, DIR, LINE_FAST>(tw, twmul, gld, lds_st, GLD_LDS, op);
     __syncthreads();
     run_stages<T, L, NL, THREADS, DIR, LINE_FAST, GLD_LDS, GST_LDS, 1>(lds, tw, twmul, gld, gst, lidx);
   } else if constexpr (S == NS - 1) {
@@ -168,27 +188,84 @@ __device__ __forceinline__ void run_stages(cx<T> *lds, const cx<T> *tw, int twmu
 }
 
 // ---- strided c2c pass (y or x) ------------------------------------------------
-// Lines of length L with element stride `ls` (complex units); a tile holds C
-// adjacent columns.  Tile t: outer = t / nkzb, column block = t % nkzb.
-template <typename T, int L, int C, int DIR, int THREADS>
-__global__ __launch_bounds__(THREADS) void k_fft_strided(cx<T> *__restrict__ data, long ls, long os, int ncol, int nkzb,
-                                                         T scale, const cx<T> *__restrict__ tw) {
+// Lines of length L along one axis; a tile holds C adjacent kz columns of one
+// `outer` index.  Input and output may differ (out-of-place) and the output
+// may keep only a window of the line (pruned inverse transforms, where the
+// line's output is a spatial coordinate).  Optional k-space operators:
+//   PRE  (inverse x pass, input indexed by kx=pt, ky=outer, kz=col):
+//        PRE_TABLE: v *= tab[kx^2+ky^2+kz^2]
+//        PRE_ZA:    v <- i k_c v / k^2 (zero at k_c = +Nyquist and k = 0), times tab if given
+//   POST (forward x pass, output indexed by kx=pt, ky=outer, kz=col):
+//        POST_TABLE: v *= tab[kx^2+ky^2+kz^2]
+enum { PRE_NONE = FFT_PRE_NONE, PRE_TABLE = FFT_PRE_TABLE, PRE_ZA = FFT_PRE_ZA };
+enum { POST_NONE = 0, POST_TABLE = 1 };
+
+template <typename T> struct StridedArgs {
+  const cx<T> *in;
+  cx<T> *out;
+  long in_ls, in_os, out_ls, out_os;
+  int ncol, nkzb;
+  int n;            // full grid size, for signed wavenumbers
+  const T *tab;     // |k|^2-indexed factor table (or nullptr)
+  int comp;         // Zel'dovich component 0,1,2
+  T kmin;
+  int win0, wlen;   // output keeps line points (pt - win0) mod n < wlen
+  T scale;
+};
+
+__device__ __forceinline__ int signed_k(int i, int n) { return i > n / 2 ? i - n : i; }
+
+// PRE_ZA multiplies by i * k_c * tab[|k|^2], where the caller's table holds
+// filter(k) / k^2 (0 at k = 0); the component at k_c = +Nyquist is zero.
+template <typename T, int L, int C, int DIR, int THREADS, int PRE, int POST>
+__global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const cx<T> *__restrict__ tw) {
   extern __shared__ char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   int tile = xcd_tile(blockIdx.x, gridDim.x);
-  int outer = tile / nkzb, kzb = tile - outer * nkzb;
+  int outer = tile / a.nkzb, kzb = tile - outer * a.nkzb;
   int col0 = kzb * C;
-  int nc = min(C, ncol - col0);
-  cx<T> *base = data + outer * os + col0;
+  int nc = min(C, a.ncol - col0);
+  const cx<T> *ib = a.in + outer * a.in_os + col0;
+  cx<T> *ob = a.out + outer * a.out_os + col0;
+  int ky = signed_k(outer, a.n);
+  int ky2 = ky * ky;
   auto gld = [&](int line, int pt) -> cx<T> {
-    if (line < nc) return base[pt * ls + line];
-    return cx<T>{0, 0};
+    if (line >= nc) return cx<T>{0, 0};
+    return ib[pt * a.in_ls + line];
+  };
+  auto op = [&](int line, int pt, cx<T> v) -> cx<T> {
+    int kx = signed_k(pt, a.n), kz = col0 + line;
+    T f = a.tab[kx * kx + ky2 + kz * kz];
+    if constexpr (PRE == PRE_ZA) {
+      int kc = a.comp == 0 ? kx : a.comp == 1 ? ky : kz;
+      if (kc == a.n / 2) return cx<T>{0, 0};
+      f *= T(kc) * a.kmin;
+      return cx<T>{-v.y * f, v.x * f};
+    } else {
+      return cx<T>{v.x * f, v.y * f};
+    }
   };
   auto gst = [&](int line, int pt, cx<T> v) {
-    if (line < nc) base[pt * ls + line] = cx<T>{v.x * scale, v.y * scale};
+    if (line >= nc) return;
+    int o = pt;
+    if (a.wlen < L) {
+      o = pt - a.win0;
+      if (o < 0) o += L;
+      if (o >= a.wlen) return;
+    }
+    T s = a.scale;
+    if constexpr (POST == POST_TABLE) {
+      int kx = signed_k(pt, a.n), kz = col0 + line;
+      s *= a.tab[kx * kx + ky2 + kz * kz];
+    }
+    ob[o * a.out_ls + line] = cx<T>{v.x * s, v.y * s};
   };
   auto lidx = [](int line, int pt) { return pt * C + line; };
-  run_stages<T, L, C, THREADS, DIR, true, false, false, 0>(lds, tw, 1, gld, gst, lidx);
+  if constexpr (PRE == PRE_NONE) {
+    run_stages<T, L, C, THREADS, DIR, true, false, false, 0>(lds, tw, 1, gld, gst, lidx);
+  } else {
+    run_stages<T, L, C, THREADS, DIR, true, false, false, 0>(lds, tw, 1, gld, gst, lidx, op);
+  }
 }
 
 // ---- z pass, r2c --------------------------------------------------------------
@@ -274,6 +351,63 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_c2r(T *__restrict__ data, lon
   }
 }
 
+// ---- z pass, c2r into a window ------------------------------------------------
+// Same transform as k_fft_z_c2r, reading complex rows from `in` (row stride
+// M+1 complex) and writing only z in [z0, z0+wz) (mod N) of each real row to
+// `out` (row stride wz).
+template <typename T, int N, int ROWS, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_fft_z_c2r_window(const T *__restrict__ in, T *__restrict__ out,
+                                                              long nrows, int z0, int wz, T scale,
+                                                              const cx<T> *__restrict__ twN) {
+  constexpr int M = N / 2;
+  extern __shared__ char smem_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  long row0 = long(blockIdx.x) * ROWS;
+  const cx<T> *rows = reinterpret_cast<const cx<T> *>(in) + row0 * (M + 1);
+  auto lidx = [](int line, int pt) { return line * (M + 1) + pt; };
+  constexpr int H = M / 2 + 1;
+  for (int q = threadIdx.x; q < ROWS * H; q += THREADS) {
+    int line = q / H, k = q - line * H;
+    if (row0 + line >= nrows) continue;
+    const cx<T> *rin = rows + long(line) * (M + 1);
+    cx<T> xk = rin[k], xm = rin[M - k];
+    if (k == 0) {
+      xk.y = 0;
+      xm.y = 0;
+    }
+    cx<T> E{xk.x + xm.x, xk.y - xm.y};
+    cx<T> D{xk.x - xm.x, xk.y + xm.y};
+    cx<T> w = twN[k];
+    w.y = -w.y;
+    cx<T> O = cmul(D, w);
+    cx<T> Zk{E.x - O.y, E.y + O.x};
+    cx<T> Zm{E.x + O.y, -E.y + O.x};
+    if (k == 0) {
+      lds[lidx(line, 0)] = Zk;
+    } else {
+      lds[lidx(line, k)] = Zk;
+      if (M - k != k) lds[lidx(line, M - k)] = Zm;
+    }
+  }
+  __syncthreads();
+  auto lld = [&](int line, int pt) -> cx<T> { return lds[lidx(line, pt)]; };
+  auto gst = [&](int line, int pt, cx<T> v) {
+    if (row0 + line >= nrows) return;
+    T *orow = out + (row0 + line) * wz;
+    // complex point pt holds real samples z = 2pt, 2pt+1
+    for (int h = 0; h < 2; ++h) {
+      int z = 2 * pt + h - z0;
+      if (z < 0) z += N;
+      if (z < wz) orow[z] = (h ? v.y : v.x) * scale;
+    }
+  };
+  if constexpr (M == 1) {
+    if (threadIdx.x < ROWS) gst(threadIdx.x, 0, lld(threadIdx.x, 0));
+  } else {
+    run_stages<T, M, ROWS, THREADS, +1, false, true, false, 0>(lds, twN, 2, lld, gst, lidx);
+  }
+}
+
 // ---- Hermitian mirror of the kz=0 and kz=N/2 planes ------------------------------
 // ensureFourierModesAreMirrored (fourier.hpp:331-357): for kx in [0,N/2] and
 // ky from N/2 down to -N/2+1 the conjugate of mode (kx,ky) is copied into
@@ -334,23 +468,41 @@ template <typename T, int N> struct FftN {
   static constexpr int THREADS = 256;
   static constexpr int C = strided_cols<T>();
   static constexpr int M = N / 2;
+  static constexpr int NZ = N / 2 + 1;
   static constexpr int ROWS0 = (8192 / N) < 1 ? 1 : (8192 / N);
   static constexpr int ROWS = ROWS0 > N * N ? N * N : ROWS0;
 
-  static void strided(cx<T> *d, long ls, long os, int nouter, T scale, int dir, hipStream_t st, TimingHook *hook,
-                      const char *name) {
-    int ncol = N / 2 + 1;
-    int nkzb = (ncol + C - 1) / C;
-    int nblk = nouter * nkzb;
+  template <int DIR, int PRE, int POST>
+  static void strided(const StridedArgs<T> &a0, int nouter, hipStream_t st, TimingHook *hook, const char *name) {
+    StridedArgs<T> a = a0;
+    a.ncol = NZ;
+    a.nkzb = (NZ + C - 1) / C;
+    a.n = N;
+    int nblk = nouter * a.nkzb;
     size_t shm = sizeof(cx<T>) * C * N;
     const cx<T> *tw = twtable<T>().get(N);
-    auto kern = dir < 0 ? k_fft_strided<T, N, C, -1, THREADS> : k_fft_strided<T, N, C, +1, THREADS>;
+    auto kern = k_fft_strided<T, N, C, DIR, THREADS, PRE, POST>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     std::string tag = std::string(name) + "." + std::to_string(N);
     if (hook) hook->begin(tag.c_str(), st);
-    hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, d, ls, os, ncol, nkzb, scale, tw);
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, a, tw);
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);
+  }
+  // in-place plain pass over a full field: y (line stride NZ) or x (line stride N*NZ)
+  static StridedArgs<T> plain(cx<T> *d, bool xpass, T scale) {
+    StridedArgs<T> a{};
+    a.in = d;
+    a.out = d;
+    a.in_ls = a.out_ls = xpass ? long(N) * NZ : NZ;
+    a.in_os = a.out_os = xpass ? NZ : long(N) * NZ;
+    a.tab = nullptr;
+    a.comp = 0;
+    a.kmin = 0;
+    a.win0 = 0;
+    a.wlen = N;
+    a.scale = scale;
+    return a;
   }
   static void zpass(T *d, T scale, int dir, hipStream_t st, TimingHook *hook, const char *name) {
     long nrows = long(N) * N;
@@ -365,18 +517,8 @@ template <typename T, int N> struct FftN {
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);
   }
-  static void r2c(T *d, hipStream_t st, TimingHook *hook) {
-    cx<T> *c = reinterpret_cast<cx<T> *>(d);
-    long nz = N / 2 + 1;
-    T scale = T(1.0 / pow(double(N), 1.5));
-    zpass(d, T(1), -1, st, hook, "fft_z_r2c");
-    strided(c, nz, long(N) * nz, N, T(1), -1, st, hook, "fft_y");
-    strided(c, long(N) * nz, nz, N, scale, -1, st, hook, "fft_x");
-  }
-  static void c2r(T *d, hipStream_t st, TimingHook *hook) {
-    cx<T> *c = reinterpret_cast<cx<T> *>(d);
-    long nz = N / 2 + 1;
-    T scale = T(1.0 / pow(double(N), 1.5));
+  static T norm() { return T(1.0 / pow(double(N), 1.5)); }
+  static void mirror(cx<T> *c, hipStream_t st, TimingHook *hook) {
     int nt = 256;
     long nn = long(N) * N;
     std::string tag = "fft_mirror." + std::to_string(N);
@@ -384,33 +526,140 @@ template <typename T, int N> struct FftN {
     hipLaunchKernelGGL(k_mirror_planes<T>, dim3((nn + nt - 1) / nt), dim3(nt), 0, st, c, N);
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);
-    strided(c, long(N) * nz, nz, N, T(1), +1, st, hook, "fft_x");
-    strided(c, nz, long(N) * nz, N, T(1), +1, st, hook, "fft_y");
-    zpass(d, scale, +1, st, hook, "fft_z_c2r");
+  }
+  static void r2c(T *d, const T *post, hipStream_t st, TimingHook *hook) {
+    cx<T> *c = reinterpret_cast<cx<T> *>(d);
+    zpass(d, T(1), -1, st, hook, "fft_z_r2c");
+    strided<-1, PRE_NONE, POST_NONE>(plain(c, false, T(1)), N, st, hook, "fft_y");
+    StridedArgs<T> ax = plain(c, true, norm());
+    if (post) {
+      ax.tab = post;
+      strided<-1, PRE_NONE, POST_TABLE>(ax, N, st, hook, "fft_x_post");
+    } else {
+      strided<-1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x");
+    }
+  }
+  static void c2r(T *d, hipStream_t st, TimingHook *hook) {
+    cx<T> *c = reinterpret_cast<cx<T> *>(d);
+    mirror(c, st, hook);
+    strided<+1, PRE_NONE, POST_NONE>(plain(c, true, T(1)), N, st, hook, "fft_x");
+    strided<+1, PRE_NONE, POST_NONE>(plain(c, false, T(1)), N, st, hook, "fft_y");
+    zpass(d, norm(), +1, st, hook, "fft_z_c2r");
+  }
+  // out <- c2r(op(spec)) with op applied while the x pass reads the spectrum
+  static void c2r_from(const T *spec, T *out, int pre, const T *tab, int comp, T kmin, hipStream_t st,
+                       TimingHook *hook) {
+    const cx<T> *ci = reinterpret_cast<const cx<T> *>(spec);
+    cx<T> *co = reinterpret_cast<cx<T> *>(out);
+    StridedArgs<T> ax = plain(co, true, T(1));
+    ax.in = ci;
+    ax.tab = tab;
+    ax.comp = comp;
+    ax.kmin = kmin;
+    if (pre == PRE_ZA) strided<+1, PRE_ZA, POST_NONE>(ax, N, st, hook, "fft_x_za");
+    else if (pre == PRE_TABLE) strided<+1, PRE_TABLE, POST_NONE>(ax, N, st, hook, "fft_x_pre");
+    else strided<+1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x");
+    strided<+1, PRE_NONE, POST_NONE>(plain(co, false, T(1)), N, st, hook, "fft_y");
+    zpass(out, norm(), +1, st, hook, "fft_z_c2r");
+  }
+  // real-space window [w0, w0+wl) (per axis, mod N) of c2r(op(spec)):
+  // bufA holds wl[0] x-planes after the x pass, bufB wl[0]*wl[1] rows after
+  // the y pass; out receives wl[0]*wl[1]*wl[2] reals.
+  static void c2r_window(const T *spec, int pre, const T *tab, int comp, T kmin, const int *w0, const int *wl, T *bufA,
+                         T *bufB, T *out, hipStream_t st, TimingHook *hook) {
+    const cx<T> *ci = reinterpret_cast<const cx<T> *>(spec);
+    cx<T> *ca = reinterpret_cast<cx<T> *>(bufA);
+    cx<T> *cb = reinterpret_cast<cx<T> *>(bufB);
+    // x pass: tiles (ky, kz block), lines over kx, keep x in window; A = [wl0][N][NZ]
+    StridedArgs<T> ax{};
+    ax.in = ci;
+    ax.out = ca;
+    ax.in_ls = long(N) * NZ;
+    ax.in_os = NZ;
+    ax.out_ls = long(N) * NZ;
+    ax.out_os = NZ;
+    ax.tab = tab;
+    ax.comp = comp;
+    ax.kmin = kmin;
+    ax.win0 = w0[0];
+    ax.wlen = wl[0];
+    ax.scale = T(1);
+    if (pre == PRE_ZA) strided<+1, PRE_ZA, POST_NONE>(ax, N, st, hook, "fft_x_za_win");
+    else if (pre == PRE_TABLE) strided<+1, PRE_TABLE, POST_NONE>(ax, N, st, hook, "fft_x_pre_win");
+    else strided<+1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x_win");
+    // y pass: tiles (x_local, kz block), lines over ky, keep y in window; B = [wl0][wl1][NZ]
+    StridedArgs<T> ay{};
+    ay.in = ca;
+    ay.out = cb;
+    ay.in_ls = NZ;
+    ay.in_os = long(N) * NZ;
+    ay.out_ls = NZ;
+    ay.out_os = long(wl[1]) * NZ;
+    ay.tab = nullptr;
+    ay.win0 = w0[1];
+    ay.wlen = wl[1];
+    ay.scale = T(1);
+    strided<+1, PRE_NONE, POST_NONE>(ay, wl[0], st, hook, "fft_y_win");
+    // z pass: rows (x_local, y_local), keep z in window
+    long nrows = long(wl[0]) * wl[1];
+    int nblk = int((nrows + ROWS - 1) / ROWS);
+    size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
+    const cx<T> *tw = twtable<T>().get(N);
+    auto kern = k_fft_z_c2r_window<T, N, ROWS, THREADS>;
+    if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    std::string tag = "fft_z_c2r_win." + std::to_string(N);
+    if (hook) hook->begin(tag.c_str(), st);
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, bufB, out, nrows, w0[2], wl[2], norm(), tw);
+    GIC_HIP(hipGetLastError());
+    if (hook) hook->end(tag.c_str(), st);
   }
 };
 
-template <typename T> void fft3d(T *d, int n, int dir, hipStream_t st, TimingHook *hook) {
-#define GIC_FFT_CASE(NN)                                   \
-  case NN:                                                 \
-    if (dir < 0) FftN<T, NN>::r2c(d, st, hook);            \
-    else FftN<T, NN>::c2r(d, st, hook);                    \
-    return;
-  switch (n) {
-    GIC_FFT_CASE(4)
-    GIC_FFT_CASE(8)
-    GIC_FFT_CASE(16)
-    GIC_FFT_CASE(32)
-    GIC_FFT_CASE(64)
-    GIC_FFT_CASE(128)
-    GIC_FFT_CASE(256)
-    GIC_FFT_CASE(512)
-    GIC_FFT_CASE(1024)
-    default:
-      throw std::runtime_error("FFT size " + std::to_string(n) + " is not supported (powers of two 4..1024)");
+#define GIC_FFT_SWITCH(n, CALL)                                                                     \
+  switch (n) {                                                                                      \
+    case 4: { using F = FftN<T, 4>; CALL; } break;                                                  \
+    case 8: { using F = FftN<T, 8>; CALL; } break;                                                  \
+    case 16: { using F = FftN<T, 16>; CALL; } break;                                                \
+    case 32: { using F = FftN<T, 32>; CALL; } break;                                                \
+    case 64: { using F = FftN<T, 64>; CALL; } break;                                                \
+    case 128: { using F = FftN<T, 128>; CALL; } break;                                              \
+    case 256: { using F = FftN<T, 256>; CALL; } break;                                              \
+    case 512: { using F = FftN<T, 512>; CALL; } break;                                              \
+    case 1024: { using F = FftN<T, 1024>; CALL; } break;                                            \
+    default:                                                                                        \
+      throw std::runtime_error("FFT size " + std::to_string(n) + " is not supported (powers of two 4..1024)"); \
   }
-#undef GIC_FFT_CASE
+
+template <typename T> void fft3d(T *d, int n, int dir, hipStream_t st, TimingHook *hook) {
+  if (dir < 0) {
+    GIC_FFT_SWITCH(n, F::r2c(d, nullptr, st, hook))
+  } else {
+    GIC_FFT_SWITCH(n, F::c2r(d, st, hook))
+  }
 }
+template <typename T> void fft_r2c_post(T *d, int n, const T *post, hipStream_t st, TimingHook *hook) {
+  GIC_FFT_SWITCH(n, F::r2c(d, post, st, hook))
+}
+template <typename T>
+void fft_c2r_from(const T *spec, T *out, int n, int pre, const T *tab, int comp, double kmin, hipStream_t st,
+                  TimingHook *hook) {
+  GIC_FFT_SWITCH(n, F::c2r_from(spec, out, pre, tab, comp, T(kmin), st, hook))
+}
+template <typename T>
+void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, double kmin, const int *w0, const int *wl,
+                    T *bufA, T *bufB, T *out, hipStream_t st, TimingHook *hook) {
+  for (int a = 0; a < 3; ++a)
+    if (wl[a] < 1 || wl[a] > n || w0[a] < 0 || w0[a] >= n) throw std::runtime_error("fft_c2r_window: bad window");
+  GIC_FFT_SWITCH(n, F::c2r_window(spec, pre, tab, comp, T(kmin), w0, wl, bufA, bufB, out, st, hook))
+}
+
+#define GIC_FFT_INST(T)                                                                                     \
+  template void fft_r2c_post<T>(T *, int, const T *, hipStream_t, TimingHook *);                            \
+  template void fft_c2r_from<T>(const T *, T *, int, int, const T *, int, double, hipStream_t, TimingHook *); \
+  template void fft_c2r_window<T>(const T *, int, int, const T *, int, double, const int *, const int *, T *, T *, \
+                                  T *, hipStream_t, TimingHook *);
+GIC_FFT_INST(float)
+GIC_FFT_INST(double)
 
 template void fft3d<float>(float *, int, int, hipStream_t, TimingHook *);
 template void fft3d<double>(double *, int, int, hipStream_t, TimingHook *);

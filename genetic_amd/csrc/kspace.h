@@ -24,6 +24,16 @@ struct InterpGeom {
   int wrap;          // coarse grid covers the box: wrap, else clamp
 };
 
+// Where the coarse values live: element (cx,cy,cz) of the coarse grid (after
+// wrapping/clamping) is p[lx*sx + ly*sy + lz] with l = (c - org) mod n < len.
+// A full padded field is org = 0, len = n, sx = n*(n+2), sy = n+2.
+struct CoarseView {
+  const void *p;
+  int org[3];
+  int len[3];
+  long sx, sy;
+};
+
 struct ParticleGeom {
   double dx;
   double o[3];
@@ -34,11 +44,13 @@ struct ParticleGeom {
 template <typename T>
 void sqrtp_table(const SplineHost &sp, double amplitude, double ns, double kmin, double norm, long nent, T *out,
                  hipStream_t st);
-template <typename T> void filter_table(int kind, double kc, double kmin, long nent, T *out, hipStream_t st);
+// kind 0 low pass, 1 high pass (sqrt(1-lp^2)), 2 unity; over_k2 divides by k^2
+template <typename T>
+void filter_table(int kind, double kc, double kmin, long nent, T *out, hipStream_t st, bool over_k2 = false);
 template <typename T> void mul_table(T *field, int n, const T *tab, hipStream_t st);
 template <typename T>
 void zeldovich(const T *delta, int n, double kmin, T *px, T *py, T *pz, const T *post, hipStream_t st);
-template <typename T> void interp_add(T *fine, int nf, const T *coarse, const InterpGeom &g, hipStream_t st);
+template <typename T> void interp_add(T *fine, int nf, const CoarseView &coarse, const InterpGeom &g, hipStream_t st);
 template <typename T>
 void particles(const T *px, const T *py, const T *pz, int n, const ParticleGeom &g, const uint64_t *cells,
                size_t start, size_t count, float *pos, float *vel, hipStream_t st);

@@ -82,17 +82,26 @@ void sqrtp_table(const SplineHost &sp, double amplitude, double ns, double kmin,
 // ---- filter tables by |k|^2 -------------------------------------------------------
 // kind 0: low pass 1/(1+exp((k-kc)/(0.1 kc)))
 // kind 1: high pass sqrt(1 - lp^2)
+// kind 2: unity
+// over_k2: divide by k^2 (zero at k = 0), the Zel'dovich kernel's 1/k^2
 template <typename T>
-__global__ void k_filter_table(int kind, double kc, double kmin, long nent, T *__restrict__ out) {
+__global__ void k_filter_table(int kind, double kc, double kmin, long nent, int over_k2, T *__restrict__ out) {
   long s = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= nent) return;
   double k = sqrt(double(s)) * kmin;
-  double lp = 1. / (1. + exp((k - kc) / (kc * 0.1)));
-  out[s] = T(kind == 0 ? lp : sqrt(1. - lp * lp));
+  double v = 1.0;
+  if (kind != 2) {
+    double lp = 1. / (1. + exp((k - kc) / (kc * 0.1)));
+    v = kind == 0 ? lp : sqrt(1. - lp * lp);
+  }
+  if (over_k2) v = s == 0 ? 0.0 : v / (double(s) * kmin * kmin);
+  out[s] = T(v);
 }
 
-template <typename T> void filter_table(int kind, double kc, double kmin, long nent, T *out, hipStream_t st) {
-  hipLaunchKernelGGL(k_filter_table<T>, dim3((nent + 255) / 256), dim3(256), 0, st, kind, kc, kmin, nent, out);
+template <typename T>
+void filter_table(int kind, double kc, double kmin, long nent, T *out, hipStream_t st, bool over_k2) {
+  hipLaunchKernelGGL(k_filter_table<T>, dim3((nent + 255) / 256), dim3(256), 0, st, kind, kc, kmin, nent,
+                     over_k2 ? 1 : 0, out);
   GIC_HIP(hipGetLastError());
 }
 
@@ -161,7 +170,8 @@ __device__ __forceinline__ void cr_weights(double t, double *w) {
 }
 
 template <typename T>
-__global__ void k_interp_add(T *__restrict__ fine, int nf, const T *__restrict__ coarse, InterpGeom g) {
+__global__ void k_interp_add(T *__restrict__ fine, int nf, CoarseView cv, InterpGeom g) {
+  const T *__restrict__ coarse = reinterpret_cast<const T *>(cv.p);
   size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   size_t nf3 = size_t(nf) * nf * nf;
   if (idx >= nf3) return;
@@ -195,19 +205,21 @@ __global__ void k_interp_add(T *__restrict__ fine, int nf, const T *__restrict__
         if (v > nc - 1) v = nc - 1;
         if (v < 0) v = 0;
       }
-      ci[a][i] = v;
+      int l = v - cv.org[a];
+      if (l < 0) l += nc;
+      ci[a][i] = l;
     }
   T acc = 0;
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j) {
       T wij = T(w[0][i] * w[1][j]);
-      const T *row = coarse + (size_t(ci[0][i]) * nc + ci[1][j]) * (nc + 2);
+      const T *row = coarse + ci[0][i] * cv.sx + ci[1][j] * cv.sy;
       for (int k = 0; k < 4; ++k) acc += wij * T(w[2][k]) * row[ci[2][k]];
     }
   fine[(size_t(x) * nf + y) * (nf + 2) + z] += acc;
 }
 
-template <typename T> void interp_add(T *fine, int nf, const T *coarse, const InterpGeom &g, hipStream_t st) {
+template <typename T> void interp_add(T *fine, int nf, const CoarseView &coarse, const InterpGeom &g, hipStream_t st) {
   size_t nf3 = size_t(nf) * nf * nf;
   hipLaunchKernelGGL(k_interp_add<T>, dim3((nf3 + 255) / 256), dim3(256), 0, st, fine, nf, coarse, g);
   GIC_HIP(hipGetLastError());
@@ -298,10 +310,10 @@ template <typename S, typename D> void convert(const S *a, size_t n, D *b, hipSt
 
 #define GIC_INST(T)                                                                                            \
   template void sqrtp_table<T>(const SplineHost &, double, double, double, double, long, T *, hipStream_t);   \
-  template void filter_table<T>(int, double, double, long, T *, hipStream_t);                                 \
+  template void filter_table<T>(int, double, double, long, T *, hipStream_t, bool);                           \
   template void mul_table<T>(T *, int, const T *, hipStream_t);                                               \
   template void zeldovich<T>(const T *, int, double, T *, T *, T *, const T *, hipStream_t);                  \
-  template void interp_add<T>(T *, int, const T *, const InterpGeom &, hipStream_t);                          \
+  template void interp_add<T>(T *, int, const CoarseView &, const InterpGeom &, hipStream_t);                 \
   template void particles<T>(const T *, const T *, const T *, int, const ParticleGeom &, const uint64_t *,    \
                              size_t, size_t, float *, float *, hipStream_t);                                  \
   template void fill<T>(T *, size_t, T, hipStream_t);

@@ -284,14 +284,15 @@ __device__ __forceinline__ int zig_iter(Get get, long pos, bool &accept, double 
 // an iteration is guaranteed to fit in the numbers generated so far.
 constexpr int RING = 48;
 
-struct Ring {
+template <int SLOTS> struct RingT {
   uint32_t *base;
   int stride;
-  __device__ __forceinline__ uint32_t get(long pos) const { return base[int(pos % RING) * stride]; }
-  __device__ __forceinline__ void put(long pos, uint32_t v) const { base[int(pos % RING) * stride] = v; }
+  __device__ __forceinline__ uint32_t get(long pos) const { return base[int(pos % SLOTS) * stride]; }
+  __device__ __forceinline__ void put(long pos, uint32_t v) const { base[int(pos % SLOTS) * stride] = v; }
 };
+using Ring = RingT<RING>;
 
-__device__ __forceinline__ void push_block(const Ring &r, long at, const uint64_t *x) {
+template <class R> __device__ __forceinline__ void push_block(const R &r, long at, const uint64_t *x) {
 #pragma unroll
   for (int k = 0; k < 12; ++k) {
     r.put(at + 2 * k, uint32_t(x[k] & 0xFFFFFF));
@@ -299,11 +300,20 @@ __device__ __forceinline__ void push_block(const Ring &r, long at, const uint64_
   }
 }
 
+// Raw numbers of the summary pass, kept for the emit pass: block q, number k
+// of chunk t at ((t/64)*(KC+1)*24 + q*24 + k)*64 + t%64, so a wavefront's 64
+// chunks store and load each number as one contiguous 256-byte segment.
+constexpr int RAWB = (RngPlan::KC + 1) * 24;
+__device__ __forceinline__ size_t raw_at(long t, int q, int k) {
+  return (size_t(t >> 6) * RAWB + q * 24 + k) * 64 + (t & 63);
+}
+
 // Pass 1: per chunk, the exit offset and Gaussian count for each of the four
 // possible entry offsets.  Once the four cursors sit on the same position
 // they share the rest of the parse, so only one is advanced and its later
 // acceptances are added to all four.
-__global__ __launch_bounds__(256) void k_rng_summary(RngDev p, long nchunks, uint32_t *__restrict__ summary) {
+__global__ __launch_bounds__(256) void k_rng_summary(RngDev p, long nchunks, uint32_t *__restrict__ summary,
+                                                    uint32_t *__restrict__ raw) {
   extern __shared__ uint32_t ring_mem[];
   long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= nchunks) return;
@@ -320,6 +330,11 @@ __global__ __launch_bounds__(256) void k_rng_summary(RngDev p, long nchunks, uin
   for (int q = 0; q <= RngPlan::KC; ++q) {
     ranlux_block(x, c);
     push_block(ring, avail, x);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) {
+      raw[raw_at(t, q, 2 * k)] = uint32_t(x[k] & 0xFFFFFF);
+      raw[raw_at(t, q, 2 * k + 1)] = uint32_t(x[k] >> 24);
+    }
     avail += 24;
     if (merged) {
       long pos = cur[0];
@@ -443,50 +458,101 @@ struct LevelTargets {
   size_t cum[4];   // cumulative Gaussian counts at the end of each level
 };
 
+// Pass 3 works one wavefront per workgroup.  Each lane parses its chunk from
+// the stored raw numbers (read through an LDS ring, one coalesced block at a
+// time) and stages its Gaussians in LDS; lanes holding 32 or more flush
+// them cooperatively, two lanes' runs of 32 consecutive cells per store
+// instruction, so every store writes whole 128-byte lines.
+constexpr int EMIT_RING = 32;
+constexpr int EMIT_STAGE = 48;
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_rng_emit(RngDev p, long nchunks, const uint32_t *__restrict__ summary,
-                                                 const uint64_t *__restrict__ segstart, LevelTargets lt,
-                                                 size_t total) {
-  extern __shared__ uint32_t ring_mem[];
-  long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= nchunks) return;
-  // entry offset and first Gaussian index of this chunk
-  long s = t / SEG;
-  uint64_t ss = segstart[s];
-  uint64_t g = ss >> 2;
-  int e = int(ss & 3);
-  for (long u = s * SEG; u < t; ++u) {
-    Map4 m = load_map(summary, u);
-    g += m.cnt[e];
-    e = m.ex[e];
+__device__ __forceinline__ T *cell_ptr(const LevelTargets &lt, uint64_t g) {
+  int L = g < lt.cum[0] ? 0 : g < lt.cum[1] ? 1 : g < lt.cum[2] ? 2 : 3;
+  size_t i = g - (L ? lt.cum[L - 1] : 0);
+  int n = lt.n[L];
+  size_t nn = size_t(n) * n;
+  size_t xx = i / nn, r = i - xx * nn, y = r / n, z = r - y * n;
+  return reinterpret_cast<T *>(lt.f[L]) + (xx * n + y) * (n + 2) + z;
+}
+
+template <typename T>
+__global__ __launch_bounds__(64) void k_rng_emit(const uint32_t *__restrict__ raw, long nchunks,
+                                                const uint32_t *__restrict__ summary,
+                                                const uint64_t *__restrict__ segstart, LevelTargets lt,
+                                                size_t total) {
+  __shared__ uint32_t ring_mem[EMIT_RING * 64];
+  __shared__ T stage_mem[EMIT_STAGE * 64];
+  const int lane = threadIdx.x;
+  long t = long(blockIdx.x) * 64 + lane;
+  bool active = t < nchunks;
+  uint64_t g = 0;
+  int e = 0;
+  if (active) {
+    long s = t / SEG;
+    uint64_t ss = segstart[s];
+    g = ss >> 2;
+    e = int(ss & 3);
+    for (long u = s * SEG; u < t; ++u) {
+      Map4 m = load_map(summary, u);
+      g += m.cnt[e];
+      e = m.ex[e];
+    }
+    if (g >= total) active = false;
   }
-  if (g >= total) return;
-  Ring ring{ring_mem + threadIdx.x, int(blockDim.x)};
-  uint64_t x[12];
-  int64_t c;
-  chunk_start_state(p, t, x, c);
+  RingT<EMIT_RING> ring{ring_mem + lane, 64};
   auto get = [&](long pos) { return ring.get(pos); };
   long pos = e;
   long avail = 0;
-  for (int q = 0; q <= RngPlan::KC && pos < RngPlan::CH && g < total; ++q) {
-    ranlux_block(x, c);
-    push_block(ring, avail, x);
-    avail += 24;
-    while (pos < RngPlan::CH && pos + 4 <= avail && g < total) {
-      bool acc;
-      double v;
-      pos += zig_iter(get, pos, acc, v);
-      if (acc) {
-        int L = g < lt.cum[0] ? 0 : g < lt.cum[1] ? 1 : g < lt.cum[2] ? 2 : 3;
-        size_t i = g - (L ? lt.cum[L - 1] : 0);
-        int n = lt.n[L];
-        size_t nn = size_t(n) * n;
-        size_t xx = i / nn, r = i - xx * nn, y = r / n, z = r - y * n;
-        reinterpret_cast<T *>(lt.f[L])[(xx * n + y) * (n + 2) + z] = T(v);
-        ++g;
+  int fill = 0;
+  uint64_t gstage = g;   // global index of staged slot 0
+  auto flush = [&](bool final) {
+    __syncthreads();
+    int want = final ? fill : (fill >= 32 ? 32 : 0);
+    unsigned long long mask = __ballot(want > 0);
+    while (mask) {
+      int s0 = __ffsll((long long)mask) - 1;
+      mask &= mask - 1;
+      int s1 = -1;
+      if (mask) {
+        s1 = __ffsll((long long)mask) - 1;
+        mask &= mask - 1;
+      }
+      int src = lane < 32 ? s0 : s1;
+      int j = lane & 31;
+      int w = __shfl(want, src < 0 ? 0 : src);
+      uint64_t gs = __shfl(gstage, src < 0 ? 0 : src);
+      if (src >= 0 && j < w) *cell_ptr<T>(lt, gs + j) = stage_mem[j * 64 + src];
+    }
+    __syncthreads();
+    if (want > 0) {
+      for (int j = 0; j + want < fill; ++j) stage_mem[j * 64 + lane] = stage_mem[(j + want) * 64 + lane];
+      fill -= want;
+      gstage += want;
+    }
+    __syncthreads();
+  };
+  for (int q = 0; q <= RngPlan::KC; ++q) {
+    bool busy = active && pos < RngPlan::CH && g < total;
+    if (!__any(busy)) break;
+    if (busy) {
+#pragma unroll
+      for (int k = 0; k < 24; ++k) ring.put(avail + k, raw[raw_at(t, q, k)]);
+      avail += 24;
+      while (pos < RngPlan::CH && pos + 4 <= avail && g < total) {
+        bool acc;
+        double v;
+        pos += zig_iter(get, pos, acc, v);
+        if (acc) {
+          stage_mem[fill * 64 + lane] = T(v);
+          ++fill;
+          ++g;
+        }
       }
     }
+    flush(false);
   }
+  flush(true);
 }
 
 // Raw 24-bit numbers in stream order (for bit-exactness checks).
@@ -629,14 +695,17 @@ void rng_plan(unsigned long seed, size_t n_gaussians, RngPlan &plan) {
   plan.nchunks = long(need / RngPlan::CH) + 2;
 }
 
+static size_t raw_bytes(long nc) { return size_t((nc + 63) / 64) * 64 * RAWB * 4; }
+
 size_t rng_workspace_bytes(const RngPlan &plan) {
   long nc = plan.nchunks;
   long nseg = (nc + SEG - 1) / SEG;
-  return (size_t(nc) * 5 * 4 + 255) / 256 * 256 + (size_t(nseg) * 5 * 4 + 255) / 256 * 256 + size_t(nseg) * 8 + 256;
+  return (size_t(nc) * 5 * 4 + 255) / 256 * 256 + (size_t(nseg) * 5 * 4 + 255) / 256 * 256 +
+         (size_t(nseg) * 8 + 255) / 256 * 256 + raw_bytes(nc) + 256;
 }
 
 struct RngBufs {
-  uint32_t *summary, *segmaps;
+  uint32_t *summary, *segmaps, *raw;
   uint64_t *segstart;
   long nseg;
 };
@@ -653,6 +722,7 @@ static RngBufs carve(const RngPlan &plan, void *ws, size_t bytes) {
   r.summary = (uint32_t *)take(size_t(nc) * 5 * 4);
   r.segmaps = (uint32_t *)take(size_t(nseg) * 5 * 4);
   r.segstart = (uint64_t *)take(size_t(nseg) * 8);
+  r.raw = (uint32_t *)take(raw_bytes(nc));
   r.nseg = nseg;
   if ((size_t)(p - (char *)ws) > bytes) throw std::runtime_error("RNG workspace too small");
   return r;
@@ -686,7 +756,7 @@ void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, 
   const int TB = 256;
   size_t ring_bytes = size_t(RING) * TB * 4;
   if (hook) hook->begin("rng_summary", st);
-  hipLaunchKernelGGL(k_rng_summary, dim3((nc + TB - 1) / TB), dim3(TB), ring_bytes, st, d, nc, b.summary);
+  hipLaunchKernelGGL(k_rng_summary, dim3((nc + TB - 1) / TB), dim3(TB), ring_bytes, st, d, nc, b.summary, b.raw);
   GIC_HIP(hipGetLastError());
   if (hook) hook->end("rng_summary", st);
   if (hook) hook->begin("rng_scan", st);
@@ -714,8 +784,8 @@ void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, 
   size_t total = acc;
   if (produced < total) throw std::runtime_error("internal error: RNG plan produced too few Gaussians");
   if (hook) hook->begin("rng_emit", st);
-  hipLaunchKernelGGL(k_rng_emit<T>, dim3((nc + TB - 1) / TB), dim3(TB), ring_bytes, st, d, nc, b.summary, b.segstart,
-                     lt, total);
+  hipLaunchKernelGGL(k_rng_emit<T>, dim3((nc + 63) / 64), dim3(64), 0, st, b.raw, nc, b.summary, b.segstart, lt,
+                     total);
   GIC_HIP(hipGetLastError());
   if (hook) hook->end("rng_emit", st);
 }
