@@ -558,6 +558,39 @@ template <typename T> void do_set_field(gic_context *c, int field, int level, in
   GIC_HIP(hipStreamSynchronize(c->st));
 }
 
+// Per-|k|^2 histogram of the overdensity's power and the theory covariance
+// P(k) N^3/V for dumpPowerSpectrum (cosmology/parameters.hpp:112-205).  The
+// overdensity is transformed on a copy, so the field keeps its space.
+template <typename T>
+void do_mode_power(gic_context *c, int level, double *sum, double *cnt, double *theory, size_t nm) {
+  Level &L = level_at(c, level);
+  long h = L.g.n / 2;
+  if (nm != size_t(3 * h * h + 1)) throw std::runtime_error("histogram length must be 3(n/2)^2+1");
+  if (!c->drawn) do_draw<T>(c);
+  if (!c->power_applied) do_apply_ps<T>(c);
+  if (L.dc_pending) ensure_noise_fourier<T>(c, L);
+  size_t bytes = L.g.padded_reals() * sizeof(T);
+  size_t hb = nm * sizeof(double);
+  c->stage.ensure(bytes + 2 * hb);
+  T *spec = (T *)L.noise.p;
+  if (!L.noise_fourier) {
+    GIC_HIP(hipMemcpyAsync(c->stage.p, L.noise.p, bytes, hipMemcpyDeviceToDevice, c->st));
+    spec = (T *)c->stage.p;
+    fft3d<T>(spec, L.g.n, -1, c->st, c->hook());
+  }
+  double *dsum = (double *)((char *)c->stage.p + bytes);
+  double *dcnt = dsum + nm;
+  fill<double>(dsum, 2 * nm, 0.0, c->st);
+  mode_power<T>(spec, L.g.n, dsum, dcnt, c->st);
+  GIC_HIP(hipMemcpyAsync(sum, dsum, hb, hipMemcpyDeviceToHost, c->st));
+  GIC_HIP(hipMemcpyAsync(cnt, dcnt, hb, hipMemcpyDeviceToHost, c->st));
+  std::vector<T> tab(nm);
+  build_sqrtp<T>(c, L);
+  GIC_HIP(hipMemcpyAsync(tab.data(), L.sqrtp.p, nm * sizeof(T), hipMemcpyDeviceToHost, c->st));
+  GIC_HIP(hipStreamSynchronize(c->st));
+  for (size_t m = 0; m < nm; ++m) theory[m] = double(tab[m]) * double(tab[m]);
+}
+
 double velocity_ratio(const Cosmo &c) {
   double f = 1.0;
   return f * 100. * sqrt(c.Om / (c.a * c.a * c.a) + c.Ol) * sqrt(c.a);
@@ -841,6 +874,10 @@ int gic_set_field(gic_context *c, int field, int level, int space, const void *h
 int gic_particles(gic_context *c, int level, const uint64_t *cells, size_t start, size_t count, float *pos,
                   float *vel) {
   return guarded(c, [&] { DISPATCH(c, do_particles, c, level, cells, start, count, pos, vel); });
+}
+
+int gic_mode_power(gic_context *c, int level, double *sum_power, double *count, double *theory, size_t nm) {
+  return guarded(c, [&] { DISPATCH(c, do_mode_power, c, level, sum_power, count, theory, nm); });
 }
 
 int gic_particle_mass_eps(gic_context *c, int level, double *mass, double *eps) {

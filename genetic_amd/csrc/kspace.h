@@ -48,6 +48,8 @@ void sqrtp_table(const SplineHost &sp, double amplitude, double ns, double kmin,
 template <typename T>
 void filter_table(int kind, double kc, double kmin, long nent, T *out, hipStream_t st, bool over_k2 = false);
 template <typename T> void mul_table(T *field, int n, const T *tab, hipStream_t st);
+// sum[m] += |spec_k|^2, cnt[m] += 1 over (ix,iy,iz) in [-N/2,N/2]^3, m = |k_int|^2
+template <typename T> void mode_power(const T *spec, int n, double *sum, double *cnt, hipStream_t st);
 template <typename T>
 void zeldovich(const T *delta, int n, double kmin, T *px, T *py, T *pz, const T *post, hipStream_t st);
 template <typename T> void interp_add(T *fine, int nf, const CoarseView &coarse, const InterpGeom &g, hipStream_t st);

@@ -125,6 +125,42 @@ template <typename T> void mul_table(T *field, int n, const T *tab, hipStream_t 
   GIC_HIP(hipGetLastError());
 }
 
+// ---- |delta_k|^2 histogram by integer |k|^2 ----------------------------------------
+// dumpPowerSpectrum (cosmology/parameters.hpp:112-205) sums over every
+// (ix,iy,iz) in [-N/2, N/2]^3 -- the Nyquist planes are visited twice -- and
+// the bin of a mode depends only on m = ix^2+iy^2+iz^2.  One thread per
+// visited triple adds |delta|^2 and 1 into the two FP64 bins of its m.
+template <typename T>
+__global__ void k_mode_power(const cx<T> *__restrict__ f, int n, double *__restrict__ sum, double *__restrict__ cnt) {
+  size_t w = size_t(n) + 1;
+  size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (idx >= w * w * w) return;
+  int h = n / 2;
+  int iz = int(idx % w) - h;
+  int iy = int((idx / w) % w) - h;
+  int ix = int(idx / (w * w)) - h;
+  int sx = ix, sy = iy, sz = iz;
+  if (sz < 0) {   // conjugate: (kx,ky,kz) = conj(-kx,-ky,-kz); the modulus is the same
+    sx = -sx;
+    sy = -sy;
+    sz = -sz;
+  }
+  if (sx < 0) sx += n;
+  if (sy < 0) sy += n;
+  cx<T> v = f[(size_t(sx) * n + sy) * (h + 1) + sz];
+  double a = double(v.x) * double(v.x) + double(v.y) * double(v.y);
+  int m = ix * ix + iy * iy + iz * iz;
+  atomicAdd(&sum[m], a);
+  atomicAdd(&cnt[m], 1.0);
+}
+
+template <typename T> void mode_power(const T *spec, int n, double *sum, double *cnt, hipStream_t st) {
+  size_t w = size_t(n) + 1, tot = w * w * w;
+  hipLaunchKernelGGL(k_mode_power<T>, dim3((tot + 255) / 256), dim3(256), 0, st,
+                     reinterpret_cast<const cx<T> *>(spec), n, sum, cnt);
+  GIC_HIP(hipGetLastError());
+}
+
 // ---- Zel'dovich: psi_j(k) = i k_j delta(k) / k^2 -----------------------------------------
 // Computed in T with k_j = int * kMin as the reference does; the component
 // at k_j = +Nyquist and the k = 0 mode are zero.  An optional table factor
@@ -312,6 +348,7 @@ template <typename S, typename D> void convert(const S *a, size_t n, D *b, hipSt
   template void sqrtp_table<T>(const SplineHost &, double, double, double, double, long, T *, hipStream_t);   \
   template void filter_table<T>(int, double, double, long, T *, hipStream_t, bool);                           \
   template void mul_table<T>(T *, int, const T *, hipStream_t);                                               \
+  template void mode_power<T>(const T *, int, double *, double *, hipStream_t);                               \
   template void zeldovich<T>(const T *, int, double, T *, T *, T *, const T *, hipStream_t);                  \
   template void interp_add<T>(T *, int, const CoarseView &, const InterpGeom &, hipStream_t);                 \
   template void particles<T>(const T *, const T *, const T *, int, const ParticleGeom &, const uint64_t *,    \

@@ -42,6 +42,7 @@ SIGNATURES = {
     "gic_zeldovich": ([_v], _i),
     "gic_copy_field": ([_v, _i, _i, _i, _v, _i], _i),
     "gic_set_field": ([_v, _i, _i, _i, _v, _i], _i),
+    "gic_mode_power": ([_v, _i, _v, _v, _v, _s], _i),
     "gic_particles": ([_v, _i, _v, _s, _s, _v, _v], _i),
     "gic_particle_mass_eps": ([_v, _i, _P(_d), _P(_d)], _i),
     "gic_velocity_ratio": ([_v, _P(_d)], _i),
@@ -223,6 +224,14 @@ class ICGenerator:
         cp = _ptr(cells) if cells is not None else None
         self._check(lib().gic_particles(self._h, level, cp, start, count, _ptr(pos), _ptr(vel)))
         return pos, vel
+
+    def mode_power(self, level):
+        """Per-|k|^2 sums of |delta_k|^2, mode counts and theory covariance."""
+        h = self.sizes[level] // 2
+        nm = 3 * h * h + 1
+        s, c, t = (np.empty(nm) for _ in range(3))
+        self._check(lib().gic_mode_power(self._h, level, _ptr(s), _ptr(c), _ptr(t), nm))
+        return s, c, t
 
     def particle_mass_eps(self, level):
         m, e = ctypes.c_double(), ctypes.c_double()

@@ -1,0 +1,1322 @@
+// genetIC-hip: paramfile front-end for libgic_hip.so.
+//
+// Keeps the reference's command surface for the field-generation path
+// (genetIC/src/main.cpp:22-172) and its parser contract
+// (tools/parser.hpp:40-211): case-insensitive command names, '#'/'%'
+// comments, arguments parsed before the call, and
+// 'Error "<what>" on line N ("<line>")' then exit 1.  Grid and flag
+// bookkeeping, the zoom-window choice, the particle mapper and the writers
+// are host integer work; every field operation is a gic_* call into the HIP
+// library.
+#include "../../include/gic.h"
+
+#include <unistd.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <fstream>
+#include <functional>
+#include <iomanip>
+#include <iostream>
+#include <map>
+#include <numeric>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <tuple>
+#include <vector>
+
+namespace fe {
+
+// ---------------------------------------------------------------- logging --
+// logging::entry() (tools/logging.hpp): local time stamp, then three spaces
+std::ostream &log() {
+  std::time_t t = std::time(nullptr);
+  std::cerr << std::put_time(std::localtime(&t), " %F %T   ");
+  return std::cerr;
+}
+
+// ---------------------------------------------------------------- geometry -
+struct I3 {
+  int x = 0, y = 0, z = 0;
+  int &operator[](int i) { return i == 0 ? x : i == 1 ? y : z; }
+  int operator[](int i) const { return i == 0 ? x : i == 1 ? y : z; }
+};
+
+// One level's grid (simulation/grid/grid.hpp): index = (x*n + y)*n + z,
+// centroid = c*dx + offset + dx/2, flags kept sorted and unique.
+struct Grid {
+  double periodic = 0, dx = 0, ox = 0, oy = 0, oz = 0;
+  int n = 0;
+  unsigned gadgetType = 1;
+  std::vector<size_t> flags;
+  size_t n3() const { return size_t(n) * n * n; }
+  double size() const { return dx * n; }
+  int simEquivalent() const { return int(std::llround(periodic / dx)); }
+  bool coversFull() const { return periodic == size(); }
+  I3 coord(size_t id) const {
+    I3 c;
+    c.x = int(id / (size_t(n) * n));
+    c.y = int((id / n) % n);
+    c.z = int(id % n);
+    return c;
+  }
+  size_t index(int x, int y, int z) const { return (size_t(x) * n + y) * n + z; }
+  std::array<double, 3> centroidOf(const I3 &c) const {
+    return {c.x * dx + ox + dx / 2, c.y * dx + oy + dx / 2, c.z * dx + oz + dx / 2};
+  }
+  std::array<double, 3> centroid(size_t id) const { return centroidOf(coord(id)); }
+  double wrap(double v) const {
+    v = std::fmod(v, periodic);
+    if (v < 0) v += periodic;
+    return v;
+  }
+  double wrappedOffset(double a, double b) const {
+    double r = a - b;
+    if (r > periodic / 2) r -= periodic;
+    if (r < -periodic / 2) r += periodic;
+    return r;
+  }
+  void flagCells(const std::vector<size_t> &ids) {
+    std::vector<size_t> u(ids);
+    u.insert(u.end(), flags.begin(), flags.end());
+    std::sort(u.begin(), u.end());
+    u.erase(std::unique(u.begin(), u.end()), u.end());
+    flags.swap(u);
+  }
+  // getCentreWrapped (grid.hpp:466-494)
+  std::array<double, 3> flaggedCentre() const {
+    if (flags.empty()) throw std::runtime_error("Cannot calculate the center of an empty region");
+    auto p0 = centroid(flags[0]);
+    double s[3] = {0, 0, 0};
+    for (size_t i = 1; i < flags.size(); ++i) {
+      auto pi = centroid(flags[i]);
+      for (int d = 0; d < 3; ++d) s[d] += wrappedOffset(pi[d], p0[d]);
+    }
+    std::array<double, 3> r;
+    for (int d = 0; d < 3; ++d) r[d] = wrap(s[d] / double(flags.size()) + p0[d]);
+    return r;
+  }
+  // insertCubeIdsIntoVector (grid.hpp:704-731): the cells whose centres lie
+  // in the cube of side dxc centred on the point
+  void cubeIds(const std::array<double, 3> &c, double dxc, std::vector<size_t> &out) const {
+    double p[3] = {wrap(c[0] - ox), wrap(c[1] - oy), wrap(c[2] - oz)};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = int(std::floor((p[d] - dxc / 2 + dx / 2) / dx));
+      hi[d] = int(std::floor((p[d] + dxc / 2 - dx / 2) / dx));
+      if (lo[d] < 0 || hi[d] >= n) throw std::runtime_error("Requested zoom region falls outside high resolution grid");
+    }
+    for (int x = lo[0]; x <= hi[0]; ++x)
+      for (int y = lo[1]; y <= hi[1]; ++y)
+        for (int z = lo[2]; z <= hi[2]; ++z) out.push_back(index(x, y, z));
+  }
+};
+
+// Integer window with wrapping (simulation/window.hpp), kept with the
+// reference's behaviour, including expandSymmetricallyToSize comparing the
+// new y corner against the old z corner.
+struct Window {
+  int wrapLength = 0;
+  I3 lo, hi;   // lower inclusive, upper exclusive
+  Window() = default;
+  Window(int wl, I3 p) : wrapLength(wl), lo(p) {
+    hi = p;
+    hi.x += 1;
+    hi.y += 1;
+    hi.z += 1;
+  }
+  int wrap(int v) const {
+    if (v < 0) v += wrapLength;
+    if (v >= wrapLength) v -= wrapLength;
+    return v;
+  }
+  static bool within(int lo, int hi, int v) {
+    if (hi >= lo) return v < hi && v >= lo;
+    return v < hi || v >= lo;
+  }
+  void wrapCorners() {
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = wrap(lo[d]);
+      hi[d] = wrap(hi[d] - 1) + 1;
+    }
+  }
+  int maxDim() const {
+    int m = 0;
+    for (int d = 0; d < 3; ++d) m = std::max(m, wrap(hi[d] - lo[d] - 1) + 1);
+    return m;
+  }
+  I3 centre() const {
+    I3 c;
+    for (int d = 0; d < 3; ++d) c[d] = wrap(lo[d] + wrap(hi[d] - 1 - lo[d]) / 2);
+    return c;
+  }
+  void expandToInclude(const I3 &p) {
+    for (int d = 0; d < 3; ++d) {
+      if (!within(lo[d], hi[d], p[d])) {
+        int sizeLower = wrap(hi[d] - p[d] - 1) + 1;
+        int sizeUpper = wrap(p[d] + 1 - lo[d] - 1) + 1;
+        bool canLower = within(p[d], hi[d], lo[d]);
+        bool canUpper = within(lo[d], p[d] + 1, hi[d] - 1);
+        if (canLower && (sizeLower < sizeUpper || !canUpper)) lo[d] = p[d];
+        else hi[d] = p[d] + 1;
+      }
+    }
+  }
+  void expandSymmetricallyToSize(int newSize) {
+    I3 oldLo = lo;
+    if (newSize == wrapLength) {
+      lo = I3{};
+      hi.x = hi.y = hi.z = newSize;
+      return;
+    }
+    I3 c = centre();
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = wrap(c[d] - newSize / 2);
+      if (newSize % 2 == 0) hi[d] = wrap(c[d] + newSize / 2 - 1) + 1;
+      else hi[d] = wrap(c[d] + newSize / 2) + 1;
+    }
+    if (lo.x > oldLo.x) {
+      lo.x -= 1;
+      hi.x -= 1;
+    }
+    if (lo.y > oldLo.z) {
+      lo.y -= 1;
+      hi.y -= 1;
+    }
+    if (lo.z > oldLo.z) {
+      lo.z -= 1;
+      hi.z -= 1;
+    }
+  }
+  bool contains(int v, int d) const { return within(lo[d], hi[d], v); }
+  int wrappedOffset(int a, int b) const {
+    int dist = b - a;
+    if (dist < -wrapLength / 2) dist += wrapLength;
+    if (dist > wrapLength / 2) dist -= wrapLength;
+    return dist;
+  }
+  void shift(const I3 &s, int sign) {
+    for (int d = 0; d < 3; ++d) {
+      lo[d] += sign * s[d];
+      hi[d] += sign * s[d];
+    }
+    wrapCorners();
+  }
+  void moveInto(const Window &o) {
+    I3 mv;
+    for (int d = 0; d < 3; ++d) {
+      if (!o.contains(lo[d], d)) mv[d] = wrappedOffset(lo[d], o.lo[d]);
+      if (!o.contains(hi[d] - 1, d)) mv[d] = wrappedOffset(hi[d], o.hi[d]);
+    }
+    shift(mv, +1);
+  }
+};
+
+int ratioInt(double a, double b) {
+  double r = a / b;
+  long ri = std::lround(r);
+  if (std::fabs(r - ri) > 1e-6 || ri < 1) throw std::runtime_error("Ratio of grid spacings must be a positive integer");
+  return int(ri);
+}
+
+std::vector<size_t> argsort(const std::vector<size_t> &v) {
+  std::vector<size_t> i(v.size());
+  std::iota(i.begin(), i.end(), 0);
+  std::sort(i.begin(), i.end(), [&](size_t a, size_t b) { return v[a] < v[b]; });
+  return i;
+}
+
+// ---------------------------------------------------------------- npy ------
+void saveNpy(const std::string &fn, int n, const std::vector<double> &d) {
+  std::ofstream f(fn, std::ios::binary);
+  if (!f) throw std::runtime_error("Can't open file " + fn);
+  std::string hdr = "{'descr': '<f8', 'fortran_order': False, 'shape': (" + std::to_string(n) + ", " +
+                    std::to_string(n) + ", " + std::to_string(n) + "), }";
+  size_t total = 10 + hdr.size() + 1;
+  hdr += std::string((64 - total % 64) % 64, ' ') + "\n";
+  unsigned short hl = (unsigned short)hdr.size();
+  f.write("\x93NUMPY\x01\x00", 8);
+  f.write((const char *)&hl, 2);
+  f.write(hdr.data(), hdr.size());
+  f.write((const char *)d.data(), d.size() * sizeof(double));
+}
+
+// ---------------------------------------------------------------- generator
+enum class Fmt { unknown = 1, gadget2 = 2, gadget3 = 3, tipsy = 4, grafic = 5, gadgethdf = 6, swift = 7 };
+
+// One TwoLevelParticleMapper (particles/mapper/twolevelmapper.hpp) of the
+// nested chain: coarse = mapper of levels [0, L-1], fine = grid L.
+struct LevelMap {
+  std::vector<size_t> cells;       // level1CellsToReplace: sorted cells of level L-1
+  std::vector<size_t> particles;   // level1ParticlesToReplace: their ids in the coarse mapper
+  std::vector<size_t> hires;       // zoomParticleArrayHiresUnsorted: level-L cells in output order
+  size_t nhr = 0, size1 = 0, first2 = 0, total = 0;
+};
+
+// The particles of one level (or of the gas copy of the finest level) in
+// mapper order
+struct Block {
+  int level = 0;
+  bool gas = false;
+  unsigned type = 1;
+  std::vector<uint64_t> cells;
+  size_t first = 0;   // mapper index of the first particle
+  double mass = 0, eps = 0;
+  std::vector<float> pos, vel;
+  size_t count() const { return cells.size(); }
+};
+
+struct Generator {
+  gic_context *ctx = nullptr;
+  bool fp32 = false;
+  double Om = 0.3, Ol = 0, Ob = 0, hubble = 0.701, s8 = 0, ns = 0.96, z = 0, TCMB = 2.725;
+  double epsNorm = 0.01075;
+  double velOffset[3] = {0, 0, 0};
+  bool haveCamb = false, drawn = false, generated = false;
+  std::vector<Grid> grids;
+  std::vector<std::vector<size_t>> zoomParticles;   // zoomParticleArray: per level >= 1, cells of level-1
+  std::vector<LevelMap> maps;
+  double x0 = 0, y0 = 0, z0 = 0;
+  Fmt fmt = Fmt::unknown;
+  std::string outdir = "./", outname;
+  int nGadgetFiles = 1;
+  unsigned mruGadgetType = 1;
+  bool usedGadgetType[6] = {false, false, false, false, false, false};
+
+  explicit Generator(bool f32) : fp32(f32) {}
+  ~Generator() {
+    if (ctx) gic_destroy(ctx);
+  }
+
+  // The device context is opened by the first command that needs it, so a
+  // paramfile's syntax errors are reported the same way with or without a GPU.
+  gic_context *c() {
+    if (!ctx) {
+      if (gic_create(&ctx, 0, fp32 ? GIC_FLOAT32 : GIC_FLOAT64) != 0) {
+        ctx = nullptr;
+        throw std::runtime_error("No usable HIP device for libgic_hip");
+      }
+      pushCosmology();
+    }
+    return ctx;
+  }
+  void check(int rc) {
+    if (rc) throw std::runtime_error(gic_last_error(ctx));
+  }
+  void pushCosmology() {
+    if (ctx) check(gic_set_cosmology(c(), Om, Ol, Ob, hubble, s8, ns, z));
+  }
+  double a() const { return 1. / (z + 1.); }
+
+  // io/input.hpp:33-84: whitespace-separated numbers; the column count is
+  // that of the first line
+  void setCamb(const std::string &fn) {
+    std::ifstream f(fn);
+    if (!f.is_open()) throw std::runtime_error("File " + fn + " not found");
+    std::vector<double> v;
+    std::string line;
+    int ncols = 0;
+    while (std::getline(f, line)) {
+      std::stringstream ss(line);
+      double t;
+      int c = 0;
+      while (ss >> t) {
+        v.push_back(t);
+        ++c;
+      }
+      if (ncols == 0) ncols = c;
+    }
+    if (ncols == 0) throw std::runtime_error("Error reading file " + fn + ". Could not determine number of columns.");
+    check(gic_set_camb_table(c(), v.data(), v.size(), ncols));
+    haveCamb = true;
+  }
+
+  // --- grids ---
+  void addLevel(double size, int n, double ox, double oy, double oz) {
+    Grid g;
+    g.periodic = grids.empty() ? size : grids[0].periodic;
+    g.n = n;
+    g.dx = size / n;
+    g.ox = ox;
+    g.oy = oy;
+    g.oz = oz;
+    g.gadgetType = mruGadgetType;
+    if (!grids.empty()) usedGadgetType[mruGadgetType] = true;
+    grids.push_back(g);
+  }
+  void initBaseGrid(double box, int n) {
+    if (!grids.empty()) throw std::runtime_error("Cannot re-initialize the base grid");
+    check(gic_init_base_grid(c(), box, n));
+    addLevel(box, n, 0, 0, 0);
+    log() << "Initialized the base grid:" << std::endl;
+    log() << "  Box length            = " << box << " Mpc/h" << std::endl;
+    log() << "  n                     = " << n << std::endl;
+    log() << "  dx                    = " << box / n << std::endl;
+  }
+
+  // initZoomGridWithLowLeftCornerAt (ic.hpp:501-601)
+  void initZoomGridWithLowLeftCornerAt(int x0i, int y0i, int z0i, int zoomfac, int n) {
+    if (drawn) throw std::runtime_error("Trying to initialize a grid after the random field was already drawn");
+    if (grids.empty()) throw std::runtime_error("Cannot initialise a zoom grid before initialising the base grid");
+    if (zoomfac < 1) throw std::runtime_error("Zoom factor must be positive");
+    Grid above = grids.back();
+    std::vector<size_t> untrimmed = above.flags;
+    int nAbove = above.n;
+    int nc = nAbove / zoomfac;
+    if (x0i < 0 || y0i < 0 || z0i < 0)
+      throw std::runtime_error("The newly initialised zoom level is not fully contained within the previous level");
+    I3 lower{x0i, y0i, z0i};
+    I3 upper{x0i + nc, y0i + nc, z0i + nc};
+    if (above.coversFull()) {
+      int sim = above.simEquivalent();
+      for (int d = 0; d < 3; ++d) {
+        int v = (upper[d] - 1) % sim;
+        if (v < 0) v += sim;
+        upper[d] = v + 1;
+      }
+    } else if (upper.x > nAbove || upper.y > nAbove || upper.z > nAbove) {
+      throw std::runtime_error("The newly initialised zoom level is not fully contained within the previous level");
+    }
+    // keep the flagged cells whose centroid lies in the zoom window (+-eps);
+    // Window<T>::wrapCorners wraps the point just inside the upper corner
+    double eps = above.dx * 1e-6;
+    I3 um1{upper.x - 1, upper.y - 1, upper.z - 1};
+    auto wlo = above.centroidOf(lower), whi = above.centroidOf(um1);
+    std::vector<size_t> trimmed;
+    size_t missed = 0;
+    for (size_t id : untrimmed) {
+      auto p = above.centroid(id);
+      bool in = true;
+      for (int d = 0; d < 3; ++d) {
+        double lo = wlo[d] - eps, hi = whi[d] + eps;
+        lo = lo < 0 ? lo + above.periodic : (lo >= above.periodic ? lo - above.periodic : lo);
+        double hin = std::nextafter(hi, -INFINITY);
+        hin = hin < 0 ? hin + above.periodic : (hin >= above.periodic ? hin - above.periodic : hin);
+        hi = std::nextafter(hin, INFINITY);
+        in &= hi >= lo ? (p[d] < hi && p[d] >= lo) : (p[d] < hi || p[d] >= lo);
+      }
+      if (in) trimmed.push_back(id);
+      else ++missed;
+    }
+    if (missed > 0) {
+      log() << "WARNING: the requested zoom particles do not all fit in the requested zoom window" << std::endl;
+      log() << "         of " << untrimmed.size() << " particles, " << missed << " have been omitted" << std::endl;
+      log() << "         to make a new zoom flag list of " << trimmed.size() << std::endl;
+    }
+    check(gic_init_zoom_grid_with_origin_at(c(), x0i, y0i, z0i, zoomfac, n));
+    zoomParticles.push_back(trimmed);
+    addLevel(above.size() / zoomfac, n, above.ox + x0i * above.dx, above.oy + y0i * above.dx,
+             above.oz + z0i * above.dx);
+    Grid &ng = grids.back();
+    log() << "Initialized a zoom region:" << std::endl;
+    log() << "  Subbox length         = " << ng.size() << " Mpc/h" << std::endl;
+    log() << "  n                     = " << ng.n << std::endl;
+    log() << "  dx                    = " << ng.dx << std::endl;
+    log() << "  Zoom factor           = " << zoomfac << std::endl;
+    log() << "  Num particles         = " << untrimmed.size() << std::endl;
+    log() << "  Low-left corner in parent grid = " << x0i << ", " << y0i << ", " << z0i << std::endl;
+    log() << "  Low-left corner (h**-1 Mpc)    = " << ng.ox << ", " << ng.oy << ", " << ng.oz << std::endl;
+    buildMapper();
+    log() << "  Total particles       = " << mapperSize(int(grids.size()) - 1) << std::endl;
+    // flag the fine cells under the flagged coarse cells (makeProxyGridToMatch:
+    // supersample, then the section at the fine grid's offset, wrapped)
+    Grid &fine = grids.back();
+    int r = ratioInt(above.dx, fine.dx);
+    int sim = fine.simEquivalent();
+    long off[3] = {std::lround((fine.ox - above.ox) / fine.dx), std::lround((fine.oy - above.oy) / fine.dx),
+                   std::lround((fine.oz - above.oz) / fine.dx)};
+    auto w = [sim](long v) {
+      v %= sim;
+      if (v < 0) v += sim;
+      return v;
+    };
+    std::vector<size_t> fineFlags;
+    for (size_t id : above.flags) {
+      I3 c = above.coord(id);
+      for (int i = 0; i < r; ++i)
+        for (int j = 0; j < r; ++j)
+          for (int k = 0; k < r; ++k) {
+            long sx = w(long(c.x) * r + i - off[0]), sy = w(long(c.y) * r + j - off[1]),
+                 sz = w(long(c.z) * r + k - off[2]);
+            if (sx < fine.n && sy < fine.n && sz < fine.n) fineFlags.push_back(fine.index(int(sx), int(sy), int(sz)));
+          }
+    }
+    fine.flagCells(fineFlags);
+  }
+
+  // initZoomGrid (ic.hpp:398-478): smallest window around the flags,
+  // expanded symmetrically to the requested size, moved inside the level
+  void initZoomGrid(int zoomfac, int n) {
+    if (grids.empty()) throw std::runtime_error("Cannot initialise a zoom grid before initialising the base grid");
+    if (zoomfac < 1) throw std::runtime_error("Zoom factor must be positive");
+    Grid &above = grids.back();
+    if (above.flags.empty()) throw std::runtime_error("Cannot initialise zoom without marking particles to be replaced");
+    int sim = above.simEquivalent();
+    Window zw(sim, above.coord(above.flags[0]));
+    for (size_t id : above.flags) zw.expandToInclude(above.coord(id));
+    int nreq = zw.maxDim();
+    int nuser = above.n / zoomfac;
+    if (nreq > nuser)
+      throw std::runtime_error("Zoom particles do not fit in specified sub-box. Decrease zoom, or choose different particles");
+    if (nreq < nuser) zw.expandSymmetricallyToSize(nuser);
+    I3 aboveLo{int(std::llround(above.ox / above.dx)), int(std::llround(above.oy / above.dx)),
+               int(std::llround(above.oz / above.dx))};
+    zw.shift(aboveLo, +1);
+    Window actual(sim, aboveLo);
+    actual.hi = I3{aboveLo.x + above.n, aboveLo.y + above.n, aboveLo.z + above.n};
+    actual.wrapCorners();
+    zw.moveInto(actual);
+    zw.shift(aboveLo, -1);
+    initZoomGridWithLowLeftCornerAt(zw.lo.x, zw.lo.y, zw.lo.z, zoomfac, n);
+  }
+
+  // --- particle mapper ---------------------------------------------------
+  size_t mapperSize(int L) const { return L == 0 ? grids[0].n3() : maps[L].total; }
+
+  // getFlaggedParticles with an explicit per-level flag set (twolevelmapper.hpp:473-513)
+  std::vector<size_t> flaggedParticles(int L, const std::vector<std::vector<size_t>> &f) const {
+    std::vector<size_t> out;
+    if (L == 0) {
+      out = f[0];
+      std::sort(out.begin(), out.end());
+      return out;
+    }
+    const LevelMap &m = maps[L];
+    auto g1 = flaggedParticles(L - 1, f);
+    size_t zi = 0;
+    for (size_t i_lr : g1) {
+      while (zi < m.particles.size() && m.particles[zi] < i_lr) ++zi;
+      if (zi == m.particles.size() || m.particles[zi] != i_lr) out.push_back(i_lr - zi);
+    }
+    std::vector<size_t> g2 = f[L];
+    std::sort(g2.begin(), g2.end());
+    auto si = argsort(m.hires);
+    size_t k = 0;
+    for (size_t i_hr : g2) {
+      while (k < m.hires.size() && m.hires[si[k]] < i_hr) ++k;
+      if (k == m.hires.size() || m.hires[si[k]] != i_hr) continue;
+      out.push_back(si[k] + m.first2);
+    }
+    std::sort(out.begin(), out.end());
+    return out;
+  }
+  std::vector<size_t> flaggedParticles() const {
+    std::vector<std::vector<size_t>> f;
+    for (auto &g : grids) f.push_back(g.flags);
+    return flaggedParticles(int(grids.size()) - 1, f);
+  }
+
+  // flagParticles (twolevelmapper.hpp:357-466); ids ascending
+  void flagParticles(int L, const std::vector<size_t> &ids) {
+    if (L == 0) {
+      for (size_t id : ids)
+        if (id >= grids[0].n3()) throw std::runtime_error("Particle ID out of range");
+      grids[0].flagCells(ids);
+      return;
+    }
+    const LevelMap &m = maps[L];
+    size_t nz = m.particles.size();
+    size_t skipped = 0, last = 0;
+    std::vector<size_t> lr, hr;
+    size_t firstHr = ids.size();
+    for (size_t i = 0; i < ids.size(); ++i) {
+      size_t p = ids[i];
+      if (p < last) throw std::runtime_error("The particle list must be in ascending order");
+      last = p;
+      if (p < m.first2) {
+        while (skipped < nz && m.particles[skipped] <= p + skipped) ++skipped;
+        lr.push_back(p + skipped);
+      } else {
+        firstHr = i;
+        break;
+      }
+    }
+    const Grid &gc = grids[L - 1];
+    const Grid &gf = grids[L];
+    size_t lastLr = SIZE_MAX;
+    std::vector<size_t> cube;
+    for (size_t i = firstHr; i < ids.size(); ++i) {
+      size_t rel = ids[i] - m.first2;
+      size_t lri = rel / m.nhr;
+      if (lri >= nz) throw std::runtime_error("Particle ID out of range");
+      if (lastLr != m.particles[lri]) {
+        lastLr = m.particles[lri];
+        lr.push_back(lastLr);
+        cube.clear();
+        gf.cubeIds(gc.centroid(m.cells[lri]), gc.dx, cube);
+      }
+      hr.push_back(cube[rel % m.nhr]);
+    }
+    std::sort(lr.begin(), lr.end());
+    std::sort(hr.begin(), hr.end());
+    flagParticles(L - 1, lr);
+    grids[L].flagCells(hr);
+  }
+
+  // TwoLevelParticleMapper constructor (twolevelmapper.hpp:270-320), per level
+  void buildMapper() {
+    maps.assign(grids.size(), LevelMap());
+    for (size_t L = 1; L < grids.size(); ++L) {
+      LevelMap &m = maps[L];
+      m.cells = zoomParticles[L - 1];
+      std::sort(m.cells.begin(), m.cells.end());
+      std::vector<std::vector<size_t>> f(L);
+      f[L - 1] = m.cells;
+      m.particles = flaggedParticles(int(L) - 1, f);
+      if (m.particles.size() != m.cells.size())
+        throw std::runtime_error("Consistency error when relating cells to particles");
+      int r = ratioInt(grids[L - 1].dx, grids[L].dx);
+      m.nhr = size_t(r) * r * r;
+      m.size1 = mapperSize(int(L) - 1);
+      m.total = m.size1 + (m.nhr - 1) * m.particles.size();
+      m.first2 = m.size1 - m.particles.size();
+      m.hires.clear();
+      m.hires.reserve(m.cells.size() * m.nhr);
+      for (size_t cid : m.cells) grids[L].cubeIds(grids[L - 1].centroid(cid), grids[L - 1].dx, m.hires);
+    }
+  }
+
+  // Cells of each level in output order: each two-level mapper emits the
+  // coarse mapper's particles minus the replaced ones, then the zoom cubes
+  // (twolevelmapper.hpp:252-319,869-892).
+  std::vector<std::vector<uint64_t>> levelOrder() const {
+    size_t nl = grids.size();
+    std::vector<std::vector<uint64_t>> lists(1);
+    lists[0].resize(grids[0].n3());
+    std::iota(lists[0].begin(), lists[0].end(), uint64_t(0));
+    for (size_t L = 1; L < nl; ++L) {
+      const LevelMap &m = maps[L];
+      size_t p = 0, k = 0;
+      for (auto &lst : lists) {
+        std::vector<uint64_t> kept;
+        kept.reserve(lst.size());
+        for (uint64_t c : lst) {
+          while (k < m.particles.size() && m.particles[k] < p) ++k;
+          if (!(k < m.particles.size() && m.particles[k] == p)) kept.push_back(c);
+          ++p;
+        }
+        lst.swap(kept);
+      }
+      lists.emplace_back(m.hires.begin(), m.hires.end());
+    }
+    return lists;
+  }
+
+  // --- selection (ic.hpp:1442-1523) ---
+  void select(const std::function<bool(double, double, double)> &inc) {
+    if (grids.empty()) throw std::runtime_error("No grid has yet been initialised");
+    for (auto &g : grids) g.flags.clear();
+    for (auto &g : grids) {
+      std::vector<size_t> ids;
+      for (size_t i = 0; i < g.n3(); ++i) {
+        auto p = g.centroid(i);
+        double dx = grids[0].wrappedOffset(p[0], x0), dy = grids[0].wrappedOffset(p[1], y0),
+               dz = grids[0].wrappedOffset(p[2], z0);
+        if (inc(dx, dy, dz)) ids.push_back(i);
+      }
+      g.flagCells(ids);
+    }
+  }
+  void selectSphere(float radius) {
+    double r2 = radius * radius;   // float product, as in the reference
+    select([r2](double a, double b, double c) { return a * a + b * b + c * c < r2; });
+  }
+  void selectCube(float side) {
+    double h = side / 2;
+    select([h](double a, double b, double c) { return std::fabs(a) < h && std::fabs(b) < h && std::fabs(c) < h; });
+  }
+  void selectEllipse(float a, float b, float c) {
+    double a2 = a * a, b2 = b * b, c2 = c * c;
+    select([=](double x, double y, double z) { return x * x / a2 + y * y / b2 + z * z / c2 < 1.; });
+  }
+  void selectNearest() {
+    if (grids.empty()) throw std::runtime_error("No grid has yet been initialised");
+    for (auto &g : grids) g.flags.clear();
+    Grid &g = grids.back();
+    double p[3] = {x0 - g.ox, y0 - g.oy, z0 - g.oz};
+    int c[3];
+    for (int d = 0; d < 3; ++d) {
+      c[d] = int(std::floor(g.wrap(p[d]) / g.dx));
+      if (c[d] < 0 || c[d] >= g.n) throw std::runtime_error("Point is outside the deepest grid");
+    }
+    g.flagCells({g.index(c[0], c[1], c[2])});
+  }
+  void centreOnFlagged() {
+    int level = -1;
+    for (int l = 0; l < int(grids.size()); ++l)
+      if (!grids[l].flags.empty()) level = l;
+    if (level < 0) throw std::runtime_error("Cannot calculate the center of an empty region");
+    auto c = grids[level].flaggedCentre();
+    x0 = c[0];
+    y0 = c[1];
+    z0 = c[2];
+    log() << "Centre of region is " << std::setprecision(12) << x0 << " " << y0 << " " << z0 << std::setprecision(6)
+          << std::endl;
+  }
+  // loadID / appendID (ic.hpp:1390-1428)
+  void loadIDFile(const std::string &fn, bool append) {
+    std::ifstream f(fn);
+    if (!f.is_open()) throw std::runtime_error("File " + fn + " not found");
+    std::vector<size_t> ids;
+    size_t v;
+    while (f >> v) ids.push_back(v);
+    std::sort(ids.begin(), ids.end());
+    ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
+    if (!append)
+      for (auto &g : grids) g.flags.clear();
+    buildMapper();
+    flagParticles(int(grids.size()) - 1, ids);
+    centreOnFlagged();
+  }
+  void dumpIDFile(const std::string &fn) {
+    buildMapper();
+    std::ofstream f(fn);
+    for (size_t id : flaggedParticles()) f << id << std::endl;
+  }
+  void setGadgetParticleType(unsigned t) {
+    if (t >= 6) throw std::runtime_error("Gadget particle type must be between 0 and 5");
+    if (t != mruGadgetType) {
+      if (usedGadgetType[t])
+        throw std::runtime_error(
+            "Cannot reuse a gadget particle type; this type was already used in a different section of the ICs");
+      mruGadgetType = t;
+    }
+    if (grids.empty())
+      throw std::runtime_error("Can't set a gadget particle type until a level has been initialised to set it for");
+    grids.back().gadgetType = t;
+  }
+
+  // --- generation ---
+  void ensureDrawn() {
+    if (grids.empty()) throw std::runtime_error("No grid has yet been initialised");
+    if (!drawn) {
+      check(gic_draw(c()));
+      drawn = true;
+    }
+  }
+  void zeroLevel(int level) {
+    log() << "*** Warning: your script calls zeroLevel(" << level << "). This is intended for testing purposes only!"
+          << std::endl;
+    if (level < 0 || level >= int(grids.size())) throw std::runtime_error("Specified level does not exist.");
+    ensureDrawn();
+    check(gic_zero_level(c(), level));
+  }
+  void generate() {
+    if (generated) return;
+    if (!haveCamb) throw std::runtime_error("No power spectrum has been set (use the camb command)");
+    ensureDrawn();
+    check(gic_apply_power_spectrum(c()));
+    check(gic_zeldovich(c()));
+    generated = true;
+  }
+  std::string outputPath() const {
+    std::ostringstream s;
+    if (outname.empty()) s << outdir << "/IC_" << (fp32 ? "float" : "doub") << "_z" << z << "_" << grids[0].n;
+    else s << outdir << "/" << outname;
+    return s.str();
+  }
+
+  // Particle blocks in mapper order: the DM levels, then (gadget) or before
+  // them (tipsy) the gas copy of the finest level when Ob > 0 (ic.hpp:1190-1214,
+  // AddGasMapper in particles/mapper/gasmapper.hpp; the split masses follow
+  // OneLevelParticleMapper::splitMass, onelevelmapper.hpp:155-173).
+  std::vector<Block> particleBlocks(bool gasFirst) {
+    buildMapper();
+    auto order = levelOrder();
+    size_t nl = grids.size();
+    std::vector<Block> dm(nl);
+    double massRatio = Ob / Om;
+    for (size_t l = 0; l < nl; ++l) {
+      Block &b = dm[l];
+      b.level = int(l);
+      b.type = grids[l].gadgetType;
+      b.cells = std::move(order[l]);
+      check(gic_particle_mass_eps(c(), int(l), &b.mass, nullptr));
+      b.eps = grids[l].dx * epsNorm;
+      if (Ob > 0 && l == nl - 1) b.mass *= 1.0 - massRatio;
+    }
+    std::vector<Block> out;
+    if (Ob > 0) {
+      Block g = dm[nl - 1];
+      g.gas = true;
+      g.type = 0;
+      check(gic_particle_mass_eps(c(), int(nl - 1), &g.mass, nullptr));
+      g.mass *= massRatio;
+      if (gasFirst) out.push_back(g);
+      for (auto &b : dm) out.push_back(std::move(b));
+      if (!gasFirst) out.push_back(g);
+    } else {
+      out = std::move(dm);
+    }
+    size_t first = 0;
+    for (auto &b : out) {
+      b.first = first;
+      first += b.count();
+      size_t m = b.count();
+      b.pos.resize(3 * m);
+      b.vel.resize(3 * m);
+      if (!m) continue;
+      bool identity = m == grids[b.level].n3();
+      for (size_t i = 0; identity && i < m; ++i) identity = b.cells[i] == i;
+      check(gic_particles(c(), b.level, identity ? nullptr : b.cells.data(), 0, m, b.pos.data(), b.vel.data()));
+      if (velOffset[0] != 0 || velOffset[1] != 0 || velOffset[2] != 0)
+        for (size_t i = 0; i < m; ++i)
+          for (int d = 0; d < 3; ++d) b.vel[3 * i + d] = float(b.vel[3 * i + d] + velOffset[d]);
+    }
+    return out;
+  }
+
+  void done() {
+    if (fmt == Fmt::unknown) throw std::runtime_error("No output format specified!");
+    generate();
+    switch (fmt) {
+      case Fmt::tipsy: {
+        auto blocks = particleBlocks(true);
+        logTotals(blocks);
+        writeTipsy(outputPath() + ".tipsy", blocks);
+        break;
+      }
+      case Fmt::gadget2:
+      case Fmt::gadget3: {
+        auto blocks = particleBlocks(false);
+        logTotals(blocks);
+        writeGadget(outputPath() + ".gadget", blocks, int(fmt));
+        break;
+      }
+      default:
+        throw std::runtime_error("Output format " + std::to_string(int(fmt)) + " is not supported by this build");
+    }
+    log() << "Finished writing initial conditions" << std::endl;
+  }
+  void logTotals(const std::vector<Block> &blocks) {
+    size_t ngas = 0, ndm = 0;
+    for (auto &b : blocks) (b.gas ? ngas : ndm) += b.count();
+    log() << "Writing output; number dm particles=" << ndm << ", number gas particles=" << ngas << std::endl;
+  }
+
+  // TipsyOutput::operator() (io/tipsy.hpp:207-266): gas first, masses scaled
+  // to sum to Om, positions in box units centred on 0.
+  void writeTipsy(const std::string &fn, const std::vector<Block> &blocks) {
+    double boxLength = grids[0].periodic;
+    double tot_mass = 0, min_mass = 1e300, max_mass = 0;
+    for (auto &b : blocks) {
+      for (size_t i = 0; i < b.count(); ++i) tot_mass += b.mass;
+      if (b.count()) {
+        min_mass = std::min(min_mass, b.mass);
+        max_mass = std::max(max_mass, b.mass);
+      }
+    }
+    double mass_factor = Om / tot_mass;
+    double pos_factor = 1. / boxLength;
+    double dKpcUnit = boxLength * 1000 / hubble;
+    double dMsolUnit = 1e10 / hubble / mass_factor;
+    double dKmsUnit = std::sqrt(4.3022682e-6 * dMsolUnit / dKpcUnit);
+    double vel_factor = std::pow(a(), -0.5) / dKmsUnit;
+    {
+      std::ofstream pf("tipsy.param");
+      pf << "dKpcUnit = " << dKpcUnit << std::endl;
+      pf << "dMsolUnit = " << dMsolUnit << std::endl;
+      pf << "dHubble0 = " << 0.1 * hubble * dKpcUnit / dKmsUnit << std::endl;
+      pf << "bComove = 1 " << std::endl;
+    }
+    std::ofstream photo;
+    if (min_mass != max_mass) photo.open("photogenic.txt");
+    std::ofstream f(fn, std::ios::binary);
+    if (!f) throw std::runtime_error("Can't open file " + fn);
+    size_t total = 0, ngas = 0;
+    for (auto &b : blocks) {
+      total += b.count();
+      if (b.gas) ngas += b.count();
+    }
+    // header {double a; int n, ndim, ngas, ndark, nstar;} padded to 32 bytes
+    unsigned char hdr[32];
+    memset(hdr, 0, sizeof hdr);
+    double sf = a();
+    int ints[5] = {int(total), 3, int(ngas), int(total - ngas), 0};
+    memcpy(hdr, &sf, 8);
+    memcpy(hdr + 8, ints, 20);
+    f.write((const char *)hdr, 32);
+    // gas temperature (cosmology/parameters.hpp:37-42), decoupling at a = 1/150
+    double aDec = 1. / 150.;
+    double temp = sf > aDec ? TCMB * aDec / (sf * sf) : TCMB / sf;
+    std::vector<float> rec;
+    size_t iord = 0;
+    for (auto &b : blocks) {
+      size_t m = b.count();
+      int w = b.gas ? 12 : 9;   // gas: mass x y z vx vy vz rho temp eps metals phi
+      rec.assign(size_t(w) * m, 0.f);
+      float mass = float(b.mass * mass_factor), eps = float(b.eps * pos_factor);
+      for (size_t i = 0; i < m; ++i) {
+        float *r = &rec[size_t(w) * i];
+        r[0] = mass;
+        for (int d = 0; d < 3; ++d) {
+          r[1 + d] = float(b.pos[3 * i + d] * pos_factor - 0.5);
+          r[4 + d] = float(b.vel[3 * i + d] * vel_factor);
+        }
+        if (b.gas) {
+          r[8] = float(temp);
+          r[9] = eps;
+        } else {
+          r[7] = eps;
+        }
+      }
+      f.write((const char *)rec.data(), rec.size() * 4);
+      if (photo.is_open() && b.mass == min_mass)
+        for (size_t i = 0; i < m; ++i) photo << iord + i << std::endl;
+      iord += m;
+    }
+  }
+
+  // GadgetOutput (io/gadget.hpp:26-232, 238-557): particles grouped by gadget
+  // type, IDs = mapper index, masses in the header unless they vary.
+  void writeGadget(const std::string &name, const std::vector<Block> &blocks, int version) {
+    std::vector<const Block *> seq;   // file order: per type, its blocks in mapper order
+    size_t npType[6] = {0, 0, 0, 0, 0, 0};
+    double mmin[6], mmax[6];
+    for (int t = 0; t < 6; ++t) {
+      mmin[t] = 1e300;
+      mmax[t] = 0;
+      for (auto &b : blocks)
+        if (int(b.type) == t && b.count()) {
+          seq.push_back(&b);
+          npType[t] += b.count();
+          mmin[t] = std::min(mmin[t], b.mass);
+          mmax[t] = std::max(mmax[t], b.mass);
+        }
+    }
+    bool variable = false;
+    double masses[6] = {0, 0, 0, 0, 0, 0};
+    size_t nTotal = 0;
+    log() << "Particles by gadget type:" << std::endl;
+    for (int t = 0; t < 6; ++t)
+      if (npType[t]) {
+        if (mmin[t] != mmax[t]) variable = true;
+        masses[t] = mmin[t];
+        nTotal += npType[t];
+        log() << "   Particle type " << t << ": " << npType[t] << " particles" << std::endl;
+      }
+    if (variable) {
+      log() << "Using variable-mass gadget format" << std::endl;
+      for (double &m : masses) m = 0;
+    } else {
+      log() << "Using fixed-mass gadget format" << std::endl;
+    }
+    int nFiles = nGadgetFiles;
+    std::vector<size_t> perFile;
+    for (int i = 0; i < nFiles; ++i)
+      perFile.push_back(i == nFiles - 1 ? nTotal - (nTotal / nFiles) * (nFiles - 1) : nTotal / nFiles);
+    // per-file per-type counts (gadget.hpp:415-447)
+    std::vector<std::array<size_t, 6>> typePerFile;
+    size_t remaining[6];
+    std::copy(npType, npType + 6, remaining);
+    for (int i = 0; i < nFiles; ++i) {
+      size_t left = perFile[i];
+      std::array<size_t, 6> c{};
+      for (int t = 0; t < 6; ++t) {
+        c[t] = std::min(remaining[t], left);
+        left -= c[t];
+        remaining[t] -= c[t];
+      }
+      typePerFile.push_back(c);
+    }
+    // walk the file-order sequence
+    size_t bi = 0, bo = 0;
+    auto next = [&](const Block *&b, size_t &i) {
+      while (bo >= seq[bi]->count()) {
+        ++bi;
+        bo = 0;
+      }
+      b = seq[bi];
+      i = bo++;
+    };
+    for (int fi = 0; fi < nFiles; ++fi) {
+      std::string fn = name + std::to_string(version) + (nFiles == 1 ? "" : "." + std::to_string(fi));
+      std::ofstream f(fn, std::ios::binary);
+      if (!f) throw std::runtime_error("Can't open file " + fn);
+      auto rec = [&](const void *p, uint32_t n) {
+        f.write((const char *)&n, 4);
+        f.write((const char *)p, n);
+        f.write((const char *)&n, 4);
+      };
+      unsigned char hdr[256];
+      memset(hdr, 0, sizeof hdr);
+      size_t o = 0;
+      auto put = [&](const void *p, size_t n) {
+        memcpy(hdr + o, p, n);
+        o += n;
+      };
+      int npart[6];
+      unsigned int nall[6], nhigh[6];
+      for (int t = 0; t < 6; ++t) {
+        npart[t] = int(typePerFile[fi][t]);
+        nall[t] = (unsigned)npType[t];
+        nhigh[t] = (unsigned)(uint64_t(npType[t]) >> 32);
+      }
+      int baryonFlag = npType[0] > 0 ? 1 : 0, zero = 0;
+      double time = a(), redshift = z, box = grids[0].periodic, om = Om, ol = Ol, hh = hubble;
+      put(npart, 24);
+      put(masses, 48);
+      put(&time, 8);
+      put(&redshift, 8);
+      put(&baryonFlag, 4);   // flag_sfr
+      put(&baryonFlag, 4);   // flag_feedback
+      put(nall, 24);
+      put(&baryonFlag, 4);   // flag_cooling
+      put(&nFiles, 4);
+      put(&box, 8);
+      put(&om, 8);
+      put(&ol, 8);
+      put(&hh, 8);
+      put(&zero, 4);   // flag_stellarage
+      put(&zero, 4);   // flag_metals
+      put(nhigh, 24);
+      put(&zero, 4);   // flag_entropy_instead_u
+      if (version == 3) {
+        put(&zero, 4);   // flag_doubleprecision: float output
+        int icInfo = 1;
+        put(&icInfo, 4);
+        float lpt = 0.f;
+        put(&lpt, 4);
+      }
+      rec(hdr, 256);
+      size_t nf = perFile[fi];
+      std::vector<float> pos(3 * nf), vel(3 * nf), mass(nf);
+      std::vector<uint64_t> ids(nf);
+      for (size_t j = 0; j < nf; ++j) {
+        const Block *b;
+        size_t i;
+        next(b, i);
+        for (int d = 0; d < 3; ++d) {
+          pos[3 * j + d] = b->pos[3 * i + d];
+          vel[3 * j + d] = b->vel[3 * i + d];
+        }
+        ids[j] = b->first + i;
+        mass[j] = float(b->mass);
+      }
+      rec(pos.data(), uint32_t(pos.size() * 4));
+      rec(vel.data(), uint32_t(vel.size() * 4));
+      rec(ids.data(), uint32_t(nf * 8));
+      if (variable) rec(mass.data(), uint32_t(nf * 4));
+      if (Ob > 0) {   // dummy internal energy of this file's gas particles
+        std::vector<float> u(typePerFile[fi][0], 0.f);
+        rec(u.data(), uint32_t(u.size() * 4));
+      }
+    }
+  }
+
+  // --- diagnostics ---
+  // dumpGridData (ic.hpp:852-883)
+  void dumpGrid(int level) {
+    if (level < 0 || level >= int(grids.size())) throw std::runtime_error("Specified level does not exist.");
+    ensureDrawn();
+    int n = grids[level].n;
+    std::vector<double> d(size_t(n) * n * n);
+    check(gic_copy_field(c(), GIC_FIELD_OVERDENSITY, level, GIC_REAL_SPACE, d.data(), GIC_DTYPE_F64));
+    saveNpy(outdir + "/grid-" + std::to_string(level) + ".npy", n, d);
+    std::ofstream inf(outdir + "/grid-info-" + std::to_string(level) + ".txt");
+    const Grid &g = grids[level];
+    inf << g.ox << " " << g.oy << " " << g.oz << " " << g.size() << std::endl;
+    inf << "The line above contains information about grid level " << level << std::endl;
+    inf << "It gives the x-offset, y-offset and z-offset of the low-left corner and also the box length" << std::endl;
+  }
+
+  // dumpPS / dumpPowerSpectrum (ic.hpp:943-963, cosmology/parameters.hpp:112-205):
+  // 100 log bins between 2pi/L and pi N/L; the GPU returns per-|k|^2 sums.
+  void dumpPS(int level) {
+    if (level < 0 || level >= int(grids.size())) throw std::runtime_error("Specified level does not exist.");
+    ensureDrawn();
+    const Grid &g = grids[level];
+    long h = g.n / 2;
+    size_t nm = size_t(3 * h * h + 1);
+    std::vector<double> sum(nm), cnt(nm), theory(nm);
+    check(gic_mode_power(c(), level, sum.data(), cnt.data(), theory.data(), nm));
+    const int nBins = 100;
+    std::vector<double> inBin(nBins, 0), kbin(nBins, 0), Gx(nBins, 0), Px(nBins, 0);
+    double boxLength = g.size();
+    double kmax = M_PI / boxLength * double(g.n), kmin = 2.0f * M_PI / boxLength;
+    double dklog = std::log10(kmax / kmin) / nBins, kw = 2.0f * M_PI / boxLength;
+    for (size_t m = 0; m < nm; ++m) {
+      if (cnt[m] == 0) continue;
+      double k = std::sqrt(double(m)) * kw;
+      int idx = (int)((1.0f / dklog * std::log10(k / kmin)));
+      if (k >= kmin && k < kmax) {
+        Gx[idx] += sum[m];
+        Px[idx] += cnt[m] * theory[m];
+        kbin[idx] += cnt[m] * k;
+        inBin[idx] += cnt[m];
+      }
+    }
+    double kwn = 2. * M_PI / boxLength;
+    double psnorm = 1 / (kwn * kwn * kwn / std::pow(2.f * M_PI, 3.f) * double(g.n3()));
+    std::string fn = outputPath() + "_" + char('0' + level) + ".ps";
+    std::ofstream ofs(fn);
+    for (int i = 0; i < nBins; ++i) {
+      if (inBin[i] > 0) {
+        ofs << std::setw(16) << std::pow(10., std::log10(kmin) + dklog * (i + 0.5)) << std::setw(16)
+            << kbin[i] / inBin[i] << std::setw(16) << (Px[i] / inBin[i]) * psnorm << std::setw(16)
+            << (Gx[i] / inBin[i]) * psnorm << std::setw(16) << inBin[i] << std::endl;
+      }
+    }
+  }
+
+  // saveFieldTipsyArray (io/tipsy.hpp:38-52): the overdensity at every
+  // particle in mapper order, as float, after a 4-byte count
+  void dumpTipsyArray(const std::string &fn) {
+    generate();
+    buildMapper();
+    auto order = levelOrder();
+    size_t total = 0;
+    for (auto &o : order) total += o.size();
+    size_t ngas = Ob > 0 ? order.back().size() : 0;
+    std::ofstream f(fn, std::ios::binary);
+    uint32_t len = uint32_t(total + ngas);
+    f.write((const char *)&len, 4);
+    auto emit = [&](size_t l) {
+      int n = grids[l].n;
+      std::vector<double> d(size_t(n) * n * n);
+      check(gic_copy_field(c(), GIC_FIELD_OVERDENSITY, int(l), GIC_REAL_SPACE, d.data(), GIC_DTYPE_F64));
+      std::vector<float> v(order[l].size());
+      for (size_t i = 0; i < v.size(); ++i) v[i] = float(d[order[l][i]]);
+      f.write((const char *)v.data(), v.size() * 4);
+    };
+    bool gasFirst = fmt == Fmt::tipsy;
+    if (ngas && gasFirst) emit(order.size() - 1);
+    for (size_t l = 0; l < order.size(); ++l) emit(l);
+    if (ngas && !gasFirst) emit(order.size() - 1);
+  }
+};
+
+// ---------------------------------------------------------------- dispatch -
+// Argument parsing with the reference's rules (tools/parser.hpp:40-105):
+// "Insufficient arguments" at the end of the line, "Error while parsing
+// arguments" on a failed conversion, "Too many arguments" for trailing
+// non-comment text -- all checked before the command runs.
+template <typename T> T parseArg(std::istream &in) {
+  T v{};
+  if (in.eof()) throw std::runtime_error("Insufficient arguments");
+  in >> v;
+  if (in.fail()) throw std::runtime_error("Error while parsing arguments");
+  return v;
+}
+
+void consumeComments(std::istream &in) {
+  std::string s;
+  in >> s;
+  if (s[0] == '#' || s[0] == '%') {
+    while (!in.eof()) in >> s;
+  } else if (!s.empty()) {
+    throw std::runtime_error("Too many arguments");
+  }
+}
+
+struct Dispatcher {
+  Generator &ic;
+  std::ostream &echo;   // IC_output.params: the commands as parsed
+  std::map<std::string, std::function<void(std::istream &, const std::string &)>> routes;
+
+  template <typename... A> void add(const std::string &name, std::function<void(A...)> f) {
+    std::string k = name;
+    std::transform(k.begin(), k.end(), k.begin(), ::tolower);
+    std::ostream *out = &echo;
+    routes[k] = [f, out](std::istream &in, const std::string &cmd) {
+      std::tuple<A...> args{parseArg<A>(in)...};   // braced init: left-to-right
+      consumeComments(in);
+      *out << cmd << " ";
+      std::apply([out](const A &...v) { ((*out << v << " "), ...); }, args);
+      *out << std::endl;
+      std::apply(f, args);
+    };
+  }
+
+  Dispatcher(Generator &g, std::ostream &e) : ic(g), echo(e) {
+    using D = std::function<void(double)>;
+    auto cosmo = [this](double Generator::*field) {
+      return D([this, field](double v) {
+        ic.*field = v;
+        ic.pushCosmology();
+      });
+    };
+    add("TCMB", cosmo(&Generator::TCMB));
+    add("Om", cosmo(&Generator::Om));
+    add("Ob", cosmo(&Generator::Ob));
+    add("Ol", cosmo(&Generator::Ol));
+    add("hubble", cosmo(&Generator::hubble));
+    add("s8", cosmo(&Generator::s8));
+    add("ns", cosmo(&Generator::ns));
+    add("zin", cosmo(&Generator::z));
+    add("z", cosmo(&Generator::z));
+    auto seed = [this](int mode) {
+      return std::function<void(int)>([this, mode](int s) { ic.check(gic_seed(ic.c(), (unsigned long)s, mode)); });
+    };
+    add("seed", seed(GIC_SEED_REAL_SPACE));
+    add("random_seed_real_space", seed(GIC_SEED_REAL_SPACE));
+    add("seedfourier", seed(GIC_SEED_FOURIER_SERIAL));
+    add("random_seed_serial", seed(GIC_SEED_FOURIER_SERIAL));
+    add("seedfourier_parallel", seed(GIC_SEED_FOURIER_PARALLEL));
+    add("random_seed", seed(GIC_SEED_FOURIER_PARALLEL));
+    add("seedfourier_reverse", seed(GIC_SEED_FOURIER_REVERSE));
+    add("seed_field_fourier_reverse", seed(GIC_SEED_FOURIER_REVERSE));
+    using S = std::function<void(std::string)>;
+    add("camb", S([this](std::string f) { ic.setCamb(f); }));
+    add("outdir", S([this](std::string s) { ic.outdir = s; }));
+    add("outname", S([this](std::string s) { ic.outname = s; }));
+    add("outformat", S([this](std::string s) {
+          // io::OutputFormat is read as a number or as a name (io.hpp)
+          static const std::map<std::string, Fmt> names{
+              {"gadget2", Fmt::gadget2}, {"gadget3", Fmt::gadget3},     {"tipsy", Fmt::tipsy},
+              {"grafic", Fmt::grafic},   {"gadgethdf", Fmt::gadgethdf}, {"swift", Fmt::swift}};
+          char *end = nullptr;
+          long v = std::strtol(s.c_str(), &end, 10);
+          if (!s.empty() && end && *end == 0) {
+            if (v < 1 || v > 7) throw std::runtime_error("Unknown output format");
+            ic.fmt = Fmt(v);
+          } else {
+            std::string l = s;
+            std::transform(l.begin(), l.end(), l.begin(), ::tolower);
+            auto it = names.find(l);
+            if (it == names.end()) throw std::runtime_error("Error while parsing arguments");
+            ic.fmt = it->second;
+          }
+        }));
+    std::function<void(double, int)> base = [this](double box, int n) { ic.initBaseGrid(box, n); };
+    add("base_grid", base);
+    add("basegrid", base);
+    std::function<void(int, int)> zoom = [this](int f, int n) { ic.initZoomGrid(f, n); };
+    add("zoom_grid", zoom);
+    add("zoomgrid", zoom);
+    std::function<void(int, int, int, int, int)> zoomAt = [this](int x, int y, int z, int f, int n) {
+      ic.initZoomGridWithLowLeftCornerAt(x, y, z, f, n);
+    };
+    add("zoom_grid_with_origin_at", zoomAt);
+    add("zoomgrid_with_origin_at", zoomAt);
+    std::function<void(double, double, double)> centre = [this](double x, double y, double z) {
+      ic.x0 = x;
+      ic.y0 = y;
+      ic.z0 = z;
+    };
+    add("centre", centre);
+    add("center", centre);
+    std::function<void(long)> centreOn = [this](long id) {
+      if (ic.grids.empty()) throw std::runtime_error("No grid has yet been initialised");
+      if (id < 0 || size_t(id) >= ic.grids[0].n3()) throw std::runtime_error("Particle ID out of range");
+      auto p = ic.grids[0].centroid(size_t(id));
+      ic.x0 = p[0];
+      ic.y0 = p[1];
+      ic.z0 = p[2];
+    };
+    add("centre_on", centreOn);
+    add("center_on", centreOn);
+    using F = std::function<void(float)>;
+    add("select_sphere", F([this](float r) { ic.selectSphere(r); }));
+    add("select_cube", F([this](float s) { ic.selectCube(s); }));
+    add("select_ellipse",
+        std::function<void(float, float, float)>([this](float a, float b, float c) { ic.selectEllipse(a, b, c); }));
+    add("select_nearest", std::function<void()>([this] { ic.selectNearest(); }));
+    S idfile = [this](std::string f) { ic.loadIDFile(f, false); };
+    S mergeid = [this](std::string f) { ic.loadIDFile(f, true); };
+    add("id_file", idfile);
+    add("idfile", idfile);
+    add("merge_id_file", mergeid);
+    add("append_idfile", mergeid);
+    S dumpId = [this](std::string f) { ic.dumpIDFile(f); };
+    add("dump_id_file", dumpId);
+    add("dump_idfile", dumpId);
+    std::function<void(int)> zl = [this](int l) { ic.zeroLevel(l); };
+    add("zero_level", zl);
+    add("zerolevel", zl);
+    add("eps_norm", D([this](double v) { ic.epsNorm = v; }));
+    add("gadget_num_files", std::function<void(int)>([this](int n) {
+          if (n < 1) throw std::runtime_error("Number of gadget files must be positive");
+          ic.nGadgetFiles = n;
+        }));
+    add("gadget_particle_type", std::function<void(int)>([this](int t) { ic.setGadgetParticleType(unsigned(t)); }));
+    add("velocity_offset", std::function<void(double, double, double)>([this](double vx, double vy, double vz) {
+          double c = std::pow(ic.a(), -0.5);   // km/s -> gadget units (ic.hpp:285-289)
+          ic.velOffset[0] = vx * c;
+          ic.velOffset[1] = vy * c;
+          ic.velOffset[2] = vz * c;
+        }));
+    add("done", std::function<void()>([this] { ic.done(); }));
+    add("dump_grid", std::function<void(int)>([this](int l) { ic.dumpGrid(l); }));
+    add("dump_ps", std::function<void(int)>([this](int l) { ic.dumpPS(l); }));
+    add("dump_tipsy", S([this](std::string f) { ic.dumpTipsyArray(f); }));
+  }
+
+  // ClassDispatch::run (tools/parser.hpp:159-185)
+  void run(std::istream &in) {
+    std::string s;
+    in >> s;
+    if (s[0] == '#' || s[0] == '%') return;
+    std::transform(s.begin(), s.end(), s.begin(), ::tolower);
+    auto it = routes.find(s);
+    if (it == routes.end()) throw std::runtime_error("Unknown command");
+    it->second(in, s);
+  }
+
+  // run_loop (tools/parser.hpp:189-210)
+  void runLoop(std::istream &in) {
+    std::string str;
+    int line = 0;
+    while (std::getline(in, str)) {
+      line += 1;
+      if (str.empty()) continue;
+      std::istringstream ss(str);
+      try {
+        run(ss);
+      } catch (std::runtime_error &e) {
+        log() << "Error \"" << e.what() << "\" on line " << line << " (\"" << str << "\")" << std::endl;
+        exit(1);
+      }
+    }
+  }
+};
+
+}  // namespace fe
+
+int main(int argc, char **argv) {
+  bool f32 = false;
+  std::string fname;
+  for (int i = 1; i < argc; ++i) {
+    std::string a = argv[i];
+    if (a == "-f") f32 = true;
+    else if (a == "-c" && i + 1 < argc) ++i;   // transfer-function cache size: no cache to size here
+    else fname = a;
+  }
+  if (fname.empty()) {
+    std::cout << "Usage: genetIC-hip paramfile [-f]" << std::endl << std::endl
+              << " The paramfile is a text file of genetIC commands." << std::endl
+              << " With -f the fields are held in float (32-bit) instead of double (64-bit)." << std::endl;
+    return 0;
+  }
+  std::ifstream in(fname);
+  if (!in.is_open()) {
+    fe::log() << "Error: could not open parameter file " << fname << std::endl;
+    return 255;
+  }
+  // commands run relative to the paramfile's directory (main.cpp:205-213)
+  std::string dir = fname.find('/') == std::string::npos ? "." : fname.substr(0, fname.rfind('/'));
+  if (chdir(dir.c_str()) != 0) {
+    fe::log() << "Error: cannot change directory to " << dir << std::endl;
+    return 1;
+  }
+  std::ofstream echo("IC_output.params");
+  if (!echo.is_open()) {
+    fe::log() << "Error: could not open output file" << std::endl;
+    return 1;
+  }
+  try {
+    fe::Generator gen(f32);
+    fe::Dispatcher d(gen, echo);
+    d.runLoop(in);
+  } catch (std::exception &e) {
+    fe::log() << "Error \"" << e.what() << "\"" << std::endl;
+    return 1;
+  }
+  return 0;
+}

@@ -88,6 +88,13 @@ int gic_copy_field(gic_context *ctx, int field, int level, int space, void *host
 /* Overwrite a field (import_level, ic.hpp:809-821, and tests). */
 int gic_set_field(gic_context *ctx, int field, int level, int space, const void *host, int dtype);
 
+/* Power-spectrum histogram for dump_ps (dumpPowerSpectrum,
+ * cosmology/parameters.hpp:112-205): over every (ix,iy,iz) in [-n/2,n/2]^3,
+ * sum_power[m] += |delta_k|^2 and count[m] += 1 with m = ix^2+iy^2+iz^2;
+ * theory[m] = P(k_m) n^3/V, the covariance the reference bins beside it.
+ * nm must be 3(n/2)^2+1.  The binning into log-k bins is host work. */
+int gic_mode_power(gic_context *ctx, int level, double *sum_power, double *count, double *theory, size_t nm);
+
 /* Particle evaluation (ZeldovichParticleEvaluator::getParticle,
  * zeldovich.hpp:87-109, generator.hpp:49-53): for cells[i] (or start+i when
  * cells is NULL) of a level, pos = centroid + psi wrapped into the box

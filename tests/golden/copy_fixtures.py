@@ -24,6 +24,14 @@ FILES = [
     "test_01c_zoom_grid/reference_grid/grid-info-1.txt",
     "test_01c_zoom_grid/reference_grid/test_2_0.ps",
     "test_01c_zoom_grid/reference_grid/test_2_1.ps",
+    "test_01c_zoom_grid/zoomtest.txt",
+    "test_01c_zoom_grid/zoomtest_crosscheck.txt",
+    "test_23_doublezoom_wrap_issue_90/paramfile.txt",
+    "test_23_doublezoom_wrap_issue_90/reference_output",
+    "test_25_zoom_factor_1/paramfile.txt",
+    "test_25_zoom_factor_1/reference_output",
+    "test_01g_zoom_grid_vel_offset/paramfile.txt",
+    "test_01g_zoom_grid_vel_offset/reference_output",
     "test_01d_two_zooms/paramfile.txt",
     "test_01d_two_zooms/reference_grid/grid-2.npy",
     "test_01d_two_zooms/reference_grid/grid-info-0.txt",

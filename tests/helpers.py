@@ -47,7 +47,8 @@ def fortran_blocks(raw):
 
 
 def read_gadget(path):
-    """Single-file gadget2/3 snapshot with fixed masses: header fields, pos, vel, ids."""
+    """Single-file gadget2/3 snapshot: header fields, pos, vel, ids and, when
+    the header masses are zero for a present type, the per-particle masses."""
     b = fortran_blocks(open(path, "rb").read())
     hdr = b[0]
     npart = struct.unpack("<6i", hdr[:24])
@@ -56,17 +57,33 @@ def read_gadget(path):
     pos = np.frombuffer(b[1], dtype="<f4").reshape(-1, 3)
     vel = np.frombuffer(b[2], dtype="<f4").reshape(-1, 3)
     ids = np.frombuffer(b[3], dtype="<u8")
-    return dict(npart=npart, mass=mass, time=time, redshift=redshift, header=hdr, pos=pos, vel=vel, ids=ids)
+    variable = any(n > 0 and m == 0 for n, m in zip(npart, mass))
+    if variable:
+        pmass = np.frombuffer(b[4], dtype="<f4")
+    else:
+        pmass = np.concatenate([np.full(n, m, dtype=np.float64) for n, m in zip(npart, mass)])
+    return dict(npart=npart, mass=mass, time=time, redshift=redshift, header=hdr, pos=pos, vel=vel, ids=ids,
+                pmass=pmass, nblocks=len(b))
 
 
 TIPSY_DARK = np.dtype([("mass", "<f4"), ("pos", "<f4", 3), ("vel", "<f4", 3), ("eps", "<f4"), ("phi", "<f4")])
+TIPSY_GAS = np.dtype([("mass", "<f4"), ("pos", "<f4", 3), ("vel", "<f4", 3), ("rho", "<f4"), ("temp", "<f4"),
+                      ("eps", "<f4"), ("metals", "<f4"), ("phi", "<f4")])
 
 
 def read_tipsy(path):
     raw = open(path, "rb").read()
     a, n, ndim, ngas, ndark, nstar = struct.unpack("<diiiii", raw[:28])
+    gas = np.frombuffer(raw[32:32 + 48 * ngas], dtype=TIPSY_GAS)
     dark = np.frombuffer(raw[32 + 48 * ngas:32 + 48 * ngas + 36 * ndark], dtype=TIPSY_DARK)
-    return dict(a=a, n=n, ngas=ngas, ndark=ndark, nstar=nstar, dark=dark)
+    return dict(a=a, n=n, ngas=ngas, ndark=ndark, nstar=nstar, dark=dark, gas=gas, size=len(raw))
+
+
+def read_tipsy_array(path):
+    """Field array written by dump_tipsy: 4-byte count, then float32 values."""
+    raw = open(path, "rb").read()
+    n = struct.unpack("<i", raw[:4])[0]
+    return np.frombuffer(raw[4:4 + 4 * n], dtype="<f4")
 
 
 def read_ps(path):
@@ -78,3 +95,75 @@ def assert_decimal(actual, desired, decimal, what=""):
     abs(desired-actual) < 1.5 * 10**(-decimal)."""
     err = np.max(np.abs(np.asarray(actual, dtype=np.float64) - np.asarray(desired, dtype=np.float64)))
     assert err < 1.5 * 10.0 ** (-decimal), "%s: max abs error %g exceeds decimal=%d" % (what, err, decimal)
+
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FRONTEND = os.path.join(ROOT, "genetic_amd", "genetIC-hip")
+_CAMBS = ("camb_transfer_kmax40_z0.dat", "camb_transfer_kmax40_z0_post2015.dat", "camb_transfer_kmax40_z500.dat")
+
+
+def stage_test(test, tmp_path, paramfile_text=None):
+    """Lay out a reference test directory (paramfile and side files) under
+    tmp_path with the CAMB tables one level up, as in genetIC/tests."""
+    import shutil
+    for c in _CAMBS:
+        shutil.copyfile(golden(c), os.path.join(str(tmp_path), c))
+    d = os.path.join(str(tmp_path), test)
+    os.makedirs(d, exist_ok=True)
+    src = golden(test)
+    if os.path.isdir(src):
+        for f in os.listdir(src):
+            p = os.path.join(src, f)
+            if os.path.isfile(p) and not f.startswith("reference"):
+                shutil.copyfile(p, os.path.join(d, f))
+    if paramfile_text is not None:
+        with open(os.path.join(d, "paramfile.txt"), "w") as f:
+            f.write(paramfile_text)
+    return d
+
+
+def run_frontend(testdir, args=(), timeout=300):
+    """Run genetIC-hip on testdir/paramfile.txt; returns (returncode, stderr
+    with the log time stamps stripped)."""
+    import re
+    import subprocess
+    if not os.path.exists(FRONTEND):
+        raise RuntimeError("genetIC-hip is not built (%s); run __graft_entry__.build() first" % FRONTEND)
+    p = subprocess.run([FRONTEND, os.path.join(testdir, "paramfile.txt")] + list(args), capture_output=True,
+                       text=True, timeout=timeout)
+    err = "\n".join(re.sub(r"^ [0-9]{4}-[0-9]{2}-[0-9]{2} [0-9]{2}:[0-9]{2}:[0-9]{2}   ", "", ln)
+                    for ln in p.stderr.splitlines())
+    return p.returncode, err
+
+
+def compare_particles(test_file, ref_file):
+    """tools/compare.py:compare for tipsy or gadget output written as float:
+    mass and eps to 6 decimals, pos and vel to 4, gadget IDs exactly."""
+    if test_file.endswith(".tipsy"):
+        a, b = read_tipsy(test_file), read_tipsy(ref_file)
+        assert (a["ngas"], a["ndark"], a["nstar"]) == (b["ngas"], b["ndark"], b["nstar"])
+        assert a["size"] == b["size"]
+        for fam in ("gas", "dark"):
+            x, y = a[fam], b[fam]
+            assert len(x) == len(y)
+            if len(x) == 0:
+                continue
+            assert_decimal(x["mass"], y["mass"], 6, fam + " mass")
+            assert_decimal(x["eps"], y["eps"], 6, fam + " eps")
+            assert_decimal(x["vel"], y["vel"], 4, fam + " vel")
+            assert_decimal(x["pos"], y["pos"], 4, fam + " pos")
+        if len(a["gas"]):
+            assert_decimal(a["gas"]["temp"], b["gas"]["temp"], 4, "gas temp")
+    else:
+        a, b = read_gadget(test_file), read_gadget(ref_file)
+        assert a["npart"] == b["npart"]
+        assert a["nblocks"] == b["nblocks"]
+        assert_decimal(a["pmass"], b["pmass"], 6, "mass")
+        assert_decimal(a["vel"], b["vel"], 4, "vel")
+        assert_decimal(a["pos"], b["pos"], 4, "pos")
+        np.testing.assert_array_equal(a["ids"], b["ids"])
+        # defined header fields; the reference leaves the trailing fill bytes
+        # uninitialised, and the mass table is compared as numbers
+        ha, hb = a["header"], b["header"]
+        assert ha[:24] == hb[:24] and ha[72:208] == hb[72:208]
+        np.testing.assert_allclose(a["mass"], b["mass"], rtol=1e-12)
