@@ -315,7 +315,59 @@ template <typename T> void do_draw(gic_context *c) {
       L->dc_pending = true;
     }
   } else {
-    throw std::runtime_error("Fourier-space seeding is not implemented on the GPU path yet");
+    // Fourier-space draws: the Gaussians of every level in draw order into a
+    // dense buffer (one stream continued across levels, or in parallel mode a
+    // stream seeded seed+ks per shell and level), then placed mode by mode.
+    const double sigma = double(T(1.0 / sqrt(2.0)));
+    bool parallel = c->seed_mode == GIC_SEED_FOURIER_PARALLEL;
+    int reverse = c->seed_mode == GIC_SEED_FOURIER_REVERSE ? 1 : 0;
+    std::vector<size_t> off;
+    size_t total = 0, maxslots = 0;
+    for (auto &L : c->levels) {
+      off.push_back(total);
+      total += 2 * size_t(fourier_draw_count(L->g.n));
+      maxslots = std::max(maxslots, size_t(L->g.n) * L->g.n * (L->g.n / 2 + 1));
+    }
+    size_t gbytes = (total * sizeof(T) + 255) / 256 * 256;
+    c->stage.ensure(gbytes + maxslots * 4);
+    T *gauss = (T *)c->stage.p;
+    uint32_t *owner = (uint32_t *)((char *)c->stage.p + gbytes);
+    if (!parallel) {
+      if (total) {
+        RngPlan plan;
+        rng_plan(c->seed, total, plan);
+        c->work.ensure(rng_workspace_bytes(plan));
+        std::vector<RngLevelOut> outs{{gauss, 0, total}};
+        rng_draw_real<T>(plan, outs, c->work.p, c->work.bytes, c->st, c->hook(), sigma);
+      }
+    } else {
+      int hmax = 0;
+      for (auto &L : c->levels) hmax = std::max(hmax, L->g.n / 2);
+      if (hmax > 1) {
+        RngPlan big;
+        rng_plan(c->seed, 24 * size_t(hmax - 1) * (hmax - 1), big);
+        c->work.ensure(rng_workspace_bytes(big));
+      }
+      for (size_t l = 0; l < c->levels.size(); ++l) {
+        int h = c->levels[l]->g.n / 2;
+        size_t pos = off[l];
+        for (int ks = 1; ks < h; ++ks) {
+          size_t cnt = 24 * size_t(ks) * ks;
+          RngPlan plan;
+          rng_plan(c->seed + ks, cnt, plan);
+          c->work.ensure(rng_workspace_bytes(plan));
+          std::vector<RngLevelOut> outs{{gauss + pos, 0, cnt}};
+          rng_draw_real<T>(plan, outs, c->work.p, c->work.bytes, c->st, c->hook(), sigma);
+          pos += cnt;
+        }
+      }
+    }
+    for (size_t l = 0; l < c->levels.size(); ++l) {
+      Level &L = *c->levels[l];
+      fourier_draw_scatter<T>(gauss + off[l], L.g.n, (T *)L.noise.p, owner, reverse, c->st);
+      L.noise_fourier = true;
+      L.dc_pending = false;
+    }
   }
   c->drawn = true;
 }
@@ -917,20 +969,14 @@ int gic_raw_stream(gic_context *c, unsigned long seed, uint32_t *host_out, size_
 
 int gic_gaussian_stream(gic_context *c, unsigned long seed, double *host_out, size_t n) {
   return guarded(c, [&] {
-    // draw into a dense pseudo-level: n values as rows of a padded field
-    // of width 1 are not possible, so use an (n x 1) layout through a
-    // temporary 1-level target of size m^3 >= n and copy the first n.
-    int m = 4;
-    while (size_t(m) * m * m < n) m *= 2;
+    if (n == 0) return;
     RngPlan plan;
-    rng_plan(seed, size_t(m) * m * m, plan);
+    rng_plan(seed, n, plan);
     c->work.ensure(rng_workspace_bytes(plan));
-    DevBuf f, dense;
-    f.ensure(size_t(m) * m * (m + 2) * 8);
-    dense.ensure(size_t(m) * m * m * 8);
-    std::vector<RngLevelOut> outs{{f.p, m, size_t(m) * m * m}};
+    DevBuf dense;
+    dense.ensure(n * 8);
+    std::vector<RngLevelOut> outs{{dense.p, 0, n}};
     rng_draw_real<double>(plan, outs, c->work.p, c->work.bytes, c->st, c->hook());
-    unpad<double, double>((const double *)f.p, m, (double *)dense.p, c->st);
     GIC_HIP(hipMemcpyAsync(host_out, dense.p, n * 8, hipMemcpyDeviceToHost, c->st));
     GIC_HIP(hipStreamSynchronize(c->st));
   });

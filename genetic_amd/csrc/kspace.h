@@ -57,6 +57,12 @@ template <typename T>
 void particles(const T *px, const T *py, const T *pz, int n, const ParticleGeom &g, const uint64_t *cells,
                size_t start, size_t count, float *pos, float *vel, hipStream_t st);
 template <typename T> void fill(T *p, size_t n, T v, hipStream_t st);
+// Fourier-space white noise: number of (a, b) draws of an n^3 level, and the
+// placement of gauss[0 .. 2*count) (already scaled by 1/sqrt 2) into the
+// padded spectrum 'field'; owner holds n*n*(n/2+1) words of scratch.
+long fourier_draw_count(int n);
+template <typename T>
+void fourier_draw_scatter(const T *gauss, int n, T *field, uint32_t *owner, int reverse, hipStream_t st);
 template <typename S, typename D> void unpad(const S *src, int n, D *dst, hipStream_t st);
 template <typename S, typename D> void pad(const S *src, int n, D *dst, hipStream_t st);
 template <typename S, typename D> void convert(const S *a, size_t n, D *b, hipStream_t st);

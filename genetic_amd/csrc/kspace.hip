@@ -125,6 +125,102 @@ template <typename T> void mul_table(T *field, int n, const T *tab, hipStream_t 
   GIC_HIP(hipGetLastError());
 }
 
+// ---- Fourier-space white noise ------------------------------------------------------
+// drawRandomForSpecifiedGridFourier / drawOneFourierMode
+// (field/randomfieldgenerator.hpp:144-171,203-260): square shells ks < N/2 in
+// order; in shell ks, for k1, k2 in [-ks, ks), the modes (ks,k1,k2),
+// (k1,ks,k2), (k1,k2,ks) each take two Gaussians (a, b) of the stream and
+// store (a, b) -- (b, a) in reverse mode -- through setFourierCoefficient
+// (tools/numerics/fourier.hpp:300-319,439-452), plus the conjugate at -k when
+// a component is 0.  Later writes to a slot win; the owner pass finds, per
+// slot, the last write (key 2d+1 for draw d, 2d+2 for its mirror write) and
+// the write pass stores only the winners.
+
+// draws before shell ks: sum_{j<ks} 3 (2j)^2
+__device__ __forceinline__ long shell_start(long ks) { return 2 * (ks - 1) * ks * (2 * ks - 1); }
+
+__device__ __forceinline__ void decode_draw(long d, int h, int &k1, int &k2, int &k3) {
+  long lo = 1, hi = h - 1;   // shell_start(lo) <= d < shell_start(hi + 1)
+  while (lo < hi) {
+    long mid = (lo + hi + 1) / 2;
+    if (shell_start(mid) <= d) lo = mid;
+    else hi = mid - 1;
+  }
+  long ks = lo;
+  long r = d - shell_start(ks);
+  long p = r / 3;
+  int which = int(r - 3 * p);
+  int a = int(-ks + p / (2 * ks)), b = int(-ks + p % (2 * ks));
+  if (which == 0) {
+    k1 = int(ks), k2 = a, k3 = b;
+  } else if (which == 1) {
+    k1 = a, k2 = int(ks), k3 = b;
+  } else {
+    k1 = a, k2 = b, k3 = int(ks);
+  }
+}
+
+// storage slot of mode (kx,ky,kz) and whether the stored value is conjugated
+__device__ __forceinline__ size_t mode_slot(int kx, int ky, int kz, int n, bool &conj) {
+  conj = kz < 0;
+  if (conj) {
+    kx = -kx;
+    ky = -ky;
+    kz = -kz;
+  }
+  if (ky < 0) ky += n;
+  if (kx < 0) kx += n;
+  return (size_t(kx) * n + ky) * (n / 2 + 1) + kz;
+}
+
+__global__ void k_fourier_owner(int n, long ndraw, uint32_t *__restrict__ owner) {
+  long d = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (d >= ndraw) return;
+  int k1, k2, k3;
+  decode_draw(d, n / 2, k1, k2, k3);
+  bool cj;
+  atomicMax(&owner[mode_slot(k1, k2, k3, n, cj)], uint32_t(2 * d + 1));
+  if (k1 == 0 || k2 == 0 || k3 == 0) atomicMax(&owner[mode_slot(-k1, -k2, -k3, n, cj)], uint32_t(2 * d + 2));
+}
+
+template <typename T>
+__global__ void k_fourier_write(int n, long ndraw, const T *__restrict__ gauss, const uint32_t *__restrict__ owner,
+                                cx<T> *__restrict__ out, int reverse) {
+  long d = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (d >= ndraw) return;
+  int k1, k2, k3;
+  decode_draw(d, n / 2, k1, k2, k3);
+  T a = gauss[2 * d], b = gauss[2 * d + 1];
+  T re = reverse ? b : a, im = reverse ? a : b;
+  bool cj;
+  size_t s = mode_slot(k1, k2, k3, n, cj);
+  if (owner[s] == uint32_t(2 * d + 1)) out[s] = cx<T>{re, cj ? -im : im};
+  if (k1 == 0 || k2 == 0 || k3 == 0) {
+    s = mode_slot(-k1, -k2, -k3, n, cj);
+    if (owner[s] == uint32_t(2 * d + 2)) out[s] = cx<T>{re, cj ? im : -im};
+  }
+}
+
+long fourier_draw_count(int n) {
+  long h = n / 2;
+  return h >= 1 ? 2 * (h - 1) * h * (2 * h - 1) : 0;
+}
+
+template <typename T>
+void fourier_draw_scatter(const T *gauss, int n, T *field, uint32_t *owner, int reverse, hipStream_t st) {
+  long nd = fourier_draw_count(n);
+  size_t nslot = size_t(n) * n * (n / 2 + 1);
+  if (nd * 2 + 2 >= (1L << 32)) throw std::runtime_error("grid too large for the Fourier-space draw");
+  fill<uint32_t>(owner, nslot, 0u, st);
+  fill<T>(field, 2 * nslot, T(0), st);
+  if (nd == 0) return;
+  hipLaunchKernelGGL(k_fourier_owner, dim3((nd + 255) / 256), dim3(256), 0, st, n, nd, owner);
+  GIC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_fourier_write<T>, dim3((nd + 255) / 256), dim3(256), 0, st, n, nd, gauss, owner,
+                     reinterpret_cast<cx<T> *>(field), reverse);
+  GIC_HIP(hipGetLastError());
+}
+
 // ---- |delta_k|^2 histogram by integer |k|^2 ----------------------------------------
 // dumpPowerSpectrum (cosmology/parameters.hpp:112-205) sums over every
 // (ix,iy,iz) in [-N/2, N/2]^3 -- the Nyquist planes are visited twice -- and
@@ -349,6 +445,7 @@ template <typename S, typename D> void convert(const S *a, size_t n, D *b, hipSt
   template void filter_table<T>(int, double, double, long, T *, hipStream_t, bool);                           \
   template void mul_table<T>(T *, int, const T *, hipStream_t);                                               \
   template void mode_power<T>(const T *, int, double *, double *, hipStream_t);                               \
+  template void fourier_draw_scatter<T>(const T *, int, T *, uint32_t *, int, hipStream_t);                   \
   template void zeldovich<T>(const T *, int, double, T *, T *, T *, const T *, hipStream_t);                  \
   template void interp_add<T>(T *, int, const CoarseView &, const InterpGeom &, hipStream_t);                 \
   template void particles<T>(const T *, const T *, const T *, int, const ParticleGeom &, const uint64_t *,    \
@@ -356,6 +453,7 @@ template <typename S, typename D> void convert(const S *a, size_t n, D *b, hipSt
   template void fill<T>(T *, size_t, T, hipStream_t);
 GIC_INST(float)
 GIC_INST(double)
+template void fill<uint32_t>(uint32_t *, size_t, uint32_t, hipStream_t);
 template void unpad<float, float>(const float *, int, float *, hipStream_t);
 template void unpad<float, double>(const float *, int, double *, hipStream_t);
 template void unpad<double, double>(const double *, int, double *, hipStream_t);

@@ -24,16 +24,18 @@ struct RngPlan {
 
 void rng_plan(unsigned long seed, size_t n_gaussians, RngPlan &plan);
 
-// Draw n_gaussians of the stream into the listed level fields (padded real
-// layout), in level order, as the reference's serial loop does.
+// Draw the stream's Gaussians into the listed targets in order, as the
+// reference's serial loop does: a padded real field (n > 0, count n^3, cell
+// order x, y, z) or a dense array (n == 0, count values).  Each value is
+// stored as T(v * scale).
 struct RngLevelOut {
-  void *field;       // T* padded real field
+  void *field;       // T*
   int n;
-  size_t count;      // n^3
+  size_t count;
 };
 template <typename T>
 void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, void *workspace, size_t ws_bytes,
-                   hipStream_t st, struct TimingHook *hook);
+                   hipStream_t st, struct TimingHook *hook, double scale = 1.0);
 size_t rng_workspace_bytes(const RngPlan &plan);
 
 // Raw 24-bit stream (testing / bit-exactness checks): the first n numbers.

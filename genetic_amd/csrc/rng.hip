@@ -454,8 +454,9 @@ __global__ __launch_bounds__(1024) void k_rng_seg_scan(const uint32_t *__restric
 // Pass 3: regenerate each chunk, parse from its true entry, write Gaussians.
 struct LevelTargets {
   void *f[4];
-  int n[4];
-  size_t cum[4];   // cumulative Gaussian counts at the end of each level
+  int n[4];        // grid size of a padded real field; 0 for a dense array
+  size_t cum[4];   // cumulative Gaussian counts at the end of each target
+  double scale;    // every value is stored as T(v * scale)
 };
 
 // Pass 3 works one wavefront per workgroup.  Each lane parses its chunk from
@@ -471,6 +472,7 @@ __device__ __forceinline__ T *cell_ptr(const LevelTargets &lt, uint64_t g) {
   int L = g < lt.cum[0] ? 0 : g < lt.cum[1] ? 1 : g < lt.cum[2] ? 2 : 3;
   size_t i = g - (L ? lt.cum[L - 1] : 0);
   int n = lt.n[L];
+  if (n == 0) return reinterpret_cast<T *>(lt.f[L]) + i;
   size_t nn = size_t(n) * n;
   size_t xx = i / nn, r = i - xx * nn, y = r / n, z = r - y * n;
   return reinterpret_cast<T *>(lt.f[L]) + (xx * n + y) * (n + 2) + z;
@@ -544,7 +546,7 @@ __global__ __launch_bounds__(64) void k_rng_emit(const uint32_t *__restrict__ ra
         double v;
         pos += zig_iter(get, pos, acc, v);
         if (acc) {
-          stage_mem[fill * 64 + lane] = T(v);
+          stage_mem[fill * 64 + lane] = T(v * lt.scale);
           ++fill;
           ++g;
         }
@@ -747,7 +749,7 @@ static RngDev to_dev(const RngPlan &plan) {
 
 template <typename T>
 void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, void *ws, size_t ws_bytes,
-                   hipStream_t st, TimingHook *hook) {
+                   hipStream_t st, TimingHook *hook, double scale) {
   if (levels.empty() || levels.size() > 4) throw std::runtime_error("real-space draw supports 1 to 4 levels");
   upload_tables();
   RngBufs b = carve(plan, ws, ws_bytes);
@@ -781,6 +783,7 @@ void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, 
     if (i < levels.size()) acc += levels[i].count;
     lt.cum[i] = acc;
   }
+  lt.scale = scale;
   size_t total = acc;
   if (produced < total) throw std::runtime_error("internal error: RNG plan produced too few Gaussians");
   if (hook) hook->begin("rng_emit", st);
@@ -791,9 +794,9 @@ void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, 
 }
 
 template void rng_draw_real<float>(const RngPlan &, const std::vector<RngLevelOut> &, void *, size_t, hipStream_t,
-                                   TimingHook *);
+                                   TimingHook *, double);
 template void rng_draw_real<double>(const RngPlan &, const std::vector<RngLevelOut> &, void *, size_t, hipStream_t,
-                                    TimingHook *);
+                                    TimingHook *, double);
 
 void rng_raw(const RngPlan &plan, uint32_t *d_out, size_t n, void *, size_t, hipStream_t st) {
   if (n > size_t(plan.nchunks) * RngPlan::CH) throw std::runtime_error("rng_raw: plan too small");

@@ -106,6 +106,7 @@ def stage_test(test, tmp_path, paramfile_text=None):
     """Lay out a reference test directory (paramfile and side files) under
     tmp_path with the CAMB tables one level up, as in genetIC/tests."""
     import shutil
+    os.makedirs(str(tmp_path), exist_ok=True)
     for c in _CAMBS:
         shutil.copyfile(golden(c), os.path.join(str(tmp_path), c))
     d = os.path.join(str(tmp_path), test)

@@ -268,3 +268,49 @@ def test_oracle_tipsy_fixture_07a():
     assert_decimal(vel * a ** -0.5 / dkms, d["vel"], 4, "tipsy vel")
     assert_decimal(np.full(32 ** 3, mass * mass_factor), d["mass"], 6, "tipsy mass")
     assert_decimal(np.full(32 ** 3, eps / box), d["eps"], 6, "tipsy eps")
+
+
+def _tipsy_units(pos, vel, mass, eps, n, box, c):
+    a = 1 / (1 + c["z"])
+    mass_factor = c["Om"] / (mass * n)
+    dkpc = box * 1000 / c["hubble"]
+    dmsol = 1e10 / c["hubble"] / mass_factor
+    dkms = np.sqrt(4.3022682e-6 * dmsol / dkpc)
+    return pos / box - 0.5, vel * a ** -0.5 / dkms, mass * mass_factor, eps / box
+
+
+@pytest.mark.parametrize("test,box,seed,mode", [
+    ("test_13_parallel_seed", 10.0, 42, po.SEED_FOURIER_PARALLEL),
+    ("test_05a_seed_old", 50.0, 8896131, po.SEED_FOURIER_REVERSE),
+])
+def test_oracle_fourier_seeding_fixtures(test, box, seed, mode):
+    """Fourier-space draws (randomfieldgenerator.hpp:144-260) pinned by the
+    reference's particle outputs: test_13 (random_seed: one stream per
+    k-shell, seeded seed+ks) and test_05a (seedfourier_reverse: one stream,
+    (b, a) order)."""
+    ref = read_tipsy(golden(test, "reference_output"))
+    c = cosmology_from_paramfile(golden(test, "paramfile.txt"))
+    o = po.OracleIC()
+    o.set_cosmology(Om=c["Om"], Ol=c["Ol"], s8=c["s8"], ns=c["ns"], z=c["z"], hubble=c["hubble"])
+    tab, nc = po.read_camb(golden("camb_transfer_kmax40_z0.dat"))
+    o.set_camb(tab, nc)
+    o.init_base_grid(box, 8)
+    o.seed(seed, mode)
+    o.zeldovich()
+    pos, vel, mass, eps = o.particles(0)
+    p, v, m, e = _tipsy_units(pos, vel, mass, eps, 512, box, c)
+    d = ref["dark"]
+    assert_decimal(p, d["pos"], 4, "pos")
+    assert_decimal(v, d["vel"], 4, "vel")
+    assert_decimal(np.full(512, m), d["mass"], 6, "mass")
+    # a plausible slip -- the non-reversed order, or the serial stream --
+    # moves the particles far outside that tolerance
+    for wrong in {po.SEED_FOURIER_SERIAL, po.SEED_FOURIER_PARALLEL, po.SEED_FOURIER_REVERSE} - {mode}:
+        o2 = po.OracleIC()
+        o2.set_cosmology(Om=c["Om"], Ol=c["Ol"], s8=c["s8"], ns=c["ns"], z=c["z"], hubble=c["hubble"])
+        o2.set_camb(tab, nc)
+        o2.init_base_grid(box, 8)
+        o2.seed(seed, wrong)
+        o2.zeldovich()
+        v2 = _tipsy_units(*o2.particles(0), 512, box, c)[1]
+        assert np.abs(v2 - d["vel"]).max() > 1e-2

@@ -52,7 +52,7 @@ def test_fft_matches_oracle(n, precision, tol):
     assert np.abs(back - x).max() <= tol * np.abs(x).max() * 8
 
 
-def _setup_pair(paramfile, camb, levels, precision=gic.FLOAT64, seed=SEED):
+def _setup_pair(paramfile, camb, levels, precision=gic.FLOAT64, seed=SEED, mode=gic.SEED_REAL_SPACE):
     c = cosmology_from_paramfile(golden(paramfile))
     ic = _ic(precision)
     ic.set_cosmology(**c)
@@ -67,8 +67,8 @@ def _setup_pair(paramfile, camb, levels, precision=gic.FLOAT64, seed=SEED):
     for (x0, y0, z0, fac, nz) in levels[1:]:
         ic.init_zoom_grid_with_origin_at(x0, y0, z0, fac, nz)
         o.init_zoom_grid_at(x0, y0, z0, fac, nz)
-    ic.set_seed(seed)
-    o.seed(seed)
+    ic.set_seed(seed, mode)
+    o.seed(seed, mode)
     return ic, o
 
 
@@ -121,3 +121,51 @@ def test_gadget_fixture_particles():
     assert mass == ref["mass"][1]
     assert_decimal(pos, ref["pos"], 4, "pos")
     assert_decimal(vel, ref["vel"], 4, "vel")
+
+
+FOURIER_MODES = [gic.SEED_FOURIER_SERIAL, gic.SEED_FOURIER_PARALLEL, gic.SEED_FOURIER_REVERSE]
+
+
+@pytest.mark.parametrize("mode", FOURIER_MODES)
+@pytest.mark.parametrize("n", [4, 8, 16, 64])
+def test_fourier_seeded_noise_matches_oracle(mode, n):
+    """Fourier-space white noise: every stored mode, including the slots
+    written twice through the conjugate mirror, equals the serial oracle's
+    (FP64: a few ulp from the ziggurat tails)."""
+    print("fourier noise", mode, n, flush=True)
+    ic, o = _setup_pair("test_10a_gadget/paramfile.txt", "camb_transfer_kmax40_z0.dat", [(50.0, n)],
+                        seed=8896131, mode=mode)
+    ic.draw()
+    o.draw()
+    a = ic.field(gic.FIELD_OVERDENSITY, 0, fourier=True)
+    b = o.field(0, 0, fourier=True)
+    np.testing.assert_allclose(a, b, rtol=0, atol=1e-13)
+    assert np.count_nonzero(b) > 0.8 * b.size * (1 - 3.0 / n)
+
+
+@pytest.mark.parametrize("mode", FOURIER_MODES)
+@pytest.mark.parametrize("precision,rtol", [(gic.FLOAT64, 1e-10), (gic.FLOAT32, 2e-5)])
+def test_fourier_seeded_delta_psi(mode, precision, rtol):
+    ic, o = _setup_pair("test_10a_gadget/paramfile.txt", "camb_transfer_kmax40_z0.dat", [(50.0, 64)], precision,
+                        seed=42, mode=mode)
+    ic.zeldovich()
+    o.zeldovich()
+    for which in range(4):
+        a = ic.field(which, 0)
+        b = o.field(which, 0)
+        assert np.abs(a - b).max() <= rtol * np.abs(b).max() * 10, which
+
+
+@pytest.mark.parametrize("mode", [gic.SEED_FOURIER_SERIAL, gic.SEED_FOURIER_PARALLEL])
+def test_fourier_seeded_two_levels(mode):
+    """Serial mode continues one stream into the zoom level; parallel mode
+    restarts the per-shell seeds on every level."""
+    ic, o = _setup_pair("test_01c_zoom_grid/paramfile.txt", "camb_transfer_kmax40_z0.dat",
+                        [(50.0, 32), (10, 10, 10, 2, 32)], seed=1234, mode=mode)
+    ic.zeldovich()
+    o.zeldovich()
+    for level in range(2):
+        for which in range(4):
+            a = ic.field(which, level)
+            b = o.field(which, level)
+            assert np.abs(a - b).max() <= 1e-9 * np.abs(b).max(), (level, which)

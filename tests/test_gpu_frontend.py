@@ -37,6 +37,8 @@ def particle_file(d):
     "test_01g_zoom_grid_vel_offset",
     "test_23_doublezoom_wrap_issue_90",
     "test_25_zoom_factor_1",
+    "test_13_parallel_seed",
+    "test_05a_seed_old",
 ])
 def test_particle_output_matches_reference(tmp_path, test):
     d, _ = _run(test, tmp_path)
