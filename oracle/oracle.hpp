@@ -165,6 +165,35 @@ struct IC {
   void particle(int level, size_t id, double pos[3], double vel[3]) const;
   double particleMass(int level) const;
   double particleEps(int level) const;
+
+  // --- modifications (modif.cpp) ---
+  struct Mod {
+    std::string name;
+    bool quadratic = false;
+    double target = 0, scale = -1, precision = 0.001;
+    int initSteps = 10;
+    std::vector<std::vector<size_t>> flags;
+  };
+  std::vector<std::vector<size_t>> flags;   // flagged cells per level, sorted
+  double varianceScale = -1;
+  std::vector<Mod> linear, quadratic;
+  double levelFilter(int level, double k) const;
+  double innerProduct(const std::vector<Field> &a, const std::vector<Field> &b) const;
+  void applyMetric(std::vector<Field> &f, bool toCovector) const;
+  void transferToDm(std::vector<Field> &f, bool fromWhiteNoise) const;
+  int finestFlaggedCheck(const std::string &name, const std::vector<std::vector<size_t>> &fl) const;
+  std::vector<Field> linearCovector(const std::string &name, const std::vector<std::vector<size_t>> &fl) const;
+  std::vector<Field> pushVariance(const std::vector<Field> &field, const std::vector<std::vector<size_t>> &fl,
+                                  double scale) const;
+  double varianceValue(const std::vector<Field> &field, const std::vector<std::vector<size_t>> &fl,
+                       double scale) const;
+  void checkVarianceScale(double scale) const;
+  double calculate(const std::string &name);
+  void addModification(const std::string &name, bool relative, double target);
+  void applyModifications(double *preChi2, double *postChi2);
+  void clearModifications();
+  double chi2();
+  size_t numDof() const;
 };
 
 // tricubic (Catmull-Rom tensor) interpolation of a real-space field at a point

@@ -1,6 +1,7 @@
 // C entry points of the CPU oracle, for ctypes use by tests/ and bench.py's
 // cpu_baseline leg.  TEST INFRASTRUCTURE ONLY (see oracle.hpp).
 #include "oracle.hpp"
+#include <algorithm>
 #include <cstring>
 #include <stdexcept>
 #include <string>
@@ -152,6 +153,44 @@ int orc_ic_particles(void *h, int level, double *pos, double *vel, double *mass,
     particles_level(ic, level, pos, vel);
     *mass = ic.particleMass(level);
     *eps = ic.particleEps(level);
+  })
+}
+
+int orc_ic_set_flags(void *h, int level, const uint64_t *cells, size_t n) {
+  ORC_TRY({
+    IC &ic = *(IC *)h;
+    if (ic.flags.size() < ic.grids.size()) ic.flags.resize(ic.grids.size());
+    if (level < 0 || level >= (int)ic.grids.size()) throw std::runtime_error("Specified level does not exist.");
+    std::vector<size_t> f(cells, cells + n);
+    std::sort(f.begin(), f.end());
+    f.erase(std::unique(f.begin(), f.end()), f.end());
+    ic.flags[level] = f;
+  })
+}
+int orc_ic_set_variance_scale(void *h, double s) { ORC_TRY(((IC *)h)->varianceScale = s) }
+int orc_ic_calculate(void *h, const char *name, double *out) {
+  ORC_TRY({
+    IC &ic = *(IC *)h;
+    ic.flags.resize(ic.grids.size());
+    *out = ic.calculate(name);
+  })
+}
+int orc_ic_add_modification(void *h, const char *name, int relative, double target) {
+  ORC_TRY({
+    IC &ic = *(IC *)h;
+    ic.flags.resize(ic.grids.size());
+    ic.addModification(name, relative != 0, target);
+  })
+}
+int orc_ic_apply_modifications(void *h, double *pre, double *post) {
+  ORC_TRY(((IC *)h)->applyModifications(pre, post))
+}
+int orc_ic_clear_modifications(void *h) { ORC_TRY(((IC *)h)->clearModifications()) }
+int orc_ic_chi2(void *h, double *out, size_t *dof) {
+  ORC_TRY({
+    IC &ic = *(IC *)h;
+    *out = ic.chi2();
+    if (dof) *dof = ic.numDof();
   })
 }
 

@@ -52,6 +52,13 @@ def lib():
             "orc_ic_get_field": [v, i, i, i, v],
             "orc_ic_particles": [v, i, v, v, P(d), P(d)],
             "orc_ic_velocity_ratio": [v, P(d)],
+            "orc_ic_set_flags": [v, i, v, s],
+            "orc_ic_set_variance_scale": [v, d],
+            "orc_ic_calculate": [v, ctypes.c_char_p, P(d)],
+            "orc_ic_add_modification": [v, ctypes.c_char_p, i, d],
+            "orc_ic_apply_modifications": [v, P(d), P(d)],
+            "orc_ic_clear_modifications": [v],
+            "orc_ic_chi2": [v, P(d), P(s)],
             "orc_time_pipeline": [d, i, u, v, s, i, d, d, d, d, d, P(d), P(d)],
         }
         for name, args in sigs.items():
@@ -195,6 +202,35 @@ class OracleIC:
         m, e = ctypes.c_double(), ctypes.c_double()
         _check(lib().orc_ic_particles(self.h, level, _ptr(pos), _ptr(vel), ctypes.byref(m), ctypes.byref(e)))
         return pos, vel, m.value, e.value
+
+    # --- modifications ---------------------------------------------------
+    def set_flags(self, level, cells):
+        c = np.ascontiguousarray(np.asarray(cells, dtype=np.uint64))
+        _check(lib().orc_ic_set_flags(self.h, level, _ptr(c), c.size))
+
+    def set_variance_scale(self, scale):
+        _check(lib().orc_ic_set_variance_scale(self.h, scale))
+
+    def calculate(self, name):
+        out = ctypes.c_double()
+        _check(lib().orc_ic_calculate(self.h, name.encode(), ctypes.byref(out)))
+        return out.value
+
+    def add_modification(self, name, relative, target):
+        _check(lib().orc_ic_add_modification(self.h, name.encode(), int(relative), float(target)))
+
+    def apply_modifications(self):
+        pre, post = ctypes.c_double(), ctypes.c_double()
+        _check(lib().orc_ic_apply_modifications(self.h, ctypes.byref(pre), ctypes.byref(post)))
+        return pre.value, post.value
+
+    def clear_modifications(self):
+        _check(lib().orc_ic_clear_modifications(self.h))
+
+    def chi2(self):
+        out, dof = ctypes.c_double(), ctypes.c_size_t()
+        _check(lib().orc_ic_chi2(self.h, ctypes.byref(out), ctypes.byref(dof)))
+        return out.value, dof.value
 
     def velocity_ratio(self):
         r = ctypes.c_double()

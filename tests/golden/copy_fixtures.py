@@ -51,6 +51,25 @@ FILES = [
     "test_05a_seed_old/reference_output",
     "test_02k_chisquare_onelevel/paramfile.txt",
     "test_02k_chisquare_onelevel/reference.txt",
+    "test_02l_chisquare_modification/paramfile.txt",
+    "test_02l_chisquare_modification/reference.txt",
+    "test_02m_chisquare_twolevels/paramfile.txt",
+    "test_02m_chisquare_twolevels/reference.txt",
+    "test_02f_onelevelvariance/paramfile.txt",
+    "test_02f_onelevelvariance/reference.txt",
+    "test_02g_multilevelvariance/paramfile.txt",
+    "test_02g_multilevelvariance/reference.txt",
+    "test_02j_invertible_modifs/paramfile.txt",
+    "test_02j_invertible_modifs/reference.txt",
+    "test_14_velocity_modif/paramfile.txt",
+    "test_14_velocity_modif/ids.txt",
+    "test_14_velocity_modif/reference_output",
+    "test_22_angular_momentum_modif/paramfile.txt",
+    "test_22_angular_momentum_modif/reference_output",
+    "test_20_velmodif_zoom/paramfile.txt",
+    "test_20_velmodif_zoom/reference_output",
+    "test_20_velmodif_zoom/reference_grid/grid-0.npy",
+    "test_20_velmodif_zoom/reference_grid/grid-1.npy",
     "test_07a_CAMB_new_interface/paramfile.txt",
     "test_07a_CAMB_new_interface/reference_output",
 ]

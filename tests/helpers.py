@@ -168,3 +168,49 @@ def compare_particles(test_file, ref_file):
         ha, hb = a["header"], b["header"]
         assert ha[:24] == hb[:24] and ha[72:208] == hb[72:208]
         np.testing.assert_allclose(a["mass"], b["mass"], rtol=1e-12)
+
+
+def select_sphere_cells(n, box, centre, radius, offset=(0.0, 0.0, 0.0), dx=None, periodic=None):
+    """Cells of a grid whose centroids lie within radius of centre, with the
+    box-wrapped offsets of ICGenerator::select (ic.hpp:1458-1500)."""
+    dx = box / n if dx is None else dx
+    periodic = box if periodic is None else periodic
+    i = np.arange(n)
+    cs = [i * dx + offset[a] + dx / 2 for a in range(3)]
+
+    def woff(p, c):
+        r = p - c
+        r = np.where(r > periodic / 2, r - periodic, r)
+        return np.where(r < -periodic / 2, r + periodic, r)
+    dx_, dy_, dz_ = (woff(cs[a], centre[a]) for a in range(3))
+    r2 = np.float64(np.float32(radius) * np.float32(radius))
+    d2 = dx_[:, None, None] ** 2 + dy_[None, :, None] ** 2 + dz_[None, None, :] ** 2
+    return np.flatnonzero((d2 < r2).ravel())
+
+
+def select_cube_cells(n, box, centre, side, offset=(0.0, 0.0, 0.0), dx=None, periodic=None):
+    dx = box / n if dx is None else dx
+    periodic = box if periodic is None else periodic
+    i = np.arange(n)
+    h = np.float64(np.float32(side) / np.float32(2))
+
+    def woff(p, c):
+        r = p - c
+        r = np.where(r > periodic / 2, r - periodic, r)
+        return np.where(r < -periodic / 2, r + periodic, r)
+    m = [np.abs(woff(i * dx + offset[a] + dx / 2, centre[a])) < h for a in range(3)]
+    return np.flatnonzero((m[0][:, None, None] & m[1][None, :, None] & m[2][None, None, :]).ravel())
+
+
+def select_ellipse_cells(n, box, centre, a, b, c):
+    dx = box / n
+    i = np.arange(n)
+
+    def woff(p, c0):
+        r = p - c0
+        r = np.where(r > box / 2, r - box, r)
+        return np.where(r < -box / 2, r + box, r)
+    d = [woff(i * dx + dx / 2, centre[k]) for k in range(3)]
+    a2, b2, c2 = (np.float64(np.float32(v) * np.float32(v)) for v in (a, b, c))
+    r = d[0][:, None, None] ** 2 / a2 + d[1][None, :, None] ** 2 / b2 + d[2][None, None, :] ** 2 / c2
+    return np.flatnonzero((r < 1.0).ravel())

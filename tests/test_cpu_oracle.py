@@ -8,7 +8,8 @@ import pytest
 import scipy.interpolate
 
 import pyoracle as po
-from helpers import golden, read_gadget, read_tipsy, assert_decimal, cosmology_from_paramfile
+from helpers import (golden, read_gadget, read_tipsy, assert_decimal, cosmology_from_paramfile, select_sphere_cells,
+                     select_cube_cells, select_ellipse_cells)
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -314,3 +315,173 @@ def test_oracle_fourier_seeding_fixtures(test, box, seed, mode):
         o2.zeldovich()
         v2 = _tipsy_units(*o2.particles(0), 512, box, c)[1]
         assert np.abs(v2 - d["vel"]).max() > 1e-2
+
+
+# ------------------------------------------------------------ modifications --
+def _oracle_cosmo(paramfile, camb="camb_transfer_kmax40_z0.dat"):
+    c = cosmology_from_paramfile(golden(paramfile))
+    o = po.OracleIC()
+    o.set_cosmology(Om=c["Om"], Ol=c["Ol"], s8=c["s8"], ns=c["ns"], z=c["z"], hubble=c["hubble"])
+    tab, nc = po.read_camb(golden(camb))
+    o.set_camb(tab, nc)
+    return o, c
+
+
+def _ref_value(test, prefix):
+    """Numbers of the log lines of a reference.txt that start with prefix."""
+    out = []
+    for ln in open(golden(test, "reference.txt")):
+        if ln.strip().startswith(prefix):
+            out.append(float(ln.split("=")[1].split()[0]))
+    return out
+
+
+def test_oracle_chi2_one_level():
+    """test_02k: chi^2 = <w|w> of the white noise, 64^3 (multilevelfield.hpp:461-487)."""
+    o, _ = _oracle_cosmo("test_02k_chisquare_onelevel/paramfile.txt")
+    o.init_base_grid(64.0, 64)
+    o.seed(13842314)
+    chi2, dof = o.chi2()
+    assert dof == 262144
+    assert "%.10g" % chi2 == "261733.3011"
+    # the same number from the stream directly: sum w^2 - N^3 mean(w)^2
+    w, _ = po.gaussians(13842314, 64 ** 3)
+    assert abs(chi2 - (np.sum(w ** 2) - w.size * w.mean() ** 2)) < 1e-6 * chi2
+
+
+def test_oracle_chi2_two_levels():
+    """test_02m: the two-level metric (windowed filters, cross-level terms).
+    The zoom origin (8, 15, 7) is where zoomgrid 2 64 puts the 32-cell window
+    around the cube of 16 cells selected at (25, 32, 24)."""
+    o, _ = _oracle_cosmo("test_02m_chisquare_twolevels/paramfile.txt")
+    o.init_base_grid(64.0, 64)
+    o.init_zoom_grid_at(8, 15, 7, 2, 64)
+    o.seed(13842314)
+    chi2, dof = o.chi2()
+    assert dof == 491520
+    assert "%.10g" % chi2 == "492005.8359"
+
+
+def test_oracle_overdensity_modification_delta_chi2():
+    """test_02l: modify overdensity relative 2 -- the value doubles and
+    Delta chi^2 is the reference's."""
+    o, _ = _oracle_cosmo("test_02l_chisquare_modification/paramfile.txt")
+    o.init_base_grid(64.0, 64)
+    o.seed(13842314)
+    o.set_flags(0, select_sphere_cells(64, 64.0, (25, 32, 24), 10))
+    v0 = o.calculate("overdensity")
+    o.add_modification("overdensity", True, 2.0)
+    pre, post = o.apply_modifications()
+    assert abs(o.calculate("overdensity") - 2 * v0) < 1e-12
+    assert "%g" % (post - pre) == "%g" % _ref_value("test_02l_chisquare_modification", "Modification Delta chi^2")[0]
+
+
+def test_oracle_variance_values():
+    """test_02f (one level) and test_02g (two levels): the filtered variance
+    and the overdensity, before and after the output is generated."""
+    o, _ = _oracle_cosmo("test_02f_onelevelvariance/paramfile.txt")
+    o.init_base_grid(64.0, 64)
+    o.seed(8896131)
+    o.set_flags(0, select_cube_cells(64, 64.0, (8, 8, 8), 20))
+    o.set_variance_scale(10000)
+    ref = _ref_value("test_02f_onelevelvariance", "variance: calculated value")
+    assert "%g" % o.calculate("variance") == "%g" % ref[0]
+    o.zeldovich()
+    assert "%g" % o.calculate("variance") == "%g" % ref[0]
+
+    o, _ = _oracle_cosmo("test_02g_multilevelvariance/paramfile.txt")
+    o.init_base_grid(64.0, 64)
+    o.init_zoom_grid_at(0, 0, 0, 4, 64)
+    o.seed(8896131)
+    o.set_flags(0, select_cube_cells(64, 64.0, (8, 8, 8), 16))
+    o.set_flags(1, select_cube_cells(64, 64.0, (8, 8, 8), 16, dx=0.25))
+    o.set_variance_scale(10000)
+    od = _ref_value("test_02g_multilevelvariance", "overdensity: calculated value")
+    var = _ref_value("test_02g_multilevelvariance", "variance: calculated value")
+    assert "%g" % o.calculate("overdensity") == "%g" % od[0]
+    assert "%g" % o.calculate("variance") == "%g" % var[0]
+    o.zeldovich()
+    assert "%g" % o.calculate("variance") == "%g" % var[1]
+
+
+def test_oracle_invertible_linear_and_quadratic_modifications():
+    """test_02j: overdensity and variance doubled, then halved, on two
+    levels (64^3 + x4 zoom at (24, 24, 24)); the logged values return to
+    the reference's, and after the output the variance is the reference's."""
+    o, _ = _oracle_cosmo("test_02j_invertible_modifs/paramfile.txt")
+    o.init_base_grid(64.0, 64)
+    o.init_zoom_grid_at(24, 24, 24, 4, 64)
+    o.seed(13842314)
+    o.set_flags(0, select_sphere_cells(64, 64.0, (32.5, 32.5, 32.5), 16))
+    o.set_flags(1, select_sphere_cells(64, 64.0, (32.5, 32.5, 32.5), 16, offset=(24, 24, 24), dx=0.25))
+    o.set_variance_scale(3.0)
+    od = _ref_value("test_02j_invertible_modifs", "overdensity: calculated value")
+    var = _ref_value("test_02j_invertible_modifs", "variance: calculated value")
+    assert "%g" % o.calculate("overdensity") == "%g" % od[0]
+    assert "%g" % o.calculate("variance") == "%g" % var[0]
+    for f in (2.0, 0.5):
+        o.add_modification("overdensity", True, f)
+        o.add_modification("variance", True, f)
+        o.apply_modifications()
+        o.clear_modifications()
+    assert "%g" % o.calculate("overdensity") == "%g" % od[1]
+    assert "%g" % o.calculate("variance") == "%g" % var[1]
+    o.zeldovich()
+    assert "%g" % o.calculate("variance") == "%g" % var[2]
+
+
+def _tipsy_compare(o, ref, n, box, c, level=0):
+    pos, vel, mass, eps = o.particles(level)
+    p, v, m, e = _tipsy_units(pos, vel, mass, eps, n, box, c)
+    d = ref["dark"]
+    assert_decimal(p, d["pos"], 4, "pos")
+    assert_decimal(v, d["vel"], 4, "vel")
+
+
+def test_oracle_modified_particles_test02():
+    """test_02: zeroed noise plus one overdensity constraint, so the particle
+    output is the constraint response alone (16^3, tipsy)."""
+    o, c = _oracle_cosmo("test_02_simple_modif/paramfile.txt")
+    o.init_base_grid(64.0, 16)
+    o.seed(8896131)
+    o.zero_level(0)
+    o.set_flags(0, select_sphere_cells(16, 64.0, (32.5, 32.5, 32.5), 6))
+    o.add_modification("overdensity", False, 1.0)
+    o.apply_modifications()
+    assert abs(o.calculate("overdensity") - 1.0) < 1e-10
+    o.zeldovich()
+    _tipsy_compare(o, read_tipsy(golden("test_02_simple_modif/reference_output")), 16 ** 3, 64.0, c)
+
+
+def test_oracle_velocity_modification_gadget():
+    """test_14: vx of 56 flagged particles set to 0 (4th-order stencil,
+    -s/k^2 covector), gadget3 8^3."""
+    o, c = _oracle_cosmo("test_14_velocity_modif/paramfile.txt")
+    o.init_base_grid(50.0, 8)
+    o.seed(8896131)
+    ids = np.loadtxt(golden("test_14_velocity_modif/ids.txt"), dtype=np.int64)
+    o.set_flags(0, np.unique(ids))
+    o.add_modification("vx", False, 0.0)
+    o.apply_modifications()
+    assert abs(o.calculate("vx")) < 1e-9
+    o.zeldovich()
+    ref = read_gadget(golden("test_14_velocity_modif/reference_output"))
+    pos, vel, mass, eps = o.particles(0)
+    assert_decimal(pos, ref["pos"], 4, "pos")
+    assert_decimal(vel, ref["vel"], 4, "vel")
+
+
+def test_oracle_angular_momentum_modification_gadget():
+    """test_22: Ly over an ellipsoid set to 100, gadget3 32^3."""
+    o, c = _oracle_cosmo("test_22_angular_momentum_modif/paramfile.txt")
+    o.init_base_grid(50.0, 32)
+    o.seed(8896131)
+    o.set_flags(0, select_ellipse_cells(32, 50.0, (25, 25, 25), 10, 7.5, 5))
+    o.add_modification("Ly", False, 100.0)
+    o.apply_modifications()
+    assert abs(o.calculate("Ly") - 100.0) < 1e-8
+    o.zeldovich()
+    ref = read_gadget(golden("test_22_angular_momentum_modif/reference_output"))
+    pos, vel, mass, eps = o.particles(0)
+    assert_decimal(pos, ref["pos"], 4, "pos")
+    assert_decimal(vel, ref["vel"], 4, "vel")
