@@ -3,224 +3,11 @@
 // The orchestration follows the reference's ICGenerator (genetIC/src/ic.hpp)
 // and MultiLevelParticleGenerator (particles/multilevelgenerator.hpp:78-128);
 // the arithmetic runs in the kernels of fft.hip, rng.hip and kspace.hip.
-#include "../../include/gic.h"
-#include "gic_common.h"
-#include "fft.h"
-#include "rng.h"
-#include "kspace.h"
-#include <vector>
-#include <string>
-#include <map>
-#include <memory>
-#include <cmath>
-#include <cstring>
-#include <algorithm>
-#include <limits>
+#include "context.h"
 
 using namespace gic;
 
-namespace {
-
-// ---------------------------------------------------------------- host maths --
-struct Cosmo {
-  double Om = 0.3, Ol = 0, Ob = 0, h = 0.701, s8 = 0, ns = 0.96, z = 0, a = 1;
-};
-
-// growthFactor, cosmology/parameters.hpp:55-70 (a^3 through single-precision powf there)
-double growth_factor(double a, double Om, double Ol) {
-  double Hsq = Om / (double)powf((float)a, 3.0f) + (1. - Om - Ol) / a / a + Ol;
-  return 2.5 * a * Om / pow(a, 3.0) / Hsq /
-         (pow(Om / Hsq / a / a / a, (double)(4.f / 7.f)) - Ol / Hsq + (1. + 0.5 * Om / pow(a, 3.0) / Hsq) * (1. + 1. / 70. * Ol / Hsq));
-}
-
-// Natural cubic spline (the gsl_interp_cspline construction): second-derivative
-// coefficients from a symmetric tridiagonal system with zero end conditions.
-struct Spline {
-  std::vector<double> x, y, c;
-  void init(const std::vector<double> &xs, const std::vector<double> &ys) {
-    x = xs;
-    y = ys;
-    size_t n = x.size();
-    c.assign(n, 0.0);
-    if (n < 3) return;
-    size_t m = n - 2;
-    std::vector<double> dg(m), of(m), g(m);
-    for (size_t i = 0; i < m; ++i) {
-      double h0 = x[i + 1] - x[i], h1 = x[i + 2] - x[i + 1];
-      of[i] = h1;
-      dg[i] = 2.0 * (h0 + h1);
-      g[i] = 3.0 * ((y[i + 2] - y[i + 1]) * (h1 != 0 ? 1.0 / h1 : 0.0) - (y[i + 1] - y[i]) * (h0 != 0 ? 1.0 / h0 : 0.0));
-    }
-    if (m == 1) {
-      c[1] = g[0] / dg[0];
-      return;
-    }
-    std::vector<double> gam(m), al(m), zz(m);
-    al[0] = dg[0];
-    gam[0] = of[0] / al[0];
-    for (size_t i = 1; i + 1 < m; ++i) {
-      al[i] = dg[i] - of[i - 1] * gam[i - 1];
-      gam[i] = of[i] / al[i];
-    }
-    al[m - 1] = dg[m - 1] - of[m - 2] * gam[m - 2];
-    zz[0] = g[0];
-    for (size_t i = 1; i < m; ++i) zz[i] = g[i] - gam[i - 1] * zz[i - 1];
-    c[m] = zz[m - 1] / al[m - 1];
-    for (long i = long(m) - 2; i >= 0; --i) c[i + 1] = zz[i] / al[i] - gam[i] * c[i + 2];
-  }
-  double eval(double xv) const {
-    size_t n = x.size();
-    if (xv < x[0] || xv > x[n - 1]) return 0;
-    size_t lo = 0, hi = n - 1;
-    while (hi > lo + 1) {
-      size_t i = (hi + lo) / 2;
-      if (x[i] > xv) hi = i;
-      else lo = i;
-    }
-    double dx = x[lo + 1] - x[lo];
-    if (!(dx > 0)) return y[lo];
-    double dy = y[lo + 1] - y[lo], dl = xv - x[lo];
-    double bi = dy / dx - dx * (c[lo + 1] + 2.0 * c[lo]) / 3.0, di = (c[lo + 1] - c[lo]) / (3.0 * dx);
-    return y[lo] + dl * (bi + dl * (c[lo] + dl * di));
-  }
-};
-
-struct LogSpline {
-  Spline s;
-  double minx = 0, maxx = 0;
-  void init(const std::vector<double> &xv, const std::vector<double> &yv) {
-    std::vector<double> lx, ly;
-    for (size_t i = 0; i < xv.size(); ++i) {
-      double a = std::log(xv[i]), b = std::log(yv[i]);
-      if (std::isfinite(a) && std::isfinite(b)) {
-        lx.push_back(a);
-        ly.push_back(b);
-      }
-    }
-    if (lx.size() < 2)
-      throw std::range_error(
-          "Could not find enough valid values in the transfer function. This may mean that one of the transfer "
-          "function columns is full of zeros.");
-    s.init(lx, ly);
-    minx = lx.front();
-    maxx = lx.back();
-  }
-  double operator()(double k) const {
-    double l = std::log(k);
-    if (l < minx || l > maxx) return 0;
-    return std::exp(s.eval(l));
-  }
-};
-
-// ---------------------------------------------------------------- timing -----
-struct EventTimer : TimingHook {
-  struct Rec {
-    std::string name;
-    hipEvent_t a, b;
-  };
-  std::vector<Rec> pending;
-  std::vector<hipEvent_t> pool;
-  std::map<std::string, std::pair<double, long>> totals;
-  hipEvent_t get() {
-    if (!pool.empty()) {
-      hipEvent_t e = pool.back();
-      pool.pop_back();
-      return e;
-    }
-    hipEvent_t e;
-    GIC_HIP(hipEventCreate(&e));
-    return e;
-  }
-  void begin(const char *name, hipStream_t st) override {
-    Rec r{name, get(), nullptr};
-    GIC_HIP(hipEventRecord(r.a, st));
-    pending.push_back(r);
-  }
-  void end(const char *name, hipStream_t st) override {
-    for (auto it = pending.rbegin(); it != pending.rend(); ++it)
-      if (it->name == name && it->b == nullptr) {
-        it->b = get();
-        GIC_HIP(hipEventRecord(it->b, st));
-        break;
-      }
-  }
-  void collect() {
-    for (auto &r : pending) {
-      if (!r.b) continue;
-      GIC_HIP(hipEventSynchronize(r.b));
-      float ms = 0;
-      GIC_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-      auto &t = totals[r.name];
-      t.first += ms;
-      t.second += 1;
-      pool.push_back(r.a);
-      pool.push_back(r.b);
-    }
-    pending.clear();
-  }
-  ~EventTimer() override {
-    for (auto &r : pending) {
-      (void)hipEventDestroy(r.a);
-      if (r.b) (void)hipEventDestroy(r.b);
-    }
-    for (auto e : pool) (void)hipEventDestroy(e);
-  }
-};
-
-struct DevBuf {
-  void *p = nullptr;
-  size_t bytes = 0;
-  void ensure(size_t b) {
-    if (b <= bytes) return;
-    release();
-    GIC_HIP(hipMalloc(&p, b));
-    bytes = b;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
-  }
-  ~DevBuf() { release(); }
-};
-
-struct Level {
-  LevelGeom g;
-  DevBuf noise;            // white noise, then the overdensity
-  DevBuf psi[3];
-  DevBuf sqrtp;            // sqrt P by |k|^2, element type T
-  bool noise_fourier = false;
-  bool dc_pending = false;   // Re(w_0) still to be zeroed after the first r2c
-  bool psi_fourier = false;
-  bool have_psi = false;
-};
-
-}  // namespace
-
-struct gic_context {
-  int device = 0;
-  int prec = 64;
-  hipStream_t st = nullptr;
-  std::string err;
-  Cosmo cosmo;
-  // frozen at the camb command
-  bool have_camb = false;
-  LogSpline tf_all;
-  std::vector<double> kpts;
-  double amplitude = 0, ns_frozen = 0;
-  SplineHost spline_dev;
-  DevBuf spl_x, spl_y, spl_c;
-  std::vector<std::unique_ptr<Level>> levels;
-  unsigned long seed = 0;
-  int seed_mode = 0;
-  bool seeded = false, drawn = false, power_applied = false, zeldovich_done = false;
-  DevBuf work, stage, tab_a, tab_b, tab_c;
-  std::unique_ptr<EventTimer> timer;
-  TimingHook *hook() { return timer.get(); }
-  size_t tsize() const { return prec == 32 ? 4 : 8; }
-};
-
-namespace {
+namespace gic {
 
 size_t total_bytes(gic_context *c) {
   size_t b = c->work.bytes + c->stage.bytes + c->tab_a.bytes + c->tab_b.bytes + c->tab_c.bytes + c->spl_x.bytes * 3;
@@ -236,16 +23,6 @@ Level &level_at(gic_context *c, int level) {
   return *c->levels[level];
 }
 
-template <typename T> void to_fourier(gic_context *c, DevBuf &b, int n, bool &flag) {
-  if (flag) return;
-  fft3d<T>((T *)b.p, n, -1, c->st, c->hook());
-  flag = true;
-}
-template <typename T> void to_real(gic_context *c, DevBuf &b, int n, bool &flag) {
-  if (!flag) return;
-  fft3d<T>((T *)b.p, n, +1, c->st, c->hook());
-  flag = false;
-}
 
 void alloc_field(gic_context *c, DevBuf &b, const LevelGeom &g) {
   b.ensure(g.padded_reals() * c->tsize());
@@ -275,6 +52,18 @@ template <typename T> void build_sqrtp(gic_context *c, Level &L) {
   double kw = 2. * M_PI / L.g.box;
   double norm = kw * kw * kw / pow(2.f * M_PI, 3.f) * double(L.g.n3());
   sqrtp_table<T>(c->spline_dev, c->amplitude, c->ns_frozen, L.g.kmin(), norm, nent, (T *)L.sqrtp.p, c->st);
+}
+
+// the dark-matter column of the transfer table (camb.hpp:181-185): the
+// covectors of the modifications carry this spectrum
+template <typename T> void build_sqrtp_dm(gic_context *c, Level &L) {
+  if (!c->have_camb) throw std::runtime_error("No power spectrum has been set (use the camb command)");
+  long h = L.g.n / 2;
+  long nent = 3 * h * h + 1;
+  L.sqrtp_dm.ensure(nent * sizeof(T));
+  double kw = 2. * M_PI / L.g.box;
+  double norm = kw * kw * kw / pow(2.f * M_PI, 3.f) * double(L.g.n3());
+  sqrtp_table<T>(c->spline_dm_dev, c->amplitude, c->ns_frozen, L.g.kmin(), norm, nent, (T *)L.sqrtp_dm.p, c->st);
 }
 
 // The white noise stays in real space after the draw; the first transform
@@ -713,7 +502,16 @@ template <typename F> int guarded(gic_context *c, F &&f) {
     else fn<double>(__VA_ARGS__);             \
   } while (0)
 
-}  // namespace
+template void build_sqrtp<float>(gic_context *, Level &);
+template void build_sqrtp<double>(gic_context *, Level &);
+template void build_sqrtp_dm<float>(gic_context *, Level &);
+template void build_sqrtp_dm<double>(gic_context *, Level &);
+template void do_draw<float>(gic_context *);
+template void do_draw<double>(gic_context *);
+template void ensure_noise_fourier<float>(gic_context *, Level &);
+template void ensure_noise_fourier<double>(gic_context *, Level &);
+
+}  // namespace gic
 
 extern "C" {
 
@@ -775,9 +573,14 @@ int gic_set_camb_table(gic_context *c, const double *v, size_t nv, int ncols) {
         tall.push_back(v[ncols * j + 6] / tnorm);
       }
     }
+    std::vector<double> tdm;
+    for (size_t j = 0; j < nv / ncols; ++j)
+      if (v[ncols * j] > 0) tdm.push_back(v[ncols * j + 1] / tnorm);
     c->kpts = kp;
     c->tf_all = LogSpline();
     c->tf_all.init(kp, tall);
+    c->tf_dm = LogSpline();
+    c->tf_dm.init(kp, tdm);
     c->ns_frozen = c->cosmo.ns;
     double gf = growth_factor(c->cosmo.a, c->cosmo.Om, c->cosmo.Ol) / growth_factor(1.0, c->cosmo.Om, c->cosmo.Ol);
     double s8pre = variance_in_sphere(c, 8.);
@@ -793,6 +596,16 @@ int gic_set_camb_table(gic_context *c, const double *v, size_t nv, int ncols) {
     GIC_HIP(hipMemcpy(c->spl_c.p, s.c.data(), n * 8, hipMemcpyHostToDevice));
     c->spline_dev = SplineHost{(double *)c->spl_x.p, (double *)c->spl_y.p, (double *)c->spl_c.p, n, c->tf_all.minx,
                                c->tf_all.maxx};
+    const Spline &sd = c->tf_dm.s;
+    int nd = (int)sd.x.size();
+    c->spl_dm_x.ensure(nd * 8);
+    c->spl_dm_y.ensure(nd * 8);
+    c->spl_dm_c.ensure(nd * 8);
+    GIC_HIP(hipMemcpy(c->spl_dm_x.p, sd.x.data(), nd * 8, hipMemcpyHostToDevice));
+    GIC_HIP(hipMemcpy(c->spl_dm_y.p, sd.y.data(), nd * 8, hipMemcpyHostToDevice));
+    GIC_HIP(hipMemcpy(c->spl_dm_c.p, sd.c.data(), nd * 8, hipMemcpyHostToDevice));
+    c->spline_dm_dev = SplineHost{(double *)c->spl_dm_x.p, (double *)c->spl_dm_y.p, (double *)c->spl_dm_c.p, nd,
+                                  c->tf_dm.minx, c->tf_dm.maxx};
     c->have_camb = true;
   });
 }
@@ -938,6 +751,54 @@ int gic_particle_mass_eps(gic_context *c, int level, double *mass, double *eps) 
     double boxMass = 27.744948 * c->cosmo.Om * pow(g.periodic, 3.0);
     if (mass) *mass = boxMass * pow(g.dx / g.periodic, 3.0);
     if (eps) *eps = g.dx * 1.0 * 0.01075;
+  });
+}
+
+int gic_set_flags(gic_context *c, int level, const uint64_t *cells, size_t n) {
+  return guarded(c, [&] {
+    Level &L = level_at(c, level);
+    c->flags.resize(c->levels.size());
+    std::vector<uint64_t> f(cells, cells + n);
+    for (uint64_t v : f)
+      if (v >= L.g.n3()) throw std::out_of_range("flagged cell index out of range for the level");
+    std::sort(f.begin(), f.end());
+    f.erase(std::unique(f.begin(), f.end()), f.end());
+    c->flags[level] = f;
+  });
+}
+
+int gic_set_variance_filter_scale(gic_context *c, double scale) {
+  return guarded(c, [&] { c->variance_scale = scale; });
+}
+
+int gic_calculate(gic_context *c, const char *name, double *out) {
+  return guarded(c, [&] {
+    if (c->prec == 32) *out = mod_calculate<float>(c, name);
+    else *out = mod_calculate<double>(c, name);
+  });
+}
+
+int gic_add_modification(gic_context *c, const char *name, int relative, double target, int init_steps,
+                         double precision) {
+  return guarded(c, [&] {
+    if (c->prec == 32) mod_add<float>(c, name, relative != 0, target, init_steps, precision);
+    else mod_add<double>(c, name, relative != 0, target, init_steps, precision);
+  });
+}
+
+int gic_apply_modifications(gic_context *c, double *pre_chi2, double *post_chi2) {
+  return guarded(c, [&] { DISPATCH(c, mod_apply, c, pre_chi2, post_chi2); });
+}
+
+int gic_clear_modifications(gic_context *c) {
+  return guarded(c, [&] { mod_clear(c); });
+}
+
+int gic_chi2(gic_context *c, double *chi2, uint64_t *dof) {
+  return guarded(c, [&] {
+    if (c->prec == 32) *chi2 = mod_chi2<float>(c);
+    else *chi2 = mod_chi2<double>(c);
+    if (dof) *dof = num_dof(c);
   });
 }
 

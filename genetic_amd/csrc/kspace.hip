@@ -454,6 +454,7 @@ template <typename S, typename D> void convert(const S *a, size_t n, D *b, hipSt
 GIC_INST(float)
 GIC_INST(double)
 template void fill<uint32_t>(uint32_t *, size_t, uint32_t, hipStream_t);
+template void fill<uint8_t>(uint8_t *, size_t, uint8_t, hipStream_t);
 template void unpad<float, float>(const float *, int, float *, hipStream_t);
 template void unpad<float, double>(const float *, int, double *, hipStream_t);
 template void unpad<double, double>(const double *, int, double *, hipStream_t);

@@ -1,0 +1,1050 @@
+// Constrained realisations on the GPU: multi-level covectors, the metric,
+// inner products, chi^2 and the Hoffman-Ribak linear and quadratic
+// (filtered-variance) modifications of the white-noise field.
+//
+//   filters per level        filters/filterfamily.hpp:79-143
+//   inner product            field/multilevelfield.hpp:248-294 (weights of fourier.hpp:110-151)
+//   metric                   field/multilevelfield.hpp:318-384, field.hpp:752-800
+//   cross-level transfer     field.hpp:804-849 (FILTER_ON_COARSE_GRID), evaluator.hpp:166-206
+//   deinterpolation          field.hpp:420-480, multilevelgrid.hpp:246-321
+//   covectors                modifications/linearmodification.hpp:22-412
+//   variance                 modifications/quadraticmodification.hpp:50-221
+//   manager                  modifications/modificationmanager.hpp:80-331
+//   chi^2                    multilevelfield.hpp:461-487
+//
+// Every multi-level field is one padded buffer per level on the device; the
+// inner products reduce in FP64 whatever the field precision.
+#include "context.h"
+
+#include <strings.h>
+#include <unordered_map>
+
+namespace gic {
+
+// ===================================================================== kernels ==
+namespace {
+
+constexpr int RED_BLOCKS = 1024;
+constexpr int RED_THREADS = 256;
+
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double sh[RED_THREADS];
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int s = RED_THREADS / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  return sh[0];
+}
+
+// Sum over stored modes of w * Re(conj(a) b).  The weights reproduce the
+// reference's half-space loop: 2 for 0 < kz < N/2; on the kz = 0 and N/2
+// planes 2 for 0 < kx < N/2, 0 for kx > N/2, and on the kx = 0, N/2 lines
+// 2 / 1 / 0 for ky inside / on / above (0, N/2).
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_inner_partial(const cx<T> *__restrict__ a, const cx<T> *__restrict__ b,
+                                                               int n, double *__restrict__ part) {
+  int h = n / 2;
+  size_t nm = size_t(n) * n * (h + 1);
+  double acc = 0;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nm; i += size_t(gridDim.x) * blockDim.x) {
+    int z = int(i % (h + 1));
+    size_t r = i / (h + 1);
+    int y = int(r % n), x = int(r / n);
+    double w;
+    if (z > 0 && z < h) w = 2;
+    else if (x > 0 && x < h) w = 2;
+    else if (x > h) w = 0;
+    else if (y > 0 && y < h) w = 2;
+    else if (y > h) w = 0;
+    else w = 1;
+    if (w != 0) {
+      cx<T> u = a[i], v = b[i];
+      acc += w * (double(u.x) * double(v.x) + double(u.y) * double(v.y));
+    }
+  }
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__restrict__ part, int np,
+                                                              double *__restrict__ out) {
+  double acc = 0;
+  for (int i = threadIdx.x; i < np; i += blockDim.x) acc += part[i];
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) *out = s;
+}
+
+template <typename T> __global__ void k_axpy(T *__restrict__ y, const T *__restrict__ x, T s, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += s * x[i];
+}
+template <typename T> __global__ void k_scale(T *__restrict__ x, T s, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= s;
+}
+
+// product filters as tables over |k|^2 = m kMin^2: A(k) * B(k) * scale
+struct FilterSpec {
+  int na = 0, nb = 0;
+  int ka[6], kb[6];   // 0 low-pass Fermi, 1 sqrt(1-LP^2), 2 1-LP
+  double ca[6], cb[6];
+  double scale = 1.0;
+  bool zero = false;
+};
+__device__ __forceinline__ double filter_factor(int kind, double kc, double k) {
+  double lp = 1. / (1. + exp((k - kc) / (kc * 0.1)));
+  if (kind == 0) return lp;
+  if (kind == 1) return sqrt(1. - lp * lp);
+  return 1. - lp;
+}
+template <typename T> __global__ void k_filter_spec(FilterSpec s, double kmin, long nent, T *__restrict__ out) {
+  long m = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (m >= nent) return;
+  if (s.zero) {
+    out[m] = T(0);
+    return;
+  }
+  double k = sqrt(double(m)) * kmin;
+  double fa = 1.0, fb = 1.0;
+  for (int i = 0; i < s.na; ++i) fa *= filter_factor(s.ka[i], s.ca[i], k);
+  for (int i = 0; i < s.nb; ++i) fb *= filter_factor(s.kb[i], s.cb[i], k);
+  out[m] = T(fa * fb * s.scale);
+}
+
+// pow(P_all, -1/2) * pow(P_dm, 1/2) from the two sqrt tables (0 where P = 0)
+template <typename T>
+__global__ void k_ratio_table(const T *__restrict__ all, const T *__restrict__ dm, long nent, T *__restrict__ out) {
+  long m = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (m >= nent) return;
+  T a = all[m];
+  out[m] = (a != T(0) ? T(1) / a : T(0)) * dm[m];
+}
+
+// densityToPotential (cosmology/parameters.hpp:209-225) -- zero where ky == 0,
+// as the reference tests k2 (the y component) -- and the velocity covector
+// -s delta / k^2 (linearmodification.hpp:223-263)
+template <typename T> __global__ void k_mode_op(cx<T> *__restrict__ f, int n, double kmin, int kind, double coef) {
+  int h = n / 2;
+  size_t nm = size_t(n) * n * (h + 1);
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  int iz = int(i % (h + 1));
+  size_t r = i / (h + 1);
+  int iy = int(r % n), ix = int(r / n);
+  if (ix > h) ix -= n;
+  if (iy > h) iy -= n;
+  T k1 = T(ix * kmin), k2 = T(iy * kmin), k3 = T(iz * kmin);
+  T ksq = k1 * k1 + k2 * k2 + k3 * k3;
+  cx<T> v = f[i];
+  T c = T(coef);
+  if (kind == 1) {
+    T inv = T(1.0) / ksq;
+    if (k2 == T(0)) inv = T(0);
+    f[i] = cx<T>{(-c * v.x) * inv, (-c * v.y) * inv};
+  } else {
+    if (ksq == T(0)) f[i] = cx<T>{T(0), T(0)};
+    else f[i] = cx<T>{(-c * v.x) / ksq, (-c * v.y) / ksq};
+  }
+}
+
+// Window<T> of a grid in physical coordinates (simulation/window.hpp)
+struct WinDev {
+  double L, lo[3], hi[3];
+};
+__device__ __forceinline__ bool win_contains(const WinDev &w, double x, double y, double z) {
+  double p[3] = {x, y, z};
+  for (int d = 0; d < 3; ++d) {
+    bool in = w.hi[d] >= w.lo[d] ? (p[d] < w.hi[d] && p[d] >= w.lo[d]) : (p[d] < w.hi[d] || p[d] >= w.lo[d]);
+    if (!in) return false;
+  }
+  return true;
+}
+// zero the cells (padded real layout) whose centroid is inside / outside w
+template <typename T>
+__global__ void k_window_zero(T *__restrict__ f, int n, double dx, double ox, double oy, double oz, WinDev w,
+                              int inside) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t n3 = size_t(n) * n * n;
+  if (i >= n3) return;
+  int z = int(i % n);
+  size_t r = i / n;
+  int y = int(r % n), x = int(r / n);
+  bool c = win_contains(w, x * dx + ox + dx / 2, y * dx + oy + dx / 2, z * dx + oz + dx / 2);
+  if (c == (inside != 0)) f[(size_t(x) * n + y) * (n + 2) + z] = T(0);
+}
+
+// keep only the listed cells (windowOperator, quadraticmodification.hpp:180-186)
+template <typename T> __global__ void k_mask_keep(T *__restrict__ f, int n, const uint8_t *__restrict__ mask) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t n3 = size_t(n) * n * n;
+  if (i >= n3) return;
+  if (!mask[i]) {
+    size_t x = i / (size_t(n) * n), y = (i / n) % n, z = i % n;
+    f[(x * n + y) * (n + 2) + z] = T(0);
+  }
+}
+__global__ void k_mask_set(uint8_t *__restrict__ mask, const uint64_t *__restrict__ idx, size_t m) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < m) mask[idx[i]] = 1;
+}
+
+__device__ __forceinline__ size_t padded_of(uint64_t c, int n) {
+  size_t x = c / (size_t(n) * n), y = (c / n) % n, z = c % n;
+  return (x * n + y) * (n + 2) + z;
+}
+
+// varianceOperator (quadraticmodification.hpp:202-218): sum over the
+// flagged cells, then x = (x - sum/m) / m on each of them
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_flag_sum(const T *__restrict__ f, int n, const uint64_t *__restrict__ idx,
+                                                          size_t m, double *__restrict__ part) {
+  double acc = 0;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < m; i += size_t(gridDim.x) * blockDim.x)
+    acc += double(f[padded_of(idx[i], n)]);
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+template <typename T>
+__global__ void k_flag_center(T *__restrict__ f, int n, const uint64_t *__restrict__ idx, size_t m,
+                              const double *__restrict__ sum) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  T mean = T(*sum) / T(m);
+  size_t p = padded_of(idx[i], n);
+  T v = f[p] - mean;
+  f[p] = v / T(m);
+}
+
+// sparse covector values into a zeroed padded field
+template <typename T>
+__global__ void k_scatter(T *__restrict__ f, int n, const uint64_t *__restrict__ idx, const double *__restrict__ val,
+                          size_t m) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < m) f[padded_of(idx[i], n)] = T(val[i]);
+}
+
+// fine -> coarse: the mean of the contained fine subcells of each coarse
+// cell whose first or last subcell lies in the fine grid (SubSampleEvaluator
+// over a SectionOfGrid of the fine grid)
+struct AvgGeom {
+  int r, cell_off[3], simf, nsec, nf, nc;
+};
+__device__ __forceinline__ bool sub_under(const AvgGeom &g, int sx, int sy, int sz, int *u) {
+  if (sx < 0 || sy < 0 || sz < 0 || sx >= g.nsec || sy >= g.nsec || sz >= g.nsec) return false;
+  int s[3] = {sx, sy, sz};
+  for (int a = 0; a < 3; ++a) {
+    int v = s[a] + g.cell_off[a];
+    if (v > g.simf - 1) v -= g.simf;
+    if (v < 0) v += g.simf;
+    if (v < 0 || v >= g.nf) return false;
+    u[a] = v;
+  }
+  return true;
+}
+template <typename T>
+__global__ void k_fine_to_coarse(T *__restrict__ coarse, const T *__restrict__ fine, AvgGeom g) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t nc3 = size_t(g.nc) * g.nc * g.nc;
+  if (i >= nc3) return;
+  int Z = int(i % g.nc);
+  size_t rr = i / g.nc;
+  int Y = int(rr % g.nc), X = int(rr / g.nc);
+  int u[3];
+  int r = g.r;
+  bool c0 = sub_under(g, X * r, Y * r, Z * r, u);
+  bool c1 = sub_under(g, X * r + r - 1, Y * r + r - 1, Z * r + r - 1, u);
+  if (!c0 && !c1) return;
+  T sum = T(0);
+  int cnt = 0;
+  for (int a = 0; a < r; ++a)
+    for (int b = 0; b < r; ++b)
+      for (int c = 0; c < r; ++c)
+        if (sub_under(g, X * r + a, Y * r + b, Z * r + c, u)) {
+          sum += fine[(size_t(u[0]) * g.nf + u[1]) * (g.nf + 2) + u[2]];
+          ++cnt;
+        }
+  coarse[(size_t(X) * g.nc + Y) * (g.nc + 2) + Z] += sum / T(cnt);
+}
+
+// Field::deInterpolate of every fine cell (value / r^3) into the coarse grid,
+// accumulated in the reference's contiguous real layout (n^3 cells plus the
+// tail the reference's vector has): coordinates wrap by the grid size only
+// below 0 or above n, as there, so the same cells alias.
+struct DeinterpGeom {
+  int nf, nc, simc;
+  double fdx, fo[3], cdx, co[3], periodic, inv_vol;
+};
+__device__ __forceinline__ double crw(int i, double t) {
+  double t2 = t * t, t3 = t2 * t;
+  if (i == 0) return -0.5 * t3 + t2 - 0.5 * t;
+  if (i == 1) return 1.5 * t3 - 2.5 * t2 + 1.0;
+  if (i == 2) return -1.5 * t3 + 2.0 * t2 + 0.5 * t;
+  return 0.5 * t3 - 0.5 * t2;
+}
+template <typename T, typename S>
+__global__ void k_deinterp(const S *__restrict__ fine, int fine_padded, DeinterpGeom g, T *__restrict__ acc,
+                           size_t acc_len) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t nf3 = size_t(g.nf) * g.nf * g.nf;
+  if (i >= nf3) return;
+  int z = int(i % g.nf);
+  size_t rr = i / g.nf;
+  int y = int(rr % g.nf), x = int(rr / g.nf);
+  double v = fine_padded ? double(fine[(size_t(x) * g.nf + y) * (g.nf + 2) + z]) : double(fine[i]);
+  if (v == 0) return;
+  double val = v * g.inv_vol;
+  int cf[3] = {x, y, z};
+  int p0[3];
+  double t[3];
+  for (int a = 0; a < 3; ++a) {
+    double loc = cf[a] * g.fdx + g.fo[a] + g.fdx / 2 - g.co[a];
+    double w = fmod(loc, g.periodic);
+    if (w < 0) w += g.periodic;
+    double u = w / g.cdx - 0.5;
+    p0[a] = int(floor(u));
+    t[a] = u - p0[a];
+  }
+  long n = g.nc;
+  int ix[3][4];
+  for (int a = 0; a < 3; ++a)
+    for (int q = 0; q < 4; ++q) {
+      int c = p0[a] + q - 1;
+      if (c < 0) c += int(n);
+      if (c > n) c -= int(n);
+      ix[a][q] = c;
+    }
+  long key[3];
+  for (int a = 0; a < 3; ++a) {
+    long c = ix[a][0];
+    if (c > g.simc - 1) c -= g.simc;
+    if (c < 0) c += g.simc;
+    key[a] = c;
+  }
+  long kidx = (key[0] * n + key[1]) * n + key[2];
+  double wx[4], wy[4], wz[4];
+  for (int q = 0; q < 4; ++q) {
+    wx[q] = crw(q, t[0]);
+    wy[q] = crw(q, t[1]);
+    wz[q] = crw(q, t[2]);
+  }
+  for (int a = 0; a < 4; ++a)
+    for (int b = 0; b < 4; ++b)
+      for (int c = 0; c < 4; ++c) {
+        long idx = kidx + (ix[0][a] - ix[0][0]) * n * n + (ix[1][b] - ix[1][0]) * n + (ix[2][c] - ix[2][0]);
+        if (idx >= 0 && size_t(idx) < acc_len) atomicAdd(&acc[idx], T(val * (wx[a] * wy[b] * wz[c])));
+      }
+}
+
+int blocks(size_t n) { return int((n + 255) / 256); }
+
+}  // namespace
+
+// ============================================================ multi-level fields ==
+struct ModRec {
+  std::string name;
+  bool quadratic = false;
+  double target = 0, scale = -1, precision = 0.001;
+  int init_steps = 10;
+  std::vector<std::vector<uint64_t>> flags;
+};
+
+}  // namespace gic
+
+struct ModState {
+  std::vector<gic::ModRec> linear, quadratic;
+  gic::DevBuf part, scalar;
+};
+
+gic_context::gic_context() = default;
+gic_context::~gic_context() = default;
+
+namespace gic {
+
+namespace {
+
+size_t table_len(const LevelGeom &g) {
+  long h = g.n / 2;
+  return size_t(3 * h * h + 1);
+}
+
+template <typename T> struct MLF {
+  gic_context *c = nullptr;
+  std::vector<DevBuf> lv;
+  std::vector<bool> fourier;
+  explicit MLF(gic_context *cc) : c(cc) {
+    size_t nl = c->levels.size();
+    lv.resize(nl);
+    fourier.assign(nl, false);
+    for (size_t l = 0; l < nl; ++l) lv[l].ensure(c->levels[l]->g.padded_reals() * sizeof(T));
+  }
+  MLF(const MLF &o) : c(o.c), fourier(o.fourier) {
+    lv.resize(o.lv.size());
+    for (size_t l = 0; l < lv.size(); ++l) {
+      lv[l].ensure(o.lv[l].bytes);
+      GIC_HIP(hipMemcpyAsync(lv[l].p, o.lv[l].p, o.lv[l].bytes, hipMemcpyDeviceToDevice, c->st));
+    }
+  }
+  MLF &operator=(const MLF &) = delete;
+  size_t nl() const { return lv.size(); }
+  T *p(size_t l) { return (T *)lv[l].p; }
+  const T *p(size_t l) const { return (const T *)lv[l].p; }
+  const LevelGeom &g(size_t l) const { return c->levels[l]->g; }
+  size_t reals(size_t l) const { return g(l).padded_reals(); }
+  void zero() {
+    for (size_t l = 0; l < nl(); ++l) fill<T>(p(l), reals(l), T(0), c->st);
+  }
+  void toF(size_t l) {
+    bool f = fourier[l];
+    to_fourier<T>(c, lv[l], g(l).n, f);
+    fourier[l] = f;
+  }
+  void toR(size_t l) {
+    bool f = fourier[l];
+    to_real<T>(c, lv[l], g(l).n, f);
+    fourier[l] = f;
+  }
+  void toF() {
+    for (size_t l = 0; l < nl(); ++l) toF(l);
+  }
+  void axpy(const MLF &x, double s) {   // this += s x, same spaces
+    for (size_t l = 0; l < nl(); ++l) {
+      size_t n = reals(l);
+      hipLaunchKernelGGL(k_axpy<T>, dim3(blocks(n)), dim3(256), 0, c->st, p(l), x.p(l), T(s), n);
+    }
+  }
+  void scale(double s) {
+    for (size_t l = 0; l < nl(); ++l) {
+      size_t n = reals(l);
+      hipLaunchKernelGGL(k_scale<T>, dim3(blocks(n)), dim3(256), 0, c->st, p(l), T(s), n);
+    }
+  }
+};
+
+// the output field (noise buffers) as a multi-level field
+template <typename T> MLF<T> copy_output(gic_context *c) {
+  MLF<T> m(c);
+  for (size_t l = 0; l < m.nl(); ++l) {
+    Level &L = *c->levels[l];
+    if (L.dc_pending) ensure_noise_fourier<T>(c, L);
+    GIC_HIP(hipMemcpyAsync(m.lv[l].p, L.noise.p, m.reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
+    m.fourier[l] = L.noise_fourier;
+  }
+  return m;
+}
+template <typename T> void store_output(gic_context *c, MLF<T> &m) {
+  m.toF();
+  for (size_t l = 0; l < m.nl(); ++l) {
+    Level &L = *c->levels[l];
+    GIC_HIP(hipMemcpyAsync(L.noise.p, m.lv[l].p, m.reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
+    L.noise_fourier = true;
+    L.dc_pending = false;
+  }
+}
+
+bool combined(gic_context *c) { return c->zeldovich_done && c->levels.size() >= 2; }
+
+// filter of level l (FilterFamily): prod_{j<l} HP(kc_j) * LP(kc_l) (not on the last)
+void level_factors(gic_context *c, size_t l, int &nf, int *kind, double *kc, bool &zero) {
+  size_t nl = c->levels.size();
+  nf = 0;
+  zero = false;
+  if (combined(c)) {
+    zero = l + 1 < nl;
+    return;
+  }
+  for (size_t j = 0; j < l; ++j) {
+    kind[nf] = 1;
+    kc[nf++] = kcut_for_level(c, int(j));
+  }
+  if (l + 1 < nl) {
+    kind[nf] = 0;
+    kc[nf++] = kcut_for_level(c, int(l));
+  }
+}
+
+FilterSpec spec_levels(gic_context *c, long la, long lb, double scale) {
+  FilterSpec s;
+  bool za = false, zb = false;
+  if (la >= 0) level_factors(c, size_t(la), s.na, s.ka, s.ca, za);
+  if (lb >= 0) level_factors(c, size_t(lb), s.nb, s.kb, s.cb, zb);
+  s.zero = za || zb;
+  s.scale = scale;
+  return s;
+}
+
+// multiply level l of m (Fourier) by the filter spec evaluated on that level's k
+template <typename T> void apply_spec(MLF<T> &m, size_t l, const FilterSpec &s) {
+  gic_context *c = m.c;
+  const LevelGeom &g = m.g(l);
+  size_t nent = table_len(g);
+  c->tab_a.ensure(nent * sizeof(T));
+  m.toF(l);
+  hipLaunchKernelGGL(k_filter_spec<T>, dim3(blocks(nent)), dim3(256), 0, c->st, s, g.kmin(), long(nent),
+                     (T *)c->tab_a.p);
+  mul_table<T>(m.p(l), g.n, (const T *)c->tab_a.p, c->st);
+}
+
+// sum_l sum_k w Re(conj(a) b), both in Fourier space
+template <typename T> double inner(gic_context *c, MLF<T> &a, MLF<T> &b) {
+  a.toF();
+  b.toF();
+  c->mod->part.ensure(RED_BLOCKS * sizeof(double) * 8);
+  c->mod->scalar.ensure(sizeof(double) * 8);
+  double total = 0;
+  for (size_t l = 0; l < a.nl(); ++l) {
+    hipLaunchKernelGGL(k_inner_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st,
+                       reinterpret_cast<const cx<T> *>(a.p(l)), reinterpret_cast<const cx<T> *>(b.p(l)), a.g(l).n,
+                       (double *)c->mod->part.p);
+    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, c->st, (const double *)c->mod->part.p,
+                       RED_BLOCKS, (double *)c->mod->scalar.p);
+    double v;
+    GIC_HIP(hipMemcpyAsync(&v, c->mod->scalar.p, sizeof v, hipMemcpyDeviceToHost, c->st));
+    GIC_HIP(hipStreamSynchronize(c->st));
+    total += v;
+  }
+  return total;
+}
+
+WinDev grid_window(const LevelGeom &g) {   // Grid::getWindow (grid.hpp:502-504)
+  WinDev w;
+  w.L = g.periodic;
+  double lo[3] = {g.ox, g.oy, g.oz};
+  for (int d = 0; d < 3; ++d) {
+    double a = lo[d], b = lo[d] + g.box;
+    auto wrap = [&](double v) {
+      if (v < 0) v += w.L;
+      if (v >= w.L) v -= w.L;
+      return v;
+    };
+    w.lo[d] = wrap(a);
+    w.hi[d] = std::nextafter(wrap(std::nextafter(b, -INFINITY)), INFINITY);
+  }
+  return w;
+}
+
+template <typename T> void window_zero(MLF<T> &m, size_t l, const WinDev &w, bool inside) {
+  m.toR(l);
+  const LevelGeom &g = m.g(l);
+  size_t n3 = g.n3();
+  hipLaunchKernelGGL(k_window_zero<T>, dim3(blocks(n3)), dim3(256), 0, m.c->st, m.p(l), g.n, g.dx, g.ox, g.oy, g.oz,
+                     w, inside ? 1 : 0);
+}
+
+// applyFilterInWindow (field.hpp:752-775) on level l of m
+template <typename T> void filter_in_window(MLF<T> &m, size_t l, const FilterSpec &s, const WinDev &w, bool first) {
+  MLF<T> in(m);
+  if (first) {
+    window_zero(in, l, w, false);
+    apply_spec(in, l, s);
+  } else {
+    apply_spec(in, l, s);
+    window_zero(in, l, w, false);
+  }
+  window_zero(m, l, w, true);
+  if (in.fourier[l]) m.toF(l);
+  else m.toR(l);
+  size_t n = m.reals(l);
+  hipLaunchKernelGGL(k_axpy<T>, dim3(blocks(n)), dim3(256), 0, m.c->st, m.p(l), in.p(l), T(1), n);
+}
+
+// target level lt += transfer of source level ls of src (already filtered, real)
+template <typename T> void add_other_level(gic_context *c, T *target, const LevelGeom &gt, const T *source,
+                                           const LevelGeom &gs) {
+  if (gs.dx > gt.dx) {
+    CoarseView v;
+    v.p = source;
+    for (int a = 0; a < 3; ++a) {
+      v.org[a] = 0;
+      v.len[a] = gs.n;
+    }
+    v.sx = long(gs.n) * (gs.n + 2);
+    v.sy = gs.n + 2;
+    interp_add<T>(target, gt.n, v, interp_geom(gt, gs), c->st);
+  } else {
+    AvgGeom a;
+    a.r = (int)llround(gt.dx / gs.dx);
+    a.cell_off[0] = (int)llround((gt.ox - gs.ox) / gs.dx);
+    a.cell_off[1] = (int)llround((gt.oy - gs.oy) / gs.dx);
+    a.cell_off[2] = (int)llround((gt.oz - gs.oz) / gs.dx);
+    a.simf = (int)llround(gs.periodic / gs.dx);
+    a.nsec = gt.n * a.r;
+    a.nf = gs.n;
+    a.nc = gt.n;
+    size_t nc3 = gt.n3();
+    hipLaunchKernelGGL(k_fine_to_coarse<T>, dim3(blocks(nc3)), dim3(256), 0, c->st, target, source, a);
+  }
+  GIC_HIP(hipGetLastError());
+}
+
+double level_weight(gic_context *c, size_t l) {
+  double r = c->levels[l]->g.dx / c->levels[0]->g.dx;
+  return l == 0 ? 1.0 : pow(r, 3.0);
+}
+
+// applyMetric (multilevelfield.hpp:335-384)
+template <typename T> MLF<T> metric(MLF<T> &f, bool to_covector) {
+  gic_context *c = f.c;
+  size_t nl = f.nl();
+  f.toF();
+  MLF<T> out(c);
+  for (size_t l = 0; l < nl; ++l) {
+    MLF<T> res(f);   // only level l is used
+    if (to_covector && l + 1 < nl) {
+      WinDev w = grid_window(c->levels[l + 1]->g);
+      FilterSpec fl = spec_levels(c, long(l), -1, 1.0);
+      filter_in_window(res, l, fl, w, true);
+      filter_in_window(res, l, fl, w, false);
+    } else {
+      apply_spec(res, l, spec_levels(c, long(l), long(l), 1.0));
+    }
+    for (size_t s = 0; s < nl; ++s) {
+      if (s == l) continue;
+      double sq = sqrt(level_weight(c, l) / level_weight(c, s));
+      MLF<T> tmp(f);
+      apply_spec(tmp, s, spec_levels(c, long(l), long(s), sq));
+      tmp.toR(s);
+      res.toR(l);
+      add_other_level<T>(c, res.p(l), res.g(l), tmp.p(s), tmp.g(s));
+    }
+    res.toF(l);
+    GIC_HIP(hipMemcpyAsync(out.p(l), res.p(l), out.reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
+    out.fourier[l] = true;
+  }
+  return out;
+}
+
+// applyTransferRatio(field species -> dm): white noise gets sqrt P_dm; an
+// output field (species 'all') gets sqrt(P_dm / P_all)
+template <typename T> void transfer_to_dm(MLF<T> &m) {
+  gic_context *c = m.c;
+  for (size_t l = 0; l < m.nl(); ++l) {
+    Level &L = *c->levels[l];
+    build_sqrtp_dm<T>(c, L);
+    m.toF(l);
+    if (!c->power_applied) {
+      mul_table<T>(m.p(l), L.g.n, (const T *)L.sqrtp_dm.p, c->st);
+    } else {
+      build_sqrtp<T>(c, L);
+      size_t nent = table_len(L.g);
+      c->tab_b.ensure(nent * sizeof(T));
+      hipLaunchKernelGGL(k_ratio_table<T>, dim3(blocks(nent)), dim3(256), 0, c->st, (const T *)L.sqrtp.p,
+                         (const T *)L.sqrtp_dm.p, long(nent), (T *)c->tab_b.p);
+      mul_table<T>(m.p(l), L.g.n, (const T *)c->tab_b.p, c->st);
+    }
+  }
+}
+
+struct DevList {
+  DevBuf idx, val;
+  size_t n = 0;
+};
+void upload_list(gic_context *c, const std::vector<uint64_t> &idx, const std::vector<double> *val, DevList &out) {
+  out.n = idx.size();
+  if (!out.n) return;
+  out.idx.ensure(out.n * 8);
+  GIC_HIP(hipMemcpyAsync(out.idx.p, idx.data(), out.n * 8, hipMemcpyHostToDevice, c->st));
+  if (val) {
+    out.val.ensure(out.n * 8);
+    GIC_HIP(hipMemcpyAsync(out.val.p, val->data(), out.n * 8, hipMemcpyHostToDevice, c->st));
+  }
+  GIC_HIP(hipStreamSynchronize(c->st));
+}
+
+int mod_kind(const std::string &name) {
+  static const char *names[] = {"overdensity", "potential", "vx", "vy", "vz", "Lx", "Ly", "Lz"};
+  for (int i = 0; i < 8; ++i)
+    if (!strcasecmp(name.c_str(), names[i])) return i;
+  return -1;
+}
+
+// localisation covector of the finest level (linearmodification.hpp:60-70,
+// 198-226, 330-370) as (cell, value) pairs, summed in the reference's order
+void localisation(gic_context *c, int kind, const std::vector<uint64_t> &cells, std::vector<uint64_t> &idx,
+                  std::vector<double> &val) {
+  const LevelGeom &g = c->levels.back()->g;
+  long n = g.n;
+  int sim = (int)llround(g.periodic / g.dx);
+  std::unordered_map<uint64_t, size_t> pos;
+  auto add = [&](long i, double v) {
+    if (i < 0 || size_t(i) >= g.n3()) return;   // steps leaving a zoom grid
+    auto it = pos.find(uint64_t(i));
+    if (it == pos.end()) {
+      pos.emplace(uint64_t(i), idx.size());
+      idx.push_back(uint64_t(i));
+      val.push_back(v);
+    } else {
+      val[it->second] += v;
+    }
+  };
+  auto step = [&](uint64_t index, int dir, int s) -> long {
+    long q[3] = {long(index / (n * n)), long((index / n) % n), long(index % n)};
+    q[dir] += s;
+    for (int a = 0; a < 3; ++a) {
+      if (q[a] > sim - 1) q[a] -= sim;
+      if (q[a] < 0) q[a] += sim;
+    }
+    return (q[0] * n + q[1]) * n + q[2];
+  };
+  double w = 1.0 / cells.size();
+  auto centroid = [&](uint64_t i, double *p) {
+    p[0] = double(i / (n * n)) * g.dx + g.ox + g.dx / 2;
+    p[1] = double((i / n) % n) * g.dx + g.oy + g.dx / 2;
+    p[2] = double(i % n) * g.dx + g.oz + g.dx / 2;
+  };
+  if (kind <= 1) {
+    for (uint64_t cl : cells) add(long(cl), w);
+  } else if (kind <= 4) {
+    int dir = kind - 2;
+    double a = -w / 12. / g.dx, b = w * 2. / 3. / g.dx;
+    for (uint64_t cl : cells) {
+      long m1 = step(cl, dir, -1), p1 = step(cl, dir, +1);
+      long m2 = step(uint64_t(m1), dir, -1), p2 = step(uint64_t(p1), dir, +1);
+      add(m2, a);
+      add(m1, b);
+      add(p1, -b);
+      add(p2, -a);
+    }
+  } else {
+    int dir = kind - 5;
+    auto woff = [&](double x0, double x1) {
+      double r = x0 - x1;
+      if (r > g.periodic / 2) r -= g.periodic;
+      if (r < -g.periodic / 2) r += g.periodic;
+      return r;
+    };
+    double p0[3], q[3], s[3] = {0, 0, 0}, qc[3];
+    centroid(cells[0], p0);
+    for (size_t i = 1; i < cells.size(); ++i) {
+      centroid(cells[i], q);
+      for (int d = 0; d < 3; ++d) s[d] += woff(q[d], p0[d]);
+    }
+    for (int d = 0; d < 3; ++d) {
+      double v = fmod(s[d] / double(cells.size()) + p0[d], g.periodic);
+      if (v < 0) v += g.periodic;
+      qc[d] = v;
+    }
+    double norm = double(cells.size()) * g.dx;
+    double a = 1. / 12. / norm, b = -2. / 3. / norm;
+    int dp1 = (dir + 1) % 3, dp2 = (dir + 2) % 3;
+    for (uint64_t cl : cells) {
+      centroid(cl, q);
+      double dq[3], cr[3];
+      for (int d = 0; d < 3; ++d) dq[d] = woff(qc[d], q[d]);
+      cr[dir] = 0;
+      cr[dp2] = dq[dp1];
+      cr[dp1] = -dq[dp2];
+      for (int d = 0; d < 3; ++d) {
+        if (d == dir) continue;
+        long m1 = step(cl, d, -1), m2 = step(uint64_t(m1), d, -1);
+        long p1 = step(cl, d, +1), p2 = step(uint64_t(p1), d, +1);
+        add(m2, cr[d] * a);
+        add(m1, cr[d] * b);
+        add(p1, -cr[d] * b);
+        add(p2, -cr[d] * a);
+      }
+    }
+  }
+}
+
+// LinearModification::getCovector with generateMultilevelCovectorFromHiresCovector
+template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &name,
+                                             const std::vector<std::vector<uint64_t>> &flags) {
+  int kind = mod_kind(name);
+  if (kind < 0) throw std::runtime_error(name + " is an unknown modification name");
+  size_t nl = c->levels.size(), L = nl - 1;
+  if (flags.size() < nl || flags[L].empty())
+    throw std::runtime_error(
+        "Linear modifications use exclusively information from the highest resolution grid but no cells have been "
+        "flagged on this grid.\nChange flagged cells selection to use linear modifications.");
+  if (kind >= 2 && kind <= 4 && combined(c))
+    throw std::runtime_error(
+        "Cannot create velocity covector for recombined fields (i.e. after output generation is complete)");
+  std::vector<uint64_t> idx;
+  std::vector<double> val;
+  localisation(c, kind, flags[L], idx, val);
+  MLF<T> m(c);
+  m.zero();
+  DevList dl;
+  upload_list(c, idx, &val, dl);
+  if (dl.n)
+    hipLaunchKernelGGL(k_scatter<T>, dim3(blocks(dl.n)), dim3(256), 0, c->st, m.p(L), m.g(L).n,
+                       (const uint64_t *)dl.idx.p, (const double *)dl.val.p, dl.n);
+  // deinterpolate down the levels, one level at a time, in the contiguous layout
+  if (L > 0) {
+    DevBuf accs[2];
+    const T *src = m.p(L);
+    int src_padded = 1;
+    for (long l = long(L) - 1; l >= 0; --l) {
+      const LevelGeom &gh = m.g(size_t(l) + 1), &gl = m.g(size_t(l));
+      size_t len = gl.n3() + 2 * size_t(gl.n) * gl.n;
+      DevBuf &acc = accs[l & 1];
+      acc.ensure(len * sizeof(T));
+      fill<T>((T *)acc.p, len, T(0), c->st);
+      DeinterpGeom dg;
+      dg.nf = gh.n;
+      dg.nc = gl.n;
+      dg.simc = (int)llround(gl.periodic / gl.dx);
+      dg.fdx = gh.dx;
+      dg.fo[0] = gh.ox;
+      dg.fo[1] = gh.oy;
+      dg.fo[2] = gh.oz;
+      dg.cdx = gl.dx;
+      dg.co[0] = gl.ox;
+      dg.co[1] = gl.oy;
+      dg.co[2] = gl.oz;
+      dg.periodic = gl.periodic;
+      double r = double(llround(gl.dx / gh.dx));
+      dg.inv_vol = 1.0 / (r * r * r);
+      size_t nf3 = gh.n3();
+      hipLaunchKernelGGL((k_deinterp<T, T>), dim3(blocks(nf3)), dim3(256), 0, c->st, src, src_padded, dg, (T *)acc.p,
+                         len);
+      GIC_HIP(hipGetLastError());
+      pad<T, T>((const T *)acc.p, gl.n, m.p(size_t(l)), c->st);
+      src = (const T *)acc.p;   // the next level reads this level's contiguous values
+      src_padded = 0;
+    }
+  }
+  for (size_t l = 0; l < nl; ++l) {
+    double s = level_weight(c, l) / level_weight(c, nl - 1);
+    if (s != 1.0) {
+      size_t n = m.reals(l);
+      hipLaunchKernelGGL(k_scale<T>, dim3(blocks(n)), dim3(256), 0, c->st, m.p(l), T(s), n);
+    }
+  }
+  for (size_t l = 0; l < nl; ++l) apply_spec(m, l, spec_levels(c, long(l), -1, 1.0));
+  transfer_to_dm(m);
+  if (kind >= 1) {
+    double coef;
+    int op;
+    if (kind == 1) {
+      const Cosmo &cc = c->cosmo;
+      coef = 3. / 2. * cc.Om / cc.a * 100. * 100. / (299792.) / (299792.);
+      op = 1;
+    } else {
+      const Cosmo &cc = c->cosmo;
+      coef = 1.0 * 100. * sqrt(cc.Om / (cc.a * cc.a * cc.a) + cc.Ol) * sqrt(cc.a);
+      op = 2;
+    }
+    for (size_t l = 0; l < nl; ++l) {
+      const LevelGeom &g = m.g(l);
+      size_t nm = size_t(g.n) * g.n * (g.n / 2 + 1);
+      hipLaunchKernelGGL(k_mode_op<T>, dim3(blocks(nm)), dim3(256), 0, c->st, reinterpret_cast<cx<T> *>(m.p(l)), g.n,
+                         g.kmin(), op, coef);
+    }
+  }
+  GIC_HIP(hipGetLastError());
+  return m;
+}
+
+// FilteredVarianceModification::pushMultiLevelFieldThroughMatrix
+template <typename T> MLF<T> push_variance(MLF<T> &field, const std::vector<std::vector<uint64_t>> &flags,
+                                           double scale) {
+  gic_context *c = field.c;
+  MLF<T> f(field);
+  transfer_to_dm(f);
+  FilterSpec hp;
+  hp.na = 1;
+  hp.ka[0] = 2;
+  hp.ca[0] = 2 * M_PI / scale;
+  for (size_t l = 0; l < f.nl(); ++l) {
+    const LevelGeom &g = f.g(l);
+    size_t n3 = g.n3();
+    const std::vector<uint64_t> &cells = l < flags.size() ? flags[l] : std::vector<uint64_t>();
+    DevBuf mask;
+    mask.ensure(n3);
+    fill<uint8_t>((uint8_t *)mask.p, n3, 0, c->st);
+    DevList dl;
+    upload_list(c, cells, nullptr, dl);
+    if (dl.n)
+      hipLaunchKernelGGL(k_mask_set, dim3(blocks(dl.n)), dim3(256), 0, c->st, (uint8_t *)mask.p,
+                         (const uint64_t *)dl.idx.p, dl.n);
+    auto window = [&]() {
+      f.toR(l);
+      hipLaunchKernelGGL(k_mask_keep<T>, dim3(blocks(n3)), dim3(256), 0, c->st, f.p(l), g.n, (const uint8_t *)mask.p);
+    };
+    window();
+    apply_spec(f, l, hp);
+    window();
+    if (dl.n) {
+      c->mod->part.ensure(RED_BLOCKS * sizeof(double) * 8);
+      c->mod->scalar.ensure(sizeof(double) * 8);
+      hipLaunchKernelGGL(k_flag_sum<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, (const T *)f.p(l), g.n,
+                         (const uint64_t *)dl.idx.p, dl.n, (double *)c->mod->part.p);
+      hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, c->st, (const double *)c->mod->part.p,
+                         RED_BLOCKS, (double *)c->mod->scalar.p);
+      hipLaunchKernelGGL(k_flag_center<T>, dim3(blocks(dl.n)), dim3(256), 0, c->st, f.p(l), g.n,
+                         (const uint64_t *)dl.idx.p, dl.n, (const double *)c->mod->scalar.p);
+    }
+    apply_spec(f, l, hp);
+    window();
+    GIC_HIP(hipStreamSynchronize(c->st));   // the mask and lists are released here
+  }
+  MLF<T> cov = metric(f, true);
+  transfer_to_dm(cov);
+  return cov;
+}
+
+void check_variance_scale(gic_context *c, double scale) {
+  if (scale < 0.0)
+    throw std::runtime_error(
+        "Trying to calculate filtered variance without initialising variance filtering scale. Use filtering_scale "
+        "command to do this.");
+  if (scale < c->levels.back()->g.dx)
+    throw std::runtime_error("Variance high-pass filtering scale is smaller than the smallest pixel size.");
+}
+
+template <typename T> double variance_value(MLF<T> &field, const std::vector<std::vector<uint64_t>> &flags,
+                                            double scale) {
+  MLF<T> p = push_variance(field, flags, scale);
+  return inner(field.c, p, field);
+}
+
+template <typename T> void ensure_ready(gic_context *c) {
+  if (c->levels.empty()) throw std::runtime_error("No grid has yet been initialised");
+  if (!c->mod) c->mod = std::make_unique<ModState>();
+  if (!c->drawn) do_draw<T>(c);
+  c->flags.resize(c->levels.size());
+}
+
+}  // namespace
+
+// ============================================================== entry points ==
+template <typename T> double mod_calculate(gic_context *c, const std::string &name) {
+  ensure_ready<T>(c);
+  MLF<T> field = copy_output<T>(c);
+  field.toF();
+  if (!strcasecmp(name.c_str(), "variance")) {
+    check_variance_scale(c, c->variance_scale);
+    return variance_value(field, c->flags, c->variance_scale);
+  }
+  MLF<T> cov = linear_covector<T>(c, name, c->flags);
+  return inner(c, cov, field);
+}
+
+template <typename T> void mod_add(gic_context *c, const std::string &name, bool relative, double target,
+                                   int init_steps, double precision) {
+  ensure_ready<T>(c);
+  ModRec m;
+  m.name = name;
+  m.flags = c->flags;
+  m.scale = c->variance_scale;
+  m.init_steps = init_steps;
+  m.precision = precision;
+  m.quadratic = !strcasecmp(name.c_str(), "variance");
+  if (!m.quadratic && mod_kind(name) < 0) throw std::runtime_error(name + " is an unknown modification name");
+  if (m.quadratic) check_variance_scale(c, m.scale);
+  if (relative) target *= mod_calculate<T>(c, name);
+  m.target = target;
+  (m.quadratic ? c->mod->quadratic : c->mod->linear).push_back(m);
+}
+
+void mod_clear(gic_context *c) {
+  if (c->mod) {
+    c->mod->linear.clear();
+    c->mod->quadratic.clear();
+  }
+}
+
+template <typename T> double mod_chi2(gic_context *c) {
+  ensure_ready<T>(c);
+  if (c->power_applied) throw std::runtime_error("Cannot calculate the chi^2 after the output field has been generated");
+  MLF<T> field = copy_output<T>(c);
+  field.toF();
+  MLF<T> cov = metric(field, true);
+  return inner(c, cov, field);
+}
+
+size_t num_dof(gic_context *c) {
+  size_t dof = 0;
+  for (size_t i = 0; i < c->levels.size(); ++i) {
+    dof += c->levels[i]->g.n3();
+    if (i > 0) {
+      size_t f = (size_t)llround(c->levels[i - 1]->g.dx / c->levels[i]->g.dx);
+      dof -= c->levels[i]->g.n3() / (f * f * f);
+    }
+  }
+  return dof;
+}
+
+// ModificationManager::applyModifications (modificationmanager.hpp:80-331)
+template <typename T> void mod_apply(gic_context *c, double *pre, double *post) {
+  ensure_ready<T>(c);
+  double chi_pre = mod_chi2<T>(c);
+  MLF<T> field = copy_output<T>(c);
+  field.toF();
+  std::vector<MLF<T>> alphas;
+  std::vector<double> targets;
+  for (auto &m : c->mod->linear) {
+    alphas.push_back(linear_covector<T>(c, m.name, m.flags));
+    targets.push_back(m.target);
+  }
+  auto cov_dot = [&](MLF<T> &a, MLF<T> &b) {
+    MLF<T> bv = metric(b, false);
+    return inner(c, a, bv);
+  };
+  for (size_t i = 0; i < alphas.size(); ++i) {
+    for (size_t j = 0; j < i; ++j) {
+      double r = cov_dot(alphas[i], alphas[j]);
+      alphas[i].axpy(alphas[j], -r);
+      targets[i] -= r * targets[j];
+    }
+    double norm = sqrt(cov_dot(alphas[i], alphas[i]));
+    alphas[i].scale(1.0 / norm);
+    targets[i] /= norm;
+  }
+  std::vector<double> existing;
+  for (auto &a : alphas) existing.push_back(inner(c, a, field));
+  std::vector<MLF<T>> vecs;
+  for (size_t i = 0; i < alphas.size(); ++i) {
+    double dval = targets[i] - existing[i];
+    vecs.push_back(metric(alphas[i], false));
+    field.axpy(vecs.back(), dval);
+  }
+  for (auto &m : c->mod->quadratic) {
+    auto iterate = [&](MLF<T> &f, int nsteps) {
+      double start = variance_value(f, m.flags, m.scale);
+      double stepv = (m.target - start) / nsteps;
+      for (int i = 0; i < nsteps; ++i) {
+        double cur = variance_value(f, m.flags, m.scale);
+        MLF<T> pushed = push_variance(f, m.flags, m.scale);
+        double norm = sqrt(cov_dot(pushed, pushed));
+        for (auto &a : vecs) {
+          double r = inner(c, pushed, a);
+          pushed.axpy(a, -r);
+        }
+        double n2 = sqrt(cov_dot(pushed, pushed));
+        pushed.scale(1.0 / n2);
+        double target_next = start + (i + 1) * stepv;
+        double mult = 0.5 * (target_next - cur) / norm;
+        MLF<T> pv = metric(pushed, false);
+        f.axpy(pv, mult);
+      }
+    };
+    int init = m.init_steps;
+    {
+      MLF<T> test(field);
+      iterate(test, init);
+      double achieved = std::fabs(variance_value(test, m.flags, m.scale) - m.target);
+      double want = m.target * m.precision;
+      int nsteps = init * (int)std::ceil(std::sqrt(achieved / want));
+      iterate(field, nsteps > init ? nsteps : init);
+    }
+  }
+  store_output<T>(c, field);
+  double chi_post = mod_chi2<T>(c);
+  if (pre) *pre = chi_pre;
+  if (post) *post = chi_post;
+}
+
+template double mod_calculate<float>(gic_context *, const std::string &);
+template double mod_calculate<double>(gic_context *, const std::string &);
+template void mod_add<float>(gic_context *, const std::string &, bool, double, int, double);
+template void mod_add<double>(gic_context *, const std::string &, bool, double, int, double);
+template void mod_apply<float>(gic_context *, double *, double *);
+template void mod_apply<double>(gic_context *, double *, double *);
+template double mod_chi2<float>(gic_context *);
+template double mod_chi2<double>(gic_context *);
+
+}  // namespace gic

@@ -43,6 +43,13 @@ SIGNATURES = {
     "gic_copy_field": ([_v, _i, _i, _i, _v, _i], _i),
     "gic_set_field": ([_v, _i, _i, _i, _v, _i], _i),
     "gic_mode_power": ([_v, _i, _v, _v, _v, _s], _i),
+    "gic_set_flags": ([_v, _i, _v, _s], _i),
+    "gic_set_variance_filter_scale": ([_v, _d], _i),
+    "gic_calculate": ([_v, ctypes.c_char_p, _P(_d)], _i),
+    "gic_add_modification": ([_v, ctypes.c_char_p, _i, _d, _i, _d], _i),
+    "gic_apply_modifications": ([_v, _P(_d), _P(_d)], _i),
+    "gic_clear_modifications": ([_v], _i),
+    "gic_chi2": ([_v, _P(_d), _P(ctypes.c_uint64)], _i),
     "gic_particles": ([_v, _i, _v, _s, _s, _v, _v], _i),
     "gic_particle_mass_eps": ([_v, _i, _P(_d), _P(_d)], _i),
     "gic_velocity_ratio": ([_v, _P(_d)], _i),
@@ -224,6 +231,36 @@ class ICGenerator:
         cp = _ptr(cells) if cells is not None else None
         self._check(lib().gic_particles(self._h, level, cp, start, count, _ptr(pos), _ptr(vel)))
         return pos, vel
+
+    # --- constrained realisations (ic.hpp:1592-1631) ---------------------
+    def set_flags(self, level, cells):
+        c = np.ascontiguousarray(np.asarray(cells, dtype=np.uint64))
+        self._check(lib().gic_set_flags(self._h, level, _ptr(c), c.size))
+
+    def set_variance_filter_scale(self, scale):
+        self._check(lib().gic_set_variance_filter_scale(self._h, float(scale)))
+
+    def calculate(self, name):
+        out = ctypes.c_double()
+        self._check(lib().gic_calculate(self._h, name.encode(), ctypes.byref(out)))
+        return out.value
+
+    def modify(self, name, relative, target, init_steps=10, precision=0.001):
+        self._check(lib().gic_add_modification(self._h, name.encode(), int(bool(relative)), float(target),
+                                               int(init_steps), float(precision)))
+
+    def apply_modifications(self):
+        pre, post = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().gic_apply_modifications(self._h, ctypes.byref(pre), ctypes.byref(post)))
+        return pre.value, post.value
+
+    def clear_modifications(self):
+        self._check(lib().gic_clear_modifications(self._h))
+
+    def chi2(self):
+        out, dof = ctypes.c_double(), ctypes.c_uint64()
+        self._check(lib().gic_chi2(self._h, ctypes.byref(out), ctypes.byref(dof)))
+        return out.value, dof.value
 
     def mode_power(self, level):
         """Per-|k|^2 sums of |delta_k|^2, mode counts and theory covariance."""

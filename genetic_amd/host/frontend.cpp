@@ -28,6 +28,7 @@
 #include <sstream>
 #include <stdexcept>
 #include <string>
+#include <strings.h>
 #include <tuple>
 #include <vector>
 
@@ -288,6 +289,10 @@ struct Generator {
   int nGadgetFiles = 1;
   unsigned mruGadgetType = 1;
   bool usedGadgetType[6] = {false, false, false, false, false, false};
+  double varianceScale = -1;
+  size_t nModifications = 0;
+  int initialNumberSteps = 10;   // ic.hpp:155-161
+  double precision = 0.001;
 
   explicit Generator(bool f32) : fp32(f32) {}
   ~Generator() {
@@ -772,8 +777,101 @@ struct Generator {
     return out;
   }
 
+  // --- modifications (ic.hpp:1592-1631) ---
+  void syncFlags() {
+    for (size_t l = 0; l < grids.size(); ++l) {
+      std::vector<uint64_t> f(grids[l].flags.begin(), grids[l].flags.end());
+      check(gic_set_flags(c(), int(l), f.data(), f.size()));
+    }
+    check(gic_set_variance_filter_scale(c(), varianceScale));
+  }
+  // warnings of the Modification and FilteredVarianceModification constructors
+  // (modification.hpp:36-49, quadraticmodification.hpp:131-150)
+  void modificationWarnings(const std::string &name) {
+    for (size_t l = 1; l < grids.size(); ++l)
+      if (grids[l].flags.size() == grids[l].n3()) {
+        log() << "WARNING: Region selected for modification is the entire zoom grid. This is likely because the cell "
+                 "selection extends beyond the zoom boundaries."
+              << std::endl;
+        std::cerr << "By design, modifications are meant to be defined inside a zoom region. Increase the size of "
+                     "your zoom grid or decrease your selection to avoid nasty surprises."
+                  << std::endl;
+      }
+    if (strcasecmp(name.c_str(), "variance") == 0 && varianceScale >= 0 && varianceScale >= grids.back().dx) {
+      const Grid &g = grids.back();
+      double windowSize = 0;
+      if (!g.flags.empty()) {
+        Window w(g.simEquivalent(), g.coord(g.flags[0]));
+        for (size_t id : g.flags) w.expandToInclude(g.coord(id));
+        windowSize = double(w.maxDim()) * g.dx;
+      }
+      if (varianceScale > windowSize) {
+        log() << "WARNING: High-pass filtering scale: " << varianceScale
+              << " h**-1 Mpc is greater than the rough window size: " << windowSize
+              << " h**-1 Mpc used for modifications." << std::endl;
+        log() << "This is prone to numerical errors when modifying the field. Decrease filtering scale to avoid it."
+              << std::endl;
+      }
+    }
+  }
+  void calculate(const std::string &name) {
+    ensureDrawn();
+    syncFlags();
+    modificationWarnings(name);
+    double v;
+    check(gic_calculate(c(), name.c_str(), &v));
+    log() << name << ": calculated value = " << v << std::endl;
+  }
+  void modify(const std::string &name, const std::string &type, float target) {
+    ensureDrawn();
+    bool relative = strcasecmp(type.c_str(), "relative") == 0;
+    if (!relative && strcasecmp(type.c_str(), "absolute") != 0)
+      throw std::runtime_error("Modification type must be either 'relative' or 'absolute'");
+    syncFlags();
+    modificationWarnings(name);
+    check(gic_add_modification(c(), name.c_str(), relative ? 1 : 0, double(target), initialNumberSteps, precision));
+    ++nModifications;
+  }
+  void applyModifications() {
+    ensureDrawn();
+    log() << std::endl << "Applying modifications" << std::endl;
+    double pre, post;
+    check(gic_apply_modifications(c(), &pre, &post));
+    uint64_t dof = numDof();
+    log() << "   Post-modification chi^2 = " << post << std::endl;
+    log() << "  Modification Delta chi^2 = " << post - pre << std::endl;
+    log() << "           d.o.f. in field = " << dof << std::endl;
+    log() << std::endl;
+  }
+  void clearModifications() {
+    log() << "Clearing modification list" << std::endl;
+    check(gic_clear_modifications(c()));
+    nModifications = 0;
+  }
+  uint64_t numDof() const {   // MultiLevelGrid::getNumDof (multilevelgrid.hpp:160-172)
+    uint64_t dof = 0;
+    for (size_t i = 0; i < grids.size(); ++i) {
+      dof += grids[i].n3();
+      if (i > 0) {
+        uint64_t f = uint64_t(ratioInt(grids[i - 1].dx, grids[i].dx));
+        dof -= grids[i].n3() / (f * f * f);
+      }
+    }
+    return dof;
+  }
+  void chi2() {
+    ensureDrawn();
+    double v;
+    uint64_t dof;
+    check(gic_chi2(c(), &v, &dof));
+    log() << "Calculated chi^2 = " << std::setprecision(10) << v << " (dof = " << dof << ")" << std::setprecision(6)
+          << std::endl;
+  }
+
   void done() {
     if (fmt == Fmt::unknown) throw std::runtime_error("No output format specified!");
+    ensureDrawn();
+    if (nModifications) applyModifications();
     generate();
     switch (fmt) {
       case Fmt::tipsy: {
@@ -1243,6 +1341,13 @@ struct Dispatcher {
           ic.velOffset[2] = vz * c;
         }));
     add("done", std::function<void()>([this] { ic.done(); }));
+    add("calculate", S([this](std::string n) { ic.calculate(n); }));
+    add("modify", std::function<void(std::string, std::string, float)>(
+                      [this](std::string n, std::string t, float v) { ic.modify(n, t, v); }));
+    add("apply_modifications", std::function<void()>([this] { ic.applyModifications(); }));
+    add("clear_modifications", std::function<void()>([this] { ic.clearModifications(); }));
+    add("chi2", std::function<void()>([this] { ic.chi2(); }));
+    add("filtering_scale", D([this](double v) { ic.varianceScale = v; }));
     add("dump_grid", std::function<void(int)>([this](int l) { ic.dumpGrid(l); }));
     add("dump_ps", std::function<void(int)>([this](int l) { ic.dumpPS(l); }));
     add("dump_tipsy", S([this](std::string f) { ic.dumpTipsyArray(f); }));

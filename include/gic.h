@@ -88,6 +88,30 @@ int gic_copy_field(gic_context *ctx, int field, int level, int space, void *host
 /* Overwrite a field (import_level, ic.hpp:809-821, and tests). */
 int gic_set_field(gic_context *ctx, int field, int level, int space, const void *host, int dtype);
 
+/* ---- constrained realisations ------------------------------------------------
+ * Flagged cells of a level (the selection commands' result, ic.hpp:1442-1523),
+ * read when a modification or a calculation is defined. */
+int gic_set_flags(gic_context *ctx, int level, const uint64_t *cells, size_t n);
+/* filtering_scale (ic.hpp:356-358): scale of the variance's high-pass filter. */
+int gic_set_variance_filter_scale(gic_context *ctx, double scale);
+/* calculate <name> (ic.hpp:1600-1606, modificationmanager.hpp:37-44): the current
+ * value of overdensity, potential, vx|vy|vz, Lx|Ly|Lz (linearmodification.hpp)
+ * or variance (quadraticmodification.hpp:152-221) on the flagged cells. */
+int gic_calculate(gic_context *ctx, const char *name, double *out);
+/* modify <name> absolute|relative <target> (ic.hpp:1613-1619,
+ * modificationmanager.hpp:52-77); init_steps and precision drive the quadratic
+ * iterations (ic.hpp:155-161). */
+int gic_add_modification(gic_context *ctx, const char *name, int relative, double target, int init_steps,
+                         double precision);
+/* apply_modifications (modificationmanager.hpp:80-115): Hoffman-Ribak on the white
+ * noise; chi^2 before and after. */
+int gic_apply_modifications(gic_context *ctx, double *pre_chi2, double *post_chi2);
+/* clear_modifications (modificationmanager.hpp:117-121). */
+int gic_clear_modifications(gic_context *ctx);
+/* chi2 (ic.hpp:1592-1597, multilevelfield.hpp:461-487): <w|M|w> of the white noise and
+ * the degrees of freedom (multilevelgrid.hpp:160-172). */
+int gic_chi2(gic_context *ctx, double *chi2, uint64_t *dof);
+
 /* Power-spectrum histogram for dump_ps (dumpPowerSpectrum,
  * cosmology/parameters.hpp:112-205): over every (ix,iy,iz) in [-n/2,n/2]^3,
  * sum_power[m] += |delta_k|^2 and count[m] += 1 with m = ix^2+iy^2+iz^2;

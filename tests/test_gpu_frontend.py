@@ -105,3 +105,48 @@ def test_float32_fields_stay_close(tmp_path):
     np.testing.assert_array_equal(a["ids"], b["ids"])
     assert np.abs(a["pos"] - b["pos"]).max() < 1e-4
     assert np.abs(a["vel"] - b["vel"]).max() < 1e-3 * np.abs(a["vel"]).max()
+
+
+def _log_lines_present(test, err):
+    """compare.py's compare_outputlogfile: every line of reference.txt occurs
+    in the (time-stamp stripped) log."""
+    lines = err.splitlines()
+    for want in open(golden(test, "reference.txt")).read().splitlines():
+        assert want in lines, "missing log line: %r" % want
+
+
+@pytest.mark.parametrize("test", [
+    "test_02k_chisquare_onelevel",
+    "test_02l_chisquare_modification",
+    "test_02m_chisquare_twolevels",
+    "test_02f_onelevelvariance",
+    "test_02g_multilevelvariance",
+    "test_02j_invertible_modifs",
+])
+def test_modification_logs_match_reference(tmp_path, test):
+    d, err = _run(test, tmp_path)
+    _log_lines_present(test, err)
+
+
+@pytest.mark.parametrize("test", [
+    "test_02_simple_modif",
+    "test_14_velocity_modif",
+    "test_22_angular_momentum_modif",
+    "test_20_velmodif_zoom",
+])
+def test_modified_particle_output_matches_reference(tmp_path, test):
+    d, _ = _run(test, tmp_path)
+    compare_particles(particle_file(d), golden(test, "reference_output"))
+
+
+def test_modified_overdensity_array_and_grids(tmp_path):
+    """test_02's dump_tipsy array (compare.py: 5 decimals) and test_20's grids."""
+    d, _ = _run("test_02_simple_modif", tmp_path / "a")
+    got = read_tipsy_array(os.path.join(d, "test_2.tipsy.overdensity"))
+    ref = read_tipsy_array(golden("test_02_simple_modif", "reference_output.overdensity"))
+    assert_decimal(got, ref, 5, "overdensity")
+    d, _ = _run("test_20_velmodif_zoom", tmp_path / "b")
+    for lev in (0, 1):
+        g = np.load(os.path.join(d, "grid-%d.npy" % lev))
+        r = np.load(golden("test_20_velmodif_zoom", "reference_grid", "grid-%d.npy" % lev))
+        assert_decimal(g, r, 4, "grid-%d" % lev)

@@ -1,0 +1,148 @@
+"""Constrained realisations on the GPU through the C-ABI: chi^2, covector
+values and the Hoffman-Ribak modifications, against the reference's logged
+numbers (reference.txt) and the oracle on the same inputs."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from genetic_amd import gic
+from helpers import (golden, cosmology_from_paramfile, select_sphere_cells, select_cube_cells,
+                     select_ellipse_cells)
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(paramfile, levels, seed, precision=gic.FLOAT64, camb="camb_transfer_kmax40_z0.dat"):
+    c = cosmology_from_paramfile(golden(paramfile))
+    ic = gic.ICGenerator(precision=precision)
+    ic.set_cosmology(**c)
+    ic.set_camb(golden(camb))
+    o = po.OracleIC()
+    o.set_cosmology(Om=c["Om"], Ol=c["Ol"], s8=c["s8"], ns=c["ns"], z=c["z"], hubble=c["hubble"])
+    tab, nc = po.read_camb(golden(camb))
+    o.set_camb(tab, nc)
+    ic.init_base_grid(*levels[0])
+    o.init_base_grid(*levels[0])
+    for z in levels[1:]:
+        ic.init_zoom_grid_with_origin_at(*z)
+        o.init_zoom_grid_at(*z)
+    ic.set_seed(seed)
+    o.seed(seed)
+    return ic, o
+
+
+def _ref(test, prefix):
+    out = []
+    for ln in open(golden(test, "reference.txt")):
+        if ln.strip().startswith(prefix):
+            out.append(ln.split("=")[1].split()[0])
+    return out
+
+
+def test_chi2_one_and_two_levels():
+    print("chi2 64^3 and 64^3 + x2 zoom", flush=True)
+    ic, _ = _pair("test_02k_chisquare_onelevel/paramfile.txt", [(64.0, 64)], 13842314)
+    chi2, dof = ic.chi2()
+    assert ("%.10g" % chi2, dof) == ("261733.3011", 262144)
+    ic, _ = _pair("test_02m_chisquare_twolevels/paramfile.txt", [(64.0, 64), (8, 15, 7, 2, 64)], 13842314)
+    chi2, dof = ic.chi2()
+    assert ("%.10g" % chi2, dof) == ("492005.8359", 491520)
+
+
+def test_overdensity_modification_delta_chi2():
+    print("02l overdensity relative 2", flush=True)
+    ic, o = _pair("test_02l_chisquare_modification/paramfile.txt", [(64.0, 64)], 13842314)
+    cells = select_sphere_cells(64, 64.0, (25, 32, 24), 10)
+    ic.set_flags(0, cells)
+    o.set_flags(0, cells)
+    v0 = ic.calculate("overdensity")
+    assert abs(v0 - o.calculate("overdensity")) < 1e-12
+    ic.modify("overdensity", True, 2.0)
+    pre, post = ic.apply_modifications()
+    assert "%g" % (post - pre) == _ref("test_02l_chisquare_modification", "Modification Delta chi^2")[0]
+    assert abs(ic.calculate("overdensity") - 2 * v0) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["overdensity", "potential", "vx", "vz", "Lx", "Ly"])
+@pytest.mark.parametrize("zoom", [False, True])
+def test_linear_covector_values_match_oracle(name, zoom):
+    """calculate <name> on one level and on a x2 zoom: the GPU covector
+    (localisation, deinterpolation, filters, sqrt P_dm, mode operator) gives
+    the oracle's value."""
+    print("covector", name, zoom, flush=True)
+    levels = [(50.0, 32)] + ([(8, 8, 8, 2, 32)] if zoom else [])
+    ic, o = _pair("test_02l_chisquare_modification/paramfile.txt", levels, 4242)
+    if zoom:
+        cells = select_sphere_cells(32, 50.0, (25, 25, 25), 3.0, offset=(12.5, 12.5, 12.5), dx=50.0 / 64)
+        lev = 1
+    else:
+        cells = select_sphere_cells(32, 50.0, (25, 25, 25), 5.0)
+        lev = 0
+    ic.set_flags(lev, cells)
+    o.set_flags(lev, cells)
+    a, b = ic.calculate(name), o.calculate(name)
+    assert abs(a - b) <= 1e-9 * abs(b) + 1e-14, (a, b)
+
+
+def test_variance_values_match_reference():
+    print("02f/02g variance", flush=True)
+    ic, _ = _pair("test_02f_onelevelvariance/paramfile.txt", [(64.0, 64)], 8896131)
+    ic.set_flags(0, select_cube_cells(64, 64.0, (8, 8, 8), 20))
+    ic.set_variance_filter_scale(10000)
+    assert "%g" % ic.calculate("variance") == _ref("test_02f_onelevelvariance", "variance: calculated value")[0]
+    ic, _ = _pair("test_02g_multilevelvariance/paramfile.txt", [(64.0, 64), (0, 0, 0, 4, 64)], 8896131)
+    ic.set_flags(0, select_cube_cells(64, 64.0, (8, 8, 8), 16))
+    ic.set_flags(1, select_cube_cells(64, 64.0, (8, 8, 8), 16, dx=0.25))
+    ic.set_variance_filter_scale(10000)
+    assert "%g" % ic.calculate("overdensity") == _ref("test_02g_multilevelvariance", "overdensity: calculated value")[0]
+    var = _ref("test_02g_multilevelvariance", "variance: calculated value")
+    assert "%g" % ic.calculate("variance") == var[0]
+    ic.zeldovich()
+    assert "%g" % ic.calculate("variance") == var[1]
+
+
+def test_invertible_linear_and_quadratic_modifications():
+    """test_02j on the GPU: overdensity and variance doubled then halved on
+    two levels; the values return to the reference's."""
+    print("02j", flush=True)
+    ic, _ = _pair("test_02j_invertible_modifs/paramfile.txt", [(64.0, 64), (24, 24, 24, 4, 64)], 13842314)
+    ic.set_flags(0, select_sphere_cells(64, 64.0, (32.5, 32.5, 32.5), 16))
+    ic.set_flags(1, select_sphere_cells(64, 64.0, (32.5, 32.5, 32.5), 16, offset=(24, 24, 24), dx=0.25))
+    ic.set_variance_filter_scale(3.0)
+    od = _ref("test_02j_invertible_modifs", "overdensity: calculated value")
+    var = _ref("test_02j_invertible_modifs", "variance: calculated value")
+    assert "%g" % ic.calculate("overdensity") == od[0]
+    assert "%g" % ic.calculate("variance") == var[0]
+    for f in (2.0, 0.5):
+        ic.modify("overdensity", True, f)
+        ic.modify("variance", True, f)
+        ic.apply_modifications()
+        ic.clear_modifications()
+    assert "%g" % ic.calculate("overdensity") == od[1]
+    assert "%g" % ic.calculate("variance") == var[1]
+    ic.zeldovich()
+    assert "%g" % ic.calculate("variance") == var[2]
+
+
+def test_modified_field_matches_oracle_float32():
+    """FP32 fields, FP64 inner products: a velocity and an angular-momentum
+    constraint on 32^3 reach their targets and the modified overdensity
+    stays within float precision of the FP64 oracle."""
+    print("fp32 vx+Ly", flush=True)
+    ic, o = _pair("test_22_angular_momentum_modif/paramfile.txt", [(50.0, 32)], 8896131, precision=gic.FLOAT32)
+    cells = select_ellipse_cells(32, 50.0, (25, 25, 25), 10, 7.5, 5)
+    for x in (ic, o):
+        x.set_flags(0, cells)
+    ic.modify("Ly", False, 100.0)
+    ic.modify("vx", False, 50.0)
+    o.add_modification("Ly", False, 100.0)
+    o.add_modification("vx", False, 50.0)
+    ic.apply_modifications()
+    o.apply_modifications()
+    assert abs(ic.calculate("Ly") - 100.0) < 1e-3
+    assert abs(ic.calculate("vx") - 50.0) < 1e-3
+    ic.zeldovich()
+    o.zeldovich()
+    a = ic.field(gic.FIELD_OVERDENSITY, 0)
+    b = o.field(0, 0)
+    assert np.abs(a - b).max() < 1e-4 * np.abs(b).max()
