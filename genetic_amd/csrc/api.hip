@@ -241,6 +241,72 @@ void coarse_window(const LevelGeom &gf, const LevelGeom &gc, int *w0, int *wl) {
   }
 }
 
+// Second-order LPT (no counterpart in the reference, which stops at
+// Zel'dovich, zeldovich.hpp:152-253).  With phi_ij = k_i k_j delta / k^2 the
+// source is S = sum_{i<j} (phi_ii phi_jj - phi_ij^2) and
+// psi2 = (3/7) i k S / k^2, i.e. D2 = -3/7 D1^2 with delta at the output
+// redshift; the velocity uses f2 = 2 f1.  Each level is built from its
+// combined overdensity; a zoom level keeps the high-pass part of its own
+// psi2 and takes the low-pass part from the level above, as for psi.
+template <typename T> void do_second_order(gic_context *c) {
+  size_t nl = c->levels.size();
+  for (size_t l = 0; l < nl; ++l) {
+    Level &L = *c->levels[l];
+    const LevelGeom &g = L.g;
+    int n = g.n;
+    size_t reals = g.padded_reals(), bytes = reals * sizeof(T);
+    for (auto &p : L.psi2) alloc_field(c, p, g);
+    c->stage.ensure(2 * bytes);
+    T *D = (T *)c->stage.p;
+    T *S = D + reals;
+    const T *spec = (const T *)L.noise.p;
+    if (!L.noise_fourier) {
+      GIC_HIP(hipMemcpyAsync(D, L.noise.p, bytes, hipMemcpyDeviceToDevice, c->st));
+      fft3d<T>(D, n, -1, c->st, c->hook());
+      spec = D;
+    }
+    long h = n / 2, nent = 3 * h * h + 1;
+    c->tab_a.ensure(nent * sizeof(T));
+    c->tab_c.ensure(nent * sizeof(T));
+    filter_table<T>(2, 0.0, g.kmin(), nent, (T *)c->tab_a.p, c->st, true);
+    const T *inv_k2 = (const T *)c->tab_a.p;
+    T *A = (T *)L.psi2[0].p, *B = (T *)L.psi2[1].p;
+    auto phi = [&](T *out, int a, int b) {
+      fft_c2r_from<T>(spec, out, n, FFT_PRE_KK, inv_k2, a + 3 * b, g.kmin(), c->st, c->hook());
+    };
+    if (c->hook()) c->hook()->begin("lpt_source", c->st);
+    phi(A, 0, 0);
+    phi(B, 1, 1);
+    lpt_accum<T>(S, A, B, reals, 0, c->st);   // S = xx yy
+    lpt_accum<T>(S, A, B, reals, 3, c->st);   // A = xx + yy
+    phi(B, 2, 2);
+    lpt_accum<T>(S, A, B, reals, 1, c->st);   // S += (xx + yy) zz
+    const int off[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    for (auto &o : off) {
+      phi(B, o[0], o[1]);
+      lpt_accum<T>(S, B, B, reals, 2, c->st);   // S -= xy^2, xz^2, yz^2
+    }
+    if (c->hook()) c->hook()->end("lpt_source", c->st);
+    fft3d<T>(S, n, -1, c->st, c->hook());
+    double kc = l >= 1 ? kcut_for_level(c, int(l) - 1) : 0.0;
+    filter_table<T>(l >= 1 ? 1 : 2, kc, g.kmin(), nent, (T *)c->tab_c.p, c->st, true, 3.0 / 7.0);
+    for (int a = 0; a < 3; ++a)
+      fft_c2r_from<T>(S, (T *)L.psi2[a].p, n, FFT_PRE_ZA, (const T *)c->tab_c.p, a, g.kmin(), c->st, c->hook());
+    if (l >= 1) {
+      Level &C = *c->levels[l - 1];
+      long hc = C.g.n / 2, nc_ent = 3 * hc * hc + 1;
+      c->tab_b.ensure(nc_ent * sizeof(T));
+      filter_table<T>(0, kcut_for_level(c, int(l) - 1), C.g.kmin(), nc_ent, (T *)c->tab_b.p, c->st);
+      InterpGeom ig = interp_geom(g, C.g);
+      for (int a = 0; a < 3; ++a) {
+        CoarseView v = lowpass_full<T>(c, C.psi2[a], C.g, (const T *)c->tab_b.p);
+        interp_add<T>((T *)L.psi2[a].p, n, v, ig, c->st);
+      }
+    }
+    L.have_psi2 = true;
+  }
+}
+
 template <typename T> void do_zeldovich(gic_context *c) {
   if (!c->power_applied) do_apply_ps<T>(c);
   size_t nl = c->levels.size();
@@ -317,6 +383,7 @@ template <typename T> void do_zeldovich(gic_context *c) {
       if (c->hook()) c->hook()->end("interp_add", c->st);
     }
   }
+  if (c->lpt_order == 2) do_second_order<T>(c);
   c->zeldovich_done = true;
 }
 
@@ -332,6 +399,13 @@ template <typename T> void do_copy_field(gic_context *c, int field, int level, i
     if (!L.have_psi) throw std::runtime_error("Attempt to generate particles before they have been calculated");
     b = &L.psi[field - 1];
     flag = &L.psi_fourier;
+  } else if (field >= 4 && field <= 6) {
+    if (!L.have_psi2) throw std::runtime_error("Second-order displacements have not been calculated (lpt_order 2)");
+    if (space != GIC_REAL_SPACE) throw std::runtime_error("second-order displacements are held in real space");
+    b = &L.psi2[field - 4];
+    static bool real_flag;
+    real_flag = false;
+    flag = &real_flag;
   } else {
     throw std::runtime_error("unknown field");
   }
@@ -463,6 +537,7 @@ void do_particles(gic_context *c, int level, const uint64_t *cells, size_t start
   pg.o[2] = L.g.oz;
   pg.periodic = L.g.periodic;
   pg.vratio = velocity_ratio(c->cosmo);
+  pg.vratio2 = 2.0 * pg.vratio;   // f2 = 2 f1 (f1 = 1, as zeldovichVelocityToOffsetRatio)
   // device staging: cell list + outputs, processed in slices
   const size_t slice = size_t(1) << 24;
   for (size_t s0 = 0; s0 < count; s0 += slice) {
@@ -476,7 +551,9 @@ void do_particles(gic_context *c, int level, const uint64_t *cells, size_t start
       dcells = (uint64_t *)(dvel + 3 * m);
       GIC_HIP(hipMemcpyAsync(dcells, cells + s0, m * 8, hipMemcpyHostToDevice, c->st));
     }
-    particles<T>((const T *)L.psi[0].p, (const T *)L.psi[1].p, (const T *)L.psi[2].p, L.g.n, pg, dcells, start + s0, m,
+    const T *q2[3] = {(const T *)L.psi2[0].p, (const T *)L.psi2[1].p, (const T *)L.psi2[2].p};
+    particles<T>((const T *)L.psi[0].p, (const T *)L.psi[1].p, (const T *)L.psi[2].p, L.have_psi2 ? q2 : nullptr,
+                 L.g.n, pg, dcells, start + s0, m,
                  dpos, dvel, c->st);
     GIC_HIP(hipMemcpyAsync(pos + 3 * s0, dpos, m * 12, hipMemcpyDeviceToHost, c->st));
     GIC_HIP(hipMemcpyAsync(vel + 3 * s0, dvel, m * 12, hipMemcpyDeviceToHost, c->st));
@@ -751,6 +828,13 @@ int gic_particle_mass_eps(gic_context *c, int level, double *mass, double *eps) 
     double boxMass = 27.744948 * c->cosmo.Om * pow(g.periodic, 3.0);
     if (mass) *mass = boxMass * pow(g.dx / g.periodic, 3.0);
     if (eps) *eps = g.dx * 1.0 * 0.01075;
+  });
+}
+
+int gic_set_lpt_order(gic_context *c, int order) {
+  return guarded(c, [&] {
+    if (order != 1 && order != 2) throw std::runtime_error("LPT order must be 1 or 2");
+    c->lpt_order = order;
   });
 }
 

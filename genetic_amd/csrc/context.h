@@ -184,12 +184,14 @@ struct Level {
   LevelGeom g;
   DevBuf noise;            // white noise, then the overdensity
   DevBuf psi[3];
+  DevBuf psi2[3];          // second-order displacements (2LPT)
   DevBuf sqrtp;            // sqrt P by |k|^2, element type T
   DevBuf sqrtp_dm;         // the same for the dark-matter transfer function (covectors)
   bool noise_fourier = false;
   bool dc_pending = false;   // Re(w_0) still to be zeroed after the first r2c
   bool psi_fourier = false;
   bool have_psi = false;
+  bool have_psi2 = false;
 };
 
 }  // namespace gic
@@ -227,6 +229,7 @@ struct gic_context {
   unsigned long seed = 0;
   int seed_mode = 0;
   bool seeded = false, drawn = false, power_applied = false, zeldovich_done = false;
+  int lpt_order = 1;   // 1: Zel'dovich, 2: second-order LPT on top
   DevBuf work, stage, tab_a, tab_b, tab_c;
   std::unique_ptr<EventTimer> timer;
   TimingHook *hook() { return timer.get(); }

@@ -19,14 +19,15 @@ struct TimingHook {
 template <typename T> void fft3d(T *d, int n, int dir, hipStream_t st, TimingHook *hook);
 
 // k-space operators applied inside the x pass
-enum { FFT_PRE_NONE = 0, FFT_PRE_TABLE = 1, FFT_PRE_ZA = 2 };
+enum { FFT_PRE_NONE = 0, FFT_PRE_TABLE = 1, FFT_PRE_ZA = 2, FFT_PRE_KK = 3 };
 
 // r2c whose last pass multiplies each mode by post[kx^2+ky^2+kz^2] (if non-null).
 template <typename T> void fft_r2c_post(T *d, int n, const T *post, hipStream_t st, TimingHook *hook);
 // out <- c2r(op(spec)), out-of-place from a spectrum that is left untouched.
 // op: FFT_PRE_TABLE multiplies by tab[|k|^2]; FFT_PRE_ZA forms the Zel'dovich
 // component comp, i k_c tab[|k|^2], with tab = filter(k)/k^2 (0 at k=0) and
-// zero at k_c = +Nyquist.
+// zero at k_c = +Nyquist.  FFT_PRE_KK forms k_a k_b tab[|k|^2] for comp = a + 3 b
+// (zero at a +Nyquist component when a != b).
 template <typename T>
 void fft_c2r_from(const T *spec, T *out, int n, int pre, const T *tab, int comp, double kmin, hipStream_t st,
                   TimingHook *hook);

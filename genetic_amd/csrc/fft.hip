@@ -195,9 +195,10 @@ __device__ __forceinline__ void run_stages(cx<T> *lds, const cx<T> *tw, int twmu
 //   PRE  (inverse x pass, input indexed by kx=pt, ky=outer, kz=col):
 //        PRE_TABLE: v *= tab[kx^2+ky^2+kz^2]
 //        PRE_ZA:    v <- i k_c v / k^2 (zero at k_c = +Nyquist and k = 0), times tab if given
+//        PRE_KK:    v <- k_a k_b tab[|k|^2] v (second derivatives; a != b zero at a Nyquist component)
 //   POST (forward x pass, output indexed by kx=pt, ky=outer, kz=col):
 //        POST_TABLE: v *= tab[kx^2+ky^2+kz^2]
-enum { PRE_NONE = FFT_PRE_NONE, PRE_TABLE = FFT_PRE_TABLE, PRE_ZA = FFT_PRE_ZA };
+enum { PRE_NONE = FFT_PRE_NONE, PRE_TABLE = FFT_PRE_TABLE, PRE_ZA = FFT_PRE_ZA, PRE_KK = FFT_PRE_KK };
 enum { POST_NONE = 0, POST_TABLE = 1 };
 
 template <typename T> struct StridedArgs {
@@ -241,6 +242,13 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
       if (kc == a.n / 2) return cx<T>{0, 0};
       f *= T(kc) * a.kmin;
       return cx<T>{-v.y * f, v.x * f};
+    } else if constexpr (PRE == PRE_KK) {
+      int ca = a.comp % 3, cb = a.comp / 3;
+      int k1 = ca == 0 ? kx : ca == 1 ? ky : kz;
+      int k2 = cb == 0 ? kx : cb == 1 ? ky : kz;
+      if (ca != cb && (k1 == a.n / 2 || k2 == a.n / 2)) return cx<T>{0, 0};
+      f *= (T(k1) * a.kmin) * (T(k2) * a.kmin);
+      return cx<T>{v.x * f, v.y * f};
     } else {
       return cx<T>{v.x * f, v.y * f};
     }
@@ -557,6 +565,7 @@ template <typename T, int N> struct FftN {
     ax.comp = comp;
     ax.kmin = kmin;
     if (pre == PRE_ZA) strided<+1, PRE_ZA, POST_NONE>(ax, N, st, hook, "fft_x_za");
+    else if (pre == PRE_KK) strided<+1, PRE_KK, POST_NONE>(ax, N, st, hook, "fft_x_kk");
     else if (pre == PRE_TABLE) strided<+1, PRE_TABLE, POST_NONE>(ax, N, st, hook, "fft_x_pre");
     else strided<+1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x");
     strided<+1, PRE_NONE, POST_NONE>(plain(co, false, T(1)), N, st, hook, "fft_y");

@@ -38,7 +38,8 @@ struct ParticleGeom {
   double dx;
   double o[3];
   double periodic;
-  double vratio;
+  double vratio;    // velocity per unit first-order displacement
+  double vratio2;   // velocity per unit second-order displacement
 };
 
 template <typename T>
@@ -46,16 +47,20 @@ void sqrtp_table(const SplineHost &sp, double amplitude, double ns, double kmin,
                  hipStream_t st);
 // kind 0 low pass, 1 high pass (sqrt(1-lp^2)), 2 unity; over_k2 divides by k^2
 template <typename T>
-void filter_table(int kind, double kc, double kmin, long nent, T *out, hipStream_t st, bool over_k2 = false);
+void filter_table(int kind, double kc, double kmin, long nent, T *out, hipStream_t st, bool over_k2 = false,
+                  double scale = 1.0);
 template <typename T> void mul_table(T *field, int n, const T *tab, hipStream_t st);
 // sum[m] += |spec_k|^2, cnt[m] += 1 over (ix,iy,iz) in [-N/2,N/2]^3, m = |k_int|^2
 template <typename T> void mode_power(const T *spec, int n, double *sum, double *cnt, hipStream_t st);
 template <typename T>
 void zeldovich(const T *delta, int n, double kmin, T *px, T *py, T *pz, const T *post, hipStream_t st);
 template <typename T> void interp_add(T *fine, int nf, const CoarseView &coarse, const InterpGeom &g, hipStream_t st);
+// psi2: the three second-order displacement fields, or nullptr for Zel'dovich
 template <typename T>
-void particles(const T *px, const T *py, const T *pz, int n, const ParticleGeom &g, const uint64_t *cells,
-               size_t start, size_t count, float *pos, float *vel, hipStream_t st);
+void particles(const T *px, const T *py, const T *pz, const T *const *psi2, int n, const ParticleGeom &g,
+               const uint64_t *cells, size_t start, size_t count, float *pos, float *vel, hipStream_t st);
+// elementwise source accumulation: mode 0 S = A B, 1 S += A B, 2 S -= A A, 3 A += B
+template <typename T> void lpt_accum(T *S, T *A, const T *B, size_t n, int mode, hipStream_t st);
 template <typename T> void fill(T *p, size_t n, T v, hipStream_t st);
 // Fourier-space white noise: number of (a, b) draws of an n^3 level, and the
 // placement of gauss[0 .. 2*count) (already scaled by 1/sqrt 2) into the

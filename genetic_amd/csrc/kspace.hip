@@ -85,7 +85,8 @@ void sqrtp_table(const SplineHost &sp, double amplitude, double ns, double kmin,
 // kind 2: unity
 // over_k2: divide by k^2 (zero at k = 0), the Zel'dovich kernel's 1/k^2
 template <typename T>
-__global__ void k_filter_table(int kind, double kc, double kmin, long nent, int over_k2, T *__restrict__ out) {
+__global__ void k_filter_table(int kind, double kc, double kmin, long nent, int over_k2, double scale,
+                               T *__restrict__ out) {
   long s = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (s >= nent) return;
   double k = sqrt(double(s)) * kmin;
@@ -95,13 +96,13 @@ __global__ void k_filter_table(int kind, double kc, double kmin, long nent, int 
     v = kind == 0 ? lp : sqrt(1. - lp * lp);
   }
   if (over_k2) v = s == 0 ? 0.0 : v / (double(s) * kmin * kmin);
-  out[s] = T(v);
+  out[s] = T(v * scale);
 }
 
 template <typename T>
-void filter_table(int kind, double kc, double kmin, long nent, T *out, hipStream_t st, bool over_k2) {
+void filter_table(int kind, double kc, double kmin, long nent, T *out, hipStream_t st, bool over_k2, double scale) {
   hipLaunchKernelGGL(k_filter_table<T>, dim3((nent + 255) / 256), dim3(256), 0, st, kind, kc, kmin, nent,
-                     over_k2 ? 1 : 0, out);
+                     over_k2 ? 1 : 0, scale, out);
   GIC_HIP(hipGetLastError());
 }
 
@@ -359,7 +360,8 @@ template <typename T> void interp_add(T *fine, int nf, const CoarseView &coarse,
 
 // ---- particles --------------------------------------------------------------------------
 template <typename T>
-__global__ void k_particles(const T *__restrict__ px, const T *__restrict__ py, const T *__restrict__ pz, int n,
+__global__ void k_particles(const T *__restrict__ px, const T *__restrict__ py, const T *__restrict__ pz,
+                            const T *__restrict__ qx, const T *__restrict__ qy, const T *__restrict__ qz, int n,
                             ParticleGeom g, const uint64_t *__restrict__ cells, size_t start, size_t count,
                             float *__restrict__ pos, float *__restrict__ vel) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -371,10 +373,17 @@ __global__ void k_particles(const T *__restrict__ px, const T *__restrict__ py, 
   double off[3] = {double(px[p]), double(py[p]), double(pz[p])};
   double cen[3] = {double(x) * g.dx + g.o[0] + g.dx / 2, double(y) * g.dx + g.o[1] + g.dx / 2,
                    double(z) * g.dx + g.o[2] + g.dx / 2};
+  const T *q2[3] = {qx, qy, qz};
   for (int a = 0; a < 3; ++a) {
     T o = T(off[a]);
     T v = o * T(g.vratio);
-    T q = T(off[a]) + T(cen[a]);
+    T disp = T(off[a]);
+    if (qx) {   // second order: x = q + psi1 + psi2, v = r (psi1 + f2/f1 psi2)
+      T o2 = q2[a][p];
+      disp += o2;
+      v += o2 * T(g.vratio2);
+    }
+    T q = disp + T(cen[a]);
     double qd = double(q);
     qd = fmod(qd, g.periodic);
     if (qd < 0) qd += g.periodic;
@@ -384,11 +393,28 @@ __global__ void k_particles(const T *__restrict__ px, const T *__restrict__ py, 
 }
 
 template <typename T>
-void particles(const T *px, const T *py, const T *pz, int n, const ParticleGeom &g, const uint64_t *cells,
-               size_t start, size_t count, float *pos, float *vel, hipStream_t st) {
+void particles(const T *px, const T *py, const T *pz, const T *const *psi2, int n, const ParticleGeom &g,
+               const uint64_t *cells, size_t start, size_t count, float *pos, float *vel, hipStream_t st) {
   if (count == 0) return;
-  hipLaunchKernelGGL(k_particles<T>, dim3((count + 255) / 256), dim3(256), 0, st, px, py, pz, n, g, cells, start,
-                     count, pos, vel);
+  const T *qx = psi2 ? psi2[0] : nullptr, *qy = psi2 ? psi2[1] : nullptr, *qz = psi2 ? psi2[2] : nullptr;
+  hipLaunchKernelGGL(k_particles<T>, dim3((count + 255) / 256), dim3(256), 0, st, px, py, pz, qx, qy, qz, n, g, cells,
+                     start, count, pos, vel);
+  GIC_HIP(hipGetLastError());
+}
+
+// ---- second-order source -------------------------------------------------------------
+// elementwise over a padded real field: mode 0 S = A B, 1 S += A B, 2 S -= A A, 3 A += B
+template <typename T>
+__global__ void k_lpt_accum(T *__restrict__ S, T *__restrict__ A, const T *__restrict__ B, size_t n, int mode) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (mode == 0) S[i] = A[i] * B[i];
+  else if (mode == 1) S[i] += A[i] * B[i];
+  else if (mode == 2) S[i] -= A[i] * A[i];
+  else A[i] += B[i];
+}
+template <typename T> void lpt_accum(T *S, T *A, const T *B, size_t n, int mode, hipStream_t st) {
+  hipLaunchKernelGGL(k_lpt_accum<T>, dim3((n + 255) / 256), dim3(256), 0, st, S, A, B, n, mode);
   GIC_HIP(hipGetLastError());
 }
 
@@ -442,14 +468,15 @@ template <typename S, typename D> void convert(const S *a, size_t n, D *b, hipSt
 
 #define GIC_INST(T)                                                                                            \
   template void sqrtp_table<T>(const SplineHost &, double, double, double, double, long, T *, hipStream_t);   \
-  template void filter_table<T>(int, double, double, long, T *, hipStream_t, bool);                           \
+  template void filter_table<T>(int, double, double, long, T *, hipStream_t, bool, double);                   \
   template void mul_table<T>(T *, int, const T *, hipStream_t);                                               \
   template void mode_power<T>(const T *, int, double *, double *, hipStream_t);                               \
   template void fourier_draw_scatter<T>(const T *, int, T *, uint32_t *, int, hipStream_t);                   \
   template void zeldovich<T>(const T *, int, double, T *, T *, T *, const T *, hipStream_t);                  \
   template void interp_add<T>(T *, int, const CoarseView &, const InterpGeom &, hipStream_t);                 \
-  template void particles<T>(const T *, const T *, const T *, int, const ParticleGeom &, const uint64_t *,    \
-                             size_t, size_t, float *, float *, hipStream_t);                                  \
+  template void particles<T>(const T *, const T *, const T *, const T *const *, int, const ParticleGeom &,    \
+                             const uint64_t *, size_t, size_t, float *, float *, hipStream_t);                \
+  template void lpt_accum<T>(T *, T *, const T *, size_t, int, hipStream_t);                                  \
   template void fill<T>(T *, size_t, T, hipStream_t);
 GIC_INST(float)
 GIC_INST(double)

@@ -18,6 +18,7 @@ _LIB = None
 FLOAT32, FLOAT64 = 32, 64
 SEED_REAL_SPACE, SEED_FOURIER_SERIAL, SEED_FOURIER_PARALLEL, SEED_FOURIER_REVERSE = 0, 1, 2, 3
 FIELD_OVERDENSITY, FIELD_PSI_X, FIELD_PSI_Y, FIELD_PSI_Z = 0, 1, 2, 3
+FIELD_PSI2_X, FIELD_PSI2_Y, FIELD_PSI2_Z = 4, 5, 6
 REAL_SPACE, FOURIER_SPACE = 0, 1
 DTYPE_F32, DTYPE_F64 = 0, 1
 
@@ -43,6 +44,7 @@ SIGNATURES = {
     "gic_copy_field": ([_v, _i, _i, _i, _v, _i], _i),
     "gic_set_field": ([_v, _i, _i, _i, _v, _i], _i),
     "gic_mode_power": ([_v, _i, _v, _v, _v, _s], _i),
+    "gic_set_lpt_order": ([_v, _i], _i),
     "gic_set_flags": ([_v, _i, _v, _s], _i),
     "gic_set_variance_filter_scale": ([_v, _d], _i),
     "gic_calculate": ([_v, ctypes.c_char_p, _P(_d)], _i),
@@ -231,6 +233,10 @@ class ICGenerator:
         cp = _ptr(cells) if cells is not None else None
         self._check(lib().gic_particles(self._h, level, cp, start, count, _ptr(pos), _ptr(vel)))
         return pos, vel
+
+    def set_lpt_order(self, order):
+        """1: Zel'dovich (the reference), 2: plus second-order LPT."""
+        self._check(lib().gic_set_lpt_order(self._h, int(order)))
 
     # --- constrained realisations (ic.hpp:1592-1631) ---------------------
     def set_flags(self, level, cells):

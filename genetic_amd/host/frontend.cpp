@@ -1348,6 +1348,7 @@ struct Dispatcher {
     add("clear_modifications", std::function<void()>([this] { ic.clearModifications(); }));
     add("chi2", std::function<void()>([this] { ic.chi2(); }));
     add("filtering_scale", D([this](double v) { ic.varianceScale = v; }));
+    add("lpt_order", std::function<void(int)>([this](int o) { ic.check(gic_set_lpt_order(ic.c(), o)); }));
     add("dump_grid", std::function<void(int)>([this](int l) { ic.dumpGrid(l); }));
     add("dump_ps", std::function<void(int)>([this](int l) { ic.dumpPS(l); }));
     add("dump_tipsy", S([this](std::string f) { ic.dumpTipsyArray(f); }));

@@ -34,7 +34,15 @@ enum {
   GIC_SEED_FOURIER_PARALLEL = 2, /* random_seed, seedfourier_parallel */
   GIC_SEED_FOURIER_REVERSE = 3   /* seedfourier_reverse */
 };
-enum { GIC_FIELD_OVERDENSITY = 0, GIC_FIELD_PSI_X = 1, GIC_FIELD_PSI_Y = 2, GIC_FIELD_PSI_Z = 3 };
+enum {
+  GIC_FIELD_OVERDENSITY = 0,
+  GIC_FIELD_PSI_X = 1,
+  GIC_FIELD_PSI_Y = 2,
+  GIC_FIELD_PSI_Z = 3,
+  GIC_FIELD_PSI2_X = 4,   /* second-order displacements (lpt_order 2) */
+  GIC_FIELD_PSI2_Y = 5,
+  GIC_FIELD_PSI2_Z = 6
+};
 enum { GIC_REAL_SPACE = 0, GIC_FOURIER_SPACE = 1 };
 enum { GIC_DTYPE_F32 = 0, GIC_DTYPE_F64 = 1 };
 
@@ -87,6 +95,11 @@ int gic_zeldovich(gic_context *ctx);
 int gic_copy_field(gic_context *ctx, int field, int level, int space, void *host, int dtype);
 /* Overwrite a field (import_level, ic.hpp:809-821, and tests). */
 int gic_set_field(gic_context *ctx, int field, int level, int space, const void *host, int dtype);
+
+/* Order of the displacements: 1 = Zel'dovich (the reference's only option,
+ * zeldovich.hpp:152-253), 2 = Zel'dovich plus second-order LPT.  No reference
+ * counterpart; the paramfile command lpt_order sets it. */
+int gic_set_lpt_order(gic_context *ctx, int order);
 
 /* ---- constrained realisations ------------------------------------------------
  * Flagged cells of a level (the selection commands' result, ic.hpp:1442-1523),

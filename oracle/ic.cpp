@@ -269,6 +269,58 @@ void IC::zeldovich() {
     for (auto &lv : psi)
       for (auto &f : lv) f.toReal();
   }
+  if (lptOrder == 2) secondOrder();
+}
+
+// Second-order LPT, restated from its definition (Bouchet et al. 1995;
+// Scoccimarro 1998): phi_ij = k_i k_j delta / k^2, S = sum_{i<j} (phi_ii phi_jj
+// - phi_ij^2), psi2 = (3/7) i k S / k^2 (D2 = -3/7 D1^2), zero at a +Nyquist
+// component (mixed derivatives, psi2) and at k = 0; zoom levels keep the
+// high-pass part of their own psi2 and take the low-pass part of the level
+// above, as psi does.  The reference has no second order: parity unpinned.
+void IC::secondOrder() {
+  size_t nl = grids.size();
+  psi2.assign(nl, {});
+  for (size_t L = 0; L < nl; ++L) {
+    const Grid &g = *grids[L];
+    int nyq = g.n / 2;
+    double kMin = g.kmin();
+    Field D = noise[L];
+    D.toFourier();
+    auto phi = [&](int a, int b) {
+      Field f = D;
+      f.forEachK([&](cplx v, int ix, int iy, int iz) {
+        int k[3] = {ix, iy, iz};
+        double kk = double(ix) * ix + double(iy) * iy + double(iz) * iz;
+        if (kk == 0) return cplx(0);
+        if (a != b && (k[a] == nyq || k[b] == nyq)) return cplx(0);
+        return v * ((k[a] * kMin) * (k[b] * kMin) / (kk * kMin * kMin));
+      });
+      f.toReal();
+      return f;
+    };
+    Field xx = phi(0, 0), yy = phi(1, 1), zz = phi(2, 2), xy = phi(0, 1), xz = phi(0, 2), yz = phi(1, 2);
+    Field S(g, false);
+    for (size_t i = 0; i < g.n3(); ++i)
+      S.d[i] = xx.d[i] * yy.d[i] + xx.d[i] * zz.d[i] + yy.d[i] * zz.d[i] - xy.d[i] * xy.d[i] - xz.d[i] * xz.d[i] -
+               yz.d[i] * yz.d[i];
+    S.toFourier();
+    double kcHP = L >= 1 ? kcutForLevel(int(L) - 1) : 0.0;
+    for (int c = 0; c < 3; ++c) {
+      Field f = S;
+      f.forEachK([&](cplx v, int ix, int iy, int iz) {
+        int k[3] = {ix, iy, iz};
+        double kk = (double(ix) * ix + double(iy) * iy + double(iz) * iz) * kMin * kMin;
+        if (kk == 0 || k[c] == nyq) return cplx(0);
+        double filt = L >= 1 ? highPassCov(sqrt(kk), kcHP) : 1.0;
+        return cplx(0, 1) * v * (3.0 / 7.0 * k[c] * kMin / kk * filt);
+      });
+      f.toReal();
+      psi2[L].push_back(std::move(f));
+    }
+    if (L >= 1)
+      for (int c = 0; c < 3; ++c) addWithFilterCoarse(psi2[L][c], psi2[L - 1][c], kcutForLevel(int(L) - 1));
+  }
 }
 
 double IC::particleMass(int level) const {
@@ -288,6 +340,11 @@ void IC::particle(int level, size_t id, double pos[3], double vel[3]) const {
   for (int a = 0; a < 3; ++a) {
     double off = psi[level][a].d[id];
     vel[a] = off * ratio;
+    if (!psi2.empty()) {   // second order: f2 = 2 f1
+      double o2 = psi2[level][a].d[id];
+      off += o2;
+      vel[a] += 2.0 * ratio * o2;
+    }
     double p = off + c[a];
     p = fmod(p, g.periodic);
     if (p < 0) p += g.periodic;

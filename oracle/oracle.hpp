@@ -149,6 +149,8 @@ struct IC {
   std::vector<double> weights;
   std::vector<Field> noise;               // output field (white noise, then delta)
   std::vector<std::vector<Field>> psi;    // per level: psi_x, psi_y, psi_z
+  std::vector<std::vector<Field>> psi2;   // second-order displacements (lptOrder 2)
+  int lptOrder = 1;
   unsigned long seed = 0;
   SeedMode seedMode = SeedMode::Real;
   bool seeded = false, drawn = false, powerApplied = false, combined = false;
@@ -160,6 +162,7 @@ struct IC {
   void zeroLevel(int level);
   void applyPowerSpec();
   void zeldovich();       // ZA per level + level combination (multilevelgenerator.hpp:78-128)
+  void secondOrder();     // 2LPT on the combined overdensity (no reference counterpart)
   double kcutForLevel(int level) const;   // filterfamily.hpp:110-143
   // particle evaluation on level (pos in Mpc/h wrapped, vel gadget units)
   void particle(int level, size_t id, double pos[3], double vel[3]) const;

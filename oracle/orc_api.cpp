@@ -130,7 +130,8 @@ int orc_ic_zeldovich(void *h) { ORC_TRY(((IC *)h)->zeldovich()) }
 int orc_ic_get_field(void *h, int which, int level, int fourier, double *out) {
   ORC_TRY({
     IC &ic = *(IC *)h;
-    Field f = which == 0 ? ic.noise.at(level) : ic.psi.at(level).at(which - 1);
+    Field f = which == 0 ? ic.noise.at(level) : which <= 3 ? ic.psi.at(level).at(which - 1)
+                                                           : ic.psi2.at(level).at(which - 4);
     if (fourier) {
       f.toFourier();
       memcpy(out, f.d.data(), f.d.size() * sizeof(double));
@@ -156,6 +157,16 @@ int orc_ic_particles(void *h, int level, double *pos, double *vel, double *mass,
   })
 }
 
+int orc_ic_set_lpt_order(void *h, int order) { ORC_TRY(((IC *)h)->lptOrder = order) }
+// overwrite the (white noise or overdensity) field of a level with n^3 real values
+int orc_ic_set_field(void *h, int level, const double *vals) {
+  ORC_TRY({
+    IC &ic = *(IC *)h;
+    Field &f = ic.noise.at(level);
+    f.fourier = false;
+    std::copy(vals, vals + f.g->n3(), f.d.begin());
+  })
+}
 int orc_ic_set_flags(void *h, int level, const uint64_t *cells, size_t n) {
   ORC_TRY({
     IC &ic = *(IC *)h;

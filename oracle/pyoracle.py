@@ -53,6 +53,8 @@ def lib():
             "orc_ic_particles": [v, i, v, v, P(d), P(d)],
             "orc_ic_velocity_ratio": [v, P(d)],
             "orc_ic_set_flags": [v, i, v, s],
+            "orc_ic_set_lpt_order": [v, i],
+            "orc_ic_set_field": [v, i, v],
             "orc_ic_set_variance_scale": [v, d],
             "orc_ic_calculate": [v, ctypes.c_char_p, P(d)],
             "orc_ic_add_modification": [v, ctypes.c_char_p, i, d],
@@ -202,6 +204,13 @@ class OracleIC:
         m, e = ctypes.c_double(), ctypes.c_double()
         _check(lib().orc_ic_particles(self.h, level, _ptr(pos), _ptr(vel), ctypes.byref(m), ctypes.byref(e)))
         return pos, vel, m.value, e.value
+
+    def set_lpt_order(self, order):
+        _check(lib().orc_ic_set_lpt_order(self.h, int(order)))
+
+    def set_field(self, level, data):
+        d = np.ascontiguousarray(np.asarray(data, dtype=np.float64).ravel())
+        _check(lib().orc_ic_set_field(self.h, level, _ptr(d)))
 
     # --- modifications ---------------------------------------------------
     def set_flags(self, level, cells):

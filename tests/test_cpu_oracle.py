@@ -485,3 +485,42 @@ def test_oracle_angular_momentum_modification_gadget():
     pos, vel, mass, eps = o.particles(0)
     assert_decimal(pos, ref["pos"], 4, "pos")
     assert_decimal(vel, ref["vel"], 4, "vel")
+
+
+# ------------------------------------------------------------------- 2LPT ----
+def _two_wave_setup(n=32, box=50.0, m1=2, m2=3, A=0.3, B=0.2):
+    o, c = _oracle_cosmo("test_10a_gadget/paramfile.txt")
+    o.init_base_grid(box, n)
+    o.seed(1)
+    o.set_lpt_order(2)
+    o.apply_power_spectrum()
+    x = np.arange(n) * box / n
+    k1, k2 = 2 * np.pi * m1 / box, 2 * np.pi * m2 / box
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    delta = A * np.cos(k1 * X)[:, :, None] + B * np.cos(k2 * Y)[:, :, None] + np.zeros((n, n, n))
+    return o, delta, (X, Y, k1, k2, A, B)
+
+
+def test_oracle_2lpt_closed_forms():
+    """Second-order LPT has no reference: its oracle is pinned by closed
+    forms.  A plane wave has S = 0, so psi2 = 0.  For delta = A cos(k1 x) +
+    B cos(k2 y), phi_xx = A cos(k1 x), phi_yy = B cos(k2 y), S = phi_xx phi_yy
+    and psi2 = (3/7) grad del^-2 S: psi2_x = -(3/7) A B k1 sin(k1 x) cos(k2 y) / (k1^2 + k2^2)."""
+    o, delta, (X, Y, k1, k2, A, B) = _two_wave_setup()
+    o.set_field(0, delta)
+    o.zeldovich()
+    n = delta.shape[0]
+    ex = -(3 / 7) * A * B * k1 * np.sin(k1 * X) * np.cos(k2 * Y) / (k1 ** 2 + k2 ** 2)
+    ey = -(3 / 7) * A * B * k2 * np.cos(k1 * X) * np.sin(k2 * Y) / (k1 ** 2 + k2 ** 2)
+    px, py, pz = (o.field(4 + a, 0) for a in range(3))
+    scale = np.abs(ex).max()
+    assert np.abs(px - ex[:, :, None]).max() < 1e-12 * scale * n
+    assert np.abs(py - ey[:, :, None]).max() < 1e-12 * scale * n
+    assert np.abs(pz).max() < 1e-12 * scale * n
+    # the first-order part is untouched: psi = i k delta / k^2, psi_x = -A sin(k1 x) / k1
+    assert np.abs(o.field(1, 0) + (A / k1 * np.sin(k1 * X))[:, :, None]).max() < 1e-12
+    o2, _, (X, Y, k1, k2, A, B) = _two_wave_setup()
+    o2.set_field(0, delta * 0 + A * np.cos(k1 * X)[:, :, None])
+    o2.zeldovich()
+    for a in range(3):
+        assert np.abs(o2.field(4 + a, 0)).max() < 1e-15
