@@ -250,6 +250,8 @@ void coarse_window(const LevelGeom &gf, const LevelGeom &gc, int *w0, int *wl) {
 // psi2 and takes the low-pass part from the level above, as for psi.
 template <typename T> void do_second_order(gic_context *c) {
   size_t nl = c->levels.size();
+  CoarseView win_view[3];
+  bool win_ready = false;
   for (size_t l = 0; l < nl; ++l) {
     Level &L = *c->levels[l];
     const LevelGeom &g = L.g;
@@ -294,13 +296,45 @@ template <typename T> void do_second_order(gic_context *c) {
       fft_c2r_from<T>(S, (T *)L.psi2[a].p, n, FFT_PRE_ZA, (const T *)c->tab_c.p, a, g.kmin(), c->st, c->hook());
     if (l >= 1) {
       Level &C = *c->levels[l - 1];
-      long hc = C.g.n / 2, nc_ent = 3 * hc * hc + 1;
-      c->tab_b.ensure(nc_ent * sizeof(T));
-      filter_table<T>(0, kcut_for_level(c, int(l) - 1), C.g.kmin(), nc_ent, (T *)c->tab_b.p, c->st);
       InterpGeom ig = interp_geom(g, C.g);
-      for (int a = 0; a < 3; ++a) {
-        CoarseView v = lowpass_full<T>(c, C.psi2[a], C.g, (const T *)c->tab_b.p);
-        interp_add<T>((T *)L.psi2[a].p, n, v, ig, c->st);
+      if (l == 1 && win_ready) {
+        for (int a = 0; a < 3; ++a) interp_add<T>((T *)L.psi2[a].p, n, win_view[a], ig, c->st);
+      } else {
+        long hc = C.g.n / 2, nc_ent = 3 * hc * hc + 1;
+        c->tab_b.ensure(nc_ent * sizeof(T));
+        filter_table<T>(0, kcut_for_level(c, int(l) - 1), C.g.kmin(), nc_ent, (T *)c->tab_b.p, c->st);
+        for (int a = 0; a < 3; ++a) {
+          CoarseView v = lowpass_full<T>(c, C.psi2[a], C.g, (const T *)c->tab_b.p);
+          interp_add<T>((T *)L.psi2[a].p, n, v, ig, c->st);
+        }
+      }
+    }
+    if (l == 0 && nl >= 2 && g.covers_full()) {
+      // the low-pass part that level 1 takes from the base level, straight
+      // from the base source spectrum into the coarse window (pruned c2r)
+      const LevelGeom &gf = c->levels[1]->g;
+      int w0[3], wl[3];
+      coarse_window(gf, g, w0, wl);
+      if (wl[0] < n && wl[1] < n && wl[2] < n) {
+        int nz = n / 2 + 1;
+        size_t a_b = size_t(wl[0]) * n * nz * 2 * sizeof(T), b_b = size_t(wl[0]) * wl[1] * nz * 2 * sizeof(T);
+        size_t o_b = size_t(wl[0]) * wl[1] * wl[2] * sizeof(T);
+        c->lpt_win.ensure(a_b + b_b + 3 * o_b);
+        T *bufA = (T *)c->lpt_win.p, *bufB = (T *)((char *)c->lpt_win.p + a_b);
+        filter_table<T>(0, kcut_for_level(c, 0), g.kmin(), nent, (T *)c->tab_c.p, c->st, true, 3.0 / 7.0);
+        for (int a = 0; a < 3; ++a) {
+          T *out = (T *)((char *)c->lpt_win.p + a_b + b_b + a * o_b);
+          fft_c2r_window<T>(S, n, FFT_PRE_ZA, (const T *)c->tab_c.p, a, g.kmin(), w0, wl, bufA, bufB, out, c->st,
+                            c->hook());
+          win_view[a].p = out;
+          for (int d = 0; d < 3; ++d) {
+            win_view[a].org[d] = w0[d];
+            win_view[a].len[d] = wl[d];
+          }
+          win_view[a].sx = long(wl[1]) * wl[2];
+          win_view[a].sy = wl[2];
+        }
+        win_ready = true;
       }
     }
     L.have_psi2 = true;

@@ -230,7 +230,7 @@ struct gic_context {
   int seed_mode = 0;
   bool seeded = false, drawn = false, power_applied = false, zeldovich_done = false;
   int lpt_order = 1;   // 1: Zel'dovich, 2: second-order LPT on top
-  DevBuf work, stage, tab_a, tab_b, tab_c;
+  DevBuf work, stage, tab_a, tab_b, tab_c, lpt_win;
   std::unique_ptr<EventTimer> timer;
   TimingHook *hook() { return timer.get(); }
   size_t tsize() const { return prec == 32 ? 4 : 8; }
@@ -244,6 +244,8 @@ Level &level_at(gic_context *c, int level);
 void alloc_field(gic_context *c, DevBuf &b, const LevelGeom &g);
 double kcut_for_level(gic_context *c, int level);
 InterpGeom interp_geom(const LevelGeom &gf, const LevelGeom &gc);
+// coarse cells [w0, w0 + wl) (mod n per axis) that the tricubic stencils of a fine level read
+void coarse_window(const LevelGeom &gf, const LevelGeom &gc, int *w0, int *wl);
 template <typename T> void to_fourier(gic_context *c, DevBuf &b, int n, bool &flag) {
   if (flag) return;
   fft3d<T>((T *)b.p, n, -1, c->st, c->hook());

@@ -218,7 +218,9 @@ __device__ __forceinline__ int signed_k(int i, int n) { return i > n / 2 ? i - n
 
 // PRE_ZA multiplies by i * k_c * tab[|k|^2], where the caller's table holds
 // filter(k) / k^2 (0 at k = 0); the component at k_c = +Nyquist is zero.
-template <typename T, int L, int C, int DIR, int THREADS, int PRE, int POST>
+// PASS only names the pass (0 x, 1 y, 2 windowed x, 3 windowed y) so that
+// profiles tell the launches apart.
+template <typename T, int L, int C, int DIR, int THREADS, int PRE, int POST, int PASS>
 __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const cx<T> *__restrict__ tw) {
   extern __shared__ char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
@@ -480,7 +482,7 @@ template <typename T, int N> struct FftN {
   static constexpr int ROWS0 = (8192 / N) < 1 ? 1 : (8192 / N);
   static constexpr int ROWS = ROWS0 > N * N ? N * N : ROWS0;
 
-  template <int DIR, int PRE, int POST>
+  template <int DIR, int PRE, int POST, int PASS = 0>
   static void strided(const StridedArgs<T> &a0, int nouter, hipStream_t st, TimingHook *hook, const char *name) {
     StridedArgs<T> a = a0;
     a.ncol = NZ;
@@ -489,7 +491,7 @@ template <typename T, int N> struct FftN {
     int nblk = nouter * a.nkzb;
     size_t shm = sizeof(cx<T>) * C * N;
     const cx<T> *tw = twtable<T>().get(N);
-    auto kern = k_fft_strided<T, N, C, DIR, THREADS, PRE, POST>;
+    auto kern = k_fft_strided<T, N, C, DIR, THREADS, PRE, POST, PASS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     std::string tag = std::string(name) + "." + std::to_string(N);
     if (hook) hook->begin(tag.c_str(), st);
@@ -538,20 +540,20 @@ template <typename T, int N> struct FftN {
   static void r2c(T *d, const T *post, hipStream_t st, TimingHook *hook) {
     cx<T> *c = reinterpret_cast<cx<T> *>(d);
     zpass(d, T(1), -1, st, hook, "fft_z_r2c");
-    strided<-1, PRE_NONE, POST_NONE>(plain(c, false, T(1)), N, st, hook, "fft_y");
+    strided<-1, PRE_NONE, POST_NONE, 1>(plain(c, false, T(1)), N, st, hook, "fft_y_fwd");
     StridedArgs<T> ax = plain(c, true, norm());
     if (post) {
       ax.tab = post;
       strided<-1, PRE_NONE, POST_TABLE>(ax, N, st, hook, "fft_x_post");
     } else {
-      strided<-1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x");
+      strided<-1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x_fwd");
     }
   }
   static void c2r(T *d, hipStream_t st, TimingHook *hook) {
     cx<T> *c = reinterpret_cast<cx<T> *>(d);
     mirror(c, st, hook);
     strided<+1, PRE_NONE, POST_NONE>(plain(c, true, T(1)), N, st, hook, "fft_x");
-    strided<+1, PRE_NONE, POST_NONE>(plain(c, false, T(1)), N, st, hook, "fft_y");
+    strided<+1, PRE_NONE, POST_NONE, 1>(plain(c, false, T(1)), N, st, hook, "fft_y");
     zpass(d, norm(), +1, st, hook, "fft_z_c2r");
   }
   // out <- c2r(op(spec)) with op applied while the x pass reads the spectrum
@@ -568,7 +570,7 @@ template <typename T, int N> struct FftN {
     else if (pre == PRE_KK) strided<+1, PRE_KK, POST_NONE>(ax, N, st, hook, "fft_x_kk");
     else if (pre == PRE_TABLE) strided<+1, PRE_TABLE, POST_NONE>(ax, N, st, hook, "fft_x_pre");
     else strided<+1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x");
-    strided<+1, PRE_NONE, POST_NONE>(plain(co, false, T(1)), N, st, hook, "fft_y");
+    strided<+1, PRE_NONE, POST_NONE, 1>(plain(co, false, T(1)), N, st, hook, "fft_y");
     zpass(out, norm(), +1, st, hook, "fft_z_c2r");
   }
   // real-space window [w0, w0+wl) (per axis, mod N) of c2r(op(spec)):
@@ -593,9 +595,9 @@ template <typename T, int N> struct FftN {
     ax.win0 = w0[0];
     ax.wlen = wl[0];
     ax.scale = T(1);
-    if (pre == PRE_ZA) strided<+1, PRE_ZA, POST_NONE>(ax, N, st, hook, "fft_x_za_win");
-    else if (pre == PRE_TABLE) strided<+1, PRE_TABLE, POST_NONE>(ax, N, st, hook, "fft_x_pre_win");
-    else strided<+1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x_win");
+    if (pre == PRE_ZA) strided<+1, PRE_ZA, POST_NONE, 2>(ax, N, st, hook, "fft_x_za_win");
+    else if (pre == PRE_TABLE) strided<+1, PRE_TABLE, POST_NONE, 2>(ax, N, st, hook, "fft_x_pre_win");
+    else strided<+1, PRE_NONE, POST_NONE, 2>(ax, N, st, hook, "fft_x_win");
     // y pass: tiles (x_local, kz block), lines over ky, keep y in window; B = [wl0][wl1][NZ]
     StridedArgs<T> ay{};
     ay.in = ca;
@@ -608,7 +610,7 @@ template <typename T, int N> struct FftN {
     ay.win0 = w0[1];
     ay.wlen = wl[1];
     ay.scale = T(1);
-    strided<+1, PRE_NONE, POST_NONE>(ay, wl[0], st, hook, "fft_y_win");
+    strided<+1, PRE_NONE, POST_NONE, 3>(ay, wl[0], st, hook, "fft_y_win");
     // z pass: rows (x_local, y_local), keep z in window
     long nrows = long(wl[0]) * wl[1];
     int nblk = int((nrows + ROWS - 1) / ROWS);

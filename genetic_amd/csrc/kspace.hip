@@ -413,8 +413,48 @@ __global__ void k_lpt_accum(T *__restrict__ S, T *__restrict__ A, const T *__res
   else if (mode == 2) S[i] -= A[i] * A[i];
   else A[i] += B[i];
 }
+// four consecutive elements per thread (n is a multiple of 4 for even grids)
+template <typename T> struct alignas(4 * sizeof(T)) Vec4 {
+  T v[4];
+};
+template <typename T>
+__global__ void k_lpt_accum4(Vec4<T> *__restrict__ S, Vec4<T> *__restrict__ A, const Vec4<T> *__restrict__ B,
+                             size_t n4, int mode) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n4) return;
+  Vec4<T> a = A[i];
+  if (mode == 3) {
+    Vec4<T> b = B[i];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) a.v[k] += b.v[k];
+    A[i] = a;
+    return;
+  }
+  Vec4<T> s;
+  if (mode == 0) {
+    Vec4<T> b = B[i];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s.v[k] = a.v[k] * b.v[k];
+  } else if (mode == 1) {
+    Vec4<T> b = B[i];
+    s = S[i];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s.v[k] += a.v[k] * b.v[k];
+  } else {
+    s = S[i];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) s.v[k] -= a.v[k] * a.v[k];
+  }
+  S[i] = s;
+}
 template <typename T> void lpt_accum(T *S, T *A, const T *B, size_t n, int mode, hipStream_t st) {
-  hipLaunchKernelGGL(k_lpt_accum<T>, dim3((n + 255) / 256), dim3(256), 0, st, S, A, B, n, mode);
+  if (n % 4 == 0) {
+    size_t n4 = n / 4;
+    hipLaunchKernelGGL(k_lpt_accum4<T>, dim3((n4 + 255) / 256), dim3(256), 0, st, reinterpret_cast<Vec4<T> *>(S),
+                       reinterpret_cast<Vec4<T> *>(A), reinterpret_cast<const Vec4<T> *>(B), n4, mode);
+  } else {
+    hipLaunchKernelGGL(k_lpt_accum<T>, dim3((n + 255) / 256), dim3(256), 0, st, S, A, B, n, mode);
+  }
   GIC_HIP(hipGetLastError());
 }
 

@@ -17,7 +17,6 @@
 #include "context.h"
 
 #include <strings.h>
-#include <unordered_map>
 
 namespace gic {
 
@@ -217,14 +216,6 @@ __global__ void k_flag_center(T *__restrict__ f, int n, const uint64_t *__restri
   f[p] = v / T(m);
 }
 
-// sparse covector values into a zeroed padded field
-template <typename T>
-__global__ void k_scatter(T *__restrict__ f, int n, const uint64_t *__restrict__ idx, const double *__restrict__ val,
-                          size_t m) {
-  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i < m) f[padded_of(idx[i], n)] = T(val[i]);
-}
-
 // fine -> coarse: the mean of the contained fine subcells of each coarse
 // cell whose first or last subcell lies in the fine grid (SubSampleEvaluator
 // over a SectionOfGrid of the fine grid)
@@ -337,6 +328,72 @@ __global__ void k_deinterp(const S *__restrict__ fine, int fine_padded, Deinterp
       }
 }
 
+
+// localisation covectors of the finest level (linearmodification.hpp:60-70,
+// 198-226, 330-370), one thread per flagged cell, summed in FP64
+struct LocArgs {
+  int kind, n, sim, dir;
+  double w, a, b, dx, o[3], periodic, qc[3];
+};
+__device__ __forceinline__ long loc_step(long index, int n, int sim, int dir, int s) {
+  long q[3] = {index / (long(n) * n), (index / n) % n, index % n};
+  q[dir] += s;
+  for (int a = 0; a < 3; ++a) {
+    if (q[a] > sim - 1) q[a] -= sim;
+    if (q[a] < 0) q[a] += sim;
+  }
+  return (q[0] * n + q[1]) * n + q[2];
+}
+__device__ __forceinline__ void loc_add(double *dense, long n3, long i, double v) {
+  if (i >= 0 && i < n3) atomicAdd(&dense[i], v);   // steps leaving a zoom grid are dropped
+}
+__global__ void k_localise(const uint64_t *__restrict__ cells, size_t m, LocArgs A, double *__restrict__ dense) {
+  size_t t = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= m) return;
+  long c = long(cells[t]);
+  long n3 = long(A.n) * A.n * A.n;
+  if (A.kind <= 1) {
+    loc_add(dense, n3, c, A.w);
+  } else if (A.kind <= 4) {
+    long m1 = loc_step(c, A.n, A.sim, A.dir, -1), p1 = loc_step(c, A.n, A.sim, A.dir, +1);
+    long m2 = loc_step(m1, A.n, A.sim, A.dir, -1), p2 = loc_step(p1, A.n, A.sim, A.dir, +1);
+    loc_add(dense, n3, m2, A.a);
+    loc_add(dense, n3, m1, A.b);
+    loc_add(dense, n3, p1, -A.b);
+    loc_add(dense, n3, p2, -A.a);
+  } else {
+    int dir = A.dir, dp1 = (dir + 1) % 3, dp2 = (dir + 2) % 3;
+    long q[3] = {c / (long(A.n) * A.n), (c / A.n) % A.n, c % A.n};
+    double dq[3], cr[3];
+    for (int d = 0; d < 3; ++d) {
+      double p = double(q[d]) * A.dx + A.o[d] + A.dx / 2;
+      double r = A.qc[d] - p;
+      if (r > A.periodic / 2) r -= A.periodic;
+      if (r < -A.periodic / 2) r += A.periodic;
+      dq[d] = r;
+    }
+    cr[dir] = 0;
+    cr[dp2] = dq[dp1];
+    cr[dp1] = -dq[dp2];
+    for (int d = 0; d < 3; ++d) {
+      if (d == dir) continue;
+      long m1 = loc_step(c, A.n, A.sim, d, -1), m2 = loc_step(m1, A.n, A.sim, d, -1);
+      long p1 = loc_step(c, A.n, A.sim, d, +1), p2 = loc_step(p1, A.n, A.sim, d, +1);
+      loc_add(dense, n3, m2, cr[d] * A.a);
+      loc_add(dense, n3, m1, cr[d] * A.b);
+      loc_add(dense, n3, p1, -cr[d] * A.b);
+      loc_add(dense, n3, p2, -cr[d] * A.a);
+    }
+  }
+}
+template <typename T> __global__ void k_dense_to_padded(T *__restrict__ f, const double *__restrict__ d, int n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t n3 = size_t(n) * n * n;
+  if (i >= n3) return;
+  size_t x = i / (size_t(n) * n), y = (i / n) % n, z = i % n;
+  f[(x * n + y) * (n + 2) + z] = T(d[i]);
+}
+
 int blocks(size_t n) { return int((n + 255) / 256); }
 
 }  // namespace
@@ -350,11 +407,39 @@ struct ModRec {
   std::vector<std::vector<uint64_t>> flags;
 };
 
+// Device memory reused across the many temporary fields of the modification
+// algebra: every buffer goes back to the free list when its field dies and
+// is taken again by the next request of a size it fits (all work runs on
+// the context's one stream, so reuse is ordered after the last use).
+struct BufPool {
+  std::multimap<size_t, void *> free_;
+  std::map<void *, size_t> size_of;
+  void *get(size_t b) {
+    auto it = free_.lower_bound(b);
+    if (it != free_.end() && it->first <= 2 * b) {
+      void *p = it->second;
+      free_.erase(it);
+      return p;
+    }
+    void *p = nullptr;
+    GIC_HIP(hipMalloc(&p, b));
+    size_of[p] = b;
+    return p;
+  }
+  void put(void *p) {
+    if (p) free_.emplace(size_of.at(p), p);
+  }
+  ~BufPool() {
+    for (auto &kv : size_of) (void)hipFree(kv.first);
+  }
+};
+
 }  // namespace gic
 
 struct ModState {
   std::vector<gic::ModRec> linear, quadratic;
   gic::DevBuf part, scalar;
+  gic::BufPool pool;
 };
 
 gic_context::gic_context() = default;
@@ -369,50 +454,84 @@ size_t table_len(const LevelGeom &g) {
   return size_t(3 * h * h + 1);
 }
 
+// A pool buffer that returns itself on destruction
+struct PBuf {
+  BufPool *pool = nullptr;
+  void *p = nullptr;
+  PBuf() = default;
+  PBuf(BufPool &pl, size_t b) : pool(&pl), p(pl.get(b)) {}
+  PBuf(PBuf &&o) noexcept : pool(o.pool), p(o.p) { o.p = nullptr; }
+  PBuf &operator=(PBuf &&o) noexcept {
+    if (this != &o) {
+      if (pool) pool->put(p);
+      pool = o.pool;
+      p = o.p;
+      o.p = nullptr;
+    }
+    return *this;
+  }
+  PBuf(const PBuf &) = delete;
+  ~PBuf() {
+    if (pool) pool->put(p);
+  }
+};
+
+// A multi-level field: one padded buffer per level, allocated on first use
 template <typename T> struct MLF {
   gic_context *c = nullptr;
-  std::vector<DevBuf> lv;
+  std::vector<PBuf> lv;
   std::vector<bool> fourier;
   explicit MLF(gic_context *cc) : c(cc) {
-    size_t nl = c->levels.size();
-    lv.resize(nl);
-    fourier.assign(nl, false);
-    for (size_t l = 0; l < nl; ++l) lv[l].ensure(c->levels[l]->g.padded_reals() * sizeof(T));
+    lv.resize(c->levels.size());
+    fourier.assign(c->levels.size(), false);
   }
-  MLF(const MLF &o) : c(o.c), fourier(o.fourier) {
-    lv.resize(o.lv.size());
-    for (size_t l = 0; l < lv.size(); ++l) {
-      lv[l].ensure(o.lv[l].bytes);
-      GIC_HIP(hipMemcpyAsync(lv[l].p, o.lv[l].p, o.lv[l].bytes, hipMemcpyDeviceToDevice, c->st));
-    }
-  }
-  MLF &operator=(const MLF &) = delete;
+  MLF(MLF &&) noexcept = default;
+  MLF &operator=(MLF &&) noexcept = default;
   size_t nl() const { return lv.size(); }
-  T *p(size_t l) { return (T *)lv[l].p; }
-  const T *p(size_t l) const { return (const T *)lv[l].p; }
   const LevelGeom &g(size_t l) const { return c->levels[l]->g; }
   size_t reals(size_t l) const { return g(l).padded_reals(); }
+  bool has(size_t l) const { return lv[l].p != nullptr; }
+  T *p(size_t l) {
+    if (!lv[l].p) lv[l] = PBuf(c->mod->pool, reals(l) * sizeof(T));
+    return (T *)lv[l].p;
+  }
+  const T *p(size_t l) const { return (const T *)lv[l].p; }
+  void copy_level(const MLF &o, size_t l) {
+    GIC_HIP(hipMemcpyAsync(p(l), o.p(l), reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
+    fourier[l] = o.fourier[l];
+  }
+  MLF clone() const {
+    MLF m(c);
+    for (size_t l = 0; l < nl(); ++l)
+      if (has(l)) m.copy_level(*this, l);
+    return m;
+  }
+  void zero(size_t l) {
+    fill<T>(p(l), reals(l), T(0), c->st);
+    fourier[l] = false;
+  }
   void zero() {
-    for (size_t l = 0; l < nl(); ++l) fill<T>(p(l), reals(l), T(0), c->st);
+    for (size_t l = 0; l < nl(); ++l) zero(l);
   }
   void toF(size_t l) {
-    bool f = fourier[l];
-    to_fourier<T>(c, lv[l], g(l).n, f);
-    fourier[l] = f;
+    if (fourier[l]) return;
+    fft3d<T>(p(l), g(l).n, -1, c->st, c->hook());
+    fourier[l] = true;
   }
   void toR(size_t l) {
-    bool f = fourier[l];
-    to_real<T>(c, lv[l], g(l).n, f);
-    fourier[l] = f;
+    if (!fourier[l]) return;
+    fft3d<T>(p(l), g(l).n, +1, c->st, c->hook());
+    fourier[l] = false;
   }
   void toF() {
     for (size_t l = 0; l < nl(); ++l) toF(l);
   }
+  void axpy_level(size_t l, const T *x, double s) {
+    size_t n = reals(l);
+    hipLaunchKernelGGL(k_axpy<T>, dim3(blocks(n)), dim3(256), 0, c->st, p(l), x, T(s), n);
+  }
   void axpy(const MLF &x, double s) {   // this += s x, same spaces
-    for (size_t l = 0; l < nl(); ++l) {
-      size_t n = reals(l);
-      hipLaunchKernelGGL(k_axpy<T>, dim3(blocks(n)), dim3(256), 0, c->st, p(l), x.p(l), T(s), n);
-    }
+    for (size_t l = 0; l < nl(); ++l) axpy_level(l, x.p(l), s);
   }
   void scale(double s) {
     for (size_t l = 0; l < nl(); ++l) {
@@ -428,7 +547,7 @@ template <typename T> MLF<T> copy_output(gic_context *c) {
   for (size_t l = 0; l < m.nl(); ++l) {
     Level &L = *c->levels[l];
     if (L.dc_pending) ensure_noise_fourier<T>(c, L);
-    GIC_HIP(hipMemcpyAsync(m.lv[l].p, L.noise.p, m.reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
+    GIC_HIP(hipMemcpyAsync(m.p(l), L.noise.p, m.reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
     m.fourier[l] = L.noise_fourier;
   }
   return m;
@@ -437,7 +556,7 @@ template <typename T> void store_output(gic_context *c, MLF<T> &m) {
   m.toF();
   for (size_t l = 0; l < m.nl(); ++l) {
     Level &L = *c->levels[l];
-    GIC_HIP(hipMemcpyAsync(L.noise.p, m.lv[l].p, m.reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
+    GIC_HIP(hipMemcpyAsync(L.noise.p, m.p(l), m.reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
     L.noise_fourier = true;
     L.dc_pending = false;
   }
@@ -474,36 +593,41 @@ FilterSpec spec_levels(gic_context *c, long la, long lb, double scale) {
   return s;
 }
 
-// multiply level l of m (Fourier) by the filter spec evaluated on that level's k
-template <typename T> void apply_spec(MLF<T> &m, size_t l, const FilterSpec &s) {
-  gic_context *c = m.c;
-  const LevelGeom &g = m.g(l);
+// the table of a filter spec on the k of a level (in c->tab_a)
+template <typename T> const T *spec_table(gic_context *c, const LevelGeom &g, const FilterSpec &s) {
   size_t nent = table_len(g);
   c->tab_a.ensure(nent * sizeof(T));
-  m.toF(l);
   hipLaunchKernelGGL(k_filter_spec<T>, dim3(blocks(nent)), dim3(256), 0, c->st, s, g.kmin(), long(nent),
                      (T *)c->tab_a.p);
-  mul_table<T>(m.p(l), g.n, (const T *)c->tab_a.p, c->st);
+  return (const T *)c->tab_a.p;
+}
+
+// multiply level l of m (Fourier) by the filter spec evaluated on that level's k
+template <typename T> void apply_spec(MLF<T> &m, size_t l, const FilterSpec &s) {
+  const T *tab = spec_table<T>(m.c, m.g(l), s);
+  m.toF(l);
+  mul_table<T>(m.p(l), m.g(l).n, tab, m.c->st);
 }
 
 // sum_l sum_k w Re(conj(a) b), both in Fourier space
 template <typename T> double inner(gic_context *c, MLF<T> &a, MLF<T> &b) {
   a.toF();
   b.toF();
-  c->mod->part.ensure(RED_BLOCKS * sizeof(double) * 8);
+  c->mod->part.ensure(RED_BLOCKS * sizeof(double));
   c->mod->scalar.ensure(sizeof(double) * 8);
-  double total = 0;
-  for (size_t l = 0; l < a.nl(); ++l) {
+  size_t nl = a.nl();
+  for (size_t l = 0; l < nl; ++l) {
     hipLaunchKernelGGL(k_inner_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st,
                        reinterpret_cast<const cx<T> *>(a.p(l)), reinterpret_cast<const cx<T> *>(b.p(l)), a.g(l).n,
                        (double *)c->mod->part.p);
     hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, c->st, (const double *)c->mod->part.p,
-                       RED_BLOCKS, (double *)c->mod->scalar.p);
-    double v;
-    GIC_HIP(hipMemcpyAsync(&v, c->mod->scalar.p, sizeof v, hipMemcpyDeviceToHost, c->st));
-    GIC_HIP(hipStreamSynchronize(c->st));
-    total += v;
+                       RED_BLOCKS, (double *)c->mod->scalar.p + l);
   }
+  double v[8];
+  GIC_HIP(hipMemcpyAsync(v, c->mod->scalar.p, nl * sizeof(double), hipMemcpyDeviceToHost, c->st));
+  GIC_HIP(hipStreamSynchronize(c->st));
+  double total = 0;
+  for (size_t l = 0; l < nl; ++l) total += v[l];
   return total;
 }
 
@@ -511,13 +635,13 @@ WinDev grid_window(const LevelGeom &g) {   // Grid::getWindow (grid.hpp:502-504)
   WinDev w;
   w.L = g.periodic;
   double lo[3] = {g.ox, g.oy, g.oz};
+  auto wrap = [&](double v) {
+    if (v < 0) v += w.L;
+    if (v >= w.L) v -= w.L;
+    return v;
+  };
   for (int d = 0; d < 3; ++d) {
     double a = lo[d], b = lo[d] + g.box;
-    auto wrap = [&](double v) {
-      if (v < 0) v += w.L;
-      if (v >= w.L) v -= w.L;
-      return v;
-    };
     w.lo[d] = wrap(a);
     w.hi[d] = std::nextafter(wrap(std::nextafter(b, -INFINITY)), INFINITY);
   }
@@ -534,7 +658,8 @@ template <typename T> void window_zero(MLF<T> &m, size_t l, const WinDev &w, boo
 
 // applyFilterInWindow (field.hpp:752-775) on level l of m
 template <typename T> void filter_in_window(MLF<T> &m, size_t l, const FilterSpec &s, const WinDev &w, bool first) {
-  MLF<T> in(m);
+  MLF<T> in(m.c);
+  in.copy_level(m, l);
   if (first) {
     window_zero(in, l, w, false);
     apply_spec(in, l, s);
@@ -545,24 +670,61 @@ template <typename T> void filter_in_window(MLF<T> &m, size_t l, const FilterSpe
   window_zero(m, l, w, true);
   if (in.fourier[l]) m.toF(l);
   else m.toR(l);
-  size_t n = m.reals(l);
-  hipLaunchKernelGGL(k_axpy<T>, dim3(blocks(n)), dim3(256), 0, m.c->st, m.p(l), in.p(l), T(1), n);
+  m.axpy_level(l, in.p(l), 1.0);
 }
 
-// target level lt += transfer of source level ls of src (already filtered, real)
-template <typename T> void add_other_level(gic_context *c, T *target, const LevelGeom &gt, const T *source,
-                                           const LevelGeom &gs) {
+double level_weight(gic_context *c, size_t l) {
+  double r = c->levels[l]->g.dx / c->levels[0]->g.dx;
+  return l == 0 ? 1.0 : pow(r, 3.0);
+}
+
+// R (real, level lt) += the level-ls field f transferred with filter spec s
+// (addFieldFromDifferentGridWithFilter, FILTER_ON_COARSE_GRID): from a
+// coarser level by tricubic interpolation -- through a pruned inverse
+// transform of the coarse window when the coarse level covers the box -- and
+// from a finer level by averaging its cells.
+template <typename T> void add_transfer(gic_context *c, T *R, size_t lt, MLF<T> &f, size_t ls, const FilterSpec &s) {
+  const LevelGeom &gt = c->levels[lt]->g, &gs = c->levels[ls]->g;
+  f.toF(ls);
   if (gs.dx > gt.dx) {
+    int w0[3], wl[3];
+    coarse_window(gt, gs, w0, wl);
+    const T *tab = spec_table<T>(c, gs, s);
     CoarseView v;
-    v.p = source;
-    for (int a = 0; a < 3; ++a) {
-      v.org[a] = 0;
-      v.len[a] = gs.n;
+    PBuf scratch;
+    if (gs.covers_full() && wl[0] < gs.n && wl[1] < gs.n && wl[2] < gs.n) {
+      int nz = gs.n / 2 + 1;
+      size_t a_b = size_t(wl[0]) * gs.n * nz * 2 * sizeof(T);
+      size_t b_b = size_t(wl[0]) * wl[1] * nz * 2 * sizeof(T);
+      size_t o_b = size_t(wl[0]) * wl[1] * wl[2] * sizeof(T);
+      scratch = PBuf(c->mod->pool, a_b + b_b + o_b);
+      T *bufA = (T *)scratch.p;
+      T *bufB = (T *)((char *)scratch.p + a_b);
+      T *out = (T *)((char *)scratch.p + a_b + b_b);
+      fft_c2r_window<T>(f.p(ls), gs.n, FFT_PRE_TABLE, tab, 0, gs.kmin(), w0, wl, bufA, bufB, out, c->st, c->hook());
+      v.p = out;
+      for (int a = 0; a < 3; ++a) {
+        v.org[a] = w0[a];
+        v.len[a] = wl[a];
+      }
+      v.sx = long(wl[1]) * wl[2];
+      v.sy = wl[2];
+    } else {
+      scratch = PBuf(c->mod->pool, gs.padded_reals() * sizeof(T));
+      fft_c2r_from<T>(f.p(ls), (T *)scratch.p, gs.n, FFT_PRE_TABLE, tab, 0, gs.kmin(), c->st, c->hook());
+      v.p = scratch.p;
+      for (int a = 0; a < 3; ++a) {
+        v.org[a] = 0;
+        v.len[a] = gs.n;
+      }
+      v.sx = long(gs.n) * (gs.n + 2);
+      v.sy = gs.n + 2;
     }
-    v.sx = long(gs.n) * (gs.n + 2);
-    v.sy = gs.n + 2;
-    interp_add<T>(target, gt.n, v, interp_geom(gt, gs), c->st);
+    interp_add<T>(R, gt.n, v, interp_geom(gt, gs), c->st);
   } else {
+    const T *tab = spec_table<T>(c, gs, s);
+    PBuf tmp(c->mod->pool, gs.padded_reals() * sizeof(T));
+    fft_c2r_from<T>(f.p(ls), (T *)tmp.p, gs.n, FFT_PRE_TABLE, tab, 0, gs.kmin(), c->st, c->hook());
     AvgGeom a;
     a.r = (int)llround(gt.dx / gs.dx);
     a.cell_off[0] = (int)llround((gt.ox - gs.ox) / gs.dx);
@@ -573,44 +735,40 @@ template <typename T> void add_other_level(gic_context *c, T *target, const Leve
     a.nf = gs.n;
     a.nc = gt.n;
     size_t nc3 = gt.n3();
-    hipLaunchKernelGGL(k_fine_to_coarse<T>, dim3(blocks(nc3)), dim3(256), 0, c->st, target, source, a);
+    hipLaunchKernelGGL(k_fine_to_coarse<T>, dim3(blocks(nc3)), dim3(256), 0, c->st, R, (const T *)tmp.p, a);
   }
   GIC_HIP(hipGetLastError());
 }
 
-double level_weight(gic_context *c, size_t l) {
-  double r = c->levels[l]->g.dx / c->levels[0]->g.dx;
-  return l == 0 ? 1.0 : pow(r, 3.0);
-}
-
-// applyMetric (multilevelfield.hpp:335-384)
+// applyMetric (multilevelfield.hpp:335-384); the cross-level terms of a
+// level are gathered in one real buffer and added in Fourier space
 template <typename T> MLF<T> metric(MLF<T> &f, bool to_covector) {
   gic_context *c = f.c;
   size_t nl = f.nl();
   f.toF();
   MLF<T> out(c);
   for (size_t l = 0; l < nl; ++l) {
-    MLF<T> res(f);   // only level l is used
+    out.copy_level(f, l);
     if (to_covector && l + 1 < nl) {
       WinDev w = grid_window(c->levels[l + 1]->g);
       FilterSpec fl = spec_levels(c, long(l), -1, 1.0);
-      filter_in_window(res, l, fl, w, true);
-      filter_in_window(res, l, fl, w, false);
+      filter_in_window(out, l, fl, w, true);
+      filter_in_window(out, l, fl, w, false);
+      out.toF(l);
     } else {
-      apply_spec(res, l, spec_levels(c, long(l), long(l), 1.0));
+      apply_spec(out, l, spec_levels(c, long(l), long(l), 1.0));
     }
-    for (size_t s = 0; s < nl; ++s) {
-      if (s == l) continue;
-      double sq = sqrt(level_weight(c, l) / level_weight(c, s));
-      MLF<T> tmp(f);
-      apply_spec(tmp, s, spec_levels(c, long(l), long(s), sq));
-      tmp.toR(s);
-      res.toR(l);
-      add_other_level<T>(c, res.p(l), res.g(l), tmp.p(s), tmp.g(s));
+    if (nl > 1) {
+      MLF<T> R(c);
+      R.zero(l);
+      for (size_t s = 0; s < nl; ++s) {
+        if (s == l) continue;
+        double sq = sqrt(level_weight(c, l) / level_weight(c, s));
+        add_transfer<T>(c, R.p(l), l, f, s, spec_levels(c, long(l), long(s), sq));
+      }
+      R.toF(l);
+      out.axpy_level(l, R.p(l), 1.0);
     }
-    res.toF(l);
-    GIC_HIP(hipMemcpyAsync(out.p(l), res.p(l), out.reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
-    out.fourier[l] = true;
   }
   return out;
 }
@@ -637,18 +795,14 @@ template <typename T> void transfer_to_dm(MLF<T> &m) {
 }
 
 struct DevList {
-  DevBuf idx, val;
+  DevBuf idx;
   size_t n = 0;
 };
-void upload_list(gic_context *c, const std::vector<uint64_t> &idx, const std::vector<double> *val, DevList &out) {
+void upload_list(gic_context *c, const std::vector<uint64_t> &idx, DevList &out) {
   out.n = idx.size();
   if (!out.n) return;
   out.idx.ensure(out.n * 8);
   GIC_HIP(hipMemcpyAsync(out.idx.p, idx.data(), out.n * 8, hipMemcpyHostToDevice, c->st));
-  if (val) {
-    out.val.ensure(out.n * 8);
-    GIC_HIP(hipMemcpyAsync(out.val.p, val->data(), out.n * 8, hipMemcpyHostToDevice, c->st));
-  }
   GIC_HIP(hipStreamSynchronize(c->st));
 }
 
@@ -657,95 +811,6 @@ int mod_kind(const std::string &name) {
   for (int i = 0; i < 8; ++i)
     if (!strcasecmp(name.c_str(), names[i])) return i;
   return -1;
-}
-
-// localisation covector of the finest level (linearmodification.hpp:60-70,
-// 198-226, 330-370) as (cell, value) pairs, summed in the reference's order
-void localisation(gic_context *c, int kind, const std::vector<uint64_t> &cells, std::vector<uint64_t> &idx,
-                  std::vector<double> &val) {
-  const LevelGeom &g = c->levels.back()->g;
-  long n = g.n;
-  int sim = (int)llround(g.periodic / g.dx);
-  std::unordered_map<uint64_t, size_t> pos;
-  auto add = [&](long i, double v) {
-    if (i < 0 || size_t(i) >= g.n3()) return;   // steps leaving a zoom grid
-    auto it = pos.find(uint64_t(i));
-    if (it == pos.end()) {
-      pos.emplace(uint64_t(i), idx.size());
-      idx.push_back(uint64_t(i));
-      val.push_back(v);
-    } else {
-      val[it->second] += v;
-    }
-  };
-  auto step = [&](uint64_t index, int dir, int s) -> long {
-    long q[3] = {long(index / (n * n)), long((index / n) % n), long(index % n)};
-    q[dir] += s;
-    for (int a = 0; a < 3; ++a) {
-      if (q[a] > sim - 1) q[a] -= sim;
-      if (q[a] < 0) q[a] += sim;
-    }
-    return (q[0] * n + q[1]) * n + q[2];
-  };
-  double w = 1.0 / cells.size();
-  auto centroid = [&](uint64_t i, double *p) {
-    p[0] = double(i / (n * n)) * g.dx + g.ox + g.dx / 2;
-    p[1] = double((i / n) % n) * g.dx + g.oy + g.dx / 2;
-    p[2] = double(i % n) * g.dx + g.oz + g.dx / 2;
-  };
-  if (kind <= 1) {
-    for (uint64_t cl : cells) add(long(cl), w);
-  } else if (kind <= 4) {
-    int dir = kind - 2;
-    double a = -w / 12. / g.dx, b = w * 2. / 3. / g.dx;
-    for (uint64_t cl : cells) {
-      long m1 = step(cl, dir, -1), p1 = step(cl, dir, +1);
-      long m2 = step(uint64_t(m1), dir, -1), p2 = step(uint64_t(p1), dir, +1);
-      add(m2, a);
-      add(m1, b);
-      add(p1, -b);
-      add(p2, -a);
-    }
-  } else {
-    int dir = kind - 5;
-    auto woff = [&](double x0, double x1) {
-      double r = x0 - x1;
-      if (r > g.periodic / 2) r -= g.periodic;
-      if (r < -g.periodic / 2) r += g.periodic;
-      return r;
-    };
-    double p0[3], q[3], s[3] = {0, 0, 0}, qc[3];
-    centroid(cells[0], p0);
-    for (size_t i = 1; i < cells.size(); ++i) {
-      centroid(cells[i], q);
-      for (int d = 0; d < 3; ++d) s[d] += woff(q[d], p0[d]);
-    }
-    for (int d = 0; d < 3; ++d) {
-      double v = fmod(s[d] / double(cells.size()) + p0[d], g.periodic);
-      if (v < 0) v += g.periodic;
-      qc[d] = v;
-    }
-    double norm = double(cells.size()) * g.dx;
-    double a = 1. / 12. / norm, b = -2. / 3. / norm;
-    int dp1 = (dir + 1) % 3, dp2 = (dir + 2) % 3;
-    for (uint64_t cl : cells) {
-      centroid(cl, q);
-      double dq[3], cr[3];
-      for (int d = 0; d < 3; ++d) dq[d] = woff(qc[d], q[d]);
-      cr[dir] = 0;
-      cr[dp2] = dq[dp1];
-      cr[dp1] = -dq[dp2];
-      for (int d = 0; d < 3; ++d) {
-        if (d == dir) continue;
-        long m1 = step(cl, d, -1), m2 = step(uint64_t(m1), d, -1);
-        long p1 = step(cl, d, +1), p2 = step(uint64_t(p1), d, +1);
-        add(m2, cr[d] * a);
-        add(m1, cr[d] * b);
-        add(p1, -cr[d] * b);
-        add(p2, -cr[d] * a);
-      }
-    }
-  }
 }
 
 // LinearModification::getCovector with generateMultilevelCovectorFromHiresCovector
@@ -761,26 +826,68 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
   if (kind >= 2 && kind <= 4 && combined(c))
     throw std::runtime_error(
         "Cannot create velocity covector for recombined fields (i.e. after output generation is complete)");
-  std::vector<uint64_t> idx;
-  std::vector<double> val;
-  localisation(c, kind, flags[L], idx, val);
+  const LevelGeom &g = c->levels[L]->g;
+  const std::vector<uint64_t> &cells = flags[L];
+  LocArgs A{};
+  A.kind = kind;
+  A.n = g.n;
+  A.sim = (int)llround(g.periodic / g.dx);
+  A.dx = g.dx;
+  A.o[0] = g.ox;
+  A.o[1] = g.oy;
+  A.o[2] = g.oz;
+  A.periodic = g.periodic;
+  A.w = 1.0 / double(cells.size());
+  if (kind >= 2 && kind <= 4) {
+    A.dir = kind - 2;
+    A.a = -A.w / 12. / g.dx;
+    A.b = A.w * 2. / 3. / g.dx;
+  } else if (kind >= 5) {
+    A.dir = kind - 5;
+    // getCentreWrapped of the flagged cells (grid.hpp:466-494)
+    auto cen = [&](uint64_t i, int d) {
+      uint64_t q = d == 0 ? i / (uint64_t(g.n) * g.n) : d == 1 ? (i / g.n) % g.n : i % g.n;
+      return double(q) * g.dx + (d == 0 ? g.ox : d == 1 ? g.oy : g.oz) + g.dx / 2;
+    };
+    for (int d = 0; d < 3; ++d) {
+      double p0 = cen(cells[0], d), s = 0;
+      for (size_t i = 1; i < cells.size(); ++i) {
+        double r = cen(cells[i], d) - p0;
+        if (r > g.periodic / 2) r -= g.periodic;
+        if (r < -g.periodic / 2) r += g.periodic;
+        s += r;
+      }
+      double v = fmod(s / double(cells.size()) + p0, g.periodic);
+      if (v < 0) v += g.periodic;
+      A.qc[d] = v;
+    }
+    double norm = double(cells.size()) * g.dx;
+    A.a = 1. / 12. / norm;
+    A.b = -2. / 3. / norm;
+  }
   MLF<T> m(c);
   m.zero();
-  DevList dl;
-  upload_list(c, idx, &val, dl);
-  if (dl.n)
-    hipLaunchKernelGGL(k_scatter<T>, dim3(blocks(dl.n)), dim3(256), 0, c->st, m.p(L), m.g(L).n,
-                       (const uint64_t *)dl.idx.p, (const double *)dl.val.p, dl.n);
+  {
+    DevList dl;
+    upload_list(c, cells, dl);
+    PBuf dense(c->mod->pool, g.n3() * sizeof(double));
+    fill<double>((double *)dense.p, g.n3(), 0.0, c->st);
+    hipLaunchKernelGGL(k_localise, dim3(blocks(dl.n)), dim3(256), 0, c->st, (const uint64_t *)dl.idx.p, dl.n, A,
+                       (double *)dense.p);
+    hipLaunchKernelGGL(k_dense_to_padded<T>, dim3(blocks(g.n3())), dim3(256), 0, c->st, m.p(L),
+                       (const double *)dense.p, g.n);
+    GIC_HIP(hipStreamSynchronize(c->st));   // the cell list is released here
+  }
   // deinterpolate down the levels, one level at a time, in the contiguous layout
   if (L > 0) {
-    DevBuf accs[2];
+    PBuf accs[2];
     const T *src = m.p(L);
     int src_padded = 1;
     for (long l = long(L) - 1; l >= 0; --l) {
       const LevelGeom &gh = m.g(size_t(l) + 1), &gl = m.g(size_t(l));
       size_t len = gl.n3() + 2 * size_t(gl.n) * gl.n;
-      DevBuf &acc = accs[l & 1];
-      acc.ensure(len * sizeof(T));
+      PBuf &acc = accs[l & 1];
+      acc = PBuf(c->mod->pool, len * sizeof(T));
       fill<T>((T *)acc.p, len, T(0), c->st);
       DeinterpGeom dg;
       dg.nf = gh.n;
@@ -816,22 +923,15 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
   for (size_t l = 0; l < nl; ++l) apply_spec(m, l, spec_levels(c, long(l), -1, 1.0));
   transfer_to_dm(m);
   if (kind >= 1) {
-    double coef;
-    int op;
-    if (kind == 1) {
-      const Cosmo &cc = c->cosmo;
-      coef = 3. / 2. * cc.Om / cc.a * 100. * 100. / (299792.) / (299792.);
-      op = 1;
-    } else {
-      const Cosmo &cc = c->cosmo;
-      coef = 1.0 * 100. * sqrt(cc.Om / (cc.a * cc.a * cc.a) + cc.Ol) * sqrt(cc.a);
-      op = 2;
-    }
+    const Cosmo &cc = c->cosmo;
+    double coef = kind == 1 ? 3. / 2. * cc.Om / cc.a * 100. * 100. / (299792.) / (299792.)
+                            : 1.0 * 100. * sqrt(cc.Om / (cc.a * cc.a * cc.a) + cc.Ol) * sqrt(cc.a);
+    int op = kind == 1 ? 1 : 2;
     for (size_t l = 0; l < nl; ++l) {
-      const LevelGeom &g = m.g(l);
-      size_t nm = size_t(g.n) * g.n * (g.n / 2 + 1);
-      hipLaunchKernelGGL(k_mode_op<T>, dim3(blocks(nm)), dim3(256), 0, c->st, reinterpret_cast<cx<T> *>(m.p(l)), g.n,
-                         g.kmin(), op, coef);
+      const LevelGeom &gg = m.g(l);
+      size_t nm = size_t(gg.n) * gg.n * (gg.n / 2 + 1);
+      hipLaunchKernelGGL(k_mode_op<T>, dim3(blocks(nm)), dim3(256), 0, c->st, reinterpret_cast<cx<T> *>(m.p(l)),
+                         gg.n, gg.kmin(), op, coef);
     }
   }
   GIC_HIP(hipGetLastError());
@@ -842,7 +942,7 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
 template <typename T> MLF<T> push_variance(MLF<T> &field, const std::vector<std::vector<uint64_t>> &flags,
                                            double scale) {
   gic_context *c = field.c;
-  MLF<T> f(field);
+  MLF<T> f = field.clone();
   transfer_to_dm(f);
   FilterSpec hp;
   hp.na = 1;
@@ -851,12 +951,12 @@ template <typename T> MLF<T> push_variance(MLF<T> &field, const std::vector<std:
   for (size_t l = 0; l < f.nl(); ++l) {
     const LevelGeom &g = f.g(l);
     size_t n3 = g.n3();
-    const std::vector<uint64_t> &cells = l < flags.size() ? flags[l] : std::vector<uint64_t>();
-    DevBuf mask;
-    mask.ensure(n3);
+    static const std::vector<uint64_t> none;
+    const std::vector<uint64_t> &cells = l < flags.size() ? flags[l] : none;
+    PBuf mask(c->mod->pool, n3);
     fill<uint8_t>((uint8_t *)mask.p, n3, 0, c->st);
     DevList dl;
-    upload_list(c, cells, nullptr, dl);
+    upload_list(c, cells, dl);
     if (dl.n)
       hipLaunchKernelGGL(k_mask_set, dim3(blocks(dl.n)), dim3(256), 0, c->st, (uint8_t *)mask.p,
                          (const uint64_t *)dl.idx.p, dl.n);
@@ -868,7 +968,7 @@ template <typename T> MLF<T> push_variance(MLF<T> &field, const std::vector<std:
     apply_spec(f, l, hp);
     window();
     if (dl.n) {
-      c->mod->part.ensure(RED_BLOCKS * sizeof(double) * 8);
+      c->mod->part.ensure(RED_BLOCKS * sizeof(double));
       c->mod->scalar.ensure(sizeof(double) * 8);
       hipLaunchKernelGGL(k_flag_sum<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, (const T *)f.p(l), g.n,
                          (const uint64_t *)dl.idx.p, dl.n, (double *)c->mod->part.p);
@@ -879,7 +979,7 @@ template <typename T> MLF<T> push_variance(MLF<T> &field, const std::vector<std:
     }
     apply_spec(f, l, hp);
     window();
-    GIC_HIP(hipStreamSynchronize(c->st));   // the mask and lists are released here
+    GIC_HIP(hipStreamSynchronize(c->st));   // the cell list is released here
   }
   MLF<T> cov = metric(f, true);
   transfer_to_dm(cov);
@@ -984,23 +1084,61 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
     MLF<T> bv = metric(b, false);
     return inner(c, a, bv);
   };
-  for (size_t i = 0; i < alphas.size(); ++i) {
-    for (size_t j = 0; j < i; ++j) {
-      double r = cov_dot(alphas[i], alphas[j]);
-      alphas[i].axpy(alphas[j], -r);
-      targets[i] -= r * targets[j];
-    }
-    double norm = sqrt(cov_dot(alphas[i], alphas[i]));
-    alphas[i].scale(1.0 / norm);
-    targets[i] /= norm;
-  }
-  std::vector<double> existing;
-  for (auto &a : alphas) existing.push_back(inner(c, a, field));
+  // orthonormaliseModifications + applyLinearModif (modificationmanager.hpp:
+  // 160-172,266-283) as one FP64 Gram solve.  The covector-covector product
+  // goes through the inverse metric, which is linear: with v_i = M^-1 a_i,
+  // G_ij = <a_i, v_j>, the Gram-Schmidt of the reference is the Cholesky
+  // factor L of G, and the correction it adds is sum_i v_i [G^-1 (d - <a, w>)]_i.
+  size_t nm = alphas.size();
   std::vector<MLF<T>> vecs;
-  for (size_t i = 0; i < alphas.size(); ++i) {
-    double dval = targets[i] - existing[i];
-    vecs.push_back(metric(alphas[i], false));
-    field.axpy(vecs.back(), dval);
+  for (auto &a : alphas) vecs.push_back(metric(a, false));
+  std::vector<double> G(nm * nm), L(nm * nm, 0.0), rhs(nm);
+  for (size_t i = 0; i < nm; ++i) {
+    for (size_t j = 0; j <= i; ++j) G[i * nm + j] = G[j * nm + i] = inner(c, alphas[i], vecs[j]);
+    rhs[i] = targets[i] - inner(c, alphas[i], field);
+  }
+  for (size_t i = 0; i < nm; ++i) {   // G = L L^T
+    for (size_t j = 0; j <= i; ++j) {
+      double v = G[i * nm + j];
+      for (size_t k = 0; k < j; ++k) v -= L[i * nm + k] * L[j * nm + k];
+      if (i == j) {
+        if (!(v > 0)) throw std::runtime_error("Modifications are not linearly independent");
+        L[i * nm + i] = sqrt(v);
+      } else {
+        L[i * nm + j] = v / L[j * nm + j];
+      }
+    }
+  }
+  std::vector<double> y(nm), coef(nm);
+  for (size_t i = 0; i < nm; ++i) {
+    double v = rhs[i];
+    for (size_t k = 0; k < i; ++k) v -= L[i * nm + k] * y[k];
+    y[i] = v / L[i * nm + i];
+  }
+  for (size_t ii = nm; ii-- > 0;) {
+    double v = y[ii];
+    for (size_t k = ii + 1; k < nm; ++k) v -= L[k * nm + ii] * coef[k];
+    coef[ii] = v / L[ii * nm + ii];
+  }
+  for (size_t i = 0; i < nm; ++i) field.axpy(vecs[i], coef[i]);
+  alphas.clear();
+  if (!c->mod->quadratic.empty() && nm) {
+    // the orthonormal vectors of the linear family: rows of L^-1 applied to v
+    std::vector<double> Li(nm * nm, 0.0);
+    for (size_t col = 0; col < nm; ++col)
+      for (size_t i = 0; i < nm; ++i) {
+        double v = i == col ? 1.0 : 0.0;
+        for (size_t k = 0; k < i; ++k) v -= L[i * nm + k] * Li[k * nm + col];
+        Li[i * nm + col] = v / L[i * nm + i];
+      }
+    std::vector<MLF<T>> ortho;
+    for (size_t i = 0; i < nm; ++i) {
+      MLF<T> o = vecs[i].clone();
+      o.scale(Li[i * nm + i]);
+      for (size_t j = 0; j < i; ++j) o.axpy(vecs[j], Li[i * nm + j]);
+      ortho.push_back(std::move(o));
+    }
+    vecs = std::move(ortho);
   }
   for (auto &m : c->mod->quadratic) {
     auto iterate = [&](MLF<T> &f, int nsteps) {
@@ -1023,14 +1161,12 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
       }
     };
     int init = m.init_steps;
-    {
-      MLF<T> test(field);
-      iterate(test, init);
-      double achieved = std::fabs(variance_value(test, m.flags, m.scale) - m.target);
-      double want = m.target * m.precision;
-      int nsteps = init * (int)std::ceil(std::sqrt(achieved / want));
-      iterate(field, nsteps > init ? nsteps : init);
-    }
+    MLF<T> test = field.clone();
+    iterate(test, init);
+    double achieved = std::fabs(variance_value(test, m.flags, m.scale) - m.target);
+    double want = m.target * m.precision;
+    int nsteps = init * (int)std::ceil(std::sqrt(achieved / want));
+    iterate(field, nsteps > init ? nsteps : init);
   }
   store_output<T>(c, field);
   double chi_post = mod_chi2<T>(c);
