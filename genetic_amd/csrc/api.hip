@@ -621,6 +621,8 @@ template void do_draw<float>(gic_context *);
 template void do_draw<double>(gic_context *);
 template void ensure_noise_fourier<float>(gic_context *, Level &);
 template void ensure_noise_fourier<double>(gic_context *, Level &);
+template void do_zeldovich<float>(gic_context *);
+template void do_zeldovich<double>(gic_context *);
 
 }  // namespace gic
 
@@ -918,6 +920,34 @@ int gic_chi2(gic_context *c, double *chi2, uint64_t *dof) {
     else *chi2 = mod_chi2<double>(c);
     if (dof) *dof = num_dof(c);
   });
+}
+
+int gic_displacement_forward(gic_context *c, const void *const *noise, void *const *psi) {
+  return guarded(c, [&] {
+    if (!noise || !psi) throw std::runtime_error("displacement forward: null pointer array");
+    if (c->prec == 32) do_displacement_forward<float>(c, (const float *const *)noise, (float *const *)psi);
+    else do_displacement_forward<double>(c, (const double *const *)noise, (double *const *)psi);
+  });
+}
+
+int gic_displacement_adjoint(gic_context *c, const void *const *psi_bar, void *const *noise_bar) {
+  return guarded(c, [&] {
+    if (!psi_bar || !noise_bar) throw std::runtime_error("displacement adjoint: null pointer array");
+    if (c->prec == 32) do_displacement_adjoint<float>(c, (const float *const *)psi_bar, (float *const *)noise_bar);
+    else do_displacement_adjoint<double>(c, (const double *const *)psi_bar, (double *const *)noise_bar);
+  });
+}
+
+int gic_variance_gradient(gic_context *c, const void *const *noise, double *value, void *const *grad) {
+  return guarded(c, [&] {
+    if (!noise || !value) throw std::runtime_error("variance gradient: null pointer");
+    if (c->prec == 32) *value = mod_variance_gradient<float>(c, (const float *const *)noise, (float *const *)grad);
+    else *value = mod_variance_gradient<double>(c, (const double *const *)noise, (double *const *)grad);
+  });
+}
+
+int gic_set_quadratic_line_search(gic_context *c, int enable) {
+  return guarded(c, [&] { mod_set_line_search(c, enable != 0); });
 }
 
 int gic_velocity_ratio(gic_context *c, double *out) {

@@ -192,6 +192,9 @@ struct Level {
   bool psi_fourier = false;
   bool have_psi = false;
   bool have_psi2 = false;
+  DevBuf adj_acc;          // adjoint: the noise gradient's spectrum
+  DevBuf adj_u;            // adjoint: gradient arriving from the finer level
+  DevBuf adj_tab, adj_lp;  // adjoint: |k|^2 tables of the level's operators
 };
 
 }  // namespace gic
@@ -230,7 +233,7 @@ struct gic_context {
   int seed_mode = 0;
   bool seeded = false, drawn = false, power_applied = false, zeldovich_done = false;
   int lpt_order = 1;   // 1: Zel'dovich, 2: second-order LPT on top
-  DevBuf work, stage, tab_a, tab_b, tab_c, lpt_win;
+  DevBuf work, stage, tab_a, tab_b, tab_c, lpt_win, adj_scratch;
   std::unique_ptr<EventTimer> timer;
   TimingHook *hook() { return timer.get(); }
   size_t tsize() const { return prec == 32 ? 4 : 8; }
@@ -260,6 +263,12 @@ template <typename T> void build_sqrtp(gic_context *c, Level &L);
 template <typename T> void build_sqrtp_dm(gic_context *c, Level &L);
 template <typename T> void do_draw(gic_context *c);
 template <typename T> void ensure_noise_fourier(gic_context *c, Level &L);
+template <typename T> void do_zeldovich(gic_context *c);
+
+// ---- displacement map and its transpose (adjoint.hip) ----------------------------------
+// noise[l]: dense n_l^3 white noise; psi[3l+a]: dense displacement components
+template <typename T> void do_displacement_forward(gic_context *c, const T *const *noise, T *const *psi);
+template <typename T> void do_displacement_adjoint(gic_context *c, const T *const *psi_bar, T *const *noise_bar);
 
 // ---- constrained realisations (modif.hip) ------------------------------------------
 template <typename T> double mod_calculate(gic_context *c, const std::string &name);
@@ -269,5 +278,10 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post);
 void mod_clear(gic_context *c);
 template <typename T> double mod_chi2(gic_context *c);
 size_t num_dof(gic_context *c);
+// filtered variance of an arbitrary white-noise field and its gradient 2 Q w
+template <typename T> double mod_variance_gradient(gic_context *c, const T *const *noise, T *const *grad);
+// quadratic modification steps: the reference's linearised steps (off) or an
+// exact line search along the same direction (on)
+void mod_set_line_search(gic_context *c, bool on);
 
 }  // namespace gic

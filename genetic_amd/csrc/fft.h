@@ -22,15 +22,29 @@ template <typename T> void fft3d(T *d, int n, int dir, hipStream_t st, TimingHoo
 enum { FFT_PRE_NONE = 0, FFT_PRE_TABLE = 1, FFT_PRE_ZA = 2, FFT_PRE_KK = 3 };
 
 // r2c whose last pass multiplies each mode by post[kx^2+ky^2+kz^2] (if non-null).
-template <typename T> void fft_r2c_post(T *d, int n, const T *post, hipStream_t st, TimingHook *hook);
+// With src non-null the real input is read from the dense n^3 array src (z
+// fastest, no padding) and the spectrum lands in d.
+template <typename T>
+void fft_r2c_post(T *d, int n, const T *post, hipStream_t st, TimingHook *hook, const T *src = nullptr);
+// In-place c2r of d; with dst non-null the real result goes to the dense n^3
+// array dst instead (d is then scratch).
+template <typename T> void fft_c2r_to(T *d, T *dst, int n, hipStream_t st, TimingHook *hook);
+// Transpose of one Zel'dovich component (FFT_PRE_ZA followed by c2r): the r2c
+// of src (dense n^3, or d in place when null; d is overwritten) whose last
+// pass forms conj(i k_c tab[|k|^2]) times each mode, zero at k_c = +Nyquist,
+// and adds it to the spectrum acc (stores it when !accumulate).
+template <typename T>
+void fft_r2c_za_adjoint(const T *src, T *d, T *acc, int n, const T *tab, int comp, double kmin, bool accumulate,
+                        hipStream_t st, TimingHook *hook);
 // out <- c2r(op(spec)), out-of-place from a spectrum that is left untouched.
 // op: FFT_PRE_TABLE multiplies by tab[|k|^2]; FFT_PRE_ZA forms the Zel'dovich
 // component comp, i k_c tab[|k|^2], with tab = filter(k)/k^2 (0 at k=0) and
 // zero at k_c = +Nyquist.  FFT_PRE_KK forms k_a k_b tab[|k|^2] for comp = a + 3 b
 // (zero at a +Nyquist component when a != b).
+// With dst non-null the real result goes to the dense n^3 array dst and out is scratch.
 template <typename T>
 void fft_c2r_from(const T *spec, T *out, int n, int pre, const T *tab, int comp, double kmin, hipStream_t st,
-                  TimingHook *hook);
+                  TimingHook *hook, T *dst = nullptr);
 // Only the real-space window [w0, w0+wl) (per axis, wrapping mod n) of
 // c2r(op(spec)), via pruned passes: bufA holds wl[0]*n*(n/2+1) complex,
 // bufB wl[0]*wl[1]*(n/2+1) complex, out wl[0]*wl[1]*wl[2] reals.

@@ -198,8 +198,10 @@ __device__ __forceinline__ void run_stages(cx<T> *lds, const cx<T> *tw, int twmu
 //        PRE_KK:    v <- k_a k_b tab[|k|^2] v (second derivatives; a != b zero at a Nyquist component)
 //   POST (forward x pass, output indexed by kx=pt, ky=outer, kz=col):
 //        POST_TABLE: v *= tab[kx^2+ky^2+kz^2]
+//        POST_ZA_ADJ: out (+)= -i k_c tab[|k|^2] v, zero at k_c = +Nyquist: the
+//                     transpose of PRE_ZA, accumulated over components in out
 enum { PRE_NONE = FFT_PRE_NONE, PRE_TABLE = FFT_PRE_TABLE, PRE_ZA = FFT_PRE_ZA, PRE_KK = FFT_PRE_KK };
-enum { POST_NONE = 0, POST_TABLE = 1 };
+enum { POST_NONE = 0, POST_TABLE = 1, POST_ZA_ADJ = 2 };
 
 template <typename T> struct StridedArgs {
   const cx<T> *in;
@@ -212,6 +214,7 @@ template <typename T> struct StridedArgs {
   T kmin;
   int win0, wlen;   // output keeps line points (pt - win0) mod n < wlen
   T scale;
+  int accum;        // POST_ZA_ADJ: add to the output instead of storing
 };
 
 __device__ __forceinline__ int signed_k(int i, int n) { return i > n / 2 ? i - n : i; }
@@ -268,6 +271,22 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
       int kx = signed_k(pt, a.n), kz = col0 + line;
       s *= a.tab[kx * kx + ky2 + kz * kz];
     }
+    if constexpr (POST == POST_ZA_ADJ) {
+      int kx = signed_k(pt, a.n), kz = col0 + line;
+      int kc = a.comp == 0 ? kx : a.comp == 1 ? ky : kz;
+      cx<T> r{0, 0};
+      if (kc != a.n / 2) {
+        T f = s * a.tab[kx * kx + ky2 + kz * kz] * (T(kc) * a.kmin);
+        r = cx<T>{v.y * f, -v.x * f};
+      }
+      cx<T> &dst = ob[o * a.out_ls + line];
+      if (a.accum) {
+        cx<T> prev = dst;
+        r = cx<T>{prev.x + r.x, prev.y + r.y};
+      }
+      dst = r;
+      return;
+    }
     ob[o * a.out_ls + line] = cx<T>{v.x * s, v.y * s};
   };
   auto lidx = [](int line, int pt) { return pt * C + line; };
@@ -283,13 +302,16 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
 // M=N/2 on z_j = x_{2j} + i x_{2j+1}, then X_k = F_k + W^k G_k.
 template <typename T, int N, int ROWS, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_fft_z_r2c(T *__restrict__ data, long nrows, T scale,
-                                                       const cx<T> *__restrict__ twN) {
+                                                       const cx<T> *__restrict__ twN, const T *__restrict__ src) {
   constexpr int M = N / 2;
   extern __shared__ char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   long row0 = long(blockIdx.x) * ROWS;
   cx<T> *rows = reinterpret_cast<cx<T> *>(data) + row0 * (M + 1);
-  auto gld = [&](int line, int pt) -> cx<T> { return rows[long(line) * (M + 1) + pt]; };
+  // rows come from the padded field itself, or from a dense array (row stride N reals)
+  const cx<T> *srows = src ? reinterpret_cast<const cx<T> *>(src) + row0 * M : rows;
+  const long srs = src ? M : M + 1;
+  auto gld = [&](int line, int pt) -> cx<T> { return srows[long(line) * srs + pt]; };
   auto lidx = [](int line, int pt) { return line * (M + 1) + pt; };
   auto lst = [&](int line, int pt, cx<T> v) { lds[lidx(line, pt)] = v; };
   if constexpr (M == 1) {
@@ -320,7 +342,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_r2c(T *__restrict__ data, lon
 // ---- z pass, c2r --------------------------------------------------------------
 template <typename T, int N, int ROWS, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_fft_z_c2r(T *__restrict__ data, long nrows, T scale,
-                                                       const cx<T> *__restrict__ twN) {
+                                                       const cx<T> *__restrict__ twN, T *__restrict__ dst) {
   constexpr int M = N / 2;
   extern __shared__ char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
@@ -353,7 +375,10 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_c2r(T *__restrict__ data, lon
   }
   __syncthreads();
   auto lld = [&](int line, int pt) -> cx<T> { return lds[lidx(line, pt)]; };
-  auto gst = [&](int line, int pt, cx<T> v) { rows[long(line) * (M + 1) + pt] = cx<T>{v.x * scale, v.y * scale}; };
+  // results go back into the padded rows, or to a dense array (row stride N reals)
+  cx<T> *drows = dst ? reinterpret_cast<cx<T> *>(dst) + row0 * M : rows;
+  const long drs = dst ? M : M + 1;
+  auto gst = [&](int line, int pt, cx<T> v) { drows[long(line) * drs + pt] = cx<T>{v.x * scale, v.y * scale}; };
   if constexpr (M == 1) {
     if (threadIdx.x < ROWS) gst(threadIdx.x, 0, lld(threadIdx.x, 0));
   } else {
@@ -514,16 +539,25 @@ template <typename T, int N> struct FftN {
     a.scale = scale;
     return a;
   }
-  static void zpass(T *d, T scale, int dir, hipStream_t st, TimingHook *hook, const char *name) {
+  // z rows of the padded field d; ext (if non-null) is the dense N^3 array the
+  // r2c reads its rows from, or the c2r writes its rows to
+  static void zpass(T *d, T scale, int dir, hipStream_t st, TimingHook *hook, const char *name, T *ext = nullptr) {
     long nrows = long(N) * N;
     int nblk = int(nrows / ROWS);
     size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
     const cx<T> *tw = twtable<T>().get(N);
-    auto kern = dir < 0 ? k_fft_z_r2c<T, N, ROWS, THREADS> : k_fft_z_c2r<T, N, ROWS, THREADS>;
-    if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     std::string tag = std::string(name) + "." + std::to_string(N);
-    if (hook) hook->begin(tag.c_str(), st);
-    hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, d, nrows, scale, tw);
+    if (dir < 0) {
+      auto kern = k_fft_z_r2c<T, N, ROWS, THREADS>;
+      if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+      if (hook) hook->begin(tag.c_str(), st);
+      hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, d, nrows, scale, tw, (const T *)ext);
+    } else {
+      auto kern = k_fft_z_c2r<T, N, ROWS, THREADS>;
+      if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+      if (hook) hook->begin(tag.c_str(), st);
+      hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, d, nrows, scale, tw, ext);
+    }
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);
   }
@@ -537,9 +571,9 @@ template <typename T, int N> struct FftN {
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);
   }
-  static void r2c(T *d, const T *post, hipStream_t st, TimingHook *hook) {
+  static void r2c(T *d, const T *post, hipStream_t st, TimingHook *hook, const T *src = nullptr) {
     cx<T> *c = reinterpret_cast<cx<T> *>(d);
-    zpass(d, T(1), -1, st, hook, "fft_z_r2c");
+    zpass(d, T(1), -1, st, hook, "fft_z_r2c", const_cast<T *>(src));
     strided<-1, PRE_NONE, POST_NONE, 1>(plain(c, false, T(1)), N, st, hook, "fft_y_fwd");
     StridedArgs<T> ax = plain(c, true, norm());
     if (post) {
@@ -549,16 +583,31 @@ template <typename T, int N> struct FftN {
       strided<-1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x_fwd");
     }
   }
-  static void c2r(T *d, hipStream_t st, TimingHook *hook) {
+  static void c2r(T *d, hipStream_t st, TimingHook *hook, T *dst = nullptr) {
     cx<T> *c = reinterpret_cast<cx<T> *>(d);
     mirror(c, st, hook);
     strided<+1, PRE_NONE, POST_NONE>(plain(c, true, T(1)), N, st, hook, "fft_x");
     strided<+1, PRE_NONE, POST_NONE, 1>(plain(c, false, T(1)), N, st, hook, "fft_y");
-    zpass(d, norm(), +1, st, hook, "fft_z_c2r");
+    zpass(d, norm(), +1, st, hook, "fft_z_c2r", dst);
+  }
+  // adjoint of one Zel'dovich component: r2c whose x pass adds
+  // conj(i k_c tab[|k|^2]) times each mode into acc (or stores it when !accum)
+  static void r2c_za_adjoint(const T *src, T *d, T *acc, const T *tab, int comp, T kmin, bool accum, hipStream_t st,
+                             TimingHook *hook) {
+    cx<T> *c = reinterpret_cast<cx<T> *>(d);
+    zpass(d, T(1), -1, st, hook, "fft_z_r2c", const_cast<T *>(src));
+    strided<-1, PRE_NONE, POST_NONE, 1>(plain(c, false, T(1)), N, st, hook, "fft_y_fwd");
+    StridedArgs<T> ax = plain(c, true, norm());
+    ax.out = reinterpret_cast<cx<T> *>(acc);
+    ax.tab = tab;
+    ax.comp = comp;
+    ax.kmin = kmin;
+    ax.accum = accum ? 1 : 0;
+    strided<-1, PRE_NONE, POST_ZA_ADJ>(ax, N, st, hook, "fft_x_za_adj");
   }
   // out <- c2r(op(spec)) with op applied while the x pass reads the spectrum
   static void c2r_from(const T *spec, T *out, int pre, const T *tab, int comp, T kmin, hipStream_t st,
-                       TimingHook *hook) {
+                       TimingHook *hook, T *dst = nullptr) {
     const cx<T> *ci = reinterpret_cast<const cx<T> *>(spec);
     cx<T> *co = reinterpret_cast<cx<T> *>(out);
     StridedArgs<T> ax = plain(co, true, T(1));
@@ -571,7 +620,7 @@ template <typename T, int N> struct FftN {
     else if (pre == PRE_TABLE) strided<+1, PRE_TABLE, POST_NONE>(ax, N, st, hook, "fft_x_pre");
     else strided<+1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x");
     strided<+1, PRE_NONE, POST_NONE, 1>(plain(co, false, T(1)), N, st, hook, "fft_y");
-    zpass(out, norm(), +1, st, hook, "fft_z_c2r");
+    zpass(out, norm(), +1, st, hook, "fft_z_c2r", dst);
   }
   // real-space window [w0, w0+wl) (per axis, mod N) of c2r(op(spec)):
   // bufA holds wl[0] x-planes after the x pass, bufB wl[0]*wl[1] rows after
@@ -648,13 +697,21 @@ template <typename T> void fft3d(T *d, int n, int dir, hipStream_t st, TimingHoo
     GIC_FFT_SWITCH(n, F::c2r(d, st, hook))
   }
 }
-template <typename T> void fft_r2c_post(T *d, int n, const T *post, hipStream_t st, TimingHook *hook) {
-  GIC_FFT_SWITCH(n, F::r2c(d, post, st, hook))
+template <typename T> void fft_r2c_post(T *d, int n, const T *post, hipStream_t st, TimingHook *hook, const T *src) {
+  GIC_FFT_SWITCH(n, F::r2c(d, post, st, hook, src))
+}
+template <typename T> void fft_c2r_to(T *d, T *dst, int n, hipStream_t st, TimingHook *hook) {
+  GIC_FFT_SWITCH(n, F::c2r(d, st, hook, dst))
+}
+template <typename T>
+void fft_r2c_za_adjoint(const T *src, T *d, T *acc, int n, const T *tab, int comp, double kmin, bool accumulate,
+                        hipStream_t st, TimingHook *hook) {
+  GIC_FFT_SWITCH(n, F::r2c_za_adjoint(src, d, acc, tab, comp, T(kmin), accumulate, st, hook))
 }
 template <typename T>
 void fft_c2r_from(const T *spec, T *out, int n, int pre, const T *tab, int comp, double kmin, hipStream_t st,
-                  TimingHook *hook) {
-  GIC_FFT_SWITCH(n, F::c2r_from(spec, out, pre, tab, comp, T(kmin), st, hook))
+                  TimingHook *hook, T *dst) {
+  GIC_FFT_SWITCH(n, F::c2r_from(spec, out, pre, tab, comp, T(kmin), st, hook, dst))
 }
 template <typename T>
 void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, double kmin, const int *w0, const int *wl,
@@ -665,8 +722,11 @@ void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, doubl
 }
 
 #define GIC_FFT_INST(T)                                                                                     \
-  template void fft_r2c_post<T>(T *, int, const T *, hipStream_t, TimingHook *);                            \
-  template void fft_c2r_from<T>(const T *, T *, int, int, const T *, int, double, hipStream_t, TimingHook *); \
+  template void fft_r2c_post<T>(T *, int, const T *, hipStream_t, TimingHook *, const T *);                 \
+  template void fft_c2r_to<T>(T *, T *, int, hipStream_t, TimingHook *);                                    \
+  template void fft_r2c_za_adjoint<T>(const T *, T *, T *, int, const T *, int, double, bool, hipStream_t,   \
+                                      TimingHook *);                                                          \
+  template void fft_c2r_from<T>(const T *, T *, int, int, const T *, int, double, hipStream_t, TimingHook *, T *); \
   template void fft_c2r_window<T>(const T *, int, int, const T *, int, double, const int *, const int *, T *, T *, \
                                   T *, hipStream_t, TimingHook *);
 GIC_FFT_INST(float)

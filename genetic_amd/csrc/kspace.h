@@ -55,6 +55,10 @@ template <typename T> void mode_power(const T *spec, int n, double *sum, double 
 template <typename T>
 void zeldovich(const T *delta, int n, double kmin, T *px, T *py, T *pz, const T *post, hipStream_t st);
 template <typename T> void interp_add(T *fine, int nf, const CoarseView &coarse, const InterpGeom &g, hipStream_t st);
+// transpose of interp_add with the coarse field whole and padded: coarse += I^T fine
+// (fine row stride frs: nf dense, nf+2 padded)
+template <typename T>
+void interp_adjoint(const T *fine, int nf, long frs, T *coarse, const InterpGeom &g, hipStream_t st);
 // psi2: the three second-order displacement fields, or nullptr for Zel'dovich
 template <typename T>
 void particles(const T *px, const T *py, const T *pz, const T *const *psi2, int n, const ParticleGeom &g,

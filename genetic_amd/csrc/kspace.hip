@@ -302,20 +302,18 @@ __device__ __forceinline__ void cr_weights(double t, double *w) {
   w[3] = (t - 1) * t * t / 2;
 }
 
-template <typename T>
-__global__ void k_interp_add(T *__restrict__ fine, int nf, CoarseView cv, InterpGeom g) {
-  const T *__restrict__ coarse = reinterpret_cast<const T *>(cv.p);
-  size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  size_t nf3 = size_t(nf) * nf * nf;
-  if (idx >= nf3) return;
-  int x = int(idx / (size_t(nf) * nf)), y = int((idx / nf) % nf), z = int(idx % nf);
+// The tricubic stencil of fine cell (x,y,z): the 4 coarse indices per axis
+// (wrapped or clamped, then relative to org mod nc) and their Catmull-Rom
+// weights.  False when the cell lies outside the coarse grid.
+__device__ __forceinline__ bool interp_stencil(int x, int y, int z, const InterpGeom &g, const int *org, int ci[3][4],
+                                               double w[3][4]) {
   int c[3] = {x + g.off[0], y + g.off[1], z + g.off[2]};
   int p0[3];
   double t[3];
   for (int a = 0; a < 3; ++a) {
     if (c[a] > g.sim - 1) c[a] -= g.sim;
     if (c[a] < 0) c[a] += g.sim;
-    if (c[a] < 0 || c[a] >= g.nsup) return;   // outside the coarse grid: nothing added
+    if (c[a] < 0 || c[a] >= g.nsup) return false;   // outside the coarse grid: nothing added
     double cen = c[a] * g.dxs + g.co[a] + g.dxs / 2;   // supersampled centroid
     double loc = cen - g.co[a];
     loc = fmod(loc, g.periodic);
@@ -324,10 +322,8 @@ __global__ void k_interp_add(T *__restrict__ fine, int nf, CoarseView cv, Interp
     p0[a] = int(floor(u));
     t[a] = u - p0[a];
   }
-  double w[3][4];
   for (int a = 0; a < 3; ++a) cr_weights(t[a], w[a]);
   int nc = g.nc;
-  int ci[3][4];
   for (int a = 0; a < 3; ++a)
     for (int i = 0; i < 4; ++i) {
       int v = p0[a] + i - 1;
@@ -338,10 +334,23 @@ __global__ void k_interp_add(T *__restrict__ fine, int nf, CoarseView cv, Interp
         if (v > nc - 1) v = nc - 1;
         if (v < 0) v = 0;
       }
-      int l = v - cv.org[a];
+      int l = v - org[a];
       if (l < 0) l += nc;
       ci[a][i] = l;
     }
+  return true;
+}
+
+template <typename T>
+__global__ void k_interp_add(T *__restrict__ fine, int nf, CoarseView cv, InterpGeom g) {
+  const T *__restrict__ coarse = reinterpret_cast<const T *>(cv.p);
+  size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t nf3 = size_t(nf) * nf * nf;
+  if (idx >= nf3) return;
+  int x = int(idx / (size_t(nf) * nf)), y = int((idx / nf) % nf), z = int(idx % nf);
+  int ci[3][4];
+  double w[3][4];
+  if (!interp_stencil(x, y, z, g, cv.org, ci, w)) return;
   T acc = 0;
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j) {
@@ -355,6 +364,36 @@ __global__ void k_interp_add(T *__restrict__ fine, int nf, CoarseView cv, Interp
 template <typename T> void interp_add(T *fine, int nf, const CoarseView &coarse, const InterpGeom &g, hipStream_t st) {
   size_t nf3 = size_t(nf) * nf * nf;
   hipLaunchKernelGGL(k_interp_add<T>, dim3((nf3 + 255) / 256), dim3(256), 0, st, fine, nf, coarse, g);
+  GIC_HIP(hipGetLastError());
+}
+
+// Transpose of k_interp_add onto a whole coarse grid: every fine value is
+// scattered, with the same 64 weights, into the padded coarse field.  The
+// fine field has row stride frs (nf for a dense array, nf+2 padded).
+template <typename T>
+__global__ void k_interp_adjoint(const T *__restrict__ fine, int nf, long frs, T *__restrict__ coarse, InterpGeom g) {
+  size_t idx = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t nf3 = size_t(nf) * nf * nf;
+  if (idx >= nf3) return;
+  int x = int(idx / (size_t(nf) * nf)), y = int((idx / nf) % nf), z = int(idx % nf);
+  int ci[3][4];
+  double w[3][4];
+  const int org[3] = {0, 0, 0};
+  if (!interp_stencil(x, y, z, g, org, ci, w)) return;
+  T v = fine[(size_t(x) * nf + y) * frs + z];
+  long sy = g.nc + 2, sx = long(g.nc) * sy;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      T wij = T(w[0][i] * w[1][j]) * v;
+      T *row = coarse + ci[0][i] * sx + ci[1][j] * sy;
+      for (int k = 0; k < 4; ++k) atomicAdd(row + ci[2][k], wij * T(w[2][k]));
+    }
+}
+
+template <typename T>
+void interp_adjoint(const T *fine, int nf, long frs, T *coarse, const InterpGeom &g, hipStream_t st) {
+  size_t nf3 = size_t(nf) * nf * nf;
+  hipLaunchKernelGGL(k_interp_adjoint<T>, dim3((nf3 + 255) / 256), dim3(256), 0, st, fine, nf, frs, coarse, g);
   GIC_HIP(hipGetLastError());
 }
 
@@ -514,6 +553,7 @@ template <typename S, typename D> void convert(const S *a, size_t n, D *b, hipSt
   template void fourier_draw_scatter<T>(const T *, int, T *, uint32_t *, int, hipStream_t);                   \
   template void zeldovich<T>(const T *, int, double, T *, T *, T *, const T *, hipStream_t);                  \
   template void interp_add<T>(T *, int, const CoarseView &, const InterpGeom &, hipStream_t);                 \
+  template void interp_adjoint<T>(const T *, int, long, T *, const InterpGeom &, hipStream_t);               \
   template void particles<T>(const T *, const T *, const T *, const T *const *, int, const ParticleGeom &,    \
                              const uint64_t *, size_t, size_t, float *, float *, hipStream_t);                \
   template void lpt_accum<T>(T *, T *, const T *, size_t, int, hipStream_t);                                  \

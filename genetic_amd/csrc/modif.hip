@@ -438,6 +438,7 @@ struct BufPool {
 
 struct ModState {
   std::vector<gic::ModRec> linear, quadratic;
+  bool line_search = false;   // quadratic steps solved exactly along the gradient
   gic::DevBuf part, scalar;
   gic::BufPool pool;
 };
@@ -481,6 +482,7 @@ template <typename T> struct MLF {
   gic_context *c = nullptr;
   std::vector<PBuf> lv;
   std::vector<bool> fourier;
+  MLF() = default;
   explicit MLF(gic_context *cc) : c(cc) {
     lv.resize(c->levels.size());
     fourier.assign(c->levels.size(), false);
@@ -1141,12 +1143,16 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
     vecs = std::move(ortho);
   }
   for (auto &m : c->mod->quadratic) {
+    // performIterations (modificationmanager.hpp:246-283): the value at f is
+    // <Q f, f>, so one push gives both the value and the gradient direction
     auto iterate = [&](MLF<T> &f, int nsteps) {
       double start = variance_value(f, m.flags, m.scale);
       double stepv = (m.target - start) / nsteps;
       for (int i = 0; i < nsteps; ++i) {
-        double cur = variance_value(f, m.flags, m.scale);
         MLF<T> pushed = push_variance(f, m.flags, m.scale);
+        double cur = inner(c, pushed, f);
+        MLF<T> qf;
+        if (c->mod->line_search) qf = pushed.clone();
         double norm = sqrt(cov_dot(pushed, pushed));
         for (auto &a : vecs) {
           double r = inner(c, pushed, a);
@@ -1155,12 +1161,35 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
         double n2 = sqrt(cov_dot(pushed, pushed));
         pushed.scale(1.0 / n2);
         double target_next = start + (i + 1) * stepv;
-        double mult = 0.5 * (target_next - cur) / norm;
         MLF<T> pv = metric(pushed, false);
+        double mult;
+        if (!c->mod->line_search) {
+          mult = 0.5 * (target_next - cur) / norm;
+        } else {
+          // exact line search (no counterpart in the reference): Q is
+          // symmetric, so along v the value is the quadratic
+          // cur + 2 t <Q f, v> + t^2 <Q v, v>; take the root of
+          // value = target_next nearest t = 0, or the vertex if none
+          MLF<T> qv = push_variance(pv, m.flags, m.scale);
+          double qa = inner(c, qv, pv), qb = 2 * inner(c, qf, pv), qc = cur - target_next;
+          double disc = qb * qb - 4 * qa * qc;
+          if (qa == 0) mult = qb != 0 ? -qc / qb : 0.0;
+          else if (disc < 0) mult = -qb / (2 * qa);
+          else {
+            double sq = sqrt(disc);
+            double q = -0.5 * (qb + (qb >= 0 ? sq : -sq));   // stable pair of roots q/qa, qc/q
+            double r1 = q / qa, r2 = q != 0 ? qc / q : r1;
+            mult = std::fabs(r1) < std::fabs(r2) ? r1 : r2;
+          }
+        }
         f.axpy(pv, mult);
       }
     };
     int init = m.init_steps;
+    if (c->mod->line_search) {
+      iterate(field, init);   // every step lands on its target: no trial run
+      continue;
+    }
     MLF<T> test = field.clone();
     iterate(test, init);
     double achieved = std::fabs(variance_value(test, m.flags, m.scale) - m.target);
@@ -1174,6 +1203,57 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
   if (post) *post = chi_post;
 }
 
+// The filtered variance V(w) = <Q w, w> of the flagged region for an
+// arbitrary white-noise field w (dense n_l^3 device arrays per level) and its
+// gradient dV/dw = 2 Q w in real space (Q is symmetric in the white-noise
+// inner product, which is the plain real-space dot product).  No
+// counterpart in the reference, which only evaluates V
+// (QuadraticModification::calculateCurrentValue, quadraticmodification.hpp:50-62).
+template <typename T> double mod_variance_gradient(gic_context *c, const T *const *noise, T *const *grad) {
+  if (c->levels.empty()) throw std::runtime_error("No grid has yet been initialised");
+  if (!c->have_camb) throw std::runtime_error("No power spectrum has been set (use the camb command)");
+  if (!c->mod) c->mod = std::make_unique<ModState>();
+  c->flags.resize(c->levels.size());
+  check_variance_scale(c, c->variance_scale);
+  bool pa = c->power_applied, zd = c->zeldovich_done;
+  c->power_applied = c->zeldovich_done = false;   // the operator acts on white noise
+  double v = 0;
+  try {
+    MLF<T> f(c);
+    for (size_t l = 0; l < f.nl(); ++l) {
+      if (!noise[l]) throw std::runtime_error("variance gradient: null device pointer");
+      pad<T, T>(noise[l], f.g(l).n, f.p(l), c->st);
+      f.fourier[l] = false;
+    }
+    MLF<T> p = push_variance(f, c->flags, c->variance_scale);
+    v = inner(c, p, f);
+    if (grad) {
+      for (size_t l = 0; l < p.nl(); ++l) {
+        if (!grad[l]) throw std::runtime_error("variance gradient: null device pointer");
+        p.toR(l);
+        size_t n3 = p.g(l).n3();
+        unpad<T, T>(p.p(l), p.g(l).n, grad[l], c->st);
+        hipLaunchKernelGGL(k_scale<T>, dim3(blocks(n3)), dim3(256), 0, c->st, grad[l], T(2), n3);
+      }
+      GIC_HIP(hipStreamSynchronize(c->st));
+    }
+  } catch (...) {
+    c->power_applied = pa;
+    c->zeldovich_done = zd;
+    throw;
+  }
+  c->power_applied = pa;
+  c->zeldovich_done = zd;
+  return v;
+}
+
+void mod_set_line_search(gic_context *c, bool on) {
+  if (!c->mod) c->mod = std::make_unique<ModState>();
+  c->mod->line_search = on;
+}
+
+template double mod_variance_gradient<float>(gic_context *, const float *const *, float *const *);
+template double mod_variance_gradient<double>(gic_context *, const double *const *, double *const *);
 template double mod_calculate<float>(gic_context *, const std::string &);
 template double mod_calculate<double>(gic_context *, const std::string &);
 template void mod_add<float>(gic_context *, const std::string &, bool, double, int, double);

@@ -52,6 +52,10 @@ SIGNATURES = {
     "gic_apply_modifications": ([_v, _P(_d), _P(_d)], _i),
     "gic_clear_modifications": ([_v], _i),
     "gic_chi2": ([_v, _P(_d), _P(ctypes.c_uint64)], _i),
+    "gic_displacement_forward": ([_v, _v, _v], _i),
+    "gic_displacement_adjoint": ([_v, _v, _v], _i),
+    "gic_variance_gradient": ([_v, _v, _P(_d), _v], _i),
+    "gic_set_quadratic_line_search": ([_v, _i], _i),
     "gic_particles": ([_v, _i, _v, _s, _s, _v, _v], _i),
     "gic_particle_mass_eps": ([_v, _i, _P(_d), _P(_d)], _i),
     "gic_velocity_ratio": ([_v, _P(_d)], _i),
@@ -267,6 +271,32 @@ class ICGenerator:
         out, dof = ctypes.c_double(), ctypes.c_uint64()
         self._check(lib().gic_chi2(self._h, ctypes.byref(out), ctypes.byref(dof)))
         return out.value, dof.value
+
+    def set_quadratic_line_search(self, on=True):
+        self._check(lib().gic_set_quadratic_line_search(self._h, int(bool(on))))
+
+    # --- differentiation: device pointers, dense n^3 arrays per level ---------
+    @staticmethod
+    def _ptrs(ptrs):
+        ptrs = [int(p) for p in ptrs]
+        return (ctypes.c_void_p * len(ptrs))(*ptrs)
+
+    def displacement_forward(self, noise_ptrs, psi_ptrs):
+        """psi[3l+a] <- component a of the Zel'dovich displacement of the white
+        noise noise[l] (device pointers of the context's precision)."""
+        self._check(lib().gic_displacement_forward(self._h, self._ptrs(noise_ptrs), self._ptrs(psi_ptrs)))
+
+    def displacement_adjoint(self, psi_bar_ptrs, noise_bar_ptrs):
+        """noise_bar[l] <- (d psi / d w_l)^T psi_bar."""
+        self._check(lib().gic_displacement_adjoint(self._h, self._ptrs(psi_bar_ptrs), self._ptrs(noise_bar_ptrs)))
+
+    def variance_gradient(self, noise_ptrs, grad_ptrs=None):
+        """Filtered variance of the flagged region for the given white noise;
+        with grad_ptrs its gradient 2 Q w is written there."""
+        v = ctypes.c_double()
+        g = self._ptrs(grad_ptrs) if grad_ptrs is not None else None
+        self._check(lib().gic_variance_gradient(self._h, self._ptrs(noise_ptrs), ctypes.byref(v), g))
+        return v.value
 
     def mode_power(self, level):
         """Per-|k|^2 sums of |delta_k|^2, mode counts and theory covariance."""

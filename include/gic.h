@@ -125,6 +125,33 @@ int gic_clear_modifications(gic_context *ctx);
  * the degrees of freedom (multilevelgrid.hpp:160-172). */
 int gic_chi2(gic_context *ctx, double *chi2, uint64_t *dof);
 
+/* ---- Differentiation (no counterpart in the reference, which has no adjoint) ----
+ * Device pointers: arrays of the context's precision (float for 32, double
+ * for 64), dense n_l^3 per level with z fastest, on the context's device.
+ * Calls are ordered on the context's stream and synchronous on return; the
+ * caller finishes its own work on the arrays first.
+ *
+ * Forward displacement map w -> psi: noise[l] is the white noise of level l,
+ * psi[3l+a] receives component a of its combined Zel'dovich displacement --
+ * the path of done (ic.hpp:1633-1648) through applyPowerSpectrum, the
+ * Zel'dovich generator (zeldovich.hpp:152-253) and the level combination
+ * (field.hpp:816-849).  Replaces the context's fields, like done. */
+int gic_displacement_forward(gic_context *ctx, const void *const *noise, void *const *psi);
+/* Transpose of that map: noise_bar[l] = sum over levels and components of
+ * (d psi / d w_l)^T psi_bar, the gradient of a loss of psi with respect to the
+ * white noise.  Independent of the context's fields. */
+int gic_displacement_adjoint(gic_context *ctx, const void *const *psi_bar, void *const *noise_bar);
+/* Filtered variance V(w) of the flagged region for the white noise noise[l]
+ * (calculate variance: QuadraticModification::calculateCurrentValue,
+ * quadraticmodification.hpp:50-62, at the current filtering_scale) and, if
+ * grad is non-null, its gradient 2 Q w into grad[l]. */
+int gic_variance_gradient(gic_context *ctx, const void *const *noise, double *value, void *const *grad);
+/* Quadratic modification steps (ModificationManager::performIterations,
+ * modificationmanager.hpp:246-283): 0 the reference's linearised steps and
+ * step-count trial (default), 1 an exact line search along the same gradient
+ * direction, so that each step lands on its intermediate target. */
+int gic_set_quadratic_line_search(gic_context *ctx, int enable);
+
 /* Power-spectrum histogram for dump_ps (dumpPowerSpectrum,
  * cosmology/parameters.hpp:112-205): over every (ix,iy,iz) in [-n/2,n/2]^3,
  * sum_power[m] += |delta_k|^2 and count[m] += 1 with m = ix^2+iy^2+iz^2;
