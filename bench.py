@@ -59,9 +59,13 @@ def make_context(wl, precision):
 
 def sphere_cells(ic, level, centre, radius):
     """Cells of a level within radius of centre (select_sphere, ic.hpp:1490-1500)."""
-    import numpy as np
     n, _, off, dx = ic.level_geometry(level)
     box0 = ic.level_geometry(0)[1]
+    return sphere_cells_geom(n, dx, off, box0, centre, radius)
+
+
+def sphere_cells_geom(n, dx, off, box0, centre, radius):
+    import numpy as np
     idx = np.arange(n)
     d = []
     for a in range(3):
@@ -85,19 +89,140 @@ def one_step(ic, seed, mods):
     ic.zeldovich()
 
 
-def cpu_baseline(n=256):
-    """The oracle's restatement of the reference's stage order (serial
-    real-space stream, threaded FFTs, sqrt P, Zel'dovich) at n^3."""
+def cpu_sample_workload(wl, scale):
+    """The CPU sample of a zoom workload: the same nesting, refinements, box
+    resolution and constraints with every grid n/scale cells a side (the box
+    shrinks with it, so cell sizes are unchanged); the constraint sphere
+    shrinks to fit the innermost level."""
+    n = wl["n"] // scale
+    zooms = []
+    for (x0, y0, z0, fac, nz) in wl["zooms"]:
+        zooms.append((x0 // scale, y0 // scale, z0 // scale, fac, nz // scale))
+    out = dict(box=wl["box"] / scale, n=n, zooms=zooms, lpt=wl.get("lpt", 1))
+    if wl.get("mods"):
+        m = wl["mods"]
+        out["mods"] = dict(centre=tuple(c / scale for c in m["centre"]), radius=m["radius"] / scale,
+                           mods=m["mods"])
+    return out
+
+
+def cpu_baseline(wl, scale, reps=3):
+    """The oracle (FP64 C++ restatement of the reference's stage order:
+    serial real-space stream, threaded FFTs, modifications, sqrt P,
+    Zel'dovich, 2LPT, level combination) timed on one realisation of the
+    workload's CPU sample, on the host's cores."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
+    s = cpu_sample_workload(wl, scale)
     tab, nc = po.read_camb(EH_TABLE)
-    tt, tr = ctypes.c_double(), ctypes.c_double()
-    po._check(po.lib().orc_time_pipeline(50.0, n, 8896131, po._ptr(tab), tab.size, nc, COSMO["Om"], COSMO["Ol"],
-                                         COSMO["s8"], COSMO["ns"], COSMO["z"], ctypes.byref(tt), ctypes.byref(tr)))
+    total = 0.0
+    for r in range(reps):
+        # the oracle, like the reference, seeds once per generator: one
+        # generator per realisation, set up outside the timed region
+        o = po.OracleIC()
+        o.set_cosmology(Om=COSMO["Om"], Ol=COSMO["Ol"], s8=COSMO["s8"], ns=COSMO["ns"], z=COSMO["z"],
+                        hubble=COSMO["hubble"])
+        o.set_camb(tab, nc)
+        o.init_base_grid(s["box"], s["n"])
+        geo = [(s["n"], s["box"] / s["n"], (0.0, 0.0, 0.0))]
+        for (x0, y0, z0, fac, nz) in s["zooms"]:
+            o.init_zoom_grid_at(x0, y0, z0, fac, nz)
+            na, dxa, offa = geo[-1]
+            dxz = dxa * (na // fac) / nz
+            geo.append((nz, dxz, tuple(offa[i] + (x0, y0, z0)[i] * dxa for i in range(3))))
+        o.set_lpt_order(s["lpt"])
+        ncells = sum(g[0] ** 3 for g in geo)
+        if s.get("mods"):
+            m = s["mods"]
+            nz, dxz, offz = geo[-1]
+            o.set_flags(len(geo) - 1, sphere_cells_geom(nz, dxz, offz, s["box"], m["centre"], m["radius"]))
+        t0 = time.perf_counter()
+        o.seed(1 + r)
+        o.draw()
+        if s.get("mods"):
+            for name, target in s["mods"]["mods"]:
+                o.add_modification(name, False, target)
+            o.apply_modifications()
+        o.apply_power_spectrum()
+        o.zeldovich()
+        total += time.perf_counter() - t0
+        del o
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=n ** 3 / tt.value / 1e6, unit="Mparticles/s", cores=threads, kind="port",
-                sample="oracle FP64 pipeline seed->delta,psi at %d^3 single level (50 Mpc/h), %.2f s total, "
-                       "%.2f s in the serial RNG + first r2c" % (n, tt.value, tr.value))
+    desc = "%d^3 in %.1f Mpc/h + zooms %s" % (s["n"], s["box"], ", ".join("%d^3" % z[4] for z in s["zooms"]))
+    return dict(value=round(reps * ncells / total / 1e6, 3), unit="Mparticles/s", cores=threads, kind="port",
+                sample="oracle FP64, %d realisations of the workload scaled by 1/%d per side (%s, same cell sizes, "
+                       "%s, constraint sphere %.2f Mpc/h): %d cells each, %.2f s in total"
+                       % (reps, scale, desc, "2LPT" if s["lpt"] == 2 else "ZA",
+                          s["mods"]["radius"] if s.get("mods") else 0.0, ncells, total))
+
+
+def adjoint_ratio(n=256, box=50.0, reps=20):
+    """Forward w -> psi and its transpose psi_bar -> w_bar on one n^3 FP32
+    level (dense device arrays in and out): median wall time of reps
+    synchronous calls each after 3 warm-up calls."""
+    import numpy as np
+    import torch
+    from genetic_amd import gic
+    g = gic.ICGenerator(precision=32)
+    g.set_cosmology(**COSMO)
+    g.set_camb(EH_TABLE)
+    g.init_base_grid(box, n)
+    gen = torch.Generator(device="cuda").manual_seed(7)
+    w = torch.randn((n, n, n), generator=gen, device="cuda", dtype=torch.float32)
+    psi = [torch.empty_like(w) for _ in range(3)]
+    bar = [torch.randn((n, n, n), generator=gen, device="cuda", dtype=torch.float32) for _ in range(3)]
+    wbar = torch.empty_like(w)
+    torch.cuda.synchronize()
+    fwd = lambda: g.displacement_forward([w.data_ptr()], [p.data_ptr() for p in psi])
+    adj = lambda: g.displacement_adjoint([b.data_ptr() for b in bar], [wbar.data_ptr()])
+    res = {}
+    for name, fn in (("forward", fwd), ("adjoint", adj)):
+        for _ in range(3):
+            fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        res[name] = float(np.median(ts)) * 1e3
+    g.close()
+    return dict(forward_ms=round(res["forward"], 4), adjoint_ms=round(res["adjoint"], 4),
+                ratio=round(res["adjoint"] / res["forward"], 3),
+                config="%d^3 in %.0f Mpc/h, FP32, dense device arrays in and out, median of %d" % (n, box, reps))
+
+
+def psi_precision(wl):
+    """rms over all cells of |psi_fp32 - psi_fp64| / dx_level: the workload's
+    Zel'dovich displacements from the same white noise in both precisions."""
+    import torch
+    from genetic_amd import gic
+    ctx = {}
+    out = {}
+    gen = torch.Generator(device="cuda").manual_seed(11)
+    for prec in (64, 32):
+        ctx[prec] = make_context(dict(wl, lpt=1), prec)
+    sizes = [wl["n"]] + [z[4] for z in wl["zooms"]]
+    noise = [torch.randn((n, n, n), generator=gen, device="cuda", dtype=torch.float64) for n in sizes]
+    psi = {}
+    for prec in (64, 32):
+        dt = torch.float64 if prec == 64 else torch.float32
+        w = [t.to(dt) for t in noise]
+        p = [torch.empty_like(w[l]) for l in range(len(w)) for _ in range(3)]
+        torch.cuda.synchronize()
+        ctx[prec].displacement_forward([t.data_ptr() for t in w], [q.data_ptr() for q in p])
+        psi[prec] = p
+        del w
+    sq, cells, worst = 0.0, 0, 0.0
+    for l, n in enumerate(sizes):
+        dx = ctx[64].level_geometry(l)[3]
+        d2 = sum((psi[32][3 * l + a].double() - psi[64][3 * l + a]) ** 2 for a in range(3))
+        sq += float(d2.sum()) / dx ** 2
+        worst = max(worst, float(d2.max().sqrt()) / dx)
+        cells += n ** 3
+    for c in ctx.values():
+        c.close()
+    return dict(rms_cells=float("%.4g" % (sq / cells) ** 0.5), max_cells=float("%.4g" % worst),
+                config="Zel'dovich psi of the workload's levels, same white noise in FP32 and FP64")
 
 
 def pmc_traffic(workload, kernel):
@@ -121,7 +246,10 @@ def main():
     ap.add_argument("--workload", default="zoom1024_2lpt", choices=sorted(WORKLOADS))
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=256)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the adjoint/forward ratio and the FP32-vs-FP64 displacement error")
+    ap.add_argument("--cpu-scale", type=int, default=8,
+                    help="the CPU baseline runs the workload with every grid 1/scale as many cells a side")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (reference reading for the FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -192,8 +320,12 @@ def main():
         roofline=roofline,
         stage_ms_per_step=stages,
     )
+    ic.close()
+    if not args.no_extras:
+        out["adjoint_over_forward"] = adjoint_ratio()
+        out["psi_fp32_error"] = psi_precision(wl)
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(args.cpu_n)
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_scale)
     print(json.dumps(out))
 
 

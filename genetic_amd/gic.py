@@ -78,6 +78,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 "libgic_hip.so is not built (%s); run __graft_entry__.build() first" % LIB_PATH)
+        # torch ships its own HIP runtime under the same soname as the
+        # system's; whichever loads first serves the whole process.  Loading
+        # torch first keeps one runtime for both when torch tensors are used.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (args, res) in SIGNATURES.items():
             fn = getattr(L, name)
