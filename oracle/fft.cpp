@@ -4,7 +4,7 @@
 // conventions restated are the padded [N][N][N/2+1] layout, the forward sign
 // e^{-2 pi i k.x/N}, the N^-1.5 scaling both ways, and the Hermitian mirroring
 // of the kz in {0, N/2} planes before c2r (positive modes win).
-// Pinned by brute-force DFT and Parseval tests (tests/test_oracle_fft.py).
+// Pinned by brute-force DFT and Parseval tests (tests/test_cpu_oracle.py).
 //
 // TEST INFRASTRUCTURE ONLY (see oracle.hpp).
 #include "oracle.hpp"

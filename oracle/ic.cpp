@@ -277,7 +277,11 @@ void IC::zeldovich() {
 // - phi_ij^2), psi2 = (3/7) i k S / k^2 (D2 = -3/7 D1^2), zero at a +Nyquist
 // component (mixed derivatives, psi2) and at k = 0; zoom levels keep the
 // high-pass part of their own psi2 and take the low-pass part of the level
-// above, as psi does.  The reference has no second order: parity unpinned.
+// above, as psi does.  The reference has no second order, so there is no
+// reference output to match; the restatement is pinned by closed forms
+// (tests/test_cpu_oracle.py): a plane wave along an axis and along (1,1,1)
+// gives S = 0, two axis waves and diagonal waves give analytic psi2 (the
+// latter only with the phi_ij^2 terms right).
 void IC::secondOrder() {
   size_t nl = grids.size();
   psi2.assign(nl, {});

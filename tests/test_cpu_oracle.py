@@ -524,3 +524,39 @@ def test_oracle_2lpt_closed_forms():
     o2.zeldovich()
     for a in range(3):
         assert np.abs(o2.field(4 + a, 0)).max() < 1e-15
+
+
+def _diagonal_waves(n, box, m=2, m2=3, A=0.3, B=0.2):
+    """delta = A cos(k (x + y)) + B cos(k2 z): phi_xx = phi_yy = phi_xy =
+    A cos(u) / 2 (u = k (x + y)) and phi_zz = B cos(k2 z), so
+    S = phi_xx phi_yy - phi_xy^2 + (phi_xx + phi_yy) phi_zz = A B cos(u) cos(k2 z)
+    (the mixed term cancels the product of the diagonal ones) and
+    psi2 = -(3/7) grad del^-2 S = (3/7) grad S / (2 k^2 + k2^2)."""
+    x = np.arange(n) * box / n
+    k, k2 = 2 * np.pi * m / box, 2 * np.pi * m2 / box
+    X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
+    u = k * (X + Y)
+    delta = A * np.cos(u) + B * np.cos(k2 * Z)
+    d = 2 * k * k + k2 * k2
+    ex = -(3 / 7) * A * B * k * np.sin(u) * np.cos(k2 * Z) / d
+    ez = -(3 / 7) * A * B * k2 * np.cos(u) * np.sin(k2 * Z) / d
+    plane = 0.4 * np.cos(k * (X + Y + Z))   # a plane wave along (1,1,1): S = 0
+    return delta, ex, ex, ez, plane
+
+
+def test_oracle_2lpt_mixed_derivative_closed_form():
+    """The mixed second derivatives enter S: diagonal waves against their
+    closed form, and a (1,1,1) plane wave (S = 0 only through phi_ij^2)."""
+    n, box = 32, 50.0
+    delta, ex, ey, ez, plane = _diagonal_waves(n, box)
+    o, *_ = _two_wave_setup(n, box)
+    o.set_field(0, delta)
+    o.zeldovich()
+    s = np.abs(ex).max()
+    for a, e in enumerate((ex, ey, ez)):
+        assert np.abs(o.field(4 + a, 0) - e).max() < 1e-12 * s * n
+    o2, *_ = _two_wave_setup(n, box)
+    o2.set_field(0, plane)
+    o2.zeldovich()
+    for a in range(3):
+        assert np.abs(o2.field(4 + a, 0)).max() < 1e-14

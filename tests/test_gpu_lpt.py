@@ -52,6 +52,30 @@ def test_2lpt_two_waves_closed_form(precision, tol):
     assert np.abs(ic.field(gic.FIELD_PSI2_Z, 0)).max() < tol * s * 10
 
 
+@pytest.mark.parametrize("precision,tol", [(gic.FLOAT64, 1e-12), (gic.FLOAT32, 2e-6)])
+def test_2lpt_mixed_derivatives_closed_form(precision, tol):
+    """delta = A cos(k (x + y)) + B cos(k2 z): the phi_xy^2 term cancels
+    phi_xx phi_yy and S = A B cos(k (x + y)) cos(k2 z); a (1,1,1) plane wave
+    gives S = 0 only through the mixed terms."""
+    print("2lpt mixed derivatives", precision, flush=True)
+    from test_cpu_oracle import _diagonal_waves
+    n, box = 32, 50.0
+    delta, ex, ey, ez, plane = _diagonal_waves(n, box)
+    ic, _ = _ic(precision, n, box)
+    ic.apply_power_spectrum()
+    ic.set_field(0, delta)
+    ic.zeldovich()
+    s = np.abs(ex).max()
+    for f, e in zip((gic.FIELD_PSI2_X, gic.FIELD_PSI2_Y, gic.FIELD_PSI2_Z), (ex, ey, ez)):
+        assert np.abs(ic.field(f, 0) - e).max() < tol * s * 10
+    ic2, _ = _ic(precision, n, box)
+    ic2.apply_power_spectrum()
+    ic2.set_field(0, plane)
+    ic2.zeldovich()
+    for f in (gic.FIELD_PSI2_X, gic.FIELD_PSI2_Y, gic.FIELD_PSI2_Z):
+        assert np.abs(ic2.field(f, 0)).max() < (1e-14 if precision == gic.FLOAT64 else 1e-6)
+
+
 def test_2lpt_plane_wave_has_no_second_order():
     n, box = 32, 50.0
     ic, _ = _ic(gic.FLOAT64, n, box)
