@@ -10,6 +10,7 @@
 // library.
 #include "../../include/gic.h"
 
+#include <sys/stat.h>
 #include <unistd.h>
 
 #include <algorithm>
@@ -427,8 +428,16 @@ struct Generator {
     log() << "  Num particles         = " << untrimmed.size() << std::endl;
     log() << "  Low-left corner in parent grid = " << x0i << ", " << y0i << ", " << z0i << std::endl;
     log() << "  Low-left corner (h**-1 Mpc)    = " << ng.ox << ", " << ng.oy << ", " << ng.oz << std::endl;
-    buildMapper();
-    log() << "  Total particles       = " << mapperSize(int(grids.size()) - 1) << std::endl;
+    if (fmt == Fmt::grafic) {
+      // grafic output keeps no particle mapper across levels: its "particles"
+      // are the cells of the grafic hierarchy (GraficMapper, graficmapper.hpp:43-72)
+      size_t total = 0;
+      for (auto &v : graficStack({x0, y0, z0})) total += v.g.n3();
+      log() << "  Total particles       = " << total << std::endl;
+    } else {
+      buildMapper();
+      log() << "  Total particles       = " << mapperSize(int(grids.size()) - 1) << std::endl;
+    }
     // flag the fine cells under the flagged coarse cells (makeProxyGridToMatch:
     // supersample, then the section at the fine grid's offset, wrapped)
     Grid &fine = grids.back();
@@ -675,11 +684,19 @@ struct Generator {
     ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
     if (!append)
       for (auto &g : grids) g.flags.clear();
+    checkGraficIds();
     buildMapper();
     flagParticles(int(grids.size()) - 1, ids);
     centreOnFlagged();
   }
+  // With grafic output, particle IDs index the cells of the grafic hierarchy;
+  // only the case where that hierarchy is the base grid alone is supported here.
+  void checkGraficIds() const {
+    if (fmt == Fmt::grafic && (grids.size() > 1 || subsample > 1 || supersample > 1))
+      throw std::runtime_error("id files over a multi-level grafic hierarchy are not supported by this build");
+  }
   void dumpIDFile(const std::string &fn) {
+    checkGraficIds();
     buildMapper();
     std::ofstream f(fn);
     for (size_t id : flaggedParticles()) f << id << std::endl;
@@ -887,6 +904,9 @@ struct Generator {
         writeGadget(outputPath() + ".gadget", blocks, int(fmt));
         break;
       }
+      case Fmt::grafic:
+        writeGrafic(outputPath() + ".grafic");
+        break;
       default:
         throw std::runtime_error("Output format " + std::to_string(int(fmt)) + " is not supported by this build");
     }
@@ -1105,6 +1125,334 @@ struct Generator {
         std::vector<float> u(typePerFile[fi][0], 0.f);
         rec(u.data(), uint32_t(u.size() * 4));
       }
+    }
+  }
+
+  // --- grafic ---
+  // The grafic hierarchy (MultiLevelGrid::copyContextWithCenteredIntermediate,
+  // multilevelgrid.hpp:338-473): extra subsampled copies of the base level
+  // ('subsample'), subsampled intermediates so that successive levels differ
+  // by a factor 2, supersampled copies of the finest level ('supersample');
+  // then every level covering the box is re-centred on a point by an integer
+  // cell shift (CenteredGrid, virtualgrid.hpp:688-800) and the others are
+  // offset by the same distance (OffsetGrid, virtualgrid.hpp:806-840).
+  struct VLevel {
+    int src = 0;         // real level holding the data
+    int factor = 1;      // > 1: subsampled by factor; < -1: supersampled by -factor
+    Grid g;              // geometry of this level (after any offset)
+    Grid gu;             // the same before the offset (where fields are evaluated)
+    bool centred = false;
+    I3 shift;            // CenteredGrid offset, in this level's cells
+  };
+  int subsample = 1, supersample = 1, lptOrder = 1;
+  double pvarValue = 1.0;
+  bool centerOnTargetRegion = false;
+
+  static int powerOf2(int v) {   // tools::findPowerOf(2, v)
+    int p = 0;
+    while (v > 1 && v % 2 == 0) {
+      v /= 2;
+      ++p;
+    }
+    if (v != 1)
+      throw std::runtime_error(
+          "Subsample and supersample cannot be converted into an integer number of grids with this base factor");
+    return p;
+  }
+  VLevel subsampled(int l, int f) const {
+    VLevel v;
+    v.src = l;
+    v.factor = f;
+    v.g = grids[l];
+    v.g.flags.clear();
+    v.g.n = grids[l].n / f;
+    v.g.dx = grids[l].dx * f;
+    return v;
+  }
+  std::vector<VLevel> graficStack(std::array<double, 3> centre) const {
+    int extraLow = powerOf2(subsample), extraHigh = powerOf2(supersample);
+    std::vector<VLevel> st;
+    for (size_t l = 0; l < grids.size(); ++l) {
+      int nint = extraLow;
+      if (l > 0) nint = int(std::round(std::log10(grids[l - 1].dx / grids[l].dx) / std::log10(2.0))) - 1;
+      for (int k = nint; k >= 1; --k) st.push_back(subsampled(int(l), 1 << k));
+      VLevel r;
+      r.src = int(l);
+      r.g = grids[l];
+      r.g.flags.clear();
+      st.push_back(r);
+    }
+    for (int k = 1; k <= extraHigh; ++k) {
+      VLevel v;
+      v.src = int(grids.size()) - 1;
+      v.factor = -(1 << k);
+      v.g = grids.back();
+      v.g.flags.clear();
+      v.g.n = grids.back().n << k;
+      v.g.dx = grids.back().dx / (1 << k);
+      st.push_back(v);
+    }
+    // copyContextAndCenter (multilevelgrid.hpp:426-445)
+    int deepest = -1;
+    for (int i = int(st.size()) - 1; i >= 0; --i)
+      if (st[i].g.coversFull()) {
+        deepest = i;
+        break;
+      }
+    if (deepest < 0) throw std::runtime_error("No coarse grid were found.");
+    double off[3] = {0, 0, 0};
+    for (int i = 0; i <= deepest; ++i) {
+      VLevel &v = st[i];
+      v.centred = true;
+      for (int a = 0; a < 3; ++a) {
+        v.shift[a] = int((0.5 * v.g.size() - centre[a]) / v.g.dx);   // Coordinate<int>: truncation
+        off[a] = v.shift[a] * v.g.dx;
+      }
+    }
+    for (auto &v : st) v.gu = v.g;
+    for (size_t i = deepest + 1; i < st.size(); ++i) {
+      Grid &g = st[i].g;
+      g.ox = g.wrap(g.ox + off[0]);
+      g.oy = g.wrap(g.oy + off[1]);
+      g.oz = g.wrap(g.oz + off[2]);
+    }
+    return st;
+  }
+  // Grid::getIndexFromCoordinateNoWrap of a level of the stack: a centred
+  // level steps back by its shift and wraps (getIndexFromIndexAndStep)
+  static size_t vIndex(const VLevel &v, int x, int y, int z) {
+    if (!v.centred) return v.g.index(x, y, z);
+    int s = v.g.simEquivalent();
+    int c[3] = {x - v.shift.x, y - v.shift.y, z - v.shift.z};
+    for (int a = 0; a < 3; ++a) {
+      if (c[a] > s - 1) c[a] -= s;
+      if (c[a] < 0) c[a] += s;
+    }
+    return v.g.index(c[0], c[1], c[2]);
+  }
+  // getCentroidFromIndex / getIndexFromPoint of a stack level
+  static std::array<double, 3> vCentroid(const VLevel &v, size_t id) {
+    auto p = v.g.centroid(id);
+    if (v.centred) {
+      p[0] = v.g.wrap(p[0] + v.shift.x * v.g.dx);
+      p[1] = v.g.wrap(p[1] + v.shift.y * v.g.dx);
+      p[2] = v.g.wrap(p[2] + v.shift.z * v.g.dx);
+    }
+    return p;
+  }
+  static size_t vIndexFromPoint(const VLevel &v, std::array<double, 3> p) {
+    if (v.centred)
+      for (int a = 0; a < 3; ++a) p[a] -= v.shift[a] * v.g.dx;
+    double o[3] = {v.g.ox, v.g.oy, v.g.oz};
+    int c[3];
+    for (int a = 0; a < 3; ++a) c[a] = int(std::floor(v.g.wrap(p[a] - o[a]) / v.g.dx));
+    return v.g.index(c[0], c[1], c[2]);
+  }
+  // GraficMask (multilevelgrid/mask.hpp:46-231): per stack level, the
+  // flagged cells; a level without flags counts as all in the mask
+  std::vector<std::vector<size_t>> graficMask(const std::vector<VLevel> &st) const {
+    size_t nl = st.size();
+    std::vector<std::vector<size_t>> flagged(nl);
+    const auto &input = zoomParticles;
+    int deepest = -1;
+    size_t il = 0;
+    for (size_t l = 0; l + 1 < nl && il < input.size(); ++l) {
+      // the first level where some cell index 0..size3-1 is in the next input list
+      if (!input[il].empty() && input[il].front() < st[l].g.n3()) {
+        flagged[l] = input[il];
+        deepest = int(l);
+        ++il;
+      }
+    }
+    if (deepest < 0) {
+      log() << "WARNING No zoom regions were ever opened. Grafic mask will not be generated in this case" << std::endl;
+      return flagged;
+    }
+    for (int l = deepest; l >= 0; --l) {
+      if (!flagged[l].empty()) continue;
+      for (size_t i : flagged[l + 1]) flagged[l].push_back(vIndexFromPoint(st[l], vCentroid(st[l + 1], i)));
+      std::sort(flagged[l].begin(), flagged[l].end());
+      flagged[l].erase(std::unique(flagged[l].begin(), flagged[l].end()), flagged[l].end());
+    }
+    for (size_t l = deepest + 1; l + 1 < nl; ++l) {
+      for (size_t i = 0; i < st[l].g.n3(); ++i) {
+        size_t above = vIndexFromPoint(st[l - 1], vCentroid(st[l], i));
+        if (std::binary_search(flagged[l - 1].begin(), flagged[l - 1].end(), above)) flagged[l].push_back(i);
+      }
+    }
+    return flagged;
+  }
+  static float inMask(const std::vector<std::vector<size_t>> &flagged, size_t l, size_t i) {
+    if (flagged[l].empty()) return 1.f;
+    return std::binary_search(flagged[l].begin(), flagged[l].end(), i) ? 1.f : 0.f;
+  }
+  std::array<double, 3> graficCentre() const {
+    if (centerOnTargetRegion) return {x0, y0, z0};
+    double h = grids[0].periodic / 2;
+    return {h, h, h};
+  }
+
+  // dumpMask (ic.hpp:975-994): mask-<l>.npy of every level of the hierarchy
+  void dumpMask() {
+    if (grids.empty()) throw std::runtime_error("No grid has yet been initialised");
+    auto st = graficStack(graficCentre());
+    auto flagged = graficMask(st);
+    for (size_t l = 0; l < st.size(); ++l) {
+      int n = st[l].g.n;
+      std::vector<double> d(size_t(n) * n * n);
+      for (int x = 0; x < n; ++x)
+        for (int y = 0; y < n; ++y)
+          for (int z = 0; z < n; ++z) d[(size_t(x) * n + y) * n + z] = inMask(flagged, l, vIndex(st[l], x, y, z));
+      saveNpy(outdir + "/mask-" + std::to_string(l) + ".npy", n, d);
+      std::ofstream inf(outdir + "/mask-info-" + std::to_string(l) + ".txt");
+      const Grid &g = st[l].g;
+      inf << g.ox << " " << g.oy << " " << g.oz << " " << g.size() << std::endl;
+      inf << "The line above contains information about grid level " << l << std::endl;
+      inf << "It gives the x-offset, y-offset and z-offset of the low-left corner and also the box length" << std::endl;
+    }
+  }
+
+  // GraficOutput::write (io/grafic.hpp:96-241): per level a directory
+  // <name>_<effective size> with ten files, each a Fortran header record and
+  // one record per z plane (x fastest).  Displacements in Mpc/h, velocities
+  // in km/s (times a^1/2), lengths in the header in Mpc.
+  void writeGrafic(const std::string &name) {
+    auto st = graficStack(graficCentre());
+    auto flagged = graficMask(st);
+    double velRatio;
+    check(gic_velocity_ratio(c(), &velRatio));
+    double velFactor = std::pow(a(), 0.5);
+    bool second = lptOrder == 2;
+    size_t iordOffset = 0;
+    const Grid &base = st[0].g;
+    // the fields of each real level, fetched once
+    std::vector<std::vector<std::vector<double>>> fields(grids.size());
+    auto fieldsOf = [&](int l) -> std::vector<std::vector<double>> & {
+      auto &f = fields[l];
+      if (f.empty()) {
+        int nf = second ? 7 : 4;
+        f.resize(nf, std::vector<double>(grids[l].n3()));
+        check(gic_copy_field(c(), GIC_FIELD_OVERDENSITY, l, GIC_REAL_SPACE, f[0].data(), GIC_DTYPE_F64));
+        for (int k = 1; k < nf; ++k)
+          check(gic_copy_field(c(), k, l, GIC_REAL_SPACE, f[k].data(), GIC_DTYPE_F64));
+      }
+      return f;
+    };
+    for (size_t l = 0; l < st.size(); ++l) {
+      const VLevel &v = st[l];
+      const Grid &g = v.g;
+      int n = g.n;
+      const Grid &rg = grids[v.src];
+      auto &f = fieldsOf(v.src);
+      size_t effective = size_t(std::llround(base.dx * base.n / g.dx));
+      std::string dir = name + "_" + std::to_string(effective);
+      mkdir(dir.c_str(), 0777);
+      static const char *names[10] = {"ic_velcx", "ic_velcy", "ic_velcz", "ic_poscx", "ic_poscy",
+                                      "ic_poscz", "ic_deltab", "ic_refmap", "ic_pvar_00001", "ic_particle_ids"};
+      std::vector<std::ofstream> out(10);
+      struct {
+        int32_t nx, ny, nz;
+        float dx, xo, yo, zo, a, om, ol, h0;
+      } hdr{n, n, n, float(g.dx / hubble), float(g.ox / hubble), float(g.oy / hubble), float(g.oz / hubble),
+            float(a()), float(Om), float(Ol), float(hubble * 100)};
+      static_assert(sizeof(hdr) == 44, "grafic header is 44 bytes");
+      int32_t hl = 44;
+      for (int k = 0; k < 10; ++k) {
+        out[k].open(dir + "/" + names[k], std::ios::binary);
+        if (!out[k]) throw std::runtime_error("Cannot open " + dir + "/" + names[k]);
+        out[k].write((const char *)&hl, 4);
+        out[k].write((const char *)&hdr, 44);
+        out[k].write((const char *)&hl, 4);
+      }
+      size_t n2 = size_t(n) * n;
+      std::vector<std::vector<float>> plane(9, std::vector<float>(n2));
+      std::vector<uint64_t> ids(n2);
+      // value of real-level field k at stack-level cell i: a subsampled level
+      // averages the real cells it covers (SubSampleGrid::forEachSubcell,
+      // virtualgrid.hpp:576-607, which clips at the grid's far edge)
+      auto eval = [&](int k, size_t i) -> double {
+        if (v.factor < -1) {
+          // SuperSampleEvaluator (field/evaluator.hpp:96-126): the real
+          // level interpolated at this cell's centroid (evaluateInterpolated,
+          // field.hpp:483-508, tricubic of field.hpp:598-617), wrapping on a
+          // level that covers the box and repeating edge values otherwise
+          auto p = v.gu.centroid(i);
+          double o[3] = {rg.ox, rg.oy, rg.oz};
+          int p0[3];
+          double w[3][4];
+          for (int a3 = 0; a3 < 3; ++a3) {
+            double u = rg.wrap(p[a3] - o[a3]) / rg.dx - 0.5;
+            p0[a3] = int(std::floor(u));
+            double t = u - p0[a3], t2 = t * t, t3 = t2 * t;
+            w[a3][0] = -0.5 * t3 + t2 - 0.5 * t;
+            w[a3][1] = 1.5 * t3 - 2.5 * t2 + 1.0;
+            w[a3][2] = -1.5 * t3 + 2.0 * t2 + 0.5 * t;
+            w[a3][3] = 0.5 * t3 - 0.5 * t2;
+          }
+          bool wrapc = rg.n == rg.simEquivalent();
+          int sim = rg.simEquivalent();
+          auto fix = [&](int c) {
+            if (wrapc) {
+              if (c > sim - 1) c -= sim;
+              if (c < 0) c += sim;
+            } else {
+              c = std::min(std::max(c, 0), rg.n - 1);
+            }
+            return c;
+          };
+          double sum = 0;
+          for (int a = 0; a < 4; ++a)
+            for (int b = 0; b < 4; ++b)
+              for (int cc = 0; cc < 4; ++cc)
+                sum += w[0][a] * w[1][b] * w[2][cc] *
+                       f[k][rg.index(fix(p0[0] + a - 1), fix(p0[1] + b - 1), fix(p0[2] + cc - 1))];
+          return sum;
+        }
+        if (v.factor <= 1) return f[k][i];
+        I3 c0 = g.coord(i);
+        double sum = 0;
+        int cnt = 0;
+        int fct = v.factor;
+        for (int x = c0.x * fct; x < std::min(c0.x * fct + fct, rg.n); ++x)
+          for (int y = c0.y * fct; y < std::min(c0.y * fct + fct, rg.n); ++y)
+            for (int z = c0.z * fct; z < std::min(c0.z * fct + fct, rg.n); ++z) {
+              sum += f[k][rg.index(x, y, z)];
+              ++cnt;
+            }
+        return sum / cnt;
+      };
+      for (int iz = 0; iz < n; ++iz) {
+        for (int iy = 0; iy < n; ++iy)
+          for (int ix = 0; ix < n; ++ix) {
+            size_t i = vIndex(v, ix, iy, iz);
+            size_t fi = size_t(iy) * n + ix;
+            for (int a3 = 0; a3 < 3; ++a3) {
+              double p = eval(1 + a3, i), vel = p * velRatio;
+              if (second) {
+                double p2 = eval(4 + a3, i);
+                p += p2;
+                vel += 2 * velRatio * p2;
+              }
+              plane[a3][fi] = float(vel * velFactor);
+              plane[3 + a3][fi] = float(p);
+            }
+            plane[6][fi] = float(eval(0, i));
+            float m = inMask(flagged, l, i);
+            plane[7][fi] = m;
+            plane[8][fi] = float(pvarValue * m);
+            ids[fi] = i + iordOffset;
+          }
+        int32_t rl = int32_t(n2 * 4), il = int32_t(n2 * 8);
+        for (int k = 0; k < 9; ++k) {
+          out[k].write((const char *)&rl, 4);
+          out[k].write((const char *)plane[k].data(), n2 * 4);
+          out[k].write((const char *)&rl, 4);
+        }
+        out[9].write((const char *)&il, 4);
+        out[9].write((const char *)ids.data(), n2 * 8);
+        out[9].write((const char *)&il, 4);
+      }
+      iordOffset += g.n3();
     }
   }
 
@@ -1348,7 +1696,17 @@ struct Dispatcher {
     add("clear_modifications", std::function<void()>([this] { ic.clearModifications(); }));
     add("chi2", std::function<void()>([this] { ic.chi2(); }));
     add("filtering_scale", D([this](double v) { ic.varianceScale = v; }));
-    add("lpt_order", std::function<void(int)>([this](int o) { ic.check(gic_set_lpt_order(ic.c(), o)); }));
+    add("lpt_order", std::function<void(int)>([this](int o) {
+          ic.check(gic_set_lpt_order(ic.c(), o));
+          ic.lptOrder = o;
+        }));
+    // grafic hierarchy and mask (ic.hpp:301-350, main.cpp:59-74)
+    add("subsample", std::function<void(int)>([this](int f) { ic.subsample = f; }));
+    add("supersample", std::function<void(int)>([this](int f) { ic.supersample = f; }));
+    add("pvar", std::function<void(double)>([this](double v) { ic.pvarValue = v; }));
+    add("center_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));
+    add("center_grafic_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));
+    add("dump_mask", std::function<void()>([this]() { ic.dumpMask(); }));
     add("dump_grid", std::function<void(int)>([this](int l) { ic.dumpGrid(l); }));
     add("dump_ps", std::function<void(int)>([this](int l) { ic.dumpPS(l); }));
     add("dump_tipsy", S([this](std::string f) { ic.dumpTipsyArray(f); }));
