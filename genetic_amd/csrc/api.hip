@@ -87,6 +87,7 @@ template <typename T> void do_draw(gic_context *c) {
   if (!c->seeded) throw std::runtime_error("The random number generator has not been seeded");
   if (c->drawn) throw std::runtime_error("Trying to re-draw the random field after it was already initialised");
   if (c->levels.empty()) throw std::runtime_error("Cannot draw a random field before the base grid is initialised");
+  Phase ph(c, "phase:draw");
   for (auto &L : c->levels) alloc_field(c, L->noise, L->g);
   if (c->seed_mode == GIC_SEED_REAL_SPACE) {
     std::vector<RngLevelOut> outs;
@@ -343,6 +344,7 @@ template <typename T> void do_second_order(gic_context *c) {
 
 template <typename T> void do_zeldovich(gic_context *c) {
   if (!c->power_applied) do_apply_ps<T>(c);
+  Phase ph(c, "phase:zeldovich");
   size_t nl = c->levels.size();
   // Zel'dovich per level, formed inside the first inverse pass from the
   // overdensity spectrum; zoom levels carry their high-pass filter (the cut
@@ -417,7 +419,10 @@ template <typename T> void do_zeldovich(gic_context *c) {
       if (c->hook()) c->hook()->end("interp_add", c->st);
     }
   }
-  if (c->lpt_order == 2) do_second_order<T>(c);
+  if (c->lpt_order == 2) {
+    Phase ph2(c, "phase:2lpt");
+    do_second_order<T>(c);
+  }
   c->zeldovich_done = true;
 }
 
@@ -907,7 +912,10 @@ int gic_add_modification(gic_context *c, const char *name, int relative, double 
 }
 
 int gic_apply_modifications(gic_context *c, double *pre_chi2, double *post_chi2) {
-  return guarded(c, [&] { DISPATCH(c, mod_apply, c, pre_chi2, post_chi2); });
+  return guarded(c, [&] {
+    Phase ph(c, "phase:modifications");
+    DISPATCH(c, mod_apply, c, pre_chi2, post_chi2);
+  });
 }
 
 int gic_clear_modifications(gic_context *c) {

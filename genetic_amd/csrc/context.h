@@ -242,6 +242,19 @@ struct gic_context {
 
 namespace gic {
 
+// A named region on the context's stream for the event timer (no-op when
+// timing is off); regions nest and overlap the per-kernel records.
+struct Phase {
+  gic_context *c;
+  const char *name;
+  Phase(gic_context *cc, const char *n) : c(cc), name(n) {
+    if (c->hook()) c->hook()->begin(name, c->st);
+  }
+  ~Phase() {
+    if (c->hook()) c->hook()->end(name, c->st);
+  }
+};
+
 // ---- pipeline steps (api.hip) ------------------------------------------------------
 Level &level_at(gic_context *c, int level);
 void alloc_field(gic_context *c, DevBuf &b, const LevelGeom &g);

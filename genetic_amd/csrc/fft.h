@@ -45,6 +45,15 @@ void fft_r2c_za_adjoint(const T *src, T *d, T *acc, int n, const T *tab, int com
 template <typename T>
 void fft_c2r_from(const T *spec, T *out, int n, int pre, const T *tab, int comp, double kmin, hipStream_t st,
                   TimingHook *hook, T *dst = nullptr);
+// Spectrum (r2c, N^-1.5) of a field that is zero outside the real-space window
+// [w0, w0+wl) (per axis, wrapping mod n), given as the dense window win
+// (wl[0]*wl[1]*wl[2] reals, z fastest): pruned z and y passes and a full x
+// pass that scales by post[|k|^2] (if non-null) and stores into, or with
+// accumulate adds to, the padded spectrum spec.  bufB holds
+// wl[0]*wl[1]*(n+2) reals, bufA wl[0]*n*(n/2+1) complex.
+template <typename T>
+void fft_r2c_window(const T *win, int n, const int *w0, const int *wl, T *bufB, T *bufA, T *spec, const T *post,
+                    bool accumulate, hipStream_t st, TimingHook *hook);
 // Only the real-space window [w0, w0+wl) (per axis, wrapping mod n) of
 // c2r(op(spec)), via pruned passes: bufA holds wl[0]*n*(n/2+1) complex,
 // bufB wl[0]*wl[1]*(n/2+1) complex, out wl[0]*wl[1]*wl[2] reals.

@@ -214,7 +214,9 @@ template <typename T> struct StridedArgs {
   T kmin;
   int win0, wlen;   // output keeps line points (pt - win0) mod n < wlen
   T scale;
-  int accum;        // POST_ZA_ADJ: add to the output instead of storing
+  int accum;        // POST ops: add to the output instead of storing
+  int in_win0, in_wlen;   // if 0 < in_wlen < n: the input holds only line points (pt - in_win0) mod n < in_wlen,
+                          // stored at index (pt - in_win0) mod n
 };
 
 __device__ __forceinline__ int signed_k(int i, int n) { return i > n / 2 ? i - n : i; }
@@ -237,6 +239,12 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
   int ky2 = ky * ky;
   auto gld = [&](int line, int pt) -> cx<T> {
     if (line >= nc) return cx<T>{0, 0};
+    if (a.in_wlen > 0 && a.in_wlen < L) {   // pruned input: zero outside the window
+      int o = pt - a.in_win0;
+      if (o < 0) o += L;
+      if (o >= a.in_wlen) return cx<T>{0, 0};
+      return ib[o * a.in_ls + line];
+    }
     return ib[pt * a.in_ls + line];
   };
   auto op = [&](int line, int pt, cx<T> v) -> cx<T> {
@@ -270,6 +278,14 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
     if constexpr (POST == POST_TABLE) {
       int kx = signed_k(pt, a.n), kz = col0 + line;
       s *= a.tab[kx * kx + ky2 + kz * kz];
+    }
+    if constexpr (POST == POST_NONE || POST == POST_TABLE) {
+      if (a.accum) {
+        cx<T> &dst = ob[o * a.out_ls + line];
+        cx<T> prev = dst;
+        dst = cx<T>{prev.x + v.x * s, prev.y + v.y * s};
+        return;
+      }
     }
     if constexpr (POST == POST_ZA_ADJ) {
       int kx = signed_k(pt, a.n), kz = col0 + line;
@@ -443,6 +459,56 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_c2r_window(const T *__restric
   }
 }
 
+// ---- z pass, r2c from a window ------------------------------------------------
+// Same transform as k_fft_z_r2c for rows whose real samples are zero except
+// z in [z0, z0+wz) (mod N), read from `in` (row stride wz); the spectra go to
+// `out` (row stride M+1 complex).
+template <typename T, int N, int ROWS, int THREADS>
+__global__ __launch_bounds__(THREADS) void k_fft_z_r2c_window(const T *__restrict__ in, T *__restrict__ out,
+                                                              long nrows, int z0, int wz, T scale,
+                                                              const cx<T> *__restrict__ twN) {
+  constexpr int M = N / 2;
+  extern __shared__ char smem_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  long row0 = long(blockIdx.x) * ROWS;
+  auto gld = [&](int line, int pt) -> cx<T> {
+    if (row0 + line >= nrows) return cx<T>{0, 0};
+    const T *irow = in + (row0 + line) * wz;
+    T v[2];
+    for (int h = 0; h < 2; ++h) {
+      int z = 2 * pt + h - z0;
+      if (z < 0) z += N;
+      v[h] = z < wz ? irow[z] : T(0);
+    }
+    return cx<T>{v[0], v[1]};
+  };
+  auto lidx = [](int line, int pt) { return line * (M + 1) + pt; };
+  auto lst = [&](int line, int pt, cx<T> v) { lds[lidx(line, pt)] = v; };
+  if constexpr (M == 1) {
+    if (threadIdx.x < ROWS) lds[lidx(threadIdx.x, 0)] = gld(threadIdx.x, 0);
+  } else {
+    run_stages<T, M, ROWS, THREADS, -1, false, false, true, 0>(lds, twN, 2, gld, lst, lidx);
+  }
+  __syncthreads();
+  constexpr int H = M / 2 + 1;
+  for (int q = threadIdx.x; q < ROWS * H; q += THREADS) {
+    int line = q / H, k = q - line * H;
+    if (row0 + line >= nrows) continue;
+    cx<T> zk = lds[lidx(line, k % M)];
+    cx<T> zm = lds[lidx(line, (M - k) % M)];
+    cx<T> F{(zk.x + zm.x) * T(0.5), (zk.y - zm.y) * T(0.5)};
+    cx<T> D{(zk.x - zm.x) * T(0.5), (zk.y + zm.y) * T(0.5)};
+    cx<T> G{D.y, -D.x};
+    cx<T> w = twN[k];
+    cx<T> wg = cmul(w, G);
+    cx<T> Xk{(F.x + wg.x) * scale, (F.y + wg.y) * scale};
+    cx<T> Xm{(F.x - wg.x) * scale, -(F.y - wg.y) * scale};
+    cx<T> *orow = reinterpret_cast<cx<T> *>(out) + (row0 + line) * (M + 1);
+    orow[k] = Xk;
+    if (M - k != k) orow[M - k] = Xm;
+  }
+}
+
 // ---- Hermitian mirror of the kz=0 and kz=N/2 planes ------------------------------
 // ensureFourierModesAreMirrored (fourier.hpp:331-357): for kx in [0,N/2] and
 // ky from N/2 down to -N/2+1 the conjugate of mode (kx,ky) is copied into
@@ -536,6 +602,8 @@ template <typename T, int N> struct FftN {
     a.kmin = 0;
     a.win0 = 0;
     a.wlen = N;
+    a.in_win0 = 0;
+    a.in_wlen = N;
     a.scale = scale;
     return a;
   }
@@ -673,6 +741,56 @@ template <typename T, int N> struct FftN {
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);
   }
+  // spectrum of a field that is zero outside the real-space window
+  // [w0, w0+wl) (per axis, mod N), given as the dense window `win`
+  // (wl[0]*wl[1]*wl[2] reals): pruned z and y passes, a full x pass whose
+  // output is scaled by post[|k|^2] (if given) and stored or added into spec.
+  // bufB holds wl[0]*wl[1] padded rows, bufA wl[0]*N*(N/2+1) complex.
+  static void r2c_window(const T *win, const int *w0, const int *wl, T *bufB, T *bufA, T *spec, const T *post,
+                         bool accum, hipStream_t st, TimingHook *hook) {
+    long nrows = long(wl[0]) * wl[1];
+    int nblk = int((nrows + ROWS - 1) / ROWS);
+    size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
+    const cx<T> *tw = twtable<T>().get(N);
+    auto kern = k_fft_z_r2c_window<T, N, ROWS, THREADS>;
+    if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    std::string tag = "fft_z_r2c_win." + std::to_string(N);
+    if (hook) hook->begin(tag.c_str(), st);
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, win, bufB, nrows, w0[2], wl[2], T(1), tw);
+    GIC_HIP(hipGetLastError());
+    if (hook) hook->end(tag.c_str(), st);
+    // y pass: tiles (x_local, kz block), lines over ky from the wl[1] window rows
+    StridedArgs<T> ay{};
+    ay.in = reinterpret_cast<const cx<T> *>(bufB);
+    ay.out = reinterpret_cast<cx<T> *>(bufA);
+    ay.in_ls = NZ;
+    ay.in_os = long(wl[1]) * NZ;
+    ay.out_ls = NZ;
+    ay.out_os = long(N) * NZ;
+    ay.win0 = 0;
+    ay.wlen = N;
+    ay.in_win0 = w0[1];
+    ay.in_wlen = wl[1];
+    ay.scale = T(1);
+    strided<-1, PRE_NONE, POST_NONE, 5>(ay, wl[0], st, hook, "fft_y_fwd_win");
+    // x pass: tiles (ky, kz block), lines over kx from the wl[0] window planes
+    StridedArgs<T> ax{};
+    ax.in = reinterpret_cast<const cx<T> *>(bufA);
+    ax.out = reinterpret_cast<cx<T> *>(spec);
+    ax.in_ls = long(N) * NZ;
+    ax.in_os = NZ;
+    ax.out_ls = long(N) * NZ;
+    ax.out_os = NZ;
+    ax.win0 = 0;
+    ax.wlen = N;
+    ax.in_win0 = w0[0];
+    ax.in_wlen = wl[0];
+    ax.scale = norm();
+    ax.tab = post;
+    ax.accum = accum ? 1 : 0;
+    if (post) strided<-1, PRE_NONE, POST_TABLE, 4>(ax, N, st, hook, "fft_x_fwd_win");
+    else strided<-1, PRE_NONE, POST_NONE, 4>(ax, N, st, hook, "fft_x_fwd_win");
+  }
 };
 
 #define GIC_FFT_SWITCH(n, CALL)                                                                     \
@@ -714,6 +832,13 @@ void fft_c2r_from(const T *spec, T *out, int n, int pre, const T *tab, int comp,
   GIC_FFT_SWITCH(n, F::c2r_from(spec, out, pre, tab, comp, T(kmin), st, hook, dst))
 }
 template <typename T>
+void fft_r2c_window(const T *win, int n, const int *w0, const int *wl, T *bufB, T *bufA, T *spec, const T *post,
+                    bool accumulate, hipStream_t st, TimingHook *hook) {
+  for (int a = 0; a < 3; ++a)
+    if (wl[a] < 1 || wl[a] > n || w0[a] < 0 || w0[a] >= n) throw std::runtime_error("fft_r2c_window: bad window");
+  GIC_FFT_SWITCH(n, F::r2c_window(win, w0, wl, bufB, bufA, spec, post, accumulate, st, hook))
+}
+template <typename T>
 void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, double kmin, const int *w0, const int *wl,
                     T *bufA, T *bufB, T *out, hipStream_t st, TimingHook *hook) {
   for (int a = 0; a < 3; ++a)
@@ -727,6 +852,8 @@ void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, doubl
   template void fft_r2c_za_adjoint<T>(const T *, T *, T *, int, const T *, int, double, bool, hipStream_t,   \
                                       TimingHook *);                                                          \
   template void fft_c2r_from<T>(const T *, T *, int, int, const T *, int, double, hipStream_t, TimingHook *, T *); \
+  template void fft_r2c_window<T>(const T *, int, const int *, const int *, T *, T *, T *, const T *, bool,      \
+                                  hipStream_t, TimingHook *);                                                  \
   template void fft_c2r_window<T>(const T *, int, int, const T *, int, double, const int *, const int *, T *, T *, \
                                   T *, hipStream_t, TimingHook *);
 GIC_FFT_INST(float)

@@ -75,6 +75,165 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_final(const double *__re
   if (threadIdx.x == 0) *out = s;
 }
 
+__device__ __forceinline__ double mode_weight(int x, int y, int z, int h) {
+  if (z > 0 && z < h) return 2;
+  if (x > 0 && x < h) return 2;
+  if (x > h) return 0;
+  if (y > 0 && y < h) return 2;
+  if (y > h) return 0;
+  return 1;
+}
+__device__ __forceinline__ long mode_ksq(int x, int y, int z, int n) {
+  int h = n / 2;
+  long kx = x > h ? x - n : x, ky = y > h ? y - n : y;
+  return kx * kx + ky * ky + long(z) * z;
+}
+
+// Several weighted inner products in one pass: sum over stored modes of
+// w Re(conj(f[pa[p]]) f[pb[p]]) for each pair p (the fields are read once
+// from HBM; repeated loads of a field hit the cache)
+constexpr int MAXF = 12, MAXP = 40;
+struct MultiDot {
+  const void *f[MAXF];
+  int pa[MAXP], pb[MAXP];
+  int np;
+};
+template <typename T, int NP>
+__global__ __launch_bounds__(RED_THREADS) void k_multi_inner_partial(MultiDot md, int n, double *__restrict__ part) {
+  int h = n / 2;
+  size_t nm = size_t(n) * n * (h + 1);
+  double acc[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) acc[p] = 0;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nm; i += size_t(gridDim.x) * blockDim.x) {
+    int z = int(i % (h + 1));
+    size_t r = i / (h + 1);
+    int y = int(r % n), x = int(r / n);
+    double w = mode_weight(x, y, z, h);
+    if (w == 0) continue;
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      cx<T> u = reinterpret_cast<const cx<T> *>(md.f[md.pa[p]])[i];
+      cx<T> v = reinterpret_cast<const cx<T> *>(md.f[md.pb[p]])[i];
+      acc[p] += w * (double(u.x) * double(v.x) + double(u.y) * double(v.y));
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NP; ++p) {
+    double s = block_sum(acc[p]);
+    if (threadIdx.x == 0) part[size_t(p) * gridDim.x + blockIdx.x] = s;
+    __syncthreads();
+  }
+}
+// pair counts of nm covectors: nm (nm + 1) / 2 Gram entries plus nm right-hand sides
+template <typename T> bool launch_multi_inner(const MultiDot &md, int n, double *part, hipStream_t st) {
+  switch (md.np) {
+#define GIC_MI(NP) \
+  case NP: hipLaunchKernelGGL((k_multi_inner_partial<T, NP>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, md, n, part); return true;
+    GIC_MI(2) GIC_MI(5) GIC_MI(9) GIC_MI(14) GIC_MI(20) GIC_MI(27) GIC_MI(35)
+#undef GIC_MI
+    default:
+      return false;
+  }
+}
+// out[p] = sum of part[p*np_blocks ...], one block per pair
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_rows(const double *__restrict__ part, int nblk,
+                                                             double *__restrict__ out) {
+  double acc = 0;
+  const double *row = part + size_t(blockIdx.x) * nblk;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) acc += row[i];
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) out[blockIdx.x] = s;
+}
+
+// The chi^2 contribution of a level whose metric filters inside the window
+// of the next level: with M the window mask, u = M x and F the level's
+// filter, <x, B(A(x))> for A = 1 - M + F M, B = 1 - M + M F (two applications
+// of Field::applyFilterInWindow, field.hpp:752-775) is
+//   |x|^2 - |u|^2 + 2 F Re(u* x) - 2 F |u|^2 + F^2 |u|^2
+// summed over modes, given the spectra of x and u.
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_chi2_window_partial(const cx<T> *__restrict__ xs,
+                                                                     const cx<T> *__restrict__ us,
+                                                                     const T *__restrict__ F, int n,
+                                                                     double *__restrict__ part) {
+  int h = n / 2;
+  size_t nm = size_t(n) * n * (h + 1);
+  double acc = 0;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nm; i += size_t(gridDim.x) * blockDim.x) {
+    int z = int(i % (h + 1));
+    size_t r = i / (h + 1);
+    int y = int(r % n), x = int(r / n);
+    double w = mode_weight(x, y, z, h);
+    if (w == 0) continue;
+    double f = double(F[mode_ksq(x, y, z, n)]);
+    cx<T> a = xs[i], b = us[i];
+    double xx = double(a.x) * a.x + double(a.y) * a.y;
+    double uu = double(b.x) * b.x + double(b.y) * b.y;
+    double ux = double(b.x) * a.x + double(b.y) * a.y;
+    acc += w * (xx - uu + 2 * f * ux - 2 * f * uu + f * f * uu);
+  }
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// sum over modes of w tab[|k|^2] Re(conj(a) b)
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_inner_tab_partial(const cx<T> *__restrict__ a,
+                                                                   const cx<T> *__restrict__ b,
+                                                                   const T *__restrict__ tab, int n,
+                                                                   double *__restrict__ part) {
+  int h = n / 2;
+  size_t nm = size_t(n) * n * (h + 1);
+  double acc = 0;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nm; i += size_t(gridDim.x) * blockDim.x) {
+    int z = int(i % (h + 1));
+    size_t r = i / (h + 1);
+    int y = int(r % n), x = int(r / n);
+    double w = mode_weight(x, y, z, h);
+    if (w == 0) continue;
+    cx<T> u = a[i], v = b[i];
+    acc += w * double(tab[mode_ksq(x, y, z, n)]) * (double(u.x) * v.x + double(u.y) * v.y);
+  }
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// plain dot product of two dense real arrays
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_dot_partial(const T *__restrict__ a, const T *__restrict__ b, size_t n,
+                                                             double *__restrict__ part) {
+  double acc = 0;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += size_t(gridDim.x) * blockDim.x)
+    acc += double(a[i]) * double(b[i]);
+  double s = block_sum(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+// out = in * tab[|k|^2] over the stored modes
+template <typename T>
+__global__ void k_copy_mul_table(cx<T> *__restrict__ out, const cx<T> *__restrict__ in, int n,
+                                 const T *__restrict__ tab) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  int h = n / 2;
+  size_t nm = size_t(n) * n * (h + 1);
+  if (i >= nm) return;
+  int z = int(i % (h + 1));
+  size_t r = i / (h + 1);
+  int y = int(r % n), x = int(r / n);
+  T v = tab[mode_ksq(x, y, z, n)];
+  cx<T> a = in[i];
+  out[i] = cx<T>{a.x * v, a.y * v};
+}
+// elementwise product of two tables
+template <typename T> __global__ void k_tab_product(T *__restrict__ out, const T *__restrict__ a, long n) {
+  long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) out[i] *= a[i];
+}
+// out[m] = -coef / k^2 (0 at k = 0), k^2 = m kmin^2: the velocity operator's factor
+template <typename T> __global__ void k_tab_vel(T *__restrict__ out, double kmin, double coef, long n) {
+  long m = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (m >= n) return;
+  out[m] = m == 0 ? T(0) : T(-coef / (double(m) * kmin * kmin)) * out[m];
+}
+
 template <typename T> __global__ void k_axpy(T *__restrict__ y, const T *__restrict__ x, T s, size_t n) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) y[i] += s * x[i];
@@ -160,6 +319,20 @@ __device__ __forceinline__ bool win_contains(const WinDev &w, double x, double y
   }
   return true;
 }
+// cell centroid along one axis; the window tests of every kernel use this one
+// expression so that they agree cell for cell
+__device__ __forceinline__ double centroid1(int x, double dx, double o) { return x * dx + o + dx / 2; }
+__device__ __forceinline__ bool win_axis(const WinDev &w, int d, double p) {
+  return w.hi[d] >= w.lo[d] ? (p < w.hi[d] && p >= w.lo[d]) : (p < w.hi[d] || p >= w.lo[d]);
+}
+// per axis d, out[d*n + x] = 1 if cells with coordinate x pass the window's test on that axis
+__global__ void k_axis_member(int n, double dx, double ox, double oy, double oz, WinDev w, uint8_t *__restrict__ out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 3 * n) return;
+  int d = t / n, x = t - d * n;
+  double o = d == 0 ? ox : d == 1 ? oy : oz;
+  out[t] = win_axis(w, d, centroid1(x, dx, o)) ? 1 : 0;
+}
 // zero the cells (padded real layout) whose centroid is inside / outside w
 template <typename T>
 __global__ void k_window_zero(T *__restrict__ f, int n, double dx, double ox, double oy, double oz, WinDev w,
@@ -170,7 +343,7 @@ __global__ void k_window_zero(T *__restrict__ f, int n, double dx, double ox, do
   int z = int(i % n);
   size_t r = i / n;
   int y = int(r % n), x = int(r / n);
-  bool c = win_contains(w, x * dx + ox + dx / 2, y * dx + oy + dx / 2, z * dx + oz + dx / 2);
+  bool c = win_contains(w, centroid1(x, dx, ox), centroid1(y, dx, oy), centroid1(z, dx, oz));
   if (c == (inside != 0)) f[(size_t(x) * n + y) * (n + 2) + z] = T(0);
 }
 
@@ -259,6 +432,38 @@ __global__ void k_fine_to_coarse(T *__restrict__ coarse, const T *__restrict__ f
   coarse[(size_t(X) * g.nc + Y) * (g.nc + 2) + Z] += sum / T(cnt);
 }
 
+// The same average for the coarse cells of a box [b0, b0+bl) (mod nc), into
+// the dense box buffer (bl[0]*bl[1]*bl[2]); cells outside the box have no
+// fine subcells.
+struct Box3 {
+  int b0[3], bl[3];
+};
+template <typename T>
+__global__ void k_fine_to_coarse_box(T *__restrict__ box, const T *__restrict__ fine, AvgGeom g, Box3 B) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t nb = size_t(B.bl[0]) * B.bl[1] * B.bl[2];
+  if (i >= nb) return;
+  int k = int(i % B.bl[2]);
+  size_t rr = i / B.bl[2];
+  int j = int(rr % B.bl[1]), l = int(rr / B.bl[1]);
+  int X = (B.b0[0] + l) % g.nc, Y = (B.b0[1] + j) % g.nc, Z = (B.b0[2] + k) % g.nc;
+  int u[3];
+  int r = g.r;
+  bool c0 = sub_under(g, X * r, Y * r, Z * r, u);
+  bool c1 = sub_under(g, X * r + r - 1, Y * r + r - 1, Z * r + r - 1, u);
+  if (!c0 && !c1) return;
+  T sum = T(0);
+  int cnt = 0;
+  for (int a = 0; a < r; ++a)
+    for (int b = 0; b < r; ++b)
+      for (int c = 0; c < r; ++c)
+        if (sub_under(g, X * r + a, Y * r + b, Z * r + c, u)) {
+          sum += fine[(size_t(u[0]) * g.nf + u[1]) * (g.nf + 2) + u[2]];
+          ++cnt;
+        }
+  box[i] += sum / T(cnt);
+}
+
 // Field::deInterpolate of every fine cell (value / r^3) into the coarse grid,
 // accumulated in the reference's contiguous real layout (n^3 cells plus the
 // tail the reference's vector has): coordinates wrap by the grid size only
@@ -276,7 +481,7 @@ __device__ __forceinline__ double crw(int i, double t) {
 }
 template <typename T, typename S>
 __global__ void k_deinterp(const S *__restrict__ fine, int fine_padded, DeinterpGeom g, T *__restrict__ acc,
-                           size_t acc_len) {
+                           size_t acc_len, size_t acc_lo, unsigned *__restrict__ dropped) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   size_t nf3 = size_t(g.nf) * g.nf * g.nf;
   if (i >= nf3) return;
@@ -324,7 +529,10 @@ __global__ void k_deinterp(const S *__restrict__ fine, int fine_padded, Deinterp
     for (int b = 0; b < 4; ++b)
       for (int c = 0; c < 4; ++c) {
         long idx = kidx + (ix[0][a] - ix[0][0]) * n * n + (ix[1][b] - ix[1][0]) * n + (ix[2][c] - ix[2][0]);
-        if (idx >= 0 && size_t(idx) < acc_len) atomicAdd(&acc[idx], T(val * (wx[a] * wy[b] * wz[c])));
+        // acc holds the contiguous entries [acc_lo, acc_lo + acc_len)
+        long rel = idx - long(acc_lo);
+        if (rel >= 0 && size_t(rel) < acc_len) atomicAdd(&acc[rel], T(val * (wx[a] * wy[b] * wz[c])));
+        else if (dropped && idx >= 0 && size_t(idx) < size_t(g.nc) * g.nc * g.nc) atomicAdd(dropped, 1u);
       }
 }
 
@@ -633,6 +841,112 @@ template <typename T> double inner(gic_context *c, MLF<T> &a, MLF<T> &b) {
   return total;
 }
 
+// FP64 reductions: launch a partial kernel into part, finish in scalar[slot]
+double finish_reduction(gic_context *c, int slot) {
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, c->st, (const double *)c->mod->part.p, RED_BLOCKS,
+                     (double *)c->mod->scalar.p + slot);
+  GIC_HIP(hipGetLastError());
+  double v;
+  GIC_HIP(hipMemcpyAsync(&v, (double *)c->mod->scalar.p + slot, sizeof(double), hipMemcpyDeviceToHost, c->st));
+  GIC_HIP(hipStreamSynchronize(c->st));
+  return v;
+}
+void ensure_red(gic_context *c) {
+  c->mod->part.ensure(size_t(RED_BLOCKS) * MAXP * sizeof(double));
+  c->mod->scalar.ensure(sizeof(double) * MAXP);
+}
+
+// a cyclic run [start, start+len) (mod n) of the indices where member[] is 1;
+// false if the members are not one run
+bool cyclic_run(const uint8_t *member, int n, int &start, int &len) {
+  int cnt = 0;
+  for (int i = 0; i < n; ++i) cnt += member[i];
+  if (cnt == 0) return false;
+  if (cnt == n) {
+    start = 0;
+    len = n;
+    return true;
+  }
+  int s = -1;
+  for (int i = 0; i < n; ++i)
+    if (member[i] && !member[(i + n - 1) % n]) {
+      if (s >= 0) return false;   // two runs
+      s = i;
+    }
+  start = s;
+  len = cnt;
+  for (int k = 0; k < len; ++k)
+    if (!member[(s + k) % n]) return false;
+  return true;
+}
+
+// The box of cells of level g whose centroids lie in window w (the cells
+// k_window_zero treats as inside), per axis a cyclic run; false if empty or
+// not a single run on some axis.
+bool window_box(gic_context *c, const LevelGeom &g, const WinDev &w, Box3 &B) {
+  DevBuf mem;
+  mem.ensure(size_t(3) * g.n);
+  hipLaunchKernelGGL(k_axis_member, dim3((3 * g.n + 255) / 256), dim3(256), 0, c->st, g.n, g.dx, g.ox, g.oy, g.oz, w,
+                     (uint8_t *)mem.p);
+  GIC_HIP(hipGetLastError());
+  std::vector<uint8_t> h(size_t(3) * g.n);
+  GIC_HIP(hipMemcpyAsync(h.data(), mem.p, h.size(), hipMemcpyDeviceToHost, c->st));
+  GIC_HIP(hipStreamSynchronize(c->st));
+  for (int d = 0; d < 3; ++d)
+    if (!cyclic_run(h.data() + size_t(d) * g.n, g.n, B.b0[d], B.bl[d])) return false;
+  return true;
+}
+
+AvgGeom avg_geom(const LevelGeom &gt, const LevelGeom &gs) {
+  AvgGeom a;
+  a.r = (int)llround(gt.dx / gs.dx);
+  a.cell_off[0] = (int)llround((gt.ox - gs.ox) / gs.dx);
+  a.cell_off[1] = (int)llround((gt.oy - gs.oy) / gs.dx);
+  a.cell_off[2] = (int)llround((gt.oz - gs.oz) / gs.dx);
+  a.simf = (int)llround(gs.periodic / gs.dx);
+  a.nsec = gt.n * a.r;
+  a.nf = gs.n;
+  a.nc = gt.n;
+  return a;
+}
+// coarse cells of level gt with subcells on the finer level gs (the cells
+// k_fine_to_coarse writes), per axis; the same integer tests on the host
+bool fine_footprint(const LevelGeom &gt, const LevelGeom &gs, std::vector<uint8_t> &member) {
+  AvgGeom a = avg_geom(gt, gs);
+  member.assign(size_t(3) * gt.n, 0);
+  for (int d = 0; d < 3; ++d)
+    for (int X = 0; X < gt.n; ++X)
+      for (int q = 0; q < a.r; ++q) {
+        int sidx = X * a.r + q;
+        if (sidx < 0 || sidx >= a.nsec) continue;
+        int v = sidx + a.cell_off[d];
+        if (v > a.simf - 1) v -= a.simf;
+        if (v < 0) v += a.simf;
+        if (v >= 0 && v < a.nf) {
+          member[size_t(d) * gt.n + X] = 1;
+          break;
+        }
+      }
+  return true;
+}
+// the box of level lt receiving averages from every finer level
+bool fine_box(gic_context *c, size_t lt, Box3 &B) {
+  const LevelGeom &gt = c->levels[lt]->g;
+  std::vector<uint8_t> un(size_t(3) * gt.n, 0), m;
+  bool any = false;
+  for (size_t s = lt + 1; s < c->levels.size(); ++s) {
+    fine_footprint(gt, c->levels[s]->g, m);
+    for (size_t i = 0; i < un.size(); ++i) un[i] |= m[i];
+    any = true;
+  }
+  if (!any) return false;
+  for (int d = 0; d < 3; ++d)
+    if (!cyclic_run(un.data() + size_t(d) * gt.n, gt.n, B.b0[d], B.bl[d])) return false;
+  return true;
+}
+size_t box_cells(const Box3 &B) { return size_t(B.bl[0]) * B.bl[1] * B.bl[2]; }
+bool box_pruned(const Box3 &B, int n) { return B.bl[0] < n || B.bl[1] < n || B.bl[2] < n; }
+
 WinDev grid_window(const LevelGeom &g) {   // Grid::getWindow (grid.hpp:502-504)
   WinDev w;
   w.L = g.periodic;
@@ -742,34 +1056,88 @@ template <typename T> void add_transfer(gic_context *c, T *R, size_t lt, MLF<T> 
   GIC_HIP(hipGetLastError());
 }
 
+// The cross-level terms of level l for a field f (Fourier on every level):
+// sum over s != l of the level-s field filtered by F_l F_s sqrt(w_l/w_s) and
+// carried to level l (FILTER_ON_COARSE_GRID).  When every other level is
+// finer and level l covers the box, they only reach the box of cells under
+// the finer grids: returned there as a dense box (and true); otherwise in R
+// as a full padded real field of level l.
+template <typename T> bool cross_terms(gic_context *c, MLF<T> &f, size_t l, PBuf &Rbox, Box3 &B, MLF<T> &R) {
+  size_t nl = f.nl();
+  const LevelGeom &g = c->levels[l]->g;
+  bool box = l == 0 && g.covers_full() && fine_box(c, l, B) && box_pruned(B, g.n);
+  if (box) {
+    size_t nb = box_cells(B);
+    Rbox = PBuf(c->mod->pool, nb * sizeof(T));
+    fill<T>((T *)Rbox.p, nb, T(0), c->st);
+    for (size_t s = l + 1; s < nl; ++s) {
+      const LevelGeom &gs = c->levels[s]->g;
+      double sq = sqrt(level_weight(c, l) / level_weight(c, s));
+      f.toF(s);
+      const T *tab = spec_table<T>(c, gs, spec_levels(c, long(l), long(s), sq));
+      PBuf tmp(c->mod->pool, gs.padded_reals() * sizeof(T));
+      fft_c2r_from<T>(f.p(s), (T *)tmp.p, gs.n, FFT_PRE_TABLE, tab, 0, gs.kmin(), c->st, c->hook());
+      hipLaunchKernelGGL(k_fine_to_coarse_box<T>, dim3(blocks(nb)), dim3(256), 0, c->st, (T *)Rbox.p,
+                         (const T *)tmp.p, avg_geom(g, gs), B);
+      GIC_HIP(hipGetLastError());
+    }
+    return true;
+  }
+  R.zero(l);
+  for (size_t s = 0; s < nl; ++s) {
+    if (s == l) continue;
+    double sq = sqrt(level_weight(c, l) / level_weight(c, s));
+    add_transfer<T>(c, R.p(l), l, f, s, spec_levels(c, long(l), long(s), sq));
+  }
+  return false;
+}
+
+// scratch for the pruned transforms of a box on a level of size n
+template <typename T> struct BoxBufs {
+  PBuf a, b;
+  BoxBufs(gic_context *c, int n, const Box3 &B) {
+    size_t nz = size_t(n / 2 + 1);
+    a = PBuf(c->mod->pool, size_t(B.bl[0]) * n * nz * 2 * sizeof(T));
+    b = PBuf(c->mod->pool, size_t(B.bl[0]) * std::max(B.bl[1], n) * nz * 2 * sizeof(T));
+  }
+};
+
 // applyMetric (multilevelfield.hpp:335-384); the cross-level terms of a
-// level are gathered in one real buffer and added in Fourier space
+// level are gathered in one real buffer (or box) and added in Fourier space
 template <typename T> MLF<T> metric(MLF<T> &f, bool to_covector) {
   gic_context *c = f.c;
   size_t nl = f.nl();
   f.toF();
   MLF<T> out(c);
   for (size_t l = 0; l < nl; ++l) {
-    out.copy_level(f, l);
+    const LevelGeom &g = c->levels[l]->g;
     if (to_covector && l + 1 < nl) {
+      out.copy_level(f, l);
       WinDev w = grid_window(c->levels[l + 1]->g);
       FilterSpec fl = spec_levels(c, long(l), -1, 1.0);
       filter_in_window(out, l, fl, w, true);
       filter_in_window(out, l, fl, w, false);
       out.toF(l);
     } else {
-      apply_spec(out, l, spec_levels(c, long(l), long(l), 1.0));
+      // out = F_l^2 f in one pass
+      const T *tab = spec_table<T>(c, g, spec_levels(c, long(l), long(l), 1.0));
+      size_t nm = size_t(g.n) * g.n * (g.n / 2 + 1);
+      hipLaunchKernelGGL(k_copy_mul_table<T>, dim3(blocks(nm)), dim3(256), 0, c->st, reinterpret_cast<cx<T> *>(out.p(l)),
+                         reinterpret_cast<const cx<T> *>(f.p(l)), g.n, tab);
+      out.fourier[l] = true;
     }
     if (nl > 1) {
+      PBuf Rbox;
+      Box3 B;
       MLF<T> R(c);
-      R.zero(l);
-      for (size_t s = 0; s < nl; ++s) {
-        if (s == l) continue;
-        double sq = sqrt(level_weight(c, l) / level_weight(c, s));
-        add_transfer<T>(c, R.p(l), l, f, s, spec_levels(c, long(l), long(s), sq));
+      if (cross_terms<T>(c, f, l, Rbox, B, R)) {
+        BoxBufs<T> bb(c, g.n, B);
+        fft_r2c_window<T>((const T *)Rbox.p, g.n, B.b0, B.bl, (T *)bb.b.p, (T *)bb.a.p, out.p(l), nullptr, true, c->st,
+                          c->hook());
+      } else {
+        R.toF(l);
+        out.axpy_level(l, R.p(l), 1.0);
       }
-      R.toF(l);
-      out.axpy_level(l, R.p(l), 1.0);
     }
   }
   return out;
@@ -868,7 +1236,6 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
     A.b = -2. / 3. / norm;
   }
   MLF<T> m(c);
-  m.zero();
   {
     DevList dl;
     upload_list(c, cells, dl);
@@ -880,17 +1247,50 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
                        (const double *)dense.p, g.n);
     GIC_HIP(hipStreamSynchronize(c->st));   // the cell list is released here
   }
+  // the factors every level gets after deinterpolation, as one |k|^2 table:
+  // weight w_l / w_L, the level filter, the transfer to dark matter and, for
+  // velocities and angular momenta, -coef / k^2
+  const Cosmo &cc = c->cosmo;
+  double coef = kind == 1 ? 3. / 2. * cc.Om / cc.a * 100. * 100. / (299792.) / (299792.)
+                          : 1.0 * 100. * sqrt(cc.Om / (cc.a * cc.a * cc.a) + cc.Ol) * sqrt(cc.a);
+  auto level_table = [&](size_t l, DevBuf &tab) {
+    const LevelGeom &gl = m.g(l);
+    size_t nent = table_len(gl);
+    double sw = level_weight(c, l) / level_weight(c, nl - 1);
+    FilterSpec fs = spec_levels(c, long(l), -1, sw);
+    tab.ensure(nent * sizeof(T));
+    hipLaunchKernelGGL(k_filter_spec<T>, dim3(blocks(nent)), dim3(256), 0, c->st, fs, gl.kmin(), long(nent),
+                       (T *)tab.p);
+    Level &Lv = *c->levels[l];
+    build_sqrtp_dm<T>(c, Lv);
+    if (!c->power_applied) {
+      hipLaunchKernelGGL(k_tab_product<T>, dim3(blocks(nent)), dim3(256), 0, c->st, (T *)tab.p,
+                         (const T *)Lv.sqrtp_dm.p, long(nent));
+    } else {
+      build_sqrtp<T>(c, Lv);
+      DevBuf ratio;
+      ratio.ensure(nent * sizeof(T));
+      hipLaunchKernelGGL(k_ratio_table<T>, dim3(blocks(nent)), dim3(256), 0, c->st, (const T *)Lv.sqrtp.p,
+                         (const T *)Lv.sqrtp_dm.p, long(nent), (T *)ratio.p);
+      hipLaunchKernelGGL(k_tab_product<T>, dim3(blocks(nent)), dim3(256), 0, c->st, (T *)tab.p, (const T *)ratio.p,
+                         long(nent));
+      GIC_HIP(hipStreamSynchronize(c->st));
+    }
+    if (kind >= 2)
+      hipLaunchKernelGGL(k_tab_vel<T>, dim3(blocks(nent)), dim3(256), 0, c->st, (T *)tab.p, gl.kmin(), coef,
+                         long(nent));
+    GIC_HIP(hipGetLastError());
+  };
+  std::vector<bool> done(nl, false);
   // deinterpolate down the levels, one level at a time, in the contiguous layout
   if (L > 0) {
     PBuf accs[2];
     const T *src = m.p(L);
     int src_padded = 1;
+    DevBuf dropped;
+    dropped.ensure(sizeof(unsigned));
     for (long l = long(L) - 1; l >= 0; --l) {
       const LevelGeom &gh = m.g(size_t(l) + 1), &gl = m.g(size_t(l));
-      size_t len = gl.n3() + 2 * size_t(gl.n) * gl.n;
-      PBuf &acc = accs[l & 1];
-      acc = PBuf(c->mod->pool, len * sizeof(T));
-      fill<T>((T *)acc.p, len, T(0), c->st);
       DeinterpGeom dg;
       dg.nf = gh.n;
       dg.nc = gl.n;
@@ -907,33 +1307,69 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
       double r = double(llround(gl.dx / gh.dx));
       dg.inv_vol = 1.0 / (r * r * r);
       size_t nf3 = gh.n3();
+      size_t nn = size_t(gl.n) * gl.n;
+      if (l == 0 && gl.covers_full()) {
+        // the base level: the finer grid's footprint (with the stencil's
+        // reach) is a range of x planes; deinterpolate into those planes
+        // only and transform them with a pruned r2c that also applies the
+        // level's factors.  Any value outside the range sends us back to
+        // the whole-level path.
+        long x_lo = long(std::floor((gh.ox - gl.ox) / gl.dx)) - 3;
+        long x_hi = long(std::ceil((gh.ox + gh.n * gh.dx - gl.ox) / gl.dx)) + 3;
+        if (x_lo >= 0 && x_hi < gl.n - 1 && x_hi - x_lo + 1 < gl.n) {
+          int wl0 = int(x_hi - x_lo + 1);
+          size_t len = size_t(wl0) * nn;
+          PBuf win(c->mod->pool, len * sizeof(T));
+          fill<T>((T *)win.p, len, T(0), c->st);
+          fill<unsigned>((unsigned *)dropped.p, 1, 0u, c->st);
+          hipLaunchKernelGGL((k_deinterp<T, T>), dim3(blocks(nf3)), dim3(256), 0, c->st, src, src_padded, dg,
+                             (T *)win.p, len, size_t(x_lo) * nn, (unsigned *)dropped.p);
+          GIC_HIP(hipGetLastError());
+          unsigned nd = 0;
+          GIC_HIP(hipMemcpyAsync(&nd, dropped.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
+          GIC_HIP(hipStreamSynchronize(c->st));
+          if (nd == 0) {
+            DevBuf tab;
+            level_table(0, tab);
+            int w0[3] = {int(x_lo), 0, 0}, wl[3] = {wl0, gl.n, gl.n};
+            Box3 B{{w0[0], 0, 0}, {wl[0], wl[1], wl[2]}};
+            BoxBufs<T> bb(c, gl.n, B);
+            fft_r2c_window<T>((const T *)win.p, gl.n, w0, wl, (T *)bb.b.p, (T *)bb.a.p, m.p(0), (const T *)tab.p,
+                              false, c->st, c->hook());
+            m.fourier[0] = true;
+            done[0] = true;
+            GIC_HIP(hipStreamSynchronize(c->st));
+            continue;
+          }
+        }
+      }
+      size_t len = gl.n3() + 2 * nn;
+      PBuf &acc = accs[l & 1];
+      acc = PBuf(c->mod->pool, len * sizeof(T));
+      fill<T>((T *)acc.p, len, T(0), c->st);
       hipLaunchKernelGGL((k_deinterp<T, T>), dim3(blocks(nf3)), dim3(256), 0, c->st, src, src_padded, dg, (T *)acc.p,
-                         len);
+                         len, size_t(0), (unsigned *)nullptr);
       GIC_HIP(hipGetLastError());
       pad<T, T>((const T *)acc.p, gl.n, m.p(size_t(l)), c->st);
       src = (const T *)acc.p;   // the next level reads this level's contiguous values
       src_padded = 0;
     }
   }
+  // every other level: transform, then the level's factors in one pass
   for (size_t l = 0; l < nl; ++l) {
-    double s = level_weight(c, l) / level_weight(c, nl - 1);
-    if (s != 1.0) {
-      size_t n = m.reals(l);
-      hipLaunchKernelGGL(k_scale<T>, dim3(blocks(n)), dim3(256), 0, c->st, m.p(l), T(s), n);
-    }
+    if (done[l]) continue;
+    DevBuf tab;
+    level_table(l, tab);
+    m.toF(l);
+    mul_table<T>(m.p(l), m.g(l).n, (const T *)tab.p, c->st);
+    GIC_HIP(hipStreamSynchronize(c->st));
   }
-  for (size_t l = 0; l < nl; ++l) apply_spec(m, l, spec_levels(c, long(l), -1, 1.0));
-  transfer_to_dm(m);
-  if (kind >= 1) {
-    const Cosmo &cc = c->cosmo;
-    double coef = kind == 1 ? 3. / 2. * cc.Om / cc.a * 100. * 100. / (299792.) / (299792.)
-                            : 1.0 * 100. * sqrt(cc.Om / (cc.a * cc.a * cc.a) + cc.Ol) * sqrt(cc.a);
-    int op = kind == 1 ? 1 : 2;
+  if (kind == 1) {
     for (size_t l = 0; l < nl; ++l) {
       const LevelGeom &gg = m.g(l);
       size_t nm = size_t(gg.n) * gg.n * (gg.n / 2 + 1);
       hipLaunchKernelGGL(k_mode_op<T>, dim3(blocks(nm)), dim3(256), 0, c->st, reinterpret_cast<cx<T> *>(m.p(l)),
-                         gg.n, gg.kmin(), op, coef);
+                         gg.n, gg.kmin(), 1, coef);
     }
   }
   GIC_HIP(hipGetLastError());
@@ -1003,6 +1439,77 @@ template <typename T> double variance_value(MLF<T> &field, const std::vector<std
   return inner(field.c, p, field);
 }
 
+// <w|M|w>, the chi^2 of getChi2 (multilevelfield.hpp:461-487), for a field
+// in Fourier space.  Only the scalar is needed, so no covector is formed:
+// per level, the windowed filter term reduces to one pass over the spectra
+// of x and of its window u = M x (k_chi2_window_partial), the last level's
+// term to sum F^2 |x|^2, and the cross terms to a dot product with x in
+// real space on the box they reach (or in Fourier space on a zoom level).
+// Falls back to the metric when a window is not a single box.
+template <typename T> double chi2_of(gic_context *c, MLF<T> &field) {
+  size_t nl = field.nl();
+  field.toF();
+  ensure_red(c);
+  std::vector<Box3> wins(nl);
+  for (size_t l = 0; l + 1 < nl; ++l)
+    if (!window_box(c, c->levels[l]->g, grid_window(c->levels[l + 1]->g), wins[l]) ||
+        !box_pruned(wins[l], c->levels[l]->g.n)) {
+      MLF<T> cov = metric(field, true);
+      return inner(c, cov, field);
+    }
+  double total = 0;
+  for (size_t l = 0; l < nl; ++l) {
+    const LevelGeom &g = c->levels[l]->g;
+    const cx<T> *xs = reinterpret_cast<const cx<T> *>(field.p(l));
+    if (l + 1 < nl) {
+      const Box3 &M = wins[l];
+      size_t nb = box_cells(M), nz = size_t(g.n / 2 + 1);
+      PBuf ubox(c->mod->pool, nb * sizeof(T));
+      {
+        PBuf A(c->mod->pool, size_t(M.bl[0]) * g.n * nz * 2 * sizeof(T));
+        PBuf Bb(c->mod->pool, size_t(M.bl[0]) * M.bl[1] * nz * 2 * sizeof(T));
+        fft_c2r_window<T>(field.p(l), g.n, FFT_PRE_NONE, nullptr, 0, g.kmin(), M.b0, M.bl, (T *)A.p, (T *)Bb.p,
+                          (T *)ubox.p, c->st, c->hook());
+      }
+      PBuf uspec(c->mod->pool, g.padded_reals() * sizeof(T));
+      {
+        BoxBufs<T> bb(c, g.n, M);
+        fft_r2c_window<T>((const T *)ubox.p, g.n, M.b0, M.bl, (T *)bb.b.p, (T *)bb.a.p, (T *)uspec.p, nullptr, false,
+                          c->st, c->hook());
+      }
+      const T *F = spec_table<T>(c, g, spec_levels(c, long(l), -1, 1.0));
+      hipLaunchKernelGGL(k_chi2_window_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, xs,
+                         reinterpret_cast<const cx<T> *>(uspec.p), F, g.n, (double *)c->mod->part.p);
+    } else {
+      const T *F2 = spec_table<T>(c, g, spec_levels(c, long(l), long(l), 1.0));
+      hipLaunchKernelGGL(k_inner_tab_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, xs, xs, F2, g.n,
+                         (double *)c->mod->part.p);
+    }
+    total += finish_reduction(c, 0);
+    if (nl > 1) {
+      PBuf Rbox;
+      Box3 B;
+      MLF<T> R(c);
+      if (cross_terms<T>(c, field, l, Rbox, B, R)) {
+        size_t nb = box_cells(B), nz = size_t(g.n / 2 + 1);
+        PBuf xbox(c->mod->pool, nb * sizeof(T));
+        PBuf A(c->mod->pool, size_t(B.bl[0]) * g.n * nz * 2 * sizeof(T));
+        PBuf Bb(c->mod->pool, size_t(B.bl[0]) * B.bl[1] * nz * 2 * sizeof(T));
+        fft_c2r_window<T>(field.p(l), g.n, FFT_PRE_NONE, nullptr, 0, g.kmin(), B.b0, B.bl, (T *)A.p, (T *)Bb.p,
+                          (T *)xbox.p, c->st, c->hook());
+        hipLaunchKernelGGL(k_dot_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, (const T *)xbox.p,
+                           (const T *)Rbox.p, nb, (double *)c->mod->part.p);
+      } else {
+        R.toF(l);
+        hipLaunchKernelGGL(k_inner_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, xs,
+                           reinterpret_cast<const cx<T> *>(R.p(l)), g.n, (double *)c->mod->part.p);
+      }
+      total += finish_reduction(c, 0);
+    }
+  }
+  return total;
+}
+
 template <typename T> void ensure_ready(gic_context *c) {
   if (c->levels.empty()) throw std::runtime_error("No grid has yet been initialised");
   if (!c->mod) c->mod = std::make_unique<ModState>();
@@ -1054,8 +1561,7 @@ template <typename T> double mod_chi2(gic_context *c) {
   if (c->power_applied) throw std::runtime_error("Cannot calculate the chi^2 after the output field has been generated");
   MLF<T> field = copy_output<T>(c);
   field.toF();
-  MLF<T> cov = metric(field, true);
-  return inner(c, cov, field);
+  return chi2_of(c, field);
 }
 
 size_t num_dof(gic_context *c) {
@@ -1073,14 +1579,21 @@ size_t num_dof(gic_context *c) {
 // ModificationManager::applyModifications (modificationmanager.hpp:80-331)
 template <typename T> void mod_apply(gic_context *c, double *pre, double *post) {
   ensure_ready<T>(c);
-  double chi_pre = mod_chi2<T>(c);
+  double chi_pre;
+  {
+    Phase ph(c, "phase:chi2_pre");
+    chi_pre = mod_chi2<T>(c);
+  }
   MLF<T> field = copy_output<T>(c);
   field.toF();
   std::vector<MLF<T>> alphas;
   std::vector<double> targets;
-  for (auto &m : c->mod->linear) {
-    alphas.push_back(linear_covector<T>(c, m.name, m.flags));
-    targets.push_back(m.target);
+  {
+    Phase ph(c, "phase:covectors");
+    for (auto &m : c->mod->linear) {
+      alphas.push_back(linear_covector<T>(c, m.name, m.flags));
+      targets.push_back(m.target);
+    }
   }
   auto cov_dot = [&](MLF<T> &a, MLF<T> &b) {
     MLF<T> bv = metric(b, false);
@@ -1093,11 +1606,60 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
   // factor L of G, and the correction it adds is sum_i v_i [G^-1 (d - <a, w>)]_i.
   size_t nm = alphas.size();
   std::vector<MLF<T>> vecs;
-  for (auto &a : alphas) vecs.push_back(metric(a, false));
+  {
+    Phase ph(c, "phase:metric_vectors");
+    for (auto &a : alphas) vecs.push_back(metric(a, false));
+  }
   std::vector<double> G(nm * nm), L(nm * nm, 0.0), rhs(nm);
-  for (size_t i = 0; i < nm; ++i) {
-    for (size_t j = 0; j <= i; ++j) G[i * nm + j] = G[j * nm + i] = inner(c, alphas[i], vecs[j]);
-    rhs[i] = targets[i] - inner(c, alphas[i], field);
+  {
+    Phase ph(c, "phase:gram");
+    // every <alpha_i, v_j> (j <= i) and <alpha_i, w> in one pass per level
+    size_t npair = nm * (nm + 1) / 2 + nm;
+    if (nm > 0 && nm <= 7 && 2 * nm + 1 <= size_t(MAXF) && npair <= size_t(MAXP)) {
+      field.toF();
+      for (size_t i = 0; i < nm; ++i) {
+        alphas[i].toF();
+        vecs[i].toF();
+      }
+      ensure_red(c);
+      std::vector<double> acc(npair, 0.0), part(npair);
+      for (size_t l = 0; l < field.nl(); ++l) {
+        MultiDot md{};
+        for (size_t i = 0; i < nm; ++i) {
+          md.f[i] = alphas[i].p(l);
+          md.f[nm + i] = vecs[i].p(l);
+        }
+        md.f[2 * nm] = field.p(l);
+        int p = 0;
+        for (size_t i = 0; i < nm; ++i) {
+          for (size_t j = 0; j <= i; ++j) {
+            md.pa[p] = int(i);
+            md.pb[p++] = int(nm + j);
+          }
+          md.pa[p] = int(i);
+          md.pb[p++] = int(2 * nm);
+        }
+        md.np = p;
+        if (!launch_multi_inner<T>(md, field.g(l).n, (double *)c->mod->part.p, c->st))
+          throw std::runtime_error("internal: no batched inner product for this many pairs");
+        hipLaunchKernelGGL(k_reduce_rows, dim3(p), dim3(RED_THREADS), 0, c->st, (const double *)c->mod->part.p,
+                           RED_BLOCKS, (double *)c->mod->scalar.p);
+        GIC_HIP(hipGetLastError());
+        GIC_HIP(hipMemcpyAsync(part.data(), c->mod->scalar.p, p * sizeof(double), hipMemcpyDeviceToHost, c->st));
+        GIC_HIP(hipStreamSynchronize(c->st));
+        for (int q = 0; q < p; ++q) acc[q] += part[q];
+      }
+      int p = 0;
+      for (size_t i = 0; i < nm; ++i) {
+        for (size_t j = 0; j <= i; ++j) G[i * nm + j] = G[j * nm + i] = acc[p++];
+        rhs[i] = targets[i] - acc[p++];
+      }
+    } else {
+      for (size_t i = 0; i < nm; ++i) {
+        for (size_t j = 0; j <= i; ++j) G[i * nm + j] = G[j * nm + i] = inner(c, alphas[i], vecs[j]);
+        rhs[i] = targets[i] - inner(c, alphas[i], field);
+      }
+    }
   }
   for (size_t i = 0; i < nm; ++i) {   // G = L L^T
     for (size_t j = 0; j <= i; ++j) {
@@ -1122,7 +1684,10 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
     for (size_t k = ii + 1; k < nm; ++k) v -= L[k * nm + ii] * coef[k];
     coef[ii] = v / L[ii * nm + ii];
   }
-  for (size_t i = 0; i < nm; ++i) field.axpy(vecs[i], coef[i]);
+  {
+    Phase ph(c, "phase:update");
+    for (size_t i = 0; i < nm; ++i) field.axpy(vecs[i], coef[i]);
+  }
   alphas.clear();
   if (!c->mod->quadratic.empty() && nm) {
     // the orthonormal vectors of the linear family: rows of L^-1 applied to v
@@ -1198,7 +1763,11 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
     iterate(field, nsteps > init ? nsteps : init);
   }
   store_output<T>(c, field);
-  double chi_post = mod_chi2<T>(c);
+  double chi_post;
+  {
+    Phase ph(c, "phase:chi2_post");
+    chi_post = mod_chi2<T>(c);
+  }
   if (pre) *pre = chi_pre;
   if (post) *post = chi_post;
 }
