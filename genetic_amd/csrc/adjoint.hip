@@ -85,8 +85,8 @@ template <typename T> void do_displacement_forward(gic_context *c, const T *cons
     filter_table<T>(2, 0.0, L.g.kmin(), nent, (T *)c->tab_c.p, c->st, true);
     alloc_field(c, L.psi[0], L.g);
     for (int a = 0; a < 3; ++a)
-      fft_c2r_from<T>((const T *)L.noise.p, (T *)L.psi[0].p, L.g.n, FFT_PRE_ZA, (const T *)c->tab_c.p, a, L.g.kmin(),
-                      c->st, c->hook(), psi[a]);
+      fft_c2r_from<T>((const T *)L.noise.p, (T *)L.psi[0].p, L.g.n, FFT_PRE_ZA, nullptr, a, L.g.kmin(), c->st,
+                      c->hook(), psi[a], 1.0);
   } else {
     int order = c->lpt_order;
     c->lpt_order = 1;

@@ -274,27 +274,32 @@ template <typename T> void do_second_order(gic_context *c) {
     filter_table<T>(2, 0.0, g.kmin(), nent, (T *)c->tab_a.p, c->st, true);
     const T *inv_k2 = (const T *)c->tab_a.p;
     T *A = (T *)L.psi2[0].p, *B = (T *)L.psi2[1].p;
+    (void)inv_k2;   // 1/k^2 is formed inside the transforms
     auto phi = [&](T *out, int a, int b) {
-      fft_c2r_from<T>(spec, out, n, FFT_PRE_KK, inv_k2, a + 3 * b, g.kmin(), c->st, c->hook());
+      fft_c2r_from<T>(spec, out, n, FFT_PRE_KK, nullptr, a + 3 * b, g.kmin(), c->st, c->hook(), nullptr, 1.0);
     };
     if (c->hook()) c->hook()->begin("lpt_source", c->st);
+    // each phi_ab is folded into S by its transform's last pass
+    auto fold = [&](int a, int b, int epi) {
+      fft_c2r_lpt<T>(spec, B, n, nullptr, a + 3 * b, g.kmin(), epi, S, A, c->st, c->hook(), 1.0);
+    };
     phi(A, 0, 0);
-    phi(B, 1, 1);
-    lpt_accum<T>(S, A, B, reals, 0, c->st);   // S = xx yy
-    lpt_accum<T>(S, A, B, reals, 3, c->st);   // A = xx + yy
-    phi(B, 2, 2);
-    lpt_accum<T>(S, A, B, reals, 1, c->st);   // S += (xx + yy) zz
-    const int off[3][2] = {{0, 1}, {0, 2}, {1, 2}};
-    for (auto &o : off) {
-      phi(B, o[0], o[1]);
-      lpt_accum<T>(S, B, B, reals, 2, c->st);   // S -= xy^2, xz^2, yz^2
-    }
+    fold(1, 1, 1);   // S = xx yy, A = xx + yy
+    fold(2, 2, 2);   // S += (xx + yy) zz
+    fold(0, 1, 3);   // S -= xy^2
+    fold(0, 2, 3);   // S -= xz^2
+    fold(1, 2, 3);   // S -= yz^2
     if (c->hook()) c->hook()->end("lpt_source", c->st);
     fft3d<T>(S, n, -1, c->st, c->hook());
     double kc = l >= 1 ? kcut_for_level(c, int(l) - 1) : 0.0;
     filter_table<T>(l >= 1 ? 1 : 2, kc, g.kmin(), nent, (T *)c->tab_c.p, c->st, true, 3.0 / 7.0);
-    for (int a = 0; a < 3; ++a)
-      fft_c2r_from<T>(S, (T *)L.psi2[a].p, n, FFT_PRE_ZA, (const T *)c->tab_c.p, a, g.kmin(), c->st, c->hook());
+    for (int a = 0; a < 3; ++a) {
+      if (l >= 1)
+        fft_c2r_from<T>(S, (T *)L.psi2[a].p, n, FFT_PRE_ZA, (const T *)c->tab_c.p, a, g.kmin(), c->st, c->hook());
+      else   // unit filter: (3/7) / k^2 formed inside the transform
+        fft_c2r_from<T>(S, (T *)L.psi2[a].p, n, FFT_PRE_ZA, nullptr, a, g.kmin(), c->st, c->hook(), nullptr,
+                        3.0 / 7.0);
+    }
     if (l >= 1) {
       Level &C = *c->levels[l - 1];
       InterpGeom ig = interp_geom(g, C.g);
@@ -358,9 +363,14 @@ template <typename T> void do_zeldovich(gic_context *c) {
     c->tab_c.ensure(nent * sizeof(T));
     double kc = l >= 1 ? kcut_for_level(c, l - 1) : 0.0;
     filter_table<T>(l >= 1 ? 1 : 2, kc, L.g.kmin(), nent, (T *)c->tab_c.p, c->st, true);
-    for (int a = 0; a < 3; ++a)
-      fft_c2r_from<T>((const T *)L.noise.p, (T *)L.psi[a].p, L.g.n, FFT_PRE_ZA, (const T *)c->tab_c.p, a,
-                      L.g.kmin(), c->st, c->hook());
+    for (int a = 0; a < 3; ++a) {
+      if (l >= 1)
+        fft_c2r_from<T>((const T *)L.noise.p, (T *)L.psi[a].p, L.g.n, FFT_PRE_ZA, (const T *)c->tab_c.p, a,
+                        L.g.kmin(), c->st, c->hook());
+      else   // unit filter: 1/k^2 formed inside the transform
+        fft_c2r_from<T>((const T *)L.noise.p, (T *)L.psi[a].p, L.g.n, FFT_PRE_ZA, nullptr, a, L.g.kmin(), c->st,
+                        c->hook(), nullptr, 1.0);
+    }
     if (l >= 1) {
       filter_table<T>(1, kc, L.g.kmin(), nent, (T *)c->tab_a.p, c->st);
       mul_table<T>((T *)L.noise.p, L.g.n, (const T *)c->tab_a.p, c->st);

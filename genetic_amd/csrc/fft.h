@@ -42,18 +42,29 @@ void fft_r2c_za_adjoint(const T *src, T *d, T *acc, int n, const T *tab, int com
 // zero at k_c = +Nyquist.  FFT_PRE_KK forms k_a k_b tab[|k|^2] for comp = a + 3 b
 // (zero at a +Nyquist component when a != b).
 // With dst non-null the real result goes to the dense n^3 array dst and out is scratch.
+// With tab null and inv_k2 non-zero the operator's factor is inv_k2 / (|k|^2 kmin^2)
+// (0 at k = 0), formed in FP64 as filter_table forms it for a unit filter.
 template <typename T>
 void fft_c2r_from(const T *spec, T *out, int n, int pre, const T *tab, int comp, double kmin, hipStream_t st,
-                  TimingHook *hook, T *dst = nullptr);
+                  TimingHook *hook, T *dst = nullptr, double inv_k2 = 0);
+// The second derivative phi_ab = c2r(k_a k_b tab[|k|^2] spec) (comp = a + 3b,
+// as FFT_PRE_KK) folded into the 2LPT source by the last pass instead of being
+// stored: epi 1: S = A phi, A += phi;  2: S += A phi;  3: S -= phi^2
+// (S, A padded real fields; scratch holds the intermediate passes).
+template <typename T>
+void fft_c2r_lpt(const T *spec, T *scratch, int n, const T *tab, int comp, double kmin, int epi, T *S, T *A,
+                 hipStream_t st, TimingHook *hook, double inv_k2 = 0);
 // Spectrum (r2c, N^-1.5) of a field that is zero outside the real-space window
 // [w0, w0+wl) (per axis, wrapping mod n), given as the dense window win
 // (wl[0]*wl[1]*wl[2] reals, z fastest): pruned z and y passes and a full x
 // pass that scales by post[|k|^2] (if non-null) and stores into, or with
 // accumulate adds to, the padded spectrum spec.  bufB holds
 // wl[0]*wl[1]*(n+2) reals, bufA wl[0]*n*(n/2+1) complex.
+// With aux (a spectrum of the same layout) the result is spec = post * aux +
+// r2c(win) instead: post then scales aux, not the transform.
 template <typename T>
 void fft_r2c_window(const T *win, int n, const int *w0, const int *wl, T *bufB, T *bufA, T *spec, const T *post,
-                    bool accumulate, hipStream_t st, TimingHook *hook);
+                    bool accumulate, hipStream_t st, TimingHook *hook, const T *aux = nullptr);
 // Only the real-space window [w0, w0+wl) (per axis, wrapping mod n) of
 // c2r(op(spec)), via pruned passes: bufA holds wl[0]*n*(n/2+1) complex,
 // bufB wl[0]*wl[1]*(n/2+1) complex, out wl[0]*wl[1]*wl[2] reals.

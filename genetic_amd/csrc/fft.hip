@@ -200,8 +200,9 @@ __device__ __forceinline__ void run_stages(cx<T> *lds, const cx<T> *tw, int twmu
 //        POST_TABLE: v *= tab[kx^2+ky^2+kz^2]
 //        POST_ZA_ADJ: out (+)= -i k_c tab[|k|^2] v, zero at k_c = +Nyquist: the
 //                     transpose of PRE_ZA, accumulated over components in out
+//        POST_AUX:   out = tab[|k|^2] aux + v, aux a spectrum laid out as out
 enum { PRE_NONE = FFT_PRE_NONE, PRE_TABLE = FFT_PRE_TABLE, PRE_ZA = FFT_PRE_ZA, PRE_KK = FFT_PRE_KK };
-enum { POST_NONE = 0, POST_TABLE = 1, POST_ZA_ADJ = 2 };
+enum { POST_NONE = 0, POST_TABLE = 1, POST_ZA_ADJ = 2, POST_AUX = 3 };
 
 template <typename T> struct StridedArgs {
   const cx<T> *in;
@@ -215,6 +216,8 @@ template <typename T> struct StridedArgs {
   int win0, wlen;   // output keeps line points (pt - win0) mod n < wlen
   T scale;
   int accum;        // POST ops: add to the output instead of storing
+  const cx<T> *aux;        // POST_AUX: the spectrum added after its table
+  double inv_k2, kmin_d;   // PRE ops without a table: factor inv_k2 / (|k|^2 kmin_d^2)
   int in_win0, in_wlen;   // if 0 < in_wlen < n: the input holds only line points (pt - in_win0) mod n < in_wlen,
                           // stored at index (pt - in_win0) mod n
 };
@@ -249,7 +252,10 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
   };
   auto op = [&](int line, int pt, cx<T> v) -> cx<T> {
     int kx = signed_k(pt, a.n), kz = col0 + line;
-    T f = a.tab[kx * kx + ky2 + kz * kz];
+    int ksq = kx * kx + ky2 + kz * kz;
+    // the factor of |k|^2: the caller's table, or inv_k2 / (|k|^2 kMin^2)
+    // formed here exactly as filter_table forms it (0 at k = 0)
+    T f = a.tab ? a.tab[ksq] : T(ksq == 0 ? 0.0 : (1.0 / (double(ksq) * a.kmin_d * a.kmin_d)) * a.inv_k2);
     if constexpr (PRE == PRE_ZA) {
       int kc = a.comp == 0 ? kx : a.comp == 1 ? ky : kz;
       if (kc == a.n / 2) return cx<T>{0, 0};
@@ -286,6 +292,14 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
         dst = cx<T>{prev.x + v.x * s, prev.y + v.y * s};
         return;
       }
+    }
+    if constexpr (POST == POST_AUX) {
+      int kx = signed_k(pt, a.n), kz = col0 + line;
+      T f = a.tab[kx * kx + ky2 + kz * kz];
+      long q = long(o) * a.out_ls + line + (ob - a.out);
+      cx<T> x = a.aux[q];
+      ob[o * a.out_ls + line] = cx<T>{x.x * f + v.x * s, x.y * f + v.y * s};
+      return;
     }
     if constexpr (POST == POST_ZA_ADJ) {
       int kx = signed_k(pt, a.n), kz = col0 + line;
@@ -356,9 +370,14 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_r2c(T *__restrict__ data, lon
 }
 
 // ---- z pass, c2r --------------------------------------------------------------
-template <typename T, int N, int ROWS, int THREADS>
+// EPI selects what the pass does with its real output v (z fastest, padded
+// rows): 0 stores it (in place, or to dst as a dense array); for the 2LPT
+// source, with S and A padded fields of the same layout:
+//   1: S = A v, A = A + v     2: S = S + A v     3: S = S - v^2
+template <typename T, int N, int ROWS, int THREADS, int EPI = 0>
 __global__ __launch_bounds__(THREADS) void k_fft_z_c2r(T *__restrict__ data, long nrows, T scale,
-                                                       const cx<T> *__restrict__ twN, T *__restrict__ dst) {
+                                                       const cx<T> *__restrict__ twN, T *__restrict__ dst,
+                                                       T *__restrict__ S = nullptr, T *__restrict__ A = nullptr) {
   constexpr int M = N / 2;
   extern __shared__ char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
@@ -394,7 +413,26 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_c2r(T *__restrict__ data, lon
   // results go back into the padded rows, or to a dense array (row stride N reals)
   cx<T> *drows = dst ? reinterpret_cast<cx<T> *>(dst) + row0 * M : rows;
   const long drs = dst ? M : M + 1;
-  auto gst = [&](int line, int pt, cx<T> v) { drows[long(line) * drs + pt] = cx<T>{v.x * scale, v.y * scale}; };
+  auto gst = [&](int line, int pt, cx<T> v) {
+    cx<T> r{v.x * scale, v.y * scale};
+    if constexpr (EPI == 0) {
+      drows[long(line) * drs + pt] = r;
+    } else {
+      long q = (row0 + line) * (M + 1) + pt;
+      cx<T> *Sc = reinterpret_cast<cx<T> *>(S), *Ac = reinterpret_cast<cx<T> *>(A);
+      if constexpr (EPI == 1) {
+        cx<T> a = Ac[q];
+        Sc[q] = cx<T>{a.x * r.x, a.y * r.y};
+        Ac[q] = cx<T>{a.x + r.x, a.y + r.y};
+      } else if constexpr (EPI == 2) {
+        cx<T> a = Ac[q], sv = Sc[q];
+        Sc[q] = cx<T>{sv.x + a.x * r.x, sv.y + a.y * r.y};
+      } else {
+        cx<T> sv = Sc[q];
+        Sc[q] = cx<T>{sv.x - r.x * r.x, sv.y - r.y * r.y};
+      }
+    }
+  };
   if constexpr (M == 1) {
     if (threadIdx.x < ROWS) gst(threadIdx.x, 0, lld(threadIdx.x, 0));
   } else {
@@ -563,7 +601,13 @@ template <typename T> TwTable<T> &twtable() {
   return t;
 }
 
-template <typename T> constexpr int strided_cols() { return sizeof(T) == 4 ? 8 : 4; }
+#ifndef GIC_STRIDED_COLS_F32
+#define GIC_STRIDED_COLS_F32 8
+#endif
+#ifndef GIC_STRIDED_THREADS
+#define GIC_STRIDED_THREADS 512
+#endif
+template <typename T> constexpr int strided_cols() { return sizeof(T) == 4 ? GIC_STRIDED_COLS_F32 : 4; }
 
 template <typename T, int N> struct FftN {
   static constexpr int THREADS = 256;
@@ -573,6 +617,7 @@ template <typename T, int N> struct FftN {
   static constexpr int ROWS0 = (8192 / N) < 1 ? 1 : (8192 / N);
   static constexpr int ROWS = ROWS0 > N * N ? N * N : ROWS0;
 
+  static constexpr int STHREADS = GIC_STRIDED_THREADS;
   template <int DIR, int PRE, int POST, int PASS = 0>
   static void strided(const StridedArgs<T> &a0, int nouter, hipStream_t st, TimingHook *hook, const char *name) {
     StridedArgs<T> a = a0;
@@ -582,11 +627,11 @@ template <typename T, int N> struct FftN {
     int nblk = nouter * a.nkzb;
     size_t shm = sizeof(cx<T>) * C * N;
     const cx<T> *tw = twtable<T>().get(N);
-    auto kern = k_fft_strided<T, N, C, DIR, THREADS, PRE, POST, PASS>;
+    auto kern = k_fft_strided<T, N, C, DIR, STHREADS, PRE, POST, PASS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     std::string tag = std::string(name) + "." + std::to_string(N);
     if (hook) hook->begin(tag.c_str(), st);
-    hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, a, tw);
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(STHREADS), shm, st, a, tw);
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);
   }
@@ -624,7 +669,8 @@ template <typename T, int N> struct FftN {
       auto kern = k_fft_z_c2r<T, N, ROWS, THREADS>;
       if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
       if (hook) hook->begin(tag.c_str(), st);
-      hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, d, nrows, scale, tw, ext);
+      hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, d, nrows, scale, tw, ext, (T *)nullptr,
+                         (T *)nullptr);
     }
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);
@@ -674,21 +720,59 @@ template <typename T, int N> struct FftN {
     strided<-1, PRE_NONE, POST_ZA_ADJ>(ax, N, st, hook, "fft_x_za_adj");
   }
   // out <- c2r(op(spec)) with op applied while the x pass reads the spectrum
-  static void c2r_from(const T *spec, T *out, int pre, const T *tab, int comp, T kmin, hipStream_t st,
-                       TimingHook *hook, T *dst = nullptr) {
+  static void c2r_from(const T *spec, T *out, int pre, const T *tab, int comp, double kmin, hipStream_t st,
+                       TimingHook *hook, T *dst = nullptr, double inv_k2 = 0) {
     const cx<T> *ci = reinterpret_cast<const cx<T> *>(spec);
     cx<T> *co = reinterpret_cast<cx<T> *>(out);
     StridedArgs<T> ax = plain(co, true, T(1));
     ax.in = ci;
     ax.tab = tab;
     ax.comp = comp;
-    ax.kmin = kmin;
+    ax.kmin = T(kmin);
+    ax.kmin_d = kmin;
+    ax.inv_k2 = inv_k2;
+    if (!tab && pre != PRE_NONE && inv_k2 == 0) throw std::runtime_error("c2r_from: operator without a table");
     if (pre == PRE_ZA) strided<+1, PRE_ZA, POST_NONE>(ax, N, st, hook, "fft_x_za");
     else if (pre == PRE_KK) strided<+1, PRE_KK, POST_NONE>(ax, N, st, hook, "fft_x_kk");
     else if (pre == PRE_TABLE) strided<+1, PRE_TABLE, POST_NONE>(ax, N, st, hook, "fft_x_pre");
     else strided<+1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x");
     strided<+1, PRE_NONE, POST_NONE, 1>(plain(co, false, T(1)), N, st, hook, "fft_y");
     zpass(out, norm(), +1, st, hook, "fft_z_c2r", dst);
+  }
+  // c2r(op(spec)) whose last pass folds its output into the 2LPT source
+  // accumulators S, A (EPI 1..3 of k_fft_z_c2r); out is scratch
+  static void c2r_from_epi(const T *spec, T *out, int pre, const T *tab, int comp, double kmin, int epi, T *S, T *A,
+                           hipStream_t st, TimingHook *hook, double inv_k2) {
+    const cx<T> *ci = reinterpret_cast<const cx<T> *>(spec);
+    cx<T> *co = reinterpret_cast<cx<T> *>(out);
+    StridedArgs<T> ax = plain(co, true, T(1));
+    ax.in = ci;
+    ax.tab = tab;
+    ax.comp = comp;
+    ax.kmin = T(kmin);
+    ax.kmin_d = kmin;
+    ax.inv_k2 = inv_k2;
+    if (!tab && inv_k2 == 0) throw std::runtime_error("c2r_from_epi: operator without a table");
+    if (pre == PRE_KK) strided<+1, PRE_KK, POST_NONE>(ax, N, st, hook, "fft_x_kk");
+    else throw std::runtime_error("c2r_from_epi: only the k_a k_b operator");
+    strided<+1, PRE_NONE, POST_NONE, 1>(plain(co, false, T(1)), N, st, hook, "fft_y");
+    long nrows = long(N) * N;
+    int nblk = int(nrows / ROWS);
+    size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
+    const cx<T> *tw = twtable<T>().get(N);
+    std::string tag = "fft_z_c2r_lpt." + std::to_string(N);
+    auto launch = [&](auto kern) {
+      if (shm > 65536)
+        GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+      if (hook) hook->begin(tag.c_str(), st);
+      hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, out, nrows, norm(), tw, (T *)nullptr, S, A);
+      GIC_HIP(hipGetLastError());
+      if (hook) hook->end(tag.c_str(), st);
+    };
+    if (epi == 1) launch(k_fft_z_c2r<T, N, ROWS, THREADS, 1>);
+    else if (epi == 2) launch(k_fft_z_c2r<T, N, ROWS, THREADS, 2>);
+    else if (epi == 3) launch(k_fft_z_c2r<T, N, ROWS, THREADS, 3>);
+    else throw std::runtime_error("c2r_from_epi: bad epilogue");
   }
   // real-space window [w0, w0+wl) (per axis, mod N) of c2r(op(spec)):
   // bufA holds wl[0] x-planes after the x pass, bufB wl[0]*wl[1] rows after
@@ -709,6 +793,7 @@ template <typename T, int N> struct FftN {
     ax.tab = tab;
     ax.comp = comp;
     ax.kmin = kmin;
+    ax.kmin_d = double(kmin);
     ax.win0 = w0[0];
     ax.wlen = wl[0];
     ax.scale = T(1);
@@ -747,7 +832,7 @@ template <typename T, int N> struct FftN {
   // output is scaled by post[|k|^2] (if given) and stored or added into spec.
   // bufB holds wl[0]*wl[1] padded rows, bufA wl[0]*N*(N/2+1) complex.
   static void r2c_window(const T *win, const int *w0, const int *wl, T *bufB, T *bufA, T *spec, const T *post,
-                         bool accum, hipStream_t st, TimingHook *hook) {
+                         bool accum, hipStream_t st, TimingHook *hook, const T *aux = nullptr) {
     long nrows = long(wl[0]) * wl[1];
     int nblk = int((nrows + ROWS - 1) / ROWS);
     size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
@@ -788,7 +873,9 @@ template <typename T, int N> struct FftN {
     ax.scale = norm();
     ax.tab = post;
     ax.accum = accum ? 1 : 0;
-    if (post) strided<-1, PRE_NONE, POST_TABLE, 4>(ax, N, st, hook, "fft_x_fwd_win");
+    ax.aux = reinterpret_cast<const cx<T> *>(aux);
+    if (aux) strided<-1, PRE_NONE, POST_AUX, 4>(ax, N, st, hook, "fft_x_fwd_win");
+    else if (post) strided<-1, PRE_NONE, POST_TABLE, 4>(ax, N, st, hook, "fft_x_fwd_win");
     else strided<-1, PRE_NONE, POST_NONE, 4>(ax, N, st, hook, "fft_x_fwd_win");
   }
 };
@@ -828,15 +915,21 @@ void fft_r2c_za_adjoint(const T *src, T *d, T *acc, int n, const T *tab, int com
 }
 template <typename T>
 void fft_c2r_from(const T *spec, T *out, int n, int pre, const T *tab, int comp, double kmin, hipStream_t st,
-                  TimingHook *hook, T *dst) {
-  GIC_FFT_SWITCH(n, F::c2r_from(spec, out, pre, tab, comp, T(kmin), st, hook, dst))
+                  TimingHook *hook, T *dst, double inv_k2) {
+  GIC_FFT_SWITCH(n, F::c2r_from(spec, out, pre, tab, comp, kmin, st, hook, dst, inv_k2))
+}
+template <typename T>
+void fft_c2r_lpt(const T *spec, T *scratch, int n, const T *tab, int comp, double kmin, int epi, T *S, T *A,
+                 hipStream_t st, TimingHook *hook, double inv_k2) {
+  GIC_FFT_SWITCH(n, F::c2r_from_epi(spec, scratch, FFT_PRE_KK, tab, comp, kmin, epi, S, A, st, hook, inv_k2))
 }
 template <typename T>
 void fft_r2c_window(const T *win, int n, const int *w0, const int *wl, T *bufB, T *bufA, T *spec, const T *post,
-                    bool accumulate, hipStream_t st, TimingHook *hook) {
+                    bool accumulate, hipStream_t st, TimingHook *hook, const T *aux) {
   for (int a = 0; a < 3; ++a)
     if (wl[a] < 1 || wl[a] > n || w0[a] < 0 || w0[a] >= n) throw std::runtime_error("fft_r2c_window: bad window");
-  GIC_FFT_SWITCH(n, F::r2c_window(win, w0, wl, bufB, bufA, spec, post, accumulate, st, hook))
+  if (aux && (!post || accumulate)) throw std::runtime_error("fft_r2c_window: aux needs a table and no accumulation");
+  GIC_FFT_SWITCH(n, F::r2c_window(win, w0, wl, bufB, bufA, spec, post, accumulate, st, hook, aux))
 }
 template <typename T>
 void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, double kmin, const int *w0, const int *wl,
@@ -851,9 +944,12 @@ void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, doubl
   template void fft_c2r_to<T>(T *, T *, int, hipStream_t, TimingHook *);                                    \
   template void fft_r2c_za_adjoint<T>(const T *, T *, T *, int, const T *, int, double, bool, hipStream_t,   \
                                       TimingHook *);                                                          \
-  template void fft_c2r_from<T>(const T *, T *, int, int, const T *, int, double, hipStream_t, TimingHook *, T *); \
+  template void fft_c2r_from<T>(const T *, T *, int, int, const T *, int, double, hipStream_t, TimingHook *, T *, \
+                                double);                                                                     \
+  template void fft_c2r_lpt<T>(const T *, T *, int, const T *, int, double, int, T *, T *, hipStream_t,       \
+                               TimingHook *, double);                                                        \
   template void fft_r2c_window<T>(const T *, int, const int *, const int *, T *, T *, T *, const T *, bool,      \
-                                  hipStream_t, TimingHook *);                                                  \
+                                  hipStream_t, TimingHook *, const T *);                                       \
   template void fft_c2r_window<T>(const T *, int, int, const T *, int, double, const int *, const int *, T *, T *, \
                                   T *, hipStream_t, TimingHook *);
 GIC_FFT_INST(float)

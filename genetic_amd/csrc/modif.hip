@@ -234,6 +234,21 @@ template <typename T> __global__ void k_tab_vel(T *__restrict__ out, double kmin
   out[m] = m == 0 ? T(0) : T(-coef / (double(m) * kmin * kmin)) * out[m];
 }
 
+// y += sum_j s_j x_j for up to 8 fields in one pass
+struct AxpyList {
+  const void *x[8];
+  double s[8];
+  int k;
+};
+template <typename T> __global__ void k_axpy_multi(T *__restrict__ y, AxpyList L, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  T v = y[i];
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (j < L.k) v += T(L.s[j]) * reinterpret_cast<const T *>(L.x[j])[i];
+  y[i] = v;
+}
 template <typename T> __global__ void k_axpy(T *__restrict__ y, const T *__restrict__ x, T s, size_t n) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i < n) y[i] += s * x[i];
@@ -1118,23 +1133,34 @@ template <typename T> MLF<T> metric(MLF<T> &f, bool to_covector) {
       filter_in_window(out, l, fl, w, true);
       filter_in_window(out, l, fl, w, false);
       out.toF(l);
-    } else {
-      // out = F_l^2 f in one pass
+    }
+    bool self_done = to_covector && l + 1 < nl;
+    auto self_term = [&]() {   // out = F_l^2 f in one pass
       const T *tab = spec_table<T>(c, g, spec_levels(c, long(l), long(l), 1.0));
       size_t nm = size_t(g.n) * g.n * (g.n / 2 + 1);
       hipLaunchKernelGGL(k_copy_mul_table<T>, dim3(blocks(nm)), dim3(256), 0, c->st, reinterpret_cast<cx<T> *>(out.p(l)),
                          reinterpret_cast<const cx<T> *>(f.p(l)), g.n, tab);
       out.fourier[l] = true;
-    }
+    };
+    if (nl == 1 && !self_done) self_term();
     if (nl > 1) {
       PBuf Rbox;
       Box3 B;
       MLF<T> R(c);
       if (cross_terms<T>(c, f, l, Rbox, B, R)) {
         BoxBufs<T> bb(c, g.n, B);
-        fft_r2c_window<T>((const T *)Rbox.p, g.n, B.b0, B.bl, (T *)bb.b.p, (T *)bb.a.p, out.p(l), nullptr, true, c->st,
-                          c->hook());
+        if (self_done) {
+          fft_r2c_window<T>((const T *)Rbox.p, g.n, B.b0, B.bl, (T *)bb.b.p, (T *)bb.a.p, out.p(l), nullptr, true,
+                            c->st, c->hook());
+        } else {
+          // out = F_l^2 f + r2c(box), the self term formed by the transform's last pass
+          const T *tab = spec_table<T>(c, g, spec_levels(c, long(l), long(l), 1.0));
+          fft_r2c_window<T>((const T *)Rbox.p, g.n, B.b0, B.bl, (T *)bb.b.p, (T *)bb.a.p, out.p(l), tab, false,
+                            c->st, c->hook(), f.p(l));
+          out.fourier[l] = true;
+        }
       } else {
+        if (!self_done) self_term();
         R.toF(l);
         out.axpy_level(l, R.p(l), 1.0);
       }
@@ -1461,10 +1487,11 @@ template <typename T> double chi2_of(gic_context *c, MLF<T> &field) {
   for (size_t l = 0; l < nl; ++l) {
     const LevelGeom &g = c->levels[l]->g;
     const cx<T> *xs = reinterpret_cast<const cx<T> *>(field.p(l));
+    PBuf ubox;   // x on the window box of the next level (u = M x there)
     if (l + 1 < nl) {
       const Box3 &M = wins[l];
       size_t nb = box_cells(M), nz = size_t(g.n / 2 + 1);
-      PBuf ubox(c->mod->pool, nb * sizeof(T));
+      ubox = PBuf(c->mod->pool, nb * sizeof(T));
       {
         PBuf A(c->mod->pool, size_t(M.bl[0]) * g.n * nz * 2 * sizeof(T));
         PBuf Bb(c->mod->pool, size_t(M.bl[0]) * M.bl[1] * nz * 2 * sizeof(T));
@@ -1492,13 +1519,22 @@ template <typename T> double chi2_of(gic_context *c, MLF<T> &field) {
       MLF<T> R(c);
       if (cross_terms<T>(c, field, l, Rbox, B, R)) {
         size_t nb = box_cells(B), nz = size_t(g.n / 2 + 1);
-        PBuf xbox(c->mod->pool, nb * sizeof(T));
-        PBuf A(c->mod->pool, size_t(B.bl[0]) * g.n * nz * 2 * sizeof(T));
-        PBuf Bb(c->mod->pool, size_t(B.bl[0]) * B.bl[1] * nz * 2 * sizeof(T));
-        fft_c2r_window<T>(field.p(l), g.n, FFT_PRE_NONE, nullptr, 0, g.kmin(), B.b0, B.bl, (T *)A.p, (T *)Bb.p,
-                          (T *)xbox.p, c->st, c->hook());
-        hipLaunchKernelGGL(k_dot_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, (const T *)xbox.p,
-                           (const T *)Rbox.p, nb, (double *)c->mod->part.p);
+        const T *xv;
+        PBuf xbox;
+        bool same = ubox.p && l + 1 < nl;
+        for (int d = 0; same && d < 3; ++d) same = B.b0[d] == wins[l].b0[d] && B.bl[d] == wins[l].bl[d];
+        if (same) {
+          xv = (const T *)ubox.p;   // the same cells: x there is u
+        } else {
+          xbox = PBuf(c->mod->pool, nb * sizeof(T));
+          PBuf A(c->mod->pool, size_t(B.bl[0]) * g.n * nz * 2 * sizeof(T));
+          PBuf Bb(c->mod->pool, size_t(B.bl[0]) * B.bl[1] * nz * 2 * sizeof(T));
+          fft_c2r_window<T>(field.p(l), g.n, FFT_PRE_NONE, nullptr, 0, g.kmin(), B.b0, B.bl, (T *)A.p, (T *)Bb.p,
+                            (T *)xbox.p, c->st, c->hook());
+          xv = (const T *)xbox.p;
+        }
+        hipLaunchKernelGGL(k_dot_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, xv, (const T *)Rbox.p,
+                           nb, (double *)c->mod->part.p);
       } else {
         R.toF(l);
         hipLaunchKernelGGL(k_inner_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, xs,
@@ -1686,7 +1722,20 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
   }
   {
     Phase ph(c, "phase:update");
-    for (size_t i = 0; i < nm; ++i) field.axpy(vecs[i], coef[i]);
+    for (size_t i0 = 0; i0 < nm; i0 += 8) {
+      size_t k = std::min<size_t>(8, nm - i0);
+      for (size_t l = 0; l < field.nl(); ++l) {
+        AxpyList al{};
+        for (size_t j = 0; j < k; ++j) {
+          al.x[j] = vecs[i0 + j].p(l);
+          al.s[j] = coef[i0 + j];
+        }
+        al.k = int(k);
+        size_t n = field.reals(l);
+        hipLaunchKernelGGL(k_axpy_multi<T>, dim3(blocks(n)), dim3(256), 0, c->st, field.p(l), al, n);
+      }
+    }
+    GIC_HIP(hipGetLastError());
   }
   alphas.clear();
   if (!c->mod->quadratic.empty() && nm) {

@@ -141,7 +141,8 @@ class ICGenerator:
 
     def close(self):
         if getattr(self, "_h", None):
-            lib().gic_destroy(self._h)
+            if _LIB is not None:   # at interpreter exit the library may already be gone
+                _LIB.gic_destroy(self._h)
             self._h = None
 
     def __del__(self):
