@@ -225,6 +225,79 @@ def psi_precision(wl):
                 config="Zel'dovich psi of the workload's levels, same white noise in FP32 and FP64")
 
 
+def splitmix64(state):
+    """The synthetic-choice stream: splitmix64 (Steele, Lea and Flood 2014)."""
+    M = (1 << 64) - 1
+    while True:
+        state = (state + 0x9E3779B97F4A7C15) & M
+        z = state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        yield z ^ (z >> 31)
+
+
+def batch_constraints(nreal=64, nmods=16, box=50.0, seed=20261202):
+    """Per realisation, nmods top-hat overdensity constraints: centre
+    U[0, box)^3, radius U[0.5, 4] Mpc/h, target N(0, 1.5) (Box-Muller), all
+    from one splitmix64 stream."""
+    import math
+    g = splitmix64(seed)
+    u = lambda: (next(g) >> 11) * 2.0 ** -53
+    out = []
+    for _ in range(nreal):
+        mods = []
+        for _ in range(nmods):
+            centre = (u() * box, u() * box, u() * box)
+            radius = 0.5 + 3.5 * u()
+            u1, u2 = u(), u()
+            target = 1.5 * math.sqrt(-2.0 * math.log(1.0 - u1)) * math.cos(2 * math.pi * u2)
+            mods.append((centre, radius, target))
+        out.append(mods)
+    return out
+
+
+def batch_throughput(nreal=64, n=128, box=50.0, nmods=16, precision=32):
+    """The batching workload: nreal independent n^3 realisations (real-space
+    seeds 1..nreal) on one context, each with nmods overdensity constraints,
+    then delta and Zel'dovich psi; FP32 fields, FP64 Gram solve."""
+    wl = dict(box=box, n=n, zooms=[])
+    ic = make_context(wl, precision)
+    plan = batch_constraints(nreal, nmods, box)
+
+    def realisation(k, mods):
+        ic.reseed(k)
+        ic.draw()
+        ic.clear_modifications()
+        for centre, radius, target in mods:
+            ic.set_flags(0, sphere_cells(ic, 0, centre, radius))
+            ic.modify("overdensity", False, target)
+        ic.apply_modifications()
+        ic.apply_power_spectrum()
+        ic.zeldovich()
+
+    realisation(1000, plan[0])   # warm-up
+    ic.synchronize()
+    cells = [[sphere_cells(ic, 0, c, r) for c, r, _ in mods] for mods in plan]
+    t0 = time.perf_counter()
+    for k, mods in enumerate(plan):
+        ic.reseed(k + 1)
+        ic.draw()
+        ic.clear_modifications()
+        for (centre, radius, target), cl in zip(mods, cells[k]):
+            ic.set_flags(0, cl)
+            ic.modify("overdensity", False, target)
+        ic.apply_modifications()
+        ic.apply_power_spectrum()
+        ic.zeldovich()
+    ic.synchronize()
+    dt = time.perf_counter() - t0
+    ic.close()
+    return dict(value=round(nreal * n ** 3 / dt / 1e6, 2), unit="Mparticles/s",
+                ms_per_realisation=round(dt / nreal * 1e3, 3),
+                config="%d realisations of %d^3 in %.0f Mpc/h (seeds 1..%d), %d overdensity constraints each "
+                       "(splitmix64 seed 20261202), FP32 fields, FP64 Gram solve" % (nreal, n, box, nreal, nmods))
+
+
 def pmc_traffic(workload, kernel):
     """HBM bytes per launch of the roofline kernel from the committed rocprofv3
     counter summary (FETCH_SIZE + WRITE_SIZE, scaled by the same counters'
@@ -323,6 +396,7 @@ def main():
     ic.close()
     if not args.no_extras:
         out["adjoint_over_forward"] = adjoint_ratio()
+        out["batch"] = batch_throughput()
         out["psi_fp32_error"] = psi_precision(wl)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_scale)

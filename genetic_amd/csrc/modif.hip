@@ -1649,46 +1649,59 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
   std::vector<double> G(nm * nm), L(nm * nm, 0.0), rhs(nm);
   {
     Phase ph(c, "phase:gram");
-    // every <alpha_i, v_j> (j <= i) and <alpha_i, w> in one pass per level
-    size_t npair = nm * (nm + 1) / 2 + nm;
-    if (nm > 0 && nm <= 7 && 2 * nm + 1 <= size_t(MAXF) && npair <= size_t(MAXP)) {
+    // every <alpha_i, v_j> (j <= i) and <alpha_i, w>, a block of up to 4
+    // alphas against 4 vectors (and w) per pass and level
+    if (nm > 0) {
       field.toF();
       for (size_t i = 0; i < nm; ++i) {
         alphas[i].toF();
         vecs[i].toF();
       }
       ensure_red(c);
-      std::vector<double> acc(npair, 0.0), part(npair);
-      for (size_t l = 0; l < field.nl(); ++l) {
-        MultiDot md{};
-        for (size_t i = 0; i < nm; ++i) {
-          md.f[i] = alphas[i].p(l);
-          md.f[nm + i] = vecs[i].p(l);
-        }
-        md.f[2 * nm] = field.p(l);
-        int p = 0;
-        for (size_t i = 0; i < nm; ++i) {
-          for (size_t j = 0; j <= i; ++j) {
-            md.pa[p] = int(i);
-            md.pb[p++] = int(nm + j);
+      std::vector<double> gsum(nm * nm, 0.0), dsum(nm, 0.0);
+      static const int sizes[] = {2, 5, 9, 14, 20, 27, 35};
+      for (size_t ib = 0; ib < nm; ib += 4)
+        for (size_t jb = 0; jb <= ib; jb += 4) {
+          size_t ie = std::min(nm, ib + 4), je = std::min(nm, jb + 4);
+          std::vector<std::pair<int, int>> pr;   // (i, j); j = -1 for w
+          for (size_t i = ib; i < ie; ++i) {
+            for (size_t j = jb; j < je && j <= i; ++j) pr.push_back({int(i), int(j)});
+            if (jb == 0) pr.push_back({int(i), -1});
           }
-          md.pa[p] = int(i);
-          md.pb[p++] = int(2 * nm);
+          int np = 0;
+          for (int sz : sizes)
+            if (sz >= int(pr.size())) {
+              np = sz;
+              break;
+            }
+          std::vector<double> part(np);
+          for (size_t l = 0; l < field.nl(); ++l) {
+            MultiDot md{};
+            for (size_t i = ib; i < ie; ++i) md.f[i - ib] = alphas[i].p(l);
+            for (size_t j = jb; j < je; ++j) md.f[4 + j - jb] = vecs[j].p(l);
+            md.f[8] = field.p(l);
+            for (int q = 0; q < np; ++q) {
+              auto pq = pr[q < int(pr.size()) ? q : 0];   // padding repeats the first pair
+              md.pa[q] = pq.first - int(ib);
+              md.pb[q] = pq.second < 0 ? 8 : 4 + pq.second - int(jb);
+            }
+            md.np = np;
+            if (!launch_multi_inner<T>(md, field.g(l).n, (double *)c->mod->part.p, c->st))
+              throw std::runtime_error("internal: no batched inner product for this many pairs");
+            hipLaunchKernelGGL(k_reduce_rows, dim3(np), dim3(RED_THREADS), 0, c->st, (const double *)c->mod->part.p,
+                               RED_BLOCKS, (double *)c->mod->scalar.p);
+            GIC_HIP(hipGetLastError());
+            GIC_HIP(hipMemcpyAsync(part.data(), c->mod->scalar.p, np * sizeof(double), hipMemcpyDeviceToHost, c->st));
+            GIC_HIP(hipStreamSynchronize(c->st));
+            for (size_t q = 0; q < pr.size(); ++q) {
+              if (pr[q].second < 0) dsum[pr[q].first] += part[q];
+              else gsum[pr[q].first * nm + pr[q].second] += part[q];
+            }
+          }
         }
-        md.np = p;
-        if (!launch_multi_inner<T>(md, field.g(l).n, (double *)c->mod->part.p, c->st))
-          throw std::runtime_error("internal: no batched inner product for this many pairs");
-        hipLaunchKernelGGL(k_reduce_rows, dim3(p), dim3(RED_THREADS), 0, c->st, (const double *)c->mod->part.p,
-                           RED_BLOCKS, (double *)c->mod->scalar.p);
-        GIC_HIP(hipGetLastError());
-        GIC_HIP(hipMemcpyAsync(part.data(), c->mod->scalar.p, p * sizeof(double), hipMemcpyDeviceToHost, c->st));
-        GIC_HIP(hipStreamSynchronize(c->st));
-        for (int q = 0; q < p; ++q) acc[q] += part[q];
-      }
-      int p = 0;
       for (size_t i = 0; i < nm; ++i) {
-        for (size_t j = 0; j <= i; ++j) G[i * nm + j] = G[j * nm + i] = acc[p++];
-        rhs[i] = targets[i] - acc[p++];
+        for (size_t j = 0; j <= i; ++j) G[i * nm + j] = G[j * nm + i] = gsum[i * nm + j];
+        rhs[i] = targets[i] - dsum[i];
       }
     } else {
       for (size_t i = 0; i < nm; ++i) {
