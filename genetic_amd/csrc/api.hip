@@ -328,10 +328,16 @@ template <typename T> void do_second_order(gic_context *c) {
         c->lpt_win.ensure(a_b + b_b + 3 * o_b);
         T *bufA = (T *)c->lpt_win.p, *bufB = (T *)((char *)c->lpt_win.p + a_b);
         filter_table<T>(0, kcut_for_level(c, 0), g.kmin(), nent, (T *)c->tab_c.p, c->st, true, 3.0 / 7.0);
+        FilterInline fil;
+        fil.nf = 1;
+        fil.kind[0] = 0;
+        fil.kc[0] = kcut_for_level(c, 0);
+        fil.scale = 3.0 / 7.0;
+        fil.over_k2 = 1;
         for (int a = 0; a < 3; ++a) {
           T *out = (T *)((char *)c->lpt_win.p + a_b + b_b + a * o_b);
           fft_c2r_window<T>(S, n, FFT_PRE_ZA, (const T *)c->tab_c.p, a, g.kmin(), w0, wl, bufA, bufB, out, c->st,
-                            c->hook());
+                            c->hook(), &fil);
           win_view[a].p = out;
           for (int d = 0; d < 3; ++d) {
             win_view[a].org[d] = w0[d];
@@ -409,8 +415,13 @@ template <typename T> void do_zeldovich(gic_context *c) {
         T *bufA = (T *)c->stage.p;
         T *bufB = (T *)((char *)c->stage.p + a_bytes);
         T *out = (T *)((char *)c->stage.p + a_bytes + b_bytes);
+        FilterInline fil;   // the same low pass (over k^2 for psi), formed in the pass
+        fil.nf = 1;
+        fil.kind[0] = 0;
+        fil.kc[0] = kc;
+        fil.over_k2 = f == 0 ? 0 : 1;
         fft_c2r_window<T>((const T *)C.noise.p, C.g.n, f == 0 ? FFT_PRE_TABLE : FFT_PRE_ZA, f == 0 ? lp : lp_k2,
-                          f - 1, C.g.kmin(), w0, wl, bufA, bufB, out, c->st, c->hook());
+                          f - 1, C.g.kmin(), w0, wl, bufA, bufB, out, c->st, c->hook(), &fil);
         v.p = out;
         for (int a = 0; a < 3; ++a) {
           v.org[a] = w0[a];

@@ -21,6 +21,20 @@ template <typename T> void fft3d(T *d, int n, int dir, hipStream_t st, TimingHoo
 // k-space operators applied inside the x pass
 enum { FFT_PRE_NONE = 0, FFT_PRE_TABLE = 1, FFT_PRE_ZA = 2, FFT_PRE_KK = 3 };
 
+// A product of Fermi level filters, formed inside a transform's pass instead
+// of read from a |k|^2 table (used by FP32 transforms, whose table reads are
+// scattered): scale * prod_i f_i(k), f = LP, sqrt(1 - LP^2) or 1 - LP for
+// kind 0, 1, 2 with LP = 1 / (1 + exp((k - kc) / (0.1 kc))); over_k2 divides
+// by k^2 (0 at k = 0); zero makes every factor 0.  nf < 0: not given.
+struct FilterInline {
+  int nf = -1;
+  int kind[8];
+  double kc[8];
+  double scale = 1.0;
+  int over_k2 = 0;
+  int zero = 0;
+};
+
 // r2c whose last pass multiplies each mode by post[kx^2+ky^2+kz^2] (if non-null).
 // With src non-null the real input is read from the dense n^3 array src (z
 // fastest, no padding) and the spectrum lands in d.
@@ -44,9 +58,10 @@ void fft_r2c_za_adjoint(const T *src, T *d, T *acc, int n, const T *tab, int com
 // With dst non-null the real result goes to the dense n^3 array dst and out is scratch.
 // With tab null and inv_k2 non-zero the operator's factor is inv_k2 / (|k|^2 kmin^2)
 // (0 at k = 0), formed in FP64 as filter_table forms it for a unit filter.
+// fil (optional) replaces tab for FP32 transforms.
 template <typename T>
 void fft_c2r_from(const T *spec, T *out, int n, int pre, const T *tab, int comp, double kmin, hipStream_t st,
-                  TimingHook *hook, T *dst = nullptr, double inv_k2 = 0);
+                  TimingHook *hook, T *dst = nullptr, double inv_k2 = 0, const FilterInline *fil = nullptr);
 // The second derivative phi_ab = c2r(k_a k_b tab[|k|^2] spec) (comp = a + 3b,
 // as FFT_PRE_KK) folded into the 2LPT source by the last pass instead of being
 // stored: epi 1: S = A phi, A += phi;  2: S += A phi;  3: S -= phi^2
@@ -64,12 +79,13 @@ void fft_c2r_lpt(const T *spec, T *scratch, int n, const T *tab, int comp, doubl
 // r2c(win) instead: post then scales aux, not the transform.
 template <typename T>
 void fft_r2c_window(const T *win, int n, const int *w0, const int *wl, T *bufB, T *bufA, T *spec, const T *post,
-                    bool accumulate, hipStream_t st, TimingHook *hook, const T *aux = nullptr);
+                    bool accumulate, hipStream_t st, TimingHook *hook, const T *aux = nullptr,
+                    const FilterInline *fil = nullptr, double kmin = 0);
 // Only the real-space window [w0, w0+wl) (per axis, wrapping mod n) of
 // c2r(op(spec)), via pruned passes: bufA holds wl[0]*n*(n/2+1) complex,
 // bufB wl[0]*wl[1]*(n/2+1) complex, out wl[0]*wl[1]*wl[2] reals.
 template <typename T>
 void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, double kmin, const int *w0, const int *wl,
-                    T *bufA, T *bufB, T *out, hipStream_t st, TimingHook *hook);
+                    T *bufA, T *bufB, T *out, hipStream_t st, TimingHook *hook, const FilterInline *fil = nullptr);
 
 }  // namespace gic

@@ -218,11 +218,34 @@ template <typename T> struct StridedArgs {
   int accum;        // POST ops: add to the output instead of storing
   const cx<T> *aux;        // POST_AUX: the spectrum added after its table
   double inv_k2, kmin_d;   // PRE ops without a table: factor inv_k2 / (|k|^2 kmin_d^2)
+  FilterInline fil;        // FP32: the table's filter product formed in the pass (fil.nf >= 0)
   int in_win0, in_wlen;   // if 0 < in_wlen < n: the input holds only line points (pt - in_win0) mod n < in_wlen,
                           // stored at index (pt - in_win0) mod n
 };
 
 __device__ __forceinline__ int signed_k(int i, int n) { return i > n / 2 ? i - n : i; }
+
+// The factor of FilterInline at |k|^2 = ksq in single precision.
+__device__ __forceinline__ float filter_inline_f(const FilterInline &F, int ksq, double kmin_d) {
+  if (F.zero) return 0.f;
+  float kmin = float(kmin_d);
+  float k = sqrtf(float(ksq)) * kmin;
+  float v = float(F.scale);
+  for (int i = 0; i < F.nf; ++i) {
+    float kc = float(F.kc[i]);
+    float lp = 1.f / (1.f + __expf((k - kc) / (kc * 0.1f)));
+    v *= F.kind[i] == 0 ? lp : F.kind[i] == 1 ? sqrtf(fmaxf(1.f - lp * lp, 0.f)) : 1.f - lp;
+  }
+  if (F.over_k2) v = ksq == 0 ? 0.f : v / (float(ksq) * kmin * kmin);
+  return v;
+}
+// the |k|^2 factor of a pass: inline filter (FP32), table, or inv_k2 / k^2
+template <typename T, class A> __device__ __forceinline__ T kfactor(const A &a, int ksq) {
+  if constexpr (sizeof(T) == 4) {
+    if (a.fil.nf >= 0) return filter_inline_f(a.fil, ksq, a.kmin_d);
+  }
+  return a.tab ? a.tab[ksq] : T(ksq == 0 ? 0.0 : (1.0 / (double(ksq) * a.kmin_d * a.kmin_d)) * a.inv_k2);
+}
 
 // PRE_ZA multiplies by i * k_c * tab[|k|^2], where the caller's table holds
 // filter(k) / k^2 (0 at k = 0); the component at k_c = +Nyquist is zero.
@@ -253,9 +276,9 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
   auto op = [&](int line, int pt, cx<T> v) -> cx<T> {
     int kx = signed_k(pt, a.n), kz = col0 + line;
     int ksq = kx * kx + ky2 + kz * kz;
-    // the factor of |k|^2: the caller's table, or inv_k2 / (|k|^2 kMin^2)
-    // formed here exactly as filter_table forms it (0 at k = 0)
-    T f = a.tab ? a.tab[ksq] : T(ksq == 0 ? 0.0 : (1.0 / (double(ksq) * a.kmin_d * a.kmin_d)) * a.inv_k2);
+    // the factor of |k|^2: the caller's table, its filter formed here (FP32),
+    // or inv_k2 / (|k|^2 kMin^2) formed exactly as filter_table forms it
+    T f = kfactor<T>(a, ksq);
     if constexpr (PRE == PRE_ZA) {
       int kc = a.comp == 0 ? kx : a.comp == 1 ? ky : kz;
       if (kc == a.n / 2) return cx<T>{0, 0};
@@ -283,7 +306,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
     T s = a.scale;
     if constexpr (POST == POST_TABLE) {
       int kx = signed_k(pt, a.n), kz = col0 + line;
-      s *= a.tab[kx * kx + ky2 + kz * kz];
+      s *= kfactor<T>(a, kx * kx + ky2 + kz * kz);
     }
     if constexpr (POST == POST_NONE || POST == POST_TABLE) {
       if (a.accum) {
@@ -295,7 +318,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
     }
     if constexpr (POST == POST_AUX) {
       int kx = signed_k(pt, a.n), kz = col0 + line;
-      T f = a.tab[kx * kx + ky2 + kz * kz];
+      T f = kfactor<T>(a, kx * kx + ky2 + kz * kz);
       long q = long(o) * a.out_ls + line + (ob - a.out);
       cx<T> x = a.aux[q];
       ob[o * a.out_ls + line] = cx<T>{x.x * f + v.x * s, x.y * f + v.y * s};
@@ -721,7 +744,7 @@ template <typename T, int N> struct FftN {
   }
   // out <- c2r(op(spec)) with op applied while the x pass reads the spectrum
   static void c2r_from(const T *spec, T *out, int pre, const T *tab, int comp, double kmin, hipStream_t st,
-                       TimingHook *hook, T *dst = nullptr, double inv_k2 = 0) {
+                       TimingHook *hook, T *dst = nullptr, double inv_k2 = 0, const FilterInline *fil = nullptr) {
     const cx<T> *ci = reinterpret_cast<const cx<T> *>(spec);
     cx<T> *co = reinterpret_cast<cx<T> *>(out);
     StridedArgs<T> ax = plain(co, true, T(1));
@@ -731,6 +754,7 @@ template <typename T, int N> struct FftN {
     ax.kmin = T(kmin);
     ax.kmin_d = kmin;
     ax.inv_k2 = inv_k2;
+    if (fil) ax.fil = *fil;
     if (!tab && pre != PRE_NONE && inv_k2 == 0) throw std::runtime_error("c2r_from: operator without a table");
     if (pre == PRE_ZA) strided<+1, PRE_ZA, POST_NONE>(ax, N, st, hook, "fft_x_za");
     else if (pre == PRE_KK) strided<+1, PRE_KK, POST_NONE>(ax, N, st, hook, "fft_x_kk");
@@ -778,7 +802,7 @@ template <typename T, int N> struct FftN {
   // bufA holds wl[0] x-planes after the x pass, bufB wl[0]*wl[1] rows after
   // the y pass; out receives wl[0]*wl[1]*wl[2] reals.
   static void c2r_window(const T *spec, int pre, const T *tab, int comp, T kmin, const int *w0, const int *wl, T *bufA,
-                         T *bufB, T *out, hipStream_t st, TimingHook *hook) {
+                         T *bufB, T *out, hipStream_t st, TimingHook *hook, const FilterInline *fil = nullptr) {
     const cx<T> *ci = reinterpret_cast<const cx<T> *>(spec);
     cx<T> *ca = reinterpret_cast<cx<T> *>(bufA);
     cx<T> *cb = reinterpret_cast<cx<T> *>(bufB);
@@ -794,6 +818,7 @@ template <typename T, int N> struct FftN {
     ax.comp = comp;
     ax.kmin = kmin;
     ax.kmin_d = double(kmin);
+    if (fil) ax.fil = *fil;
     ax.win0 = w0[0];
     ax.wlen = wl[0];
     ax.scale = T(1);
@@ -832,7 +857,8 @@ template <typename T, int N> struct FftN {
   // output is scaled by post[|k|^2] (if given) and stored or added into spec.
   // bufB holds wl[0]*wl[1] padded rows, bufA wl[0]*N*(N/2+1) complex.
   static void r2c_window(const T *win, const int *w0, const int *wl, T *bufB, T *bufA, T *spec, const T *post,
-                         bool accum, hipStream_t st, TimingHook *hook, const T *aux = nullptr) {
+                         bool accum, hipStream_t st, TimingHook *hook, const T *aux = nullptr, const FilterInline *fil = nullptr,
+                         double kmin = 0) {
     long nrows = long(wl[0]) * wl[1];
     int nblk = int((nrows + ROWS - 1) / ROWS);
     size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
@@ -874,6 +900,8 @@ template <typename T, int N> struct FftN {
     ax.tab = post;
     ax.accum = accum ? 1 : 0;
     ax.aux = reinterpret_cast<const cx<T> *>(aux);
+    if (fil) ax.fil = *fil;
+    ax.kmin_d = kmin;
     if (aux) strided<-1, PRE_NONE, POST_AUX, 4>(ax, N, st, hook, "fft_x_fwd_win");
     else if (post) strided<-1, PRE_NONE, POST_TABLE, 4>(ax, N, st, hook, "fft_x_fwd_win");
     else strided<-1, PRE_NONE, POST_NONE, 4>(ax, N, st, hook, "fft_x_fwd_win");
@@ -915,8 +943,8 @@ void fft_r2c_za_adjoint(const T *src, T *d, T *acc, int n, const T *tab, int com
 }
 template <typename T>
 void fft_c2r_from(const T *spec, T *out, int n, int pre, const T *tab, int comp, double kmin, hipStream_t st,
-                  TimingHook *hook, T *dst, double inv_k2) {
-  GIC_FFT_SWITCH(n, F::c2r_from(spec, out, pre, tab, comp, kmin, st, hook, dst, inv_k2))
+                  TimingHook *hook, T *dst, double inv_k2, const FilterInline *fil) {
+  GIC_FFT_SWITCH(n, F::c2r_from(spec, out, pre, tab, comp, kmin, st, hook, dst, inv_k2, fil))
 }
 template <typename T>
 void fft_c2r_lpt(const T *spec, T *scratch, int n, const T *tab, int comp, double kmin, int epi, T *S, T *A,
@@ -925,18 +953,19 @@ void fft_c2r_lpt(const T *spec, T *scratch, int n, const T *tab, int comp, doubl
 }
 template <typename T>
 void fft_r2c_window(const T *win, int n, const int *w0, const int *wl, T *bufB, T *bufA, T *spec, const T *post,
-                    bool accumulate, hipStream_t st, TimingHook *hook, const T *aux) {
+                    bool accumulate, hipStream_t st, TimingHook *hook, const T *aux, const FilterInline *fil,
+                    double kmin) {
   for (int a = 0; a < 3; ++a)
     if (wl[a] < 1 || wl[a] > n || w0[a] < 0 || w0[a] >= n) throw std::runtime_error("fft_r2c_window: bad window");
   if (aux && (!post || accumulate)) throw std::runtime_error("fft_r2c_window: aux needs a table and no accumulation");
-  GIC_FFT_SWITCH(n, F::r2c_window(win, w0, wl, bufB, bufA, spec, post, accumulate, st, hook, aux))
+  GIC_FFT_SWITCH(n, F::r2c_window(win, w0, wl, bufB, bufA, spec, post, accumulate, st, hook, aux, fil, kmin))
 }
 template <typename T>
 void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, double kmin, const int *w0, const int *wl,
-                    T *bufA, T *bufB, T *out, hipStream_t st, TimingHook *hook) {
+                    T *bufA, T *bufB, T *out, hipStream_t st, TimingHook *hook, const FilterInline *fil) {
   for (int a = 0; a < 3; ++a)
     if (wl[a] < 1 || wl[a] > n || w0[a] < 0 || w0[a] >= n) throw std::runtime_error("fft_c2r_window: bad window");
-  GIC_FFT_SWITCH(n, F::c2r_window(spec, pre, tab, comp, T(kmin), w0, wl, bufA, bufB, out, st, hook))
+  GIC_FFT_SWITCH(n, F::c2r_window(spec, pre, tab, comp, T(kmin), w0, wl, bufA, bufB, out, st, hook, fil))
 }
 
 #define GIC_FFT_INST(T)                                                                                     \
@@ -945,13 +974,13 @@ void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, doubl
   template void fft_r2c_za_adjoint<T>(const T *, T *, T *, int, const T *, int, double, bool, hipStream_t,   \
                                       TimingHook *);                                                          \
   template void fft_c2r_from<T>(const T *, T *, int, int, const T *, int, double, hipStream_t, TimingHook *, T *, \
-                                double);                                                                     \
+                                double, const FilterInline *);                                               \
   template void fft_c2r_lpt<T>(const T *, T *, int, const T *, int, double, int, T *, T *, hipStream_t,       \
                                TimingHook *, double);                                                        \
   template void fft_r2c_window<T>(const T *, int, const int *, const int *, T *, T *, T *, const T *, bool,      \
-                                  hipStream_t, TimingHook *, const T *);                                       \
+                                  hipStream_t, TimingHook *, const T *, const FilterInline *, double);         \
   template void fft_c2r_window<T>(const T *, int, int, const T *, int, double, const int *, const int *, T *, T *, \
-                                  T *, hipStream_t, TimingHook *);
+                                  T *, hipStream_t, TimingHook *, const FilterInline *);
 GIC_FFT_INST(float)
 GIC_FFT_INST(double)
 

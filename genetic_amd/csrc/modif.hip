@@ -827,6 +827,24 @@ template <typename T> const T *spec_table(gic_context *c, const LevelGeom &g, co
   return (const T *)c->tab_a.p;
 }
 
+// the same product for the transforms to form in their passes (FP32)
+FilterInline inline_of(const FilterSpec &s) {
+  FilterInline f;
+  if (s.na + s.nb > 8) return f;   // nf = -1: use the table
+  f.nf = 0;
+  for (int i = 0; i < s.na; ++i) {
+    f.kind[f.nf] = s.ka[i];
+    f.kc[f.nf++] = s.ca[i];
+  }
+  for (int i = 0; i < s.nb; ++i) {
+    f.kind[f.nf] = s.kb[i];
+    f.kc[f.nf++] = s.cb[i];
+  }
+  f.scale = s.scale;
+  f.zero = s.zero ? 1 : 0;
+  return f;
+}
+
 // multiply level l of m (Fourier) by the filter spec evaluated on that level's k
 template <typename T> void apply_spec(MLF<T> &m, size_t l, const FilterSpec &s) {
   const T *tab = spec_table<T>(m.c, m.g(l), s);
@@ -1021,6 +1039,7 @@ template <typename T> void add_transfer(gic_context *c, T *R, size_t lt, MLF<T> 
     int w0[3], wl[3];
     coarse_window(gt, gs, w0, wl);
     const T *tab = spec_table<T>(c, gs, s);
+    FilterInline fil = inline_of(s);
     CoarseView v;
     PBuf scratch;
     if (gs.covers_full() && wl[0] < gs.n && wl[1] < gs.n && wl[2] < gs.n) {
@@ -1032,7 +1051,8 @@ template <typename T> void add_transfer(gic_context *c, T *R, size_t lt, MLF<T> 
       T *bufA = (T *)scratch.p;
       T *bufB = (T *)((char *)scratch.p + a_b);
       T *out = (T *)((char *)scratch.p + a_b + b_b);
-      fft_c2r_window<T>(f.p(ls), gs.n, FFT_PRE_TABLE, tab, 0, gs.kmin(), w0, wl, bufA, bufB, out, c->st, c->hook());
+      fft_c2r_window<T>(f.p(ls), gs.n, FFT_PRE_TABLE, tab, 0, gs.kmin(), w0, wl, bufA, bufB, out, c->st, c->hook(),
+                        &fil);
       v.p = out;
       for (int a = 0; a < 3; ++a) {
         v.org[a] = w0[a];
@@ -1154,9 +1174,11 @@ template <typename T> MLF<T> metric(MLF<T> &f, bool to_covector) {
                             c->st, c->hook());
         } else {
           // out = F_l^2 f + r2c(box), the self term formed by the transform's last pass
-          const T *tab = spec_table<T>(c, g, spec_levels(c, long(l), long(l), 1.0));
+          FilterSpec f2 = spec_levels(c, long(l), long(l), 1.0);
+          const T *tab = spec_table<T>(c, g, f2);
+          FilterInline fil = inline_of(f2);
           fft_r2c_window<T>((const T *)Rbox.p, g.n, B.b0, B.bl, (T *)bb.b.p, (T *)bb.a.p, out.p(l), tab, false,
-                            c->st, c->hook(), f.p(l));
+                            c->st, c->hook(), f.p(l), &fil, g.kmin());
           out.fourier[l] = true;
         }
       } else {
