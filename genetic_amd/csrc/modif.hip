@@ -917,8 +917,7 @@ bool cyclic_run(const uint8_t *member, int n, int &start, int &len) {
 // k_window_zero treats as inside), per axis a cyclic run; false if empty or
 // not a single run on some axis.
 bool window_box(gic_context *c, const LevelGeom &g, const WinDev &w, Box3 &B) {
-  DevBuf mem;
-  mem.ensure(size_t(3) * g.n);
+  PBuf mem(c->mod->pool, size_t(3) * g.n);
   hipLaunchKernelGGL(k_axis_member, dim3((3 * g.n + 255) / 256), dim3(256), 0, c->st, g.n, g.dx, g.ox, g.oy, g.oz, w,
                      (uint8_t *)mem.p);
   GIC_HIP(hipGetLastError());
@@ -1213,13 +1212,13 @@ template <typename T> void transfer_to_dm(MLF<T> &m) {
 }
 
 struct DevList {
-  DevBuf idx;
+  PBuf idx;
   size_t n = 0;
 };
 void upload_list(gic_context *c, const std::vector<uint64_t> &idx, DevList &out) {
   out.n = idx.size();
   if (!out.n) return;
-  out.idx.ensure(out.n * 8);
+  out.idx = PBuf(c->mod->pool, out.n * 8);
   GIC_HIP(hipMemcpyAsync(out.idx.p, idx.data(), out.n * 8, hipMemcpyHostToDevice, c->st));
   GIC_HIP(hipStreamSynchronize(c->st));
 }
@@ -1301,12 +1300,12 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
   const Cosmo &cc = c->cosmo;
   double coef = kind == 1 ? 3. / 2. * cc.Om / cc.a * 100. * 100. / (299792.) / (299792.)
                           : 1.0 * 100. * sqrt(cc.Om / (cc.a * cc.a * cc.a) + cc.Ol) * sqrt(cc.a);
-  auto level_table = [&](size_t l, DevBuf &tab) {
+  auto level_table = [&](size_t l, PBuf &tab) {
     const LevelGeom &gl = m.g(l);
     size_t nent = table_len(gl);
     double sw = level_weight(c, l) / level_weight(c, nl - 1);
     FilterSpec fs = spec_levels(c, long(l), -1, sw);
-    tab.ensure(nent * sizeof(T));
+    tab = PBuf(c->mod->pool, nent * sizeof(T));
     hipLaunchKernelGGL(k_filter_spec<T>, dim3(blocks(nent)), dim3(256), 0, c->st, fs, gl.kmin(), long(nent),
                        (T *)tab.p);
     Level &Lv = *c->levels[l];
@@ -1316,13 +1315,11 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
                          (const T *)Lv.sqrtp_dm.p, long(nent));
     } else {
       build_sqrtp<T>(c, Lv);
-      DevBuf ratio;
-      ratio.ensure(nent * sizeof(T));
+      PBuf ratio(c->mod->pool, nent * sizeof(T));
       hipLaunchKernelGGL(k_ratio_table<T>, dim3(blocks(nent)), dim3(256), 0, c->st, (const T *)Lv.sqrtp.p,
                          (const T *)Lv.sqrtp_dm.p, long(nent), (T *)ratio.p);
       hipLaunchKernelGGL(k_tab_product<T>, dim3(blocks(nent)), dim3(256), 0, c->st, (T *)tab.p, (const T *)ratio.p,
                          long(nent));
-      GIC_HIP(hipStreamSynchronize(c->st));
     }
     if (kind >= 2)
       hipLaunchKernelGGL(k_tab_vel<T>, dim3(blocks(nent)), dim3(256), 0, c->st, (T *)tab.p, gl.kmin(), coef,
@@ -1335,8 +1332,7 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
     PBuf accs[2];
     const T *src = m.p(L);
     int src_padded = 1;
-    DevBuf dropped;
-    dropped.ensure(sizeof(unsigned));
+    PBuf dropped(c->mod->pool, sizeof(unsigned));
     for (long l = long(L) - 1; l >= 0; --l) {
       const LevelGeom &gh = m.g(size_t(l) + 1), &gl = m.g(size_t(l));
       DeinterpGeom dg;
@@ -1377,7 +1373,7 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
           GIC_HIP(hipMemcpyAsync(&nd, dropped.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
           GIC_HIP(hipStreamSynchronize(c->st));
           if (nd == 0) {
-            DevBuf tab;
+            PBuf tab;
             level_table(0, tab);
             int w0[3] = {int(x_lo), 0, 0}, wl[3] = {wl0, gl.n, gl.n};
             Box3 B{{w0[0], 0, 0}, {wl[0], wl[1], wl[2]}};
@@ -1386,8 +1382,7 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
                               false, c->st, c->hook());
             m.fourier[0] = true;
             done[0] = true;
-            GIC_HIP(hipStreamSynchronize(c->st));
-            continue;
+            continue;   // buffers return to the pool; later users are ordered on the stream
           }
         }
       }
@@ -1406,11 +1401,10 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
   // every other level: transform, then the level's factors in one pass
   for (size_t l = 0; l < nl; ++l) {
     if (done[l]) continue;
-    DevBuf tab;
+    PBuf tab;
     level_table(l, tab);
     m.toF(l);
     mul_table<T>(m.p(l), m.g(l).n, (const T *)tab.p, c->st);
-    GIC_HIP(hipStreamSynchronize(c->st));
   }
   if (kind == 1) {
     for (size_t l = 0; l < nl; ++l) {
