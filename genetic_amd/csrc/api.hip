@@ -160,6 +160,13 @@ template <typename T> void do_draw(gic_context *c) {
     }
   }
   c->drawn = true;
+  if (c->fix_power) {
+    // enforceUnitVariance (multilevelfield.hpp:395-407), right after the draw (ic.hpp:1343-1344)
+    for (auto &Lp : c->levels) {
+      ensure_noise_fourier<T>(c, *Lp);
+      unit_modulus<T>((T *)Lp->noise.p, Lp->g.n, c->st);
+    }
+  }
 }
 
 template <typename T> void ensure_noise_fourier(gic_context *c, Level &L) { noise_to_fourier<T>(c, L, nullptr); }
@@ -650,6 +657,26 @@ template void ensure_noise_fourier<double>(gic_context *, Level &);
 template void do_zeldovich<float>(gic_context *);
 template void do_zeldovich<double>(gic_context *);
 
+template <typename T> void do_import_level(gic_context *c, int level, const double *host) {
+  if (!c->drawn) do_draw<T>(c);
+  if (c->power_applied) throw std::runtime_error("import_level acts on the white noise, before the output is generated");
+  Level &L = level_at(c, level);
+  int n = L.g.n;
+  size_t n3 = L.g.n3();
+  c->stage.ensure(n3 * sizeof(double));
+  GIC_HIP(hipMemcpyAsync(c->stage.p, host, n3 * sizeof(double), hipMemcpyHostToDevice, c->st));
+  pad<double, T>((const double *)c->stage.p, n, (T *)L.noise.p, c->st);
+  // the file holds a dark-matter overdensity: back to white noise with P_dm^-1/2
+  build_sqrtp_dm<T>(c, L);
+  long h = n / 2, nent = 3 * h * h + 1;
+  c->tab_a.ensure(nent * sizeof(T));
+  inverse_table<T>((const T *)L.sqrtp_dm.p, nent, (T *)c->tab_a.p, c->st);
+  fft_r2c_post<T>((T *)L.noise.p, n, (const T *)c->tab_a.p, c->st, c->hook());
+  L.noise_fourier = true;
+  L.dc_pending = false;
+  GIC_HIP(hipStreamSynchronize(c->st));
+}
+
 }  // namespace gic
 
 extern "C" {
@@ -847,6 +874,51 @@ int gic_zero_level(gic_context *c, int level) {
     if (c->prec == 32) fill<float>((float *)L.noise.p, L.g.padded_reals(), 0.f, c->st);
     else fill<double>((double *)L.noise.p, L.g.padded_reals(), 0.0, c->st);
     GIC_HIP(hipStreamSynchronize(c->st));
+  });
+}
+
+int gic_set_exact_power_spectrum(gic_context *c, int enable) {
+  return guarded(c, [&] {
+    if (c->drawn) throw std::runtime_error("fix_power must be set before the random field is drawn");
+    c->fix_power = enable != 0;
+  });
+}
+
+int gic_reverse(gic_context *c) {
+  return guarded(c, [&] {
+    if (!c->drawn) DISPATCH(c, do_draw, c);
+    if (c->power_applied) throw std::runtime_error("reverse acts on the white noise, before the output is generated");
+    for (auto &Lp : c->levels) {
+      if (c->prec == 32) scale_all<float>((float *)Lp->noise.p, Lp->g.padded_reals(), -1.f, c->st);
+      else scale_all<double>((double *)Lp->noise.p, Lp->g.padded_reals(), -1.0, c->st);
+    }
+    GIC_HIP(hipStreamSynchronize(c->st));
+  });
+}
+
+int gic_reverse_small_k(gic_context *c, double kmax) {
+  return guarded(c, [&] {
+    if (!c->drawn) DISPATCH(c, do_draw, c);
+    if (c->power_applied) throw std::runtime_error("reverse_small_k acts on the white noise, before the output is generated");
+    for (auto &Lp : c->levels) {
+      Level &L = *Lp;
+      if (c->prec == 32) {
+        ensure_noise_fourier<float>(c, L);
+        reverse_small_k<float>((float *)L.noise.p, L.g.n, L.g.kmin(), kmax, c->st);
+      } else {
+        ensure_noise_fourier<double>(c, L);
+        reverse_small_k<double>((double *)L.noise.p, L.g.n, L.g.kmin(), kmax, c->st);
+      }
+    }
+    GIC_HIP(hipStreamSynchronize(c->st));
+  });
+}
+
+int gic_import_level(gic_context *c, int level, const double *host, size_t n) {
+  return guarded(c, [&] {
+    Level &L = level_at(c, level);
+    if (!host || n != L.g.n3()) throw std::runtime_error("import_level: the data must hold n^3 values of the level");
+    DISPATCH(c, do_import_level, c, level, host);
   });
 }
 

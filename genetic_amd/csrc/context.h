@@ -233,6 +233,7 @@ struct gic_context {
   int seed_mode = 0;
   bool seeded = false, drawn = false, power_applied = false, zeldovich_done = false;
   int lpt_order = 1;   // 1: Zel'dovich, 2: second-order LPT on top
+  bool fix_power = false;   // fix_power: unit-modulus white noise after the draw
   DevBuf work, stage, tab_a, tab_b, tab_c, lpt_win, adj_scratch;
   std::unique_ptr<EventTimer> timer;
   TimingHook *hook() { return timer.get(); }

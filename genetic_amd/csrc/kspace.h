@@ -66,6 +66,12 @@ void particles(const T *px, const T *py, const T *pz, const T *const *psi2, int 
 // elementwise source accumulation: mode 0 S = A B, 1 S += A B, 2 S -= A A, 3 A += B
 template <typename T> void lpt_accum(T *S, T *A, const T *B, size_t n, int mode, hipStream_t st);
 template <typename T> void fill(T *p, size_t n, T v, hipStream_t st);
+// white noise adjustments: every stored mode to unit modulus; sign flip of the
+// modes with 0 < k^2 < kmax^2; p *= s over n values; out = 1/tab (0 where tab is 0)
+template <typename T> void unit_modulus(T *field, int n, hipStream_t st);
+template <typename T> void reverse_small_k(T *field, int n, double kmin, double kmax, hipStream_t st);
+template <typename T> void scale_all(T *p, size_t n, T s, hipStream_t st);
+template <typename T> void inverse_table(const T *tab, long n, T *out, hipStream_t st);
 // Fourier-space white noise: number of (a, b) draws of an n^3 level, and the
 // placement of gauss[0 .. 2*count) (already scaled by 1/sqrt 2) into the
 // padded spectrum 'field'; owner holds n*n*(n/2+1) words of scratch.

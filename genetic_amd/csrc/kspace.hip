@@ -497,6 +497,61 @@ template <typename T> void lpt_accum(T *S, T *A, const T *B, size_t n, int mode,
   GIC_HIP(hipGetLastError());
 }
 
+// ---- white-noise adjustments ------------------------------------------------------------
+// enforcePowerSpectrumOneGrid (multilevelfield.hpp:498-510): every mode to
+// unit modulus, keeping its phase (0 stays 0)
+template <typename T> __global__ void k_unit_modulus(cx<T> *__restrict__ f, size_t nm) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  cx<T> v = f[i];
+  T a = sqrt(v.x * v.x + v.y * v.y);
+  f[i] = a == T(0) ? cx<T>{T(0), T(0)} : cx<T>{v.x / a, v.y / a};
+}
+template <typename T> void unit_modulus(T *field, int n, hipStream_t st) {
+  size_t nm = size_t(n) * n * (n / 2 + 1);
+  hipLaunchKernelGGL(k_unit_modulus<T>, dim3((nm + 255) / 256), dim3(256), 0, st, reinterpret_cast<cx<T> *>(field), nm);
+  GIC_HIP(hipGetLastError());
+}
+// reverseSmallK (ic.hpp:1687-1706): modes with 0 < k^2 < kmax^2 change sign,
+// k = i kMin in the field precision as the reference forms it
+template <typename T> __global__ void k_reverse_small_k(cx<T> *__restrict__ f, int n, T kmin, T k2max) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t nm = size_t(n) * n * (n / 2 + 1);
+  if (i >= nm) return;
+  int ix, iy, iz;
+  mode_ints(i, n, ix, iy, iz);
+  T kx = T(ix) * kmin, ky = T(iy) * kmin, kz = T(iz) * kmin;
+  T k2 = kx * kx + ky * ky + kz * kz;
+  if (k2 < k2max && k2 != T(0)) {
+    cx<T> v = f[i];
+    f[i] = cx<T>{-v.x, -v.y};
+  }
+}
+template <typename T> void reverse_small_k(T *field, int n, double kmin, double kmax, hipStream_t st) {
+  size_t nm = size_t(n) * n * (n / 2 + 1);
+  T km = T(kmax);
+  hipLaunchKernelGGL(k_reverse_small_k<T>, dim3((nm + 255) / 256), dim3(256), 0, st, reinterpret_cast<cx<T> *>(field),
+                     n, T(kmin), km * km);
+  GIC_HIP(hipGetLastError());
+}
+template <typename T> __global__ void k_scale_all(T *__restrict__ p, size_t n, T s) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] *= s;
+}
+template <typename T> void scale_all(T *p, size_t n, T s, hipStream_t st) {
+  hipLaunchKernelGGL(k_scale_all<T>, dim3((n + 255) / 256), dim3(256), 0, st, p, n, s);
+  GIC_HIP(hipGetLastError());
+}
+// out[m] = 1 / tab[m] (0 where tab is 0): pow(P, -1/2) from a sqrt P table
+template <typename T> __global__ void k_inverse_table(const T *__restrict__ tab, long n, T *__restrict__ out) {
+  long m = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (m < n) out[m] = tab[m] == T(0) ? T(0) : T(1) / tab[m];
+}
+template <typename T> void inverse_table(const T *tab, long n, T *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_inverse_table<T>, dim3((n + 255) / 256), dim3(256), 0, st, tab, n, out);
+  GIC_HIP(hipGetLastError());
+}
+
 // ---- small utilities -------------------------------------------------------------------
 template <typename T> __global__ void k_fill(T *p, size_t n, T v) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -557,7 +612,11 @@ template <typename S, typename D> void convert(const S *a, size_t n, D *b, hipSt
   template void particles<T>(const T *, const T *, const T *, const T *const *, int, const ParticleGeom &,    \
                              const uint64_t *, size_t, size_t, float *, float *, hipStream_t);                \
   template void lpt_accum<T>(T *, T *, const T *, size_t, int, hipStream_t);                                  \
-  template void fill<T>(T *, size_t, T, hipStream_t);
+  template void fill<T>(T *, size_t, T, hipStream_t);                                                       \
+  template void unit_modulus<T>(T *, int, hipStream_t);                                                     \
+  template void reverse_small_k<T>(T *, int, double, double, hipStream_t);                                  \
+  template void scale_all<T>(T *, size_t, T, hipStream_t);                                                  \
+  template void inverse_table<T>(const T *, long, T *, hipStream_t);
 GIC_INST(float)
 GIC_INST(double)
 template void fill<uint32_t>(uint32_t *, size_t, uint32_t, hipStream_t);

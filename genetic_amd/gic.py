@@ -39,6 +39,10 @@ SIGNATURES = {
     "gic_reseed": ([_v, _u, _i], _i),
     "gic_draw": ([_v], _i),
     "gic_zero_level": ([_v, _i], _i),
+    "gic_set_exact_power_spectrum": ([_v, _i], _i),
+    "gic_reverse": ([_v], _i),
+    "gic_reverse_small_k": ([_v, _d], _i),
+    "gic_import_level": ([_v, _i, _v, _s], _i),
     "gic_apply_power_spectrum": ([_v], _i),
     "gic_zeldovich": ([_v], _i),
     "gic_copy_field": ([_v, _i, _i, _i, _v, _i], _i),
@@ -205,6 +209,19 @@ class ICGenerator:
 
     def zero_level(self, level):
         self._check(lib().gic_zero_level(self._h, int(level)))
+
+    def fix_power(self, on=True):
+        self._check(lib().gic_set_exact_power_spectrum(self._h, int(bool(on))))
+
+    def reverse(self):
+        self._check(lib().gic_reverse(self._h))
+
+    def reverse_small_k(self, kmax):
+        self._check(lib().gic_reverse_small_k(self._h, float(kmax)))
+
+    def import_level(self, level, data):
+        d = np.ascontiguousarray(np.asarray(data, dtype=np.float64)).ravel()
+        self._check(lib().gic_import_level(self._h, int(level), _ptr(d), d.size))
 
     def apply_power_spectrum(self):
         self._check(lib().gic_apply_power_spectrum(self._h))

@@ -235,11 +235,11 @@ std::vector<size_t> argsort(const std::vector<size_t> &v) {
 }
 
 // ---------------------------------------------------------------- npy ------
-void saveNpy(const std::string &fn, int n, const std::vector<double> &d) {
+void saveNpyTyped(const std::string &fn, int n, const std::vector<double> &d, const char *descr) {
   std::ofstream f(fn, std::ios::binary);
   if (!f) throw std::runtime_error("Can't open file " + fn);
-  std::string hdr = "{'descr': '<f8', 'fortran_order': False, 'shape': (" + std::to_string(n) + ", " +
-                    std::to_string(n) + ", " + std::to_string(n) + "), }";
+  std::string hdr = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': (" +
+                    std::to_string(n) + ", " + std::to_string(n) + ", " + std::to_string(n) + "), }";
   size_t total = 10 + hdr.size() + 1;
   hdr += std::string((64 - total % 64) % 64, ' ') + "\n";
   unsigned short hl = (unsigned short)hdr.size();
@@ -248,6 +248,9 @@ void saveNpy(const std::string &fn, int n, const std::vector<double> &d) {
   f.write(hdr.data(), hdr.size());
   f.write((const char *)d.data(), d.size() * sizeof(double));
 }
+void saveNpy(const std::string &fn, int n, const std::vector<double> &d) { saveNpyTyped(fn, n, d, "<f8"); }
+// interleaved (re, im) pairs as complex128
+void saveNpyComplex(const std::string &fn, int n, const std::vector<double> &d) { saveNpyTyped(fn, n, d, "<c16"); }
 
 // ---------------------------------------------------------------- generator
 enum class Fmt { unknown = 1, gadget2 = 2, gadget3 = 3, tipsy = 4, grafic = 5, gadgethdf = 6, swift = 7 };
@@ -722,6 +725,105 @@ struct Generator {
       drawn = true;
     }
   }
+  // --- white-noise adjustments (ic.hpp:669-671, 805-820, 1652-1706) ---
+  void reverse() {
+    ensureDrawn();
+    check(gic_reverse(c()));
+  }
+  void reverseSmallK(double kmax) {
+    ensureDrawn();
+    check(gic_reverse_small_k(c(), kmax));
+  }
+  void importLevel(int level, const std::string &fn) {
+    ensureDrawn();
+    log() << "Importing random field on level " << level << " from " << fn << std::endl;
+    if (level < 0 || level >= int(grids.size())) throw std::runtime_error("Specified level does not exist.");
+    int n = grids[level].n;
+    std::vector<double> d = loadNpy(fn, n);
+    check(gic_import_level(c(), level, d.data(), d.size()));
+    log() << "... success!" << std::endl;
+  }
+  // io::numpy::LoadArrayFromNumpy: a C-ordered n x n x n array of float64 or float32
+  static std::vector<double> loadNpy(const std::string &fn, int n) {
+    std::ifstream f(fn, std::ios::binary);
+    if (!f) throw std::runtime_error("Unable to open file " + fn);
+    char magic[6];
+    f.read(magic, 6);
+    if (!f || std::string(magic, 6) != "\x93NUMPY") throw std::runtime_error(fn + " is not a numpy file");
+    unsigned char ver[2];
+    f.read((char *)ver, 2);
+    uint32_t hl = 0;
+    if (ver[0] == 1) {
+      uint16_t h16;
+      f.read((char *)&h16, 2);
+      hl = h16;
+    } else {
+      f.read((char *)&hl, 4);
+    }
+    std::string hdr(hl, ' ');
+    f.read(&hdr[0], hl);
+    bool f8 = hdr.find("'<f8'") != std::string::npos, f4 = hdr.find("'<f4'") != std::string::npos;
+    if ((!f8 && !f4) || hdr.find("'fortran_order': True") != std::string::npos)
+      throw std::runtime_error("Unsupported numpy array in " + fn);
+    size_t p = hdr.find("'shape':");
+    int dims[3] = {0, 0, 0}, nd = 0;
+    if (p != std::string::npos) {
+      size_t q = hdr.find('(', p), e = hdr.find(')', q);
+      std::string sh = hdr.substr(q + 1, e - q - 1);
+      std::stringstream ss(sh);
+      std::string tok;
+      while (std::getline(ss, tok, ',') && nd < 3)
+        if (tok.find_first_of("0123456789") != std::string::npos) dims[nd++] = std::stoi(tok);
+    }
+    if (nd != 3 || dims[0] != n || dims[1] != n || dims[2] != n)
+      throw std::runtime_error("Incorrect size for imported numpy array");
+    size_t n3 = size_t(n) * n * n;
+    std::vector<double> d(n3);
+    if (f8) {
+      f.read((char *)d.data(), n3 * 8);
+    } else {
+      std::vector<float> t(n3);
+      f.read((char *)t.data(), n3 * 4);
+      for (size_t i = 0; i < n3; ++i) d[i] = t[i];
+    }
+    if (!f) throw std::runtime_error("Truncated numpy file " + fn);
+    return d;
+  }
+  // dumpGridFourier (ic.hpp:926-936): the full complex spectrum n x n x n
+  // (kz over all n, the upper half from Hermitian symmetry) as complex128
+  void dumpGridFourier(int level) {
+    if (level < 0 || level >= int(grids.size())) throw std::runtime_error("Specified level does not exist.");
+    ensureDrawn();
+    int n = grids[level].n, nz = n / 2 + 1;
+    std::vector<double> h(size_t(n) * n * nz * 2);
+    check(gic_copy_field(c(), GIC_FIELD_OVERDENSITY, level, GIC_FOURIER_SPACE, h.data(), GIC_DTYPE_F64));
+    std::vector<double> full(size_t(n) * n * n * 2);
+    for (int x = 0; x < n; ++x)
+      for (int y = 0; y < n; ++y)
+        for (int z = 0; z < n; ++z) {
+          double re, im;
+          if (z < nz) {
+            size_t q = ((size_t(x) * n + y) * nz + z) * 2;
+            re = h[q];
+            im = h[q + 1];
+          } else {
+            int xs = (n - x) % n, ys = (n - y) % n, zs = n - z;
+            size_t q = ((size_t(xs) * n + ys) * nz + zs) * 2;
+            re = h[q];
+            im = -h[q + 1];
+          }
+          size_t o = ((size_t(x) * n + y) * n + z) * 2;
+          full[o] = re;
+          full[o + 1] = im;
+        }
+    saveNpyComplex(outdir + "/grid-" + std::to_string(level) + ".npy", n, full);
+    std::ofstream inf(outdir + "/grid-info-" + std::to_string(level) + "-fourier.txt");
+    const Grid &g = grids[level];
+    inf << g.ox << " " << g.oy << " " << g.oz << " " << g.size() << std::endl;
+    inf << "The line above contains information about grid level " << level << std::endl;
+    inf << "It gives the x-offset, y-offset and z-offset of the low-left corner and also the box length" << std::endl;
+  }
+
   void zeroLevel(int level) {
     log() << "*** Warning: your script calls zeroLevel(" << level << "). This is intended for testing purposes only!"
           << std::endl;
@@ -1708,6 +1810,13 @@ struct Dispatcher {
     add("center_grafic_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));
     add("dump_mask", std::function<void()>([this]() { ic.dumpMask(); }));
     add("dump_grid", std::function<void(int)>([this](int l) { ic.dumpGrid(l); }));
+    add("dump_grid_fourier", std::function<void(int)>([this](int l) { ic.dumpGridFourier(l); }));
+    std::function<void()> fixp = [this]() { ic.check(gic_set_exact_power_spectrum(ic.c(), 1)); };
+    add("fix_power", fixp);
+    add("exact_power_spectrum_enforcement", fixp);
+    add("reverse", std::function<void()>([this]() { ic.reverse(); }));
+    add("reverse_small_k", std::function<void(double)>([this](double k) { ic.reverseSmallK(k); }));
+    add("import_level", std::function<void(int, std::string)>([this](int l, std::string f) { ic.importLevel(l, f); }));
     add("dump_ps", std::function<void(int)>([this](int l) { ic.dumpPS(l); }));
     add("dump_tipsy", S([this](std::string f) { ic.dumpTipsyArray(f); }));
   }

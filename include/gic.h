@@ -82,6 +82,18 @@ int gic_reseed(gic_context *ctx, unsigned long seed, int mode);
 int gic_draw(gic_context *ctx);
 /* zero_level <level> (ic.hpp:787-802). */
 int gic_zero_level(gic_context *ctx, int level);
+/* fix_power (setExactPowerSpectrumEnforcement, ic.hpp:669-671): the draw is
+ * followed by enforceUnitVariance (multilevelfield.hpp:395-407,498-510), every
+ * mode set to unit modulus.  Must precede the draw. */
+int gic_set_exact_power_spectrum(gic_context *ctx, int enable);
+/* reverse (ic.hpp:1652-1656): the white noise of every level changes sign. */
+int gic_reverse(gic_context *ctx);
+/* reverse_small_k <kmax> (ic.hpp:1687-1706): modes with 0 < k^2 < kmax^2 change sign. */
+int gic_reverse_small_k(gic_context *ctx, double kmax);
+/* import_level <level> <file> (importLevel, ic.hpp:805-820): host holds the n^3
+ * real-space dark-matter overdensity of the level (z fastest); it replaces the
+ * level's white noise after the transfer ratio P_dm^-1/2. */
+int gic_import_level(gic_context *ctx, int level, const double *host, size_t n);
 /* applyPowerSpec (ic.hpp:825-843): delta_k = w_k sqrt(P(k)) on every level. */
 int gic_apply_power_spectrum(gic_context *ctx);
 /* Zel'dovich displacements of every level and the combination of levels
