@@ -1031,6 +1031,14 @@ int gic_displacement_forward(gic_context *c, const void *const *noise, void *con
   });
 }
 
+int gic_batch_constrained(gic_context *c, int nreal, const uint64_t *seeds, int ncons, const double *cons,
+                          void *out, int lanes, int use_graph, double *elapsed_ms) {
+  return guarded(c, [&] {
+    if (c->prec == 32) do_batch<float>(c, nreal, seeds, ncons, cons, (float *)out, lanes, use_graph != 0, elapsed_ms);
+    else do_batch<double>(c, nreal, seeds, ncons, cons, (double *)out, lanes, use_graph != 0, elapsed_ms);
+  });
+}
+
 int gic_displacement_adjoint(gic_context *c, const void *const *psi_bar, void *const *noise_bar) {
   return guarded(c, [&] {
     if (!psi_bar || !noise_bar) throw std::runtime_error("displacement adjoint: null pointer array");

@@ -177,6 +177,23 @@ struct DevBuf {
     p = nullptr;
     bytes = 0;
   }
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;   // one owner per allocation
+  DevBuf &operator=(const DevBuf &) = delete;
+  DevBuf(DevBuf &&o) noexcept : p(o.p), bytes(o.bytes) {
+    o.p = nullptr;
+    o.bytes = 0;
+  }
+  DevBuf &operator=(DevBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      bytes = o.bytes;
+      o.p = nullptr;
+      o.bytes = 0;
+    }
+    return *this;
+  }
   ~DevBuf() { release(); }
 };
 
@@ -200,6 +217,9 @@ struct Level {
 }  // namespace gic
 
 struct ModState;   // modif.hip
+namespace gic {
+struct BatchState;   // batch.hip
+}
 
 struct gic_context {
   using Cosmo = gic::Cosmo;
@@ -228,6 +248,7 @@ struct gic_context {
   std::vector<std::vector<uint64_t>> flags;
   double variance_scale = -1;
   std::unique_ptr<ModState> mod;
+  std::shared_ptr<gic::BatchState> batch;   // streams and buffers of batched realisations
   std::vector<std::unique_ptr<Level>> levels;
   unsigned long seed = 0;
   int seed_mode = 0;
@@ -297,5 +318,8 @@ template <typename T> double mod_variance_gradient(gic_context *c, const T *cons
 // quadratic modification steps: the reference's linearised steps (off) or an
 // exact line search along the same direction (on)
 void mod_set_line_search(gic_context *c, bool on);
+template <typename T>
+void do_batch(gic_context *c, int nreal, const uint64_t *seeds, int ncons, const double *cons, T *out, int nlanes,
+              bool graph, double *ms);
 
 }  // namespace gic

@@ -42,6 +42,11 @@ template <typename T>
 void fft_r2c_post(T *d, int n, const T *post, hipStream_t st, TimingHook *hook, const T *src = nullptr);
 // In-place c2r of d; with dst non-null the real result goes to the dense n^3
 // array dst instead (d is then scratch).
+// r2c of nb padded n^3 fields stored back to back, each mode times post[|k|^2] if post is given
+template <typename T> void fft_r2c_batch(T *d, int n, int nb, const T *post, hipStream_t st, TimingHook *hook);
+// builds the twiddle table of size n (a synchronous upload), so that later
+// launches, also those recorded into a graph, need no host work
+template <typename T> void fft_prepare(int n);
 template <typename T> void fft_c2r_to(T *d, T *dst, int n, hipStream_t st, TimingHook *hook);
 // Transpose of one Zel'dovich component (FFT_PRE_ZA followed by c2r): the r2c
 // of src (dense n^3, or d in place when null; d is overwritten) whose last

@@ -221,6 +221,8 @@ template <typename T> struct StridedArgs {
   FilterInline fil;        // FP32: the table's filter product formed in the pass (fil.nf >= 0)
   int in_win0, in_wlen;   // if 0 < in_wlen < n: the input holds only line points (pt - in_win0) mod n < in_wlen,
                           // stored at index (pt - in_win0) mod n
+  int nbatch, nouter;     // nbatch > 1: that many fields, in_bs / out_bs elements apart, nouter outer lines each
+  long in_bs, out_bs;
 };
 
 __device__ __forceinline__ int signed_k(int i, int n) { return i > n / 2 ? i - n : i; }
@@ -259,8 +261,15 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
   int outer = tile / a.nkzb, kzb = tile - outer * a.nkzb;
   int col0 = kzb * C;
   int nc = min(C, a.ncol - col0);
-  const cx<T> *ib = a.in + outer * a.in_os + col0;
-  cx<T> *ob = a.out + outer * a.out_os + col0;
+  long boff_in = 0, boff_out = 0;
+  if (a.nbatch > 1) {
+    int b = outer / a.nouter;
+    outer -= b * a.nouter;
+    boff_in = b * a.in_bs;
+    boff_out = b * a.out_bs;
+  }
+  const cx<T> *ib = a.in + boff_in + outer * a.in_os + col0;
+  cx<T> *ob = a.out + boff_out + outer * a.out_os + col0;
   int ky = signed_k(outer, a.n);
   int ky2 = ky * ky;
   auto gld = [&](int line, int pt) -> cx<T> {
@@ -647,7 +656,8 @@ template <typename T, int N> struct FftN {
     a.ncol = NZ;
     a.nkzb = (NZ + C - 1) / C;
     a.n = N;
-    int nblk = nouter * a.nkzb;
+    a.nouter = nouter;
+    int nblk = nouter * a.nkzb * (a.nbatch > 1 ? a.nbatch : 1);
     size_t shm = sizeof(cx<T>) * C * N;
     const cx<T> *tw = twtable<T>().get(N);
     auto kern = k_fft_strided<T, N, C, DIR, STHREADS, PRE, POST, PASS>;
@@ -677,8 +687,9 @@ template <typename T, int N> struct FftN {
   }
   // z rows of the padded field d; ext (if non-null) is the dense N^3 array the
   // r2c reads its rows from, or the c2r writes its rows to
-  static void zpass(T *d, T scale, int dir, hipStream_t st, TimingHook *hook, const char *name, T *ext = nullptr) {
-    long nrows = long(N) * N;
+  static void zpass(T *d, T scale, int dir, hipStream_t st, TimingHook *hook, const char *name, T *ext = nullptr,
+                    int nfields = 1) {
+    long nrows = long(N) * N * nfields;
     int nblk = int(nrows / ROWS);
     size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
     const cx<T> *tw = twtable<T>().get(N);
@@ -718,6 +729,26 @@ template <typename T, int N> struct FftN {
       strided<-1, PRE_NONE, POST_TABLE>(ax, N, st, hook, "fft_x_post");
     } else {
       strided<-1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x_fwd");
+    }
+  }
+  // r2c of nb padded fields stored back to back (no dense source), the
+  // post table applied by the last pass of each
+  static void r2c_batch(T *d, int nb, const T *post, hipStream_t st, TimingHook *hook) {
+    cx<T> *c = reinterpret_cast<cx<T> *>(d);
+    long fs = long(N) * N * NZ;
+    zpass(d, T(1), -1, st, hook, "fft_z_r2c_batch", nullptr, nb);
+    StridedArgs<T> ay = plain(c, false, T(1));
+    ay.nbatch = nb;
+    ay.in_bs = ay.out_bs = fs;
+    strided<-1, PRE_NONE, POST_NONE, 1>(ay, N, st, hook, "fft_y_fwd_batch");
+    StridedArgs<T> ax = plain(c, true, norm());
+    ax.nbatch = nb;
+    ax.in_bs = ax.out_bs = fs;
+    if (post) {
+      ax.tab = post;
+      strided<-1, PRE_NONE, POST_TABLE>(ax, N, st, hook, "fft_x_post_batch");
+    } else {
+      strided<-1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x_fwd_batch");
     }
   }
   static void c2r(T *d, hipStream_t st, TimingHook *hook, T *dst = nullptr) {
@@ -933,6 +964,14 @@ template <typename T> void fft3d(T *d, int n, int dir, hipStream_t st, TimingHoo
 template <typename T> void fft_r2c_post(T *d, int n, const T *post, hipStream_t st, TimingHook *hook, const T *src) {
   GIC_FFT_SWITCH(n, F::r2c(d, post, st, hook, src))
 }
+template <typename T> void fft_r2c_batch(T *d, int n, int nb, const T *post, hipStream_t st, TimingHook *hook) {
+  if (nb < 1) throw std::runtime_error("fft_r2c_batch: no fields");
+  GIC_FFT_SWITCH(n, F::r2c_batch(d, nb, post, st, hook))
+}
+template <typename T> void fft_prepare(int n) {
+  if (n < 4 || n > 1024 || (n & (n - 1))) throw std::runtime_error("FFT size " + std::to_string(n) + " is not supported");
+  twtable<T>().get(n);
+}
 template <typename T> void fft_c2r_to(T *d, T *dst, int n, hipStream_t st, TimingHook *hook) {
   GIC_FFT_SWITCH(n, F::c2r(d, st, hook, dst))
 }
@@ -971,6 +1010,8 @@ void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, doubl
 #define GIC_FFT_INST(T)                                                                                     \
   template void fft_r2c_post<T>(T *, int, const T *, hipStream_t, TimingHook *, const T *);                 \
   template void fft_c2r_to<T>(T *, T *, int, hipStream_t, TimingHook *);                                    \
+  template void fft_r2c_batch<T>(T *, int, int, const T *, hipStream_t, TimingHook *);                      \
+  template void fft_prepare<T>(int);                                                                        \
   template void fft_r2c_za_adjoint<T>(const T *, T *, T *, int, const T *, int, double, bool, hipStream_t,   \
                                       TimingHook *);                                                          \
   template void fft_c2r_from<T>(const T *, T *, int, int, const T *, int, double, hipStream_t, TimingHook *, T *, \

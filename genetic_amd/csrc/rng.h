@@ -13,9 +13,9 @@ struct U576 {
 // Host-side set-up of one real-space draw: the seeded RANLUX state and the
 // table of jump multipliers b^{-397*KC*2^i} mod m (b = 2^48, m = b^12-b^5+1).
 struct RngPlan {
-  static constexpr int KC = 64;            // ranlxs2 blocks (24 numbers) per chunk
-  static constexpr int CH = KC * 24;       // raw numbers per chunk
   static constexpr int MAXBITS = 32;
+  int kc = 64;                             // ranlxs2 blocks (24 numbers) per chunk: 4..64, set by rng_plan
+  int ch() const { return kc * 24; }       // raw numbers per chunk
   uint64_t seed_words[12];                 // the 12 seeded 48-bit words, oldest first
   U576 y0;                                 // LCG state 11 steps past the seeded state, times ... (see rng.hip)
   U576 jump[MAXBITS];
@@ -35,8 +35,14 @@ struct RngLevelOut {
 };
 template <typename T>
 void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, void *workspace, size_t ws_bytes,
-                   hipStream_t st, struct TimingHook *hook, double scale = 1.0);
+                   hipStream_t st, struct TimingHook *hook, double scale = 1.0, uint32_t *cover_flag = nullptr);
+// With cover_flag the draw has no host synchronisation: a device check sets
+// *cover_flag to 1 if the stream held too few Gaussians (the cells past the
+// end are then left as they were), instead of the host raising an error.
 size_t rng_workspace_bytes(const RngPlan &plan);
+// uploads the ziggurat tables (once, synchronously) ahead of draws that a
+// graph records
+void rng_prepare();
 
 // Raw 24-bit stream (testing / bit-exactness checks): the first n numbers.
 void rng_raw(const RngPlan &plan, uint32_t *d_out, size_t n, void *workspace, size_t ws_bytes, hipStream_t st);

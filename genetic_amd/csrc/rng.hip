@@ -219,6 +219,7 @@ struct RngDev {
   uint64_t seed_words[12];
   U576 y0;
   U576 jump[RngPlan::MAXBITS];
+  int kc;   // ranlxs2 blocks per chunk
 };
 
 __device__ void chunk_start_state(const RngDev &p, long t, uint64_t *x, int64_t &c) {
@@ -301,11 +302,11 @@ template <class R> __device__ __forceinline__ void push_block(const R &r, long a
 }
 
 // Raw numbers of the summary pass, kept for the emit pass: block q, number k
-// of chunk t at ((t/64)*(KC+1)*24 + q*24 + k)*64 + t%64, so a wavefront's 64
-// chunks store and load each number as one contiguous 256-byte segment.
-constexpr int RAWB = (RngPlan::KC + 1) * 24;
-__device__ __forceinline__ size_t raw_at(long t, int q, int k) {
-  return (size_t(t >> 6) * RAWB + q * 24 + k) * 64 + (t & 63);
+// of chunk t at ((t/64)*rawb + q*24 + k)*64 + t%64 with rawb = (kc+1)*24, so a
+// wavefront's 64 chunks store and load each number as one contiguous 256-byte
+// segment.
+__device__ __forceinline__ size_t raw_at(long t, int q, int k, int rawb) {
+  return (size_t(t >> 6) * rawb + q * 24 + k) * 64 + (t & 63);
 }
 
 // Pass 1: per chunk, the exit offset and Gaussian count for each of the four
@@ -318,6 +319,7 @@ __global__ __launch_bounds__(256) void k_rng_summary(RngDev p, long nchunks, uin
   long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= nchunks) return;
   Ring ring{ring_mem + threadIdx.x, int(blockDim.x)};
+  const int kc = p.kc, ch = kc * 24, rawb = (kc + 1) * 24;
   uint64_t x[12];
   int64_t c;
   chunk_start_state(p, t, x, c);
@@ -327,37 +329,37 @@ __global__ __launch_bounds__(256) void k_rng_summary(RngDev p, long nchunks, uin
   uint32_t shared_cnt = 0;   // acceptances after the cursors merged
   bool merged = false;
   long avail = 0;
-  for (int q = 0; q <= RngPlan::KC; ++q) {
+  for (int q = 0; q <= kc; ++q) {
     ranlux_block(x, c);
     push_block(ring, avail, x);
 #pragma unroll
     for (int k = 0; k < 12; ++k) {
-      raw[raw_at(t, q, 2 * k)] = uint32_t(x[k] & 0xFFFFFF);
-      raw[raw_at(t, q, 2 * k + 1)] = uint32_t(x[k] >> 24);
+      raw[raw_at(t, q, 2 * k, rawb)] = uint32_t(x[k] & 0xFFFFFF);
+      raw[raw_at(t, q, 2 * k + 1, rawb)] = uint32_t(x[k] >> 24);
     }
     avail += 24;
     if (merged) {
       long pos = cur[0];
-      while (pos < RngPlan::CH && pos + 4 <= avail) {
+      while (pos < ch && pos + 4 <= avail) {
         bool acc;
         double v;
         pos += zig_iter(get, pos, acc, v);
         shared_cnt += acc;
       }
       cur[0] = pos;
-      if (pos >= RngPlan::CH) break;
+      if (pos >= ch) break;
     } else {
       bool done = true;
       for (int e = 0; e < 4; ++e) {
         long pos = cur[e];
-        while (pos < RngPlan::CH && pos + 4 <= avail) {
+        while (pos < ch && pos + 4 <= avail) {
           bool acc;
           double v;
           pos += zig_iter(get, pos, acc, v);
           cnt[e] += acc;
         }
         cur[e] = pos;
-        done &= pos >= RngPlan::CH;
+        done &= pos >= ch;
       }
       merged = cur[0] == cur[1] && cur[0] == cur[2] && cur[0] == cur[3];
       if (done) break;
@@ -366,7 +368,7 @@ __global__ __launch_bounds__(256) void k_rng_summary(RngDev p, long nchunks, uin
   uint32_t ex = 0;
   for (int e = 0; e < 4; ++e) {
     long end = merged ? cur[0] : cur[e];
-    ex |= uint32_t(end - RngPlan::CH) << (2 * e);
+    ex |= uint32_t(end - ch) << (2 * e);
     summary[t * 5 + 1 + e] = cnt[e] + (merged ? shared_cnt : 0);
   }
   summary[t * 5] = ex;
@@ -482,7 +484,7 @@ template <typename T>
 __global__ __launch_bounds__(64) void k_rng_emit(const uint32_t *__restrict__ raw, long nchunks,
                                                 const uint32_t *__restrict__ summary,
                                                 const uint64_t *__restrict__ segstart, LevelTargets lt,
-                                                size_t total) {
+                                                size_t total, int kc) {
   __shared__ uint32_t ring_mem[EMIT_RING * 64];
   __shared__ T stage_mem[EMIT_STAGE * 64];
   const int lane = threadIdx.x;
@@ -534,14 +536,15 @@ __global__ __launch_bounds__(64) void k_rng_emit(const uint32_t *__restrict__ ra
     }
     __syncthreads();
   };
-  for (int q = 0; q <= RngPlan::KC; ++q) {
-    bool busy = active && pos < RngPlan::CH && g < total;
+  const int ch = kc * 24, rawb = (kc + 1) * 24;
+  for (int q = 0; q <= kc; ++q) {
+    bool busy = active && pos < ch && g < total;
     if (!__any(busy)) break;
     if (busy) {
 #pragma unroll
-      for (int k = 0; k < 24; ++k) ring.put(avail + k, raw[raw_at(t, q, k)]);
+      for (int k = 0; k < 24; ++k) ring.put(avail + k, raw[raw_at(t, q, k, rawb)]);
       avail += 24;
-      while (pos < RngPlan::CH && pos + 4 <= avail && g < total) {
+      while (pos < ch && pos + 4 <= avail && g < total) {
         bool acc;
         double v;
         pos += zig_iter(get, pos, acc, v);
@@ -564,8 +567,8 @@ __global__ __launch_bounds__(256) void k_rng_raw(RngDev p, long nchunks, uint32_
   uint64_t x[12];
   int64_t c;
   chunk_start_state(p, t, x, c);
-  size_t base = size_t(t) * RngPlan::CH;
-  for (int q = 0; q < RngPlan::KC; ++q) {
+  size_t base = size_t(t) * (p.kc * 24);
+  for (int q = 0; q < p.kc; ++q) {
     ranlux_block(x, c);
     for (int k = 0; k < 12; ++k) {
       size_t i = base + q * 24 + 2 * k;
@@ -672,9 +675,18 @@ void rng_plan(unsigned long seed, size_t n_gaussians, RngPlan &plan) {
       c = t >> 64;
     }
   }
-  // The chunk-t start state is step T_t = 397*KC*t; the device decodes the
+  // raw numbers needed: 2 per Gaussian plus rejections (~1.2% of iterations
+  // take 3-4 numbers and may repeat); 2.2x plus slack is never exceeded in
+  // practice and is checked after the scan.  Chunks of kc blocks: long
+  // chunks amortise the jump to each chunk's start, and a small draw takes
+  // shorter ones so that it still has about 64k chunks (one wavefront per
+  // SIMD) to run on.
+  double need = 2.2 * double(n_gaussians) + 4096.0;
+  plan.kc = 64;
+  while (plan.kc > 4 && need / (plan.kc * 24.0) < 65536.0) plan.kc /= 2;
+  // The chunk-t start state is step T_t = 397*kc*t; the device decodes the
   // LCG state V_{T-12} = Z_{T-11} = b^{-(T-12)} V_0 and runs 12 steps.
-  // y0 = b^{12} V_0 ;  jump[i] = b^{-397*KC*2^i}.
+  // y0 = b^{12} V_0 ;  jump[i] = b^{-397*kc*2^i}.
   uint64_t b12[9];
   {
     uint64_t one_b[9] = {0};
@@ -683,27 +695,23 @@ void rng_plan(unsigned long seed, size_t n_gaussians, RngPlan &plan) {
   }
   u576_mulmod(b12, V, plan.y0.w);
   uint64_t step[9];
-  u576_powmod(binv, (u128)397 * RngPlan::KC, step);
+  u576_powmod(binv, (u128)397 * plan.kc, step);
   for (int i = 0; i < RngPlan::MAXBITS; ++i) {
     memcpy(plan.jump[i].w, step, sizeof step);
     uint64_t t[9];
     u576_mulmod(step, step, t);
     memcpy(step, t, sizeof t);
   }
-  // raw numbers needed: 2 per Gaussian plus rejections (~1.2% of iterations
-  // take 3-4 numbers and may repeat); 2.2x plus slack is never exceeded in
-  // practice and is checked after the scan.
-  double need = 2.2 * double(n_gaussians) + 4096.0;
-  plan.nchunks = long(need / RngPlan::CH) + 2;
+  plan.nchunks = long(need / plan.ch()) + 2;
 }
 
-static size_t raw_bytes(long nc) { return size_t((nc + 63) / 64) * 64 * RAWB * 4; }
+static size_t raw_bytes(long nc, int kc) { return size_t((nc + 63) / 64) * 64 * size_t(kc + 1) * 24 * 4; }
 
 size_t rng_workspace_bytes(const RngPlan &plan) {
   long nc = plan.nchunks;
   long nseg = (nc + SEG - 1) / SEG;
   return (size_t(nc) * 5 * 4 + 255) / 256 * 256 + (size_t(nseg) * 5 * 4 + 255) / 256 * 256 +
-         (size_t(nseg) * 8 + 255) / 256 * 256 + raw_bytes(nc) + 256;
+         (size_t(nseg) * 8 + 255) / 256 * 256 + raw_bytes(nc, plan.kc) + 256;
 }
 
 struct RngBufs {
@@ -724,7 +732,7 @@ static RngBufs carve(const RngPlan &plan, void *ws, size_t bytes) {
   r.summary = (uint32_t *)take(size_t(nc) * 5 * 4);
   r.segmaps = (uint32_t *)take(size_t(nseg) * 5 * 4);
   r.segstart = (uint64_t *)take(size_t(nseg) * 8);
-  r.raw = (uint32_t *)take(raw_bytes(nc));
+  r.raw = (uint32_t *)take(raw_bytes(nc, plan.kc));
   r.nseg = nseg;
   if ((size_t)(p - (char *)ws) > bytes) throw std::runtime_error("RNG workspace too small");
   return r;
@@ -739,17 +747,29 @@ static void upload_tables() {
   done = true;
 }
 
+void rng_prepare() { upload_tables(); }
+
 static RngDev to_dev(const RngPlan &plan) {
   RngDev d;
   memcpy(d.seed_words, plan.seed_words, sizeof d.seed_words);
   d.y0 = plan.y0;
   memcpy(d.jump, plan.jump, sizeof d.jump);
+  d.kc = plan.kc;
   return d;
+}
+
+// the device-side form of the coverage check below
+__global__ void k_rng_cover(const uint64_t *__restrict__ last_start, const uint32_t *__restrict__ last_map,
+                            uint64_t total, uint32_t *__restrict__ flag) {
+  if (threadIdx.x != 0) return;
+  uint64_t last = *last_start;
+  uint64_t produced = (last >> 2) + last_map[1 + (last & 3)];
+  if (produced < total) *flag = 1u;
 }
 
 template <typename T>
 void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, void *ws, size_t ws_bytes,
-                   hipStream_t st, TimingHook *hook, double scale) {
+                   hipStream_t st, TimingHook *hook, double scale, uint32_t *cover_flag) {
   if (levels.empty() || levels.size() > 4) throw std::runtime_error("real-space draw supports 1 to 4 levels");
   upload_tables();
   RngBufs b = carve(plan, ws, ws_bytes);
@@ -768,13 +788,6 @@ void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, 
   hipLaunchKernelGGL(k_rng_seg_scan, dim3(1), dim3(1024), 0, st, b.segmaps, b.nseg, b.segstart);
   GIC_HIP(hipGetLastError());
   if (hook) hook->end("rng_scan", st);
-  // coverage: the stream must hold at least `total` Gaussians
-  uint64_t last = 0;
-  uint32_t lastmap[5];
-  GIC_HIP(hipMemcpyAsync(&last, b.segstart + (b.nseg - 1), 8, hipMemcpyDeviceToHost, st));
-  GIC_HIP(hipMemcpyAsync(lastmap, b.segmaps + 5 * (b.nseg - 1), 20, hipMemcpyDeviceToHost, st));
-  GIC_HIP(hipStreamSynchronize(st));
-  uint64_t produced = (last >> 2) + lastmap[1 + (last & 3)];
   LevelTargets lt;
   size_t acc = 0;
   for (size_t i = 0; i < 4; ++i) {
@@ -785,23 +798,36 @@ void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, 
   }
   lt.scale = scale;
   size_t total = acc;
-  if (produced < total) throw std::runtime_error("internal error: RNG plan produced too few Gaussians");
+  // coverage: the stream must hold at least `total` Gaussians
+  if (cover_flag) {
+    hipLaunchKernelGGL(k_rng_cover, dim3(1), dim3(64), 0, st, (const uint64_t *)(b.segstart + (b.nseg - 1)),
+                       (const uint32_t *)(b.segmaps + 5 * (b.nseg - 1)), uint64_t(total), cover_flag);
+    GIC_HIP(hipGetLastError());
+  } else {
+    uint64_t last = 0;
+    uint32_t lastmap[5];
+    GIC_HIP(hipMemcpyAsync(&last, b.segstart + (b.nseg - 1), 8, hipMemcpyDeviceToHost, st));
+    GIC_HIP(hipMemcpyAsync(lastmap, b.segmaps + 5 * (b.nseg - 1), 20, hipMemcpyDeviceToHost, st));
+    GIC_HIP(hipStreamSynchronize(st));
+    uint64_t produced = (last >> 2) + lastmap[1 + (last & 3)];
+    if (produced < total) throw std::runtime_error("internal error: RNG plan produced too few Gaussians");
+  }
   if (hook) hook->begin("rng_emit", st);
   hipLaunchKernelGGL(k_rng_emit<T>, dim3((nc + 63) / 64), dim3(64), 0, st, b.raw, nc, b.summary, b.segstart, lt,
-                     total);
+                     total, plan.kc);
   GIC_HIP(hipGetLastError());
   if (hook) hook->end("rng_emit", st);
 }
 
 template void rng_draw_real<float>(const RngPlan &, const std::vector<RngLevelOut> &, void *, size_t, hipStream_t,
-                                   TimingHook *, double);
+                                   TimingHook *, double, uint32_t *);
 template void rng_draw_real<double>(const RngPlan &, const std::vector<RngLevelOut> &, void *, size_t, hipStream_t,
-                                    TimingHook *, double);
+                                    TimingHook *, double, uint32_t *);
 
 void rng_raw(const RngPlan &plan, uint32_t *d_out, size_t n, void *, size_t, hipStream_t st) {
-  if (n > size_t(plan.nchunks) * RngPlan::CH) throw std::runtime_error("rng_raw: plan too small");
+  if (n > size_t(plan.nchunks) * plan.ch()) throw std::runtime_error("rng_raw: plan too small");
   RngDev d = to_dev(plan);
-  long nc = long((n + RngPlan::CH - 1) / RngPlan::CH);
+  long nc = long((n + plan.ch() - 1) / plan.ch());
   hipLaunchKernelGGL(k_rng_raw, dim3((nc + 255) / 256), dim3(256), 0, st, d, nc, d_out, n);
   GIC_HIP(hipGetLastError());
 }

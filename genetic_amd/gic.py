@@ -59,6 +59,7 @@ SIGNATURES = {
     "gic_displacement_forward": ([_v, _v, _v], _i),
     "gic_displacement_adjoint": ([_v, _v, _v], _i),
     "gic_variance_gradient": ([_v, _v, _P(_d), _v], _i),
+    "gic_batch_constrained": ([_v, _i, _v, _i, _v, _v, _i, _i, _P(_d)], _i),
     "gic_set_quadratic_line_search": ([_v, _i], _i),
     "gic_particles": ([_v, _i, _v, _s, _s, _v, _v], _i),
     "gic_particle_mass_eps": ([_v, _i, _P(_d), _P(_d)], _i),
@@ -322,6 +323,23 @@ class ICGenerator:
         g = self._ptrs(grad_ptrs) if grad_ptrs is not None else None
         self._check(lib().gic_variance_gradient(self._h, self._ptrs(noise_ptrs), ctypes.byref(v), g))
         return v.value
+
+    def batch_constrained(self, seeds, cons, out_ptr, lanes=4, graph=True):
+        """Batched constrained realisations of a one-level context: seeds[r]
+        (real-space draws), cons[r, i] = (x, y, z, radius, target) top-hat
+        mean-overdensity constraints with absolute targets; out_ptr (device,
+        the context's precision) receives delta, psi_x, psi_y, psi_z per
+        realisation as dense n^3 arrays at out + r*4*n^3.  Returns the elapsed
+        milliseconds from the first recorded work to the end of the batch."""
+        seeds = np.ascontiguousarray(seeds, dtype=np.uint64).ravel()
+        cons = np.ascontiguousarray(cons, dtype=np.float64)
+        if cons.ndim != 3 or cons.shape[2] != 5 or cons.shape[0] != seeds.size:
+            raise ValueError("cons must have shape (len(seeds), ncons, 5)")
+        ms = ctypes.c_double()
+        self._check(lib().gic_batch_constrained(self._h, int(seeds.size), _ptr(seeds), int(cons.shape[1]), _ptr(cons),
+                                                ctypes.c_void_p(int(out_ptr)), int(lanes), int(bool(graph)),
+                                                ctypes.byref(ms)))
+        return ms.value
 
     def mode_power(self, level):
         """Per-|k|^2 sums of |delta_k|^2, mode counts and theory covariance."""

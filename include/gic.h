@@ -164,6 +164,25 @@ int gic_variance_gradient(gic_context *ctx, const void *const *noise, double *va
  * direction, so that each step lands on its intermediate target. */
 int gic_set_quadratic_line_search(gic_context *ctx, int enable);
 
+/* ---- Batched realisations (no counterpart in the reference, which makes one
+ * realisation per run) ----
+ * For a context with exactly one level: nreal realisations, realisation r
+ * drawn in real space from seeds[r] (the stream of gic_draw), then
+ * constrained by ncons (1..32) top-hat mean-overdensity modifications with
+ * absolute targets -- cons[5 (r ncons + i) + {0,1,2,3,4}] = centre x, y, z
+ * (Mpc/h), radius (Mpc/h), target -- as select_sphere + modify overdensity
+ * absolute + apply_modifications would (ic.hpp:1458-1500,1613-1631,
+ * modificationmanager.hpp:160-312), then the transfer function and the
+ * Zel'dovich displacement.  out (device, the context's precision) receives
+ * per realisation four dense n^3 arrays: delta, psi_x, psi_y, psi_z at
+ * out + r*4*n^3.  lanes (1..16) chains run concurrently on streams of their
+ * own; use_graph != 0 records the whole batch into one HIP graph and launches
+ * it once.  elapsed_ms (optional) receives the time from the first recorded
+ * work to the end of the batch, graph recording included.  The context's own
+ * fields are untouched; no chi^2 is logged. */
+int gic_batch_constrained(gic_context *ctx, int nreal, const uint64_t *seeds, int ncons, const double *cons,
+                          void *out, int lanes, int use_graph, double *elapsed_ms);
+
 /* Power-spectrum histogram for dump_ps (dumpPowerSpectrum,
  * cosmology/parameters.hpp:112-205): over every (ix,iy,iz) in [-n/2,n/2]^3,
  * sum_power[m] += |delta_k|^2 and count[m] += 1 with m = ix^2+iy^2+iz^2;
