@@ -256,7 +256,7 @@ def batch_constraints(nreal=64, nmods=16, box=50.0, seed=20261202):
     return out
 
 
-def batch_throughput(nreal=64, n=128, box=50.0, nmods=16, precision=32, lanes=4, reps=3):
+def batch_throughput(nreal=64, n=128, box=50.0, nmods=16, precision=32, lanes=8, reps=3):
     """The batching workload: nreal independent n^3 realisations (real-space
     seeds 1..nreal), each with nmods overdensity constraints, then delta and
     Zel'dovich psi; FP32 fields, FP64 Gram solve.  The batch runs as one HIP

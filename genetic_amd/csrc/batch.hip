@@ -27,10 +27,9 @@ namespace {
 
 constexpr int MAXCONS = 32;        // constraints per realisation
 constexpr int GRAM_THREADS = 256;
-constexpr int GRAM_K = 64;         // modes per LDS chunk
-constexpr int GRAM_S = GRAM_K + 1; // LDS row stride: rows of different vectors start in different banks
 constexpr int GRAM_BLOCKS = 512;
-constexpr int GRAM_PP = 3;         // pairs per thread: (32*33/2 + 32) / 256 < 3
+constexpr int GRAM_VMAX = 36;      // MAXCONS + 1 vectors rounded up to tiles of 4
+constexpr int GRAM_NWMAX = 45;     // tile pairs of 9 tiles
 
 __device__ __forceinline__ double stored_mode_weight(int x, int y, int z, int h) {
   // each conjugate pair once: the kz = 0 and kz = N/2 planes hold both halves
@@ -43,7 +42,8 @@ __device__ __forceinline__ double stored_mode_weight(int x, int y, int z, int h)
 }
 
 // 1 on the cells of each constraint's sphere, 0 elsewhere (padding included);
-// cnt[f] += the number of cells
+// cnt[f] += the number of cells.  The whole-field form, for spheres too large
+// for a box.
 template <typename T>
 __global__ __launch_bounds__(256) void k_batch_tophat(T *__restrict__ cov, long fs, int n, double dx, double ox,
                                                       double oy, double oz, double periodic,
@@ -55,10 +55,9 @@ __global__ __launch_bounds__(256) void k_batch_tophat(T *__restrict__ cov, long 
   unsigned tot = unsigned(n) * unsigned(n) * unsigned(n + 2);
   bool inside = false;
   bool live = i < tot;
-  int z = 0;
   if (live) {
     unsigned row = i / unsigned(n + 2);
-    z = int(i - row * unsigned(n + 2));
+    int z = int(i - row * unsigned(n + 2));
     int x = int(row / unsigned(n)), y = int(row - unsigned(x) * unsigned(n));
     if (z < n) {
       const double *q = cons + 5 * f;
@@ -85,73 +84,149 @@ __global__ __launch_bounds__(256) void k_batch_tophat(T *__restrict__ cov, long 
   if (live) cov[size_t(f) * fs + i] = inside ? T(1) : T(0);
 }
 
-// Partial sums of every <c_i, c_j> (j <= i) and <c_i, w>: chunks of GRAM_K
-// modes of all vectors staged through LDS, one pair per thread slot.  Pair p
-// < nm(nm+1)/2 is (i, j) with p = i(i+1)/2 + j; the next nm pairs are (i, w).
+// The rows of each constraint's box (wl x wl cells in x, y from (winx[f],
+// winy[f]) mod n, all z): 1 on the cells of the sphere, 0 elsewhere; row
+// (f wl + i) wl + j is x = winx[f]+i, y = winy[f]+j.  Rows past nm wl^2 (up
+// to nrows) are zero.  cnt[f] += the number of cells of constraint f.  The
+// box holds every selected cell: a cell is selected only if its centroid
+// lies within the radius of the centre, and the box spans the centre's
+// +-radius with a cell to spare.
+template <typename T>
+__global__ __launch_bounds__(256) void k_batch_box_rows(T *__restrict__ rows, unsigned nrows, int nm, int wl, int n,
+                                                        double dx, double ox, double oy, double oz, double periodic,
+                                                        const double *__restrict__ cons, const int *__restrict__ winx,
+                                                        const int *__restrict__ winy, uint32_t *__restrict__ cnt) {
+#pragma clang fp contract(off)
+  unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
+  bool live = e < nrows * unsigned(n);
+  unsigned row = live ? e / unsigned(n) : 0u;
+  int z = int(e - row * unsigned(n));
+  unsigned ww = unsigned(wl) * unsigned(wl);
+  int f = int(row / ww);
+  bool inside = false;
+  if (live && f < nm) {
+    unsigned r2 = row - unsigned(f) * ww;
+    int i = int(r2 / unsigned(wl)), j = int(r2 - unsigned(i) * unsigned(wl));
+    int x = (winx[f] + i) % n, y = (winy[f] + j) % n;
+    const double *q = cons + 5 * f;
+    double half = periodic / 2;
+    auto off = [&](int idx, double o, double c) {
+      double r = double(idx) * dx;
+      r = r + o;
+      r = r + dx / 2;
+      r = r - c;
+      if (r > half) r -= periodic;
+      if (r < -half) r += periodic;
+      return r;
+    };
+    double rx = off(x, ox, q[0]), ry = off(y, oy, q[1]), rz = off(z, oz, q[2]);
+    float rf = float(q[3]);
+    float rr = rf * rf;
+    double d2 = rx * rx + ry * ry;
+    d2 = d2 + rz * rz;
+    inside = d2 < double(rr);
+  }
+  // one atomic per wavefront when its lanes share a constraint
+  int f0 = __shfl(f, 0);
+  unsigned long long m = __ballot(inside);
+  if (__all(!live || f == f0)) {
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(cnt + f0, uint32_t(__popcll(m)));
+  } else if (inside) {
+    atomicAdd(cnt + f, 1u);
+  }
+  if (live) rows[e] = inside ? T(1) : T(0);
+}
+
+// Partial sums of the products of the vectors v_0..v_nm (v_i = c_i for i <
+// nm, v_nm = w), weighted by the stored-mode weights: chunks of K modes of
+// every vector are staged through LDS, and each thread forms a 4 x 4 tile of
+// products (vectors 4ti+u against 4tj+v, tj <= ti) over every G-th mode of
+// the chunk, so that each LDS value feeds four products.  The G mode-groups
+// of a tile are summed in order of the group at the end; block b writes the
+// 16 values of tile pair p = ti(ti+1)/2 + tj at part[(b nw + p) 16 + 4u + v].
 template <typename T>
 __global__ __launch_bounds__(GRAM_THREADS) void k_batch_gram(const cx<T> *__restrict__ cov, long fs,
                                                              const cx<T> *__restrict__ w, int nm, int n,
                                                              double *__restrict__ part) {
-  __shared__ cx<T> sv[(MAXCONS + 1) * GRAM_S];
-  __shared__ double sw[GRAM_K];
+  constexpr int K = sizeof(T) == 4 ? 128 : 64;
+  constexpr int S = K + 1;   // LDS row stride: rows of different vectors start in different banks
+  __shared__ cx<T> sv[GRAM_VMAX * S];
+  __shared__ double sw[K];
+  __shared__ double red[GRAM_NWMAX * 16];
   const int nv = nm + 1;
-  const int np = nm * (nm + 1) / 2 + nm;
+  const int nt = (nv + 3) / 4;
+  const int nw = nt * (nt + 1) / 2;
+  const int G = GRAM_THREADS / nw;
   const int h = n / 2;
   const size_t nmodes = size_t(n) * n * (h + 1);
-  const size_t nchunks = (nmodes + GRAM_K - 1) / GRAM_K;
-  int pa[GRAM_PP], pb[GRAM_PP];
-  double acc[GRAM_PP];
+  const size_t nchunks = (nmodes + K - 1) / K;
+  const int t = threadIdx.x;
+  const int p = t % nw, g = t / nw;
+  const bool live = g < G;
+  int ti = 0;
+  while ((ti + 1) * (ti + 2) / 2 <= p) ++ti;
+  const int tj = p - ti * (ti + 1) / 2;
+  double acc[16];
 #pragma unroll
-  for (int u = 0; u < GRAM_PP; ++u) {
-    int p = threadIdx.x + u * GRAM_THREADS;
-    acc[u] = 0;
-    pa[u] = pb[u] = -1;
-    if (p < nm * (nm + 1) / 2) {
-      int i = 0;
-      while ((i + 1) * (i + 2) / 2 <= p) ++i;
-      pa[u] = i;
-      pb[u] = p - i * (i + 1) / 2;
-    } else if (p < np) {
-      pa[u] = p - nm * (nm + 1) / 2;
-      pb[u] = nm;
-    }
-  }
+  for (int e = 0; e < 16; ++e) acc[e] = 0;
   for (size_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
-    size_t m0 = ch * GRAM_K;
-    for (int e = threadIdx.x; e < nv * GRAM_K; e += GRAM_THREADS) {
-      int v = e / GRAM_K, q = e - v * GRAM_K;
+    size_t m0 = ch * K;
+    for (int e = t; e < nt * 4 * K; e += GRAM_THREADS) {
+      int v = e / K, q = e - v * K;
       size_t m = m0 + q;
       cx<T> val{T(0), T(0)};
-      if (m < nmodes) val = v < nm ? cov[size_t(v) * fs + m] : w[m];
-      sv[v * GRAM_S + q] = val;
+      if (m < nmodes && v < nv) val = v < nm ? cov[size_t(v) * fs + m] : w[m];
+      sv[v * S + q] = val;
     }
-    if (threadIdx.x < GRAM_K) {
-      size_t m = m0 + threadIdx.x;
+    if (t < K) {
+      size_t m = m0 + t;
       double wt = 0;
       if (m < nmodes) {
         int z = int(m % size_t(h + 1));
         size_t r = m / size_t(h + 1);
         wt = stored_mode_weight(int(r / size_t(n)), int(r % size_t(n)), z, h);
       }
-      sw[threadIdx.x] = wt;
+      sw[t] = wt;
     }
     __syncthreads();
+    if (live) {
+      const cx<T> *A = sv + 4 * ti * S;
+      const cx<T> *B = sv + 4 * tj * S;
+      for (int q = g; q < K; q += G) {
+        double wt = sw[q];
+        double ax[4], ay[4], bx[4], by[4];
 #pragma unroll
-    for (int u = 0; u < GRAM_PP; ++u) {
-      if (pa[u] < 0) continue;
-      const cx<T> *a = sv + pa[u] * GRAM_S;
-      const cx<T> *b = sv + pb[u] * GRAM_S;
-      double s = 0;
-      for (int q = 0; q < GRAM_K; ++q) s += sw[q] * (double(a[q].x) * double(b[q].x) + double(a[q].y) * double(b[q].y));
-      acc[u] += s;
+        for (int u = 0; u < 4; ++u) {
+          cx<T> va = A[u * S + q], vb = B[u * S + q];
+          ax[u] = wt * double(va.x);
+          ay[u] = wt * double(va.y);
+          bx[u] = double(vb.x);
+          by[u] = double(vb.y);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[u * 4 + v] += ax[u] * bx[v] + ay[u] * by[v];
+      }
     }
     __syncthreads();
   }
+  for (int e = t; e < nw * 16; e += GRAM_THREADS) red[e] = 0;
+  __syncthreads();
+  for (int gg = 0; gg < G; ++gg) {
+    if (live && g == gg) {
 #pragma unroll
-  for (int u = 0; u < GRAM_PP; ++u) {
-    int p = threadIdx.x + u * GRAM_THREADS;
-    if (p < np) part[size_t(blockIdx.x) * np + p] = acc[u];
+      for (int e = 0; e < 16; ++e) red[p * 16 + e] += acc[e];
+    }
+    __syncthreads();
   }
+  for (int e = t; e < nw * 16; e += GRAM_THREADS) part[size_t(blockIdx.x) * nw * 16 + e] = red[e];
+}
+
+// where the product of v_i and v_j (i >= j) sits among the tile sums
+__device__ __forceinline__ int gram_slot(int i, int j) {
+  int ti = i >> 2, tj = j >> 2;
+  return ((ti * (ti + 1) / 2 + tj) << 4) + ((i & 3) << 2) + (j & 3);
 }
 
 // sums[p] = sum over blocks of part[b * np + p], one block per pair, fixed order
@@ -189,12 +264,11 @@ __global__ __launch_bounds__(64) void k_batch_solve(const double *__restrict__ s
     return;
   }
   const double inv = 1.0 / double(ni);
-  const int ng = nm * (nm + 1) / 2;
   double a[MAXCONS];
 #pragma unroll
   for (int j = 0; j < MAXCONS; ++j) {
     double invj = __shfl(inv, j);
-    a[j] = (row && j <= i) ? sums[i * (i + 1) / 2 + j] * inv * invj : 0.0;
+    a[j] = (row && j <= i) ? sums[gram_slot(i, j)] * inv * invj : 0.0;
   }
   // constant trip counts (k < nm as a predicate) keep a[] in registers
 #pragma unroll
@@ -216,7 +290,7 @@ __global__ __launch_bounds__(64) void k_batch_solve(const double *__restrict__ s
     }
   }
   // forward substitution, column by column
-  double y = row ? cons[5 * i + 4] - sums[ng + i] * inv : 0.0;
+  double y = row ? cons[5 * i + 4] - sums[gram_slot(nm, i)] * inv : 0.0;
 #pragma unroll
   for (int k = 0; k < MAXCONS; ++k) {
     const bool act = k < nm;
@@ -274,13 +348,17 @@ struct BatchLane {
   hipStream_t st = nullptr;
   hipEvent_t done = nullptr;
   DevBuf w, cov, scratch, ws, part, sums, cnt, coef;
+  DevBuf rows, zbuf, ybuf;   // covectors from the constraints' boxes
 };
 
 struct BatchState {
   std::vector<std::unique_ptr<BatchLane>> lanes;   // lanes never move once made
   hipEvent_t fork = nullptr;
-  DevBuf cons, flags;
+  DevBuf cons, flags, win, plans;
+  hipGraphExec_t exec = nullptr;   // the last recorded batch, replayed while its shape and buffers stay
+  std::vector<uint64_t> key;
   ~BatchState() {
+    if (exec) (void)hipGraphExecDestroy(exec);
     for (auto &l : lanes) {
       if (l->done) (void)hipEventDestroy(l->done);
       if (l->st) (void)hipStreamDestroy(l->st);
@@ -308,7 +386,8 @@ void do_batch(gic_context *c, int nreal, const uint64_t *seeds, int ncons, const
   const long fs_reals = long(g.padded_reals());
   const long fs = long(g.nmodes());   // complex elements per padded field
   const size_t n3 = g.n3();
-  const int np = ncons * (ncons + 1) / 2 + ncons;
+  const int ntile = (ncons + 1 + 3) / 4;
+  const int np = ntile * (ntile + 1) / 2 * 16;   // tile sums of the Gram kernel
 
   // everything a recorded graph needs exists before the first launch
   fft_prepare<T>(n);
@@ -327,8 +406,32 @@ void do_batch(gic_context *c, int nreal, const uint64_t *seeds, int ncons, const
     GIC_HIP(hipEventCreateWithFlags(&ln.done, hipEventDisableTiming));
   }
   if (!B.fork) GIC_HIP(hipEventCreateWithFlags(&B.fork, hipEventDisableTiming));
+  // boxes around the spheres: x, y windows of wl cells from
+  // floor((c - o - R)/dx - 1/2) (mod n), which hold every selected cell
+  int wl = 0;
+  for (int q = 0; q < nreal * ncons; ++q) wl = std::max(wl, 2 * int(std::ceil(cons[5 * q + 3] / g.dx)) + 3);
+  // (the row kernel indexes rows x n values in 32 bits)
+  const bool boxes = wl < n && (double(ncons) * wl * wl + 1024.0) * n < 2e9;
+  const int rows_mult = fft_box_rows_multiple<T>(n);
+  const long nrows = boxes ? (long(ncons) * wl * wl + rows_mult - 1) / rows_mult * rows_mult : 0;
+  std::vector<int> win(boxes ? size_t(nreal) * ncons * 2 : 0);
+  if (boxes) {
+    const double org[2] = {g.ox, g.oy};
+    for (int q = 0; q < nreal * ncons; ++q)
+      for (int a = 0; a < 2; ++a) {
+        long w0 = long(std::floor((cons[5 * q + a] - org[a] - cons[5 * q + 3]) / g.dx - 0.5));
+        w0 %= n;
+        if (w0 < 0) w0 += n;
+        win[size_t(q) * 2 + a] = int(w0);
+      }
+  }
   for (int k = 0; k < nlanes; ++k) {
     BatchLane &ln = *B.lanes[k];
+    if (boxes) {
+      ln.rows.ensure(size_t(nrows) * n * sizeof(T));
+      ln.zbuf.ensure(size_t(nrows) * (n + 2) * sizeof(T));
+      ln.ybuf.ensure(size_t(ncons) * wl * n * (n + 2) * sizeof(T));
+    }
     ln.w.ensure(fs_reals * sizeof(T));
     ln.cov.ensure(size_t(ncons) * fs_reals * sizeof(T));
     ln.scratch.ensure(fs_reals * sizeof(T));
@@ -341,10 +444,54 @@ void do_batch(gic_context *c, int nreal, const uint64_t *seeds, int ncons, const
   B.cons.ensure(size_t(nreal) * ncons * 5 * sizeof(double));
   B.flags.ensure(size_t(nreal) * sizeof(uint32_t));
   GIC_HIP(hipMemcpyAsync(B.cons.p, cons, size_t(nreal) * ncons * 5 * sizeof(double), hipMemcpyHostToDevice, c->st));
+  if (boxes) {
+    // per realisation: the x origins of its boxes, then the y origins
+    std::vector<int> wxy(size_t(nreal) * ncons * 2);
+    for (int r = 0; r < nreal; ++r)
+      for (int f = 0; f < ncons; ++f) {
+        wxy[(size_t(r) * 2 + 0) * ncons + f] = win[(size_t(r) * ncons + f) * 2 + 0];
+        wxy[(size_t(r) * 2 + 1) * ncons + f] = win[(size_t(r) * ncons + f) * 2 + 1];
+      }
+    B.win.ensure(wxy.size() * sizeof(int));
+    GIC_HIP(hipMemcpy(B.win.p, wxy.data(), wxy.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
   fill<uint32_t>((uint32_t *)B.flags.p, size_t(nreal), 0u, c->st);
+  // every realisation's RNG plan in device memory: the recorded work holds
+  // no seed, so one instantiated graph serves every batch of this shape
   std::vector<RngPlan> plans(nreal);
-  for (int r = 0; r < nreal; ++r) rng_plan(seeds[r], n3, plans[r]);
+  const size_t pb = (rng_dev_bytes() + 255) / 256 * 256;
+  std::vector<char> hplans(size_t(nreal) * pb, 0);
+  for (int r = 0; r < nreal; ++r) {
+    rng_plan(seeds[r], n3, plans[r]);
+    rng_dev_pack(plans[r], hplans.data() + size_t(r) * pb);
+  }
+  B.plans.ensure(hplans.size());
+  GIC_HIP(hipMemcpy(B.plans.p, hplans.data(), hplans.size(), hipMemcpyHostToDevice));
   GIC_HIP(hipStreamSynchronize(c->st));
+  std::vector<uint64_t> key;
+  {
+    auto u = [&](double v) {
+      uint64_t b;
+      memcpy(&b, &v, 8);
+      key.push_back(b);
+    };
+    auto q = [&](const void *ptr) { key.push_back(uint64_t(uintptr_t(ptr))); };
+    for (long v : {long(sizeof(T)), long(n), long(nreal), long(ncons), long(nlanes), long(boxes), long(wl), nrows,
+                   long(np), plans[0].nchunks, long(plans[0].kc)})
+      key.push_back(uint64_t(v));
+    for (double v : {g.dx, g.ox, g.oy, g.oz, g.periodic, g.kmin()}) u(v);
+    for (const void *ptr : std::initializer_list<const void *>{out, B.cons.p, B.flags.p, B.win.p, B.plans.p, L.sqrtp.p,
+                                                               L.sqrtp_dm.p})
+      q(ptr);
+    for (int k = 0; k < nlanes; ++k) {
+      BatchLane &ln = *B.lanes[k];
+      for (const void *ptr : std::initializer_list<const void *>{ln.st, ln.w.p, ln.cov.p, ln.scratch.p, ln.ws.p,
+                                                                 ln.part.p, ln.sums.p, ln.cnt.p, ln.coef.p, ln.rows.p,
+                                                                 ln.zbuf.p, ln.ybuf.p})
+        q(ptr);
+      key.push_back(uint64_t(ln.ws.bytes));
+    }
+  }
 
   hipEvent_t t0 = nullptr, t1 = nullptr;
   if (ms) {
@@ -352,10 +499,7 @@ void do_batch(gic_context *c, int nreal, const uint64_t *seeds, int ncons, const
     GIC_HIP(hipEventCreate(&t1));
     GIC_HIP(hipEventRecord(t0, c->st));
   }
-  hipGraph_t gr = nullptr;
-  hipGraphExec_t ge = nullptr;
-  if (graph) GIC_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
-  try {
+  auto record = [&]() {
     GIC_HIP(hipEventRecord(B.fork, c->st));
     for (int k = 0; k < nlanes; ++k) GIC_HIP(hipStreamWaitEvent(B.lanes[k]->st, B.fork, 0));
     const double *dcons = (const double *)B.cons.p;
@@ -368,15 +512,27 @@ void do_batch(gic_context *c, int nreal, const uint64_t *seeds, int ncons, const
       const double *rc = dcons + size_t(r) * ncons * 5;
       // the white noise, then its spectrum with Re(w_0) = 0 (randomfieldgenerator.hpp:194-195)
       std::vector<RngLevelOut> outs{{ln.w.p, n, n3}};
-      rng_draw_real<T>(plans[r], outs, ln.ws.p, ln.ws.bytes, st, nullptr, 1.0, dflags + r);
+      rng_draw_real<T>(plans[r], outs, ln.ws.p, ln.ws.bytes, st, nullptr, 1.0, dflags + r,
+                       (const char *)B.plans.p + size_t(r) * pb);
       fft_r2c_post<T>(w, n, nullptr, st, nullptr);
       fill<T>(w, 1, T(0), st);
       // the constraints' covectors in white-noise space
       fill<uint32_t>((uint32_t *)ln.cnt.p, size_t(ncons), 0u, st);
-      hipLaunchKernelGGL(k_batch_tophat<T>, dim3(unsigned((fs_reals + 255) / 256), ncons), dim3(256), 0, st, cv,
-                         fs_reals, n, g.dx, g.ox, g.oy, g.oz, g.periodic, rc, (uint32_t *)ln.cnt.p);
-      GIC_HIP(hipGetLastError());
-      fft_r2c_batch<T>(cv, n, ncons, (const T *)L.sqrtp_dm.p, st, nullptr);
+      if (boxes) {
+        const int *wx = (const int *)B.win.p + size_t(r) * 2 * ncons, *wy = wx + ncons;
+        unsigned ne = unsigned(nrows) * unsigned(n);
+        hipLaunchKernelGGL(k_batch_box_rows<T>, dim3((ne + 255) / 256), dim3(256), 0, st, (T *)ln.rows.p,
+                           unsigned(nrows), ncons, wl, n, g.dx, g.ox, g.oy, g.oz, g.periodic, rc, wx, wy,
+                           (uint32_t *)ln.cnt.p);
+        GIC_HIP(hipGetLastError());
+        fft_r2c_boxes<T>((const T *)ln.rows.p, n, nrows, ncons, wl, wx, wy, (T *)ln.zbuf.p, (T *)ln.ybuf.p, cv, fs,
+                         (const T *)L.sqrtp_dm.p, st, nullptr);
+      } else {
+        hipLaunchKernelGGL(k_batch_tophat<T>, dim3(unsigned((fs_reals + 255) / 256), ncons), dim3(256), 0, st, cv,
+                           fs_reals, n, g.dx, g.ox, g.oy, g.oz, g.periodic, rc, (uint32_t *)ln.cnt.p);
+        GIC_HIP(hipGetLastError());
+        fft_r2c_batch<T>(cv, n, ncons, (const T *)L.sqrtp_dm.p, st, nullptr);
+      }
       // Gram matrix and current values, FP64 solve, update with the transfer function
       hipLaunchKernelGGL(k_batch_gram<T>, dim3(GRAM_BLOCKS), dim3(GRAM_THREADS), 0, st,
                          reinterpret_cast<const cx<T> *>(cv), fs, reinterpret_cast<const cx<T> *>(w), ncons, n,
@@ -403,23 +559,38 @@ void do_batch(gic_context *c, int nreal, const uint64_t *seeds, int ncons, const
       GIC_HIP(hipEventRecord(B.lanes[k]->done, B.lanes[k]->st));
       GIC_HIP(hipStreamWaitEvent(c->st, B.lanes[k]->done, 0));
     }
-  } catch (...) {
-    if (graph) {
-      hipGraph_t junk = nullptr;
-      (void)hipStreamEndCapture(c->st, &junk);
-      if (junk) (void)hipGraphDestroy(junk);
+  };
+  if (!graph) {
+    record();
+  } else {
+    if (B.exec && B.key != key) {
+      GIC_HIP(hipGraphExecDestroy(B.exec));
+      B.exec = nullptr;
     }
-    throw;
-  }
-  if (graph) {
-    GIC_HIP(hipStreamEndCapture(c->st, &gr));
-    GIC_HIP(hipGraphInstantiate(&ge, gr, nullptr, nullptr, 0));
-    GIC_HIP(hipGraphLaunch(ge, c->st));
+    if (!B.exec) {
+      hipGraph_t gr = nullptr;
+      GIC_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+      try {
+        record();
+      } catch (...) {
+        hipGraph_t junk = nullptr;
+        (void)hipStreamEndCapture(c->st, &junk);
+        if (junk) (void)hipGraphDestroy(junk);
+        throw;
+      }
+      GIC_HIP(hipStreamEndCapture(c->st, &gr));
+      hipError_t e = hipGraphInstantiate(&B.exec, gr, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(gr);
+      if (e != hipSuccess) {
+        B.exec = nullptr;
+        GIC_HIP(e);
+      }
+      B.key = key;
+    }
+    GIC_HIP(hipGraphLaunch(B.exec, c->st));
   }
   if (ms) GIC_HIP(hipEventRecord(t1, c->st));
   GIC_HIP(hipStreamSynchronize(c->st));
-  if (ge) GIC_HIP(hipGraphExecDestroy(ge));
-  if (gr) GIC_HIP(hipGraphDestroy(gr));
   if (ms) {
     float e = 0;
     GIC_HIP(hipEventElapsedTime(&e, t0, t1));

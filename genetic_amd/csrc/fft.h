@@ -47,6 +47,17 @@ template <typename T> void fft_r2c_batch(T *d, int n, int nb, const T *post, hip
 // builds the twiddle table of size n (a synchronous upload), so that later
 // launches, also those recorded into a graph, need no host work
 template <typename T> void fft_prepare(int n);
+// r2c of nb fields that vanish outside boxes of wl x wl cells in (x, y), all
+// z: box b starts at (winx[b], winy[b]) mod n (device int arrays).  rows
+// holds the boxes' rows as dense rows of n reals, row (b wl + i) wl + j for
+// x = winx[b]+i, y = winy[b]+j; nrows is a multiple of
+// fft_box_rows_multiple(n) and rows past nb wl^2 are transformed but unused.
+// zbuf (nrows (n/2+1) complex) and ybuf (nb wl n (n/2+1) complex) are
+// scratch; field b's spectrum times post[|k|^2] goes to spec + b spec_stride.
+template <typename T>
+void fft_r2c_boxes(const T *rows, int n, long nrows, int nb, int wl, const int *winx, const int *winy, T *zbuf, T *ybuf,
+                   T *spec, long spec_stride, const T *post, hipStream_t st, TimingHook *hook);
+template <typename T> int fft_box_rows_multiple(int n);
 template <typename T> void fft_c2r_to(T *d, T *dst, int n, hipStream_t st, TimingHook *hook);
 // Transpose of one Zel'dovich component (FFT_PRE_ZA followed by c2r): the r2c
 // of src (dense n^3, or d in place when null; d is overwritten) whose last

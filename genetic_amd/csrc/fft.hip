@@ -223,6 +223,7 @@ template <typename T> struct StridedArgs {
                           // stored at index (pt - in_win0) mod n
   int nbatch, nouter;     // nbatch > 1: that many fields, in_bs / out_bs elements apart, nouter outer lines each
   long in_bs, out_bs;
+  const int *in_win0_b;   // if given: in_win0 of each field of the batch (device array)
 };
 
 __device__ __forceinline__ int signed_k(int i, int n) { return i > n / 2 ? i - n : i; }
@@ -262,12 +263,14 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
   int col0 = kzb * C;
   int nc = min(C, a.ncol - col0);
   long boff_in = 0, boff_out = 0;
+  int b = 0;
   if (a.nbatch > 1) {
-    int b = outer / a.nouter;
+    b = outer / a.nouter;
     outer -= b * a.nouter;
     boff_in = b * a.in_bs;
     boff_out = b * a.out_bs;
   }
+  const int iw0 = a.in_win0_b ? a.in_win0_b[b] : a.in_win0;
   const cx<T> *ib = a.in + boff_in + outer * a.in_os + col0;
   cx<T> *ob = a.out + boff_out + outer * a.out_os + col0;
   int ky = signed_k(outer, a.n);
@@ -275,7 +278,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
   auto gld = [&](int line, int pt) -> cx<T> {
     if (line >= nc) return cx<T>{0, 0};
     if (a.in_wlen > 0 && a.in_wlen < L) {   // pruned input: zero outside the window
-      int o = pt - a.in_win0;
+      int o = pt - iw0;
       if (o < 0) o += L;
       if (o >= a.in_wlen) return cx<T>{0, 0};
       return ib[o * a.in_ls + line];
@@ -751,6 +754,63 @@ template <typename T, int N> struct FftN {
       strided<-1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x_fwd_batch");
     }
   }
+  // r2c of nb fields that are zero outside a box of wl x wl cells in (x, y)
+  // (all z), box b starting at (winx[b], winy[b]) (mod N; device arrays).
+  // rows: the boxes' rows, nrows (a multiple of ROWS) dense rows of N reals,
+  // row (b wl + i) wl + j holding x = winx[b]+i, y = winy[b]+j; zbuf (nrows
+  // rows of NZ complex) and ybuf (nb wl N NZ complex) are scratch.  Field b's
+  // spectrum, times post[|k|^2], goes to spec + b * spec_stride (complex).
+  static void r2c_boxes(const T *rows, long nrows, int nb, int wl, const int *winx, const int *winy, T *zbuf,
+                        T *ybuf, T *spec, long spec_stride, const T *post, hipStream_t st, TimingHook *hook) {
+    int nblk = int(nrows / ROWS);
+    size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
+    const cx<T> *tw = twtable<T>().get(N);
+    auto kern = k_fft_z_r2c<T, N, ROWS, THREADS>;
+    if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    std::string tag = "fft_z_r2c_boxes." + std::to_string(N);
+    if (hook) hook->begin(tag.c_str(), st);
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, zbuf, nrows, T(1), tw, rows);
+    GIC_HIP(hipGetLastError());
+    if (hook) hook->end(tag.c_str(), st);
+    // y pass: per field, tiles (i, kz block), lines over ky from the wl rows j
+    StridedArgs<T> ay{};
+    ay.in = reinterpret_cast<const cx<T> *>(zbuf);
+    ay.out = reinterpret_cast<cx<T> *>(ybuf);
+    ay.in_ls = NZ;
+    ay.in_os = long(wl) * NZ;
+    ay.out_ls = NZ;
+    ay.out_os = long(N) * NZ;
+    ay.win0 = 0;
+    ay.wlen = N;
+    ay.in_win0 = 0;
+    ay.in_wlen = wl;
+    ay.scale = T(1);
+    ay.nbatch = nb;
+    ay.in_bs = long(wl) * wl * NZ;
+    ay.out_bs = long(wl) * N * NZ;
+    ay.in_win0_b = winy;
+    strided<-1, PRE_NONE, POST_NONE, 5>(ay, wl, st, hook, "fft_y_fwd_boxes");
+    // x pass: per field, tiles (ky, kz block), lines over kx from the wl planes i
+    StridedArgs<T> ax{};
+    ax.in = reinterpret_cast<const cx<T> *>(ybuf);
+    ax.out = reinterpret_cast<cx<T> *>(spec);
+    ax.in_ls = long(N) * NZ;
+    ax.in_os = NZ;
+    ax.out_ls = long(N) * NZ;
+    ax.out_os = NZ;
+    ax.win0 = 0;
+    ax.wlen = N;
+    ax.in_win0 = 0;
+    ax.in_wlen = wl;
+    ax.scale = norm();
+    ax.tab = post;
+    ax.nbatch = nb;
+    ax.in_bs = long(wl) * N * NZ;
+    ax.out_bs = spec_stride;
+    ax.in_win0_b = winx;
+    if (post) strided<-1, PRE_NONE, POST_TABLE, 4>(ax, N, st, hook, "fft_x_fwd_boxes");
+    else strided<-1, PRE_NONE, POST_NONE, 4>(ax, N, st, hook, "fft_x_fwd_boxes");
+  }
   static void c2r(T *d, hipStream_t st, TimingHook *hook, T *dst = nullptr) {
     cx<T> *c = reinterpret_cast<cx<T> *>(d);
     mirror(c, st, hook);
@@ -968,6 +1028,18 @@ template <typename T> void fft_r2c_batch(T *d, int n, int nb, const T *post, hip
   if (nb < 1) throw std::runtime_error("fft_r2c_batch: no fields");
   GIC_FFT_SWITCH(n, F::r2c_batch(d, nb, post, st, hook))
 }
+template <typename T>
+void fft_r2c_boxes(const T *rows, int n, long nrows, int nb, int wl, const int *winx, const int *winy, T *zbuf, T *ybuf,
+                   T *spec, long spec_stride, const T *post, hipStream_t st, TimingHook *hook) {
+  if (nb < 1 || wl < 1 || wl >= n) throw std::runtime_error("fft_r2c_boxes: bad box count or size");
+  if (nrows < long(nb) * wl * wl || nrows % fft_box_rows_multiple<T>(n)) throw std::runtime_error("fft_r2c_boxes: bad row count");
+  GIC_FFT_SWITCH(n, F::r2c_boxes(rows, nrows, nb, wl, winx, winy, zbuf, ybuf, spec, spec_stride, post, st, hook))
+}
+template <typename T> int fft_box_rows_multiple(int n) {
+  int r = 0;
+  GIC_FFT_SWITCH(n, r = F::ROWS)
+  return r;
+}
 template <typename T> void fft_prepare(int n) {
   if (n < 4 || n > 1024 || (n & (n - 1))) throw std::runtime_error("FFT size " + std::to_string(n) + " is not supported");
   twtable<T>().get(n);
@@ -1012,6 +1084,9 @@ void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, doubl
   template void fft_c2r_to<T>(T *, T *, int, hipStream_t, TimingHook *);                                    \
   template void fft_r2c_batch<T>(T *, int, int, const T *, hipStream_t, TimingHook *);                      \
   template void fft_prepare<T>(int);                                                                        \
+  template int fft_box_rows_multiple<T>(int);                                                               \
+  template void fft_r2c_boxes<T>(const T *, int, long, int, int, const int *, const int *, T *, T *, T *, long,  \
+                                 const T *, hipStream_t, TimingHook *);                                       \
   template void fft_r2c_za_adjoint<T>(const T *, T *, T *, int, const T *, int, double, bool, hipStream_t,   \
                                       TimingHook *);                                                          \
   template void fft_c2r_from<T>(const T *, T *, int, int, const T *, int, double, hipStream_t, TimingHook *, T *, \

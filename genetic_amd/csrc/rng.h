@@ -35,10 +35,17 @@ struct RngLevelOut {
 };
 template <typename T>
 void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, void *workspace, size_t ws_bytes,
-                   hipStream_t st, struct TimingHook *hook, double scale = 1.0, uint32_t *cover_flag = nullptr);
+                   hipStream_t st, struct TimingHook *hook, double scale = 1.0, uint32_t *cover_flag = nullptr,
+                   const void *dev_plan = nullptr);
 // With cover_flag the draw has no host synchronisation: a device check sets
 // *cover_flag to 1 if the stream held too few Gaussians (the cells past the
 // end are then left as they were), instead of the host raising an error.
+// dev_plan: the plan already in device memory (rng_dev_pack'd, rng_dev_bytes
+// long, copied there by the caller); otherwise a kernel writes it into the
+// workspace, so that the draw's launches carry no seed-dependent arguments
+// past that one.
+size_t rng_dev_bytes();
+void rng_dev_pack(const RngPlan &plan, void *host_dst);
 size_t rng_workspace_bytes(const RngPlan &plan);
 // uploads the ziggurat tables (once, synchronously) ahead of draws that a
 // graph records

@@ -313,12 +313,14 @@ __device__ __forceinline__ size_t raw_at(long t, int q, int k, int rawb) {
 // possible entry offsets.  Once the four cursors sit on the same position
 // they share the rest of the parse, so only one is advanced and its later
 // acceptances are added to all four.
-__global__ __launch_bounds__(256) void k_rng_summary(RngDev p, long nchunks, uint32_t *__restrict__ summary,
+__global__ __launch_bounds__(256) void k_rng_summary(const RngDev *__restrict__ pp, long nchunks,
+                                                    uint32_t *__restrict__ summary,
                                                     uint32_t *__restrict__ raw) {
   extern __shared__ uint32_t ring_mem[];
   long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= nchunks) return;
   Ring ring{ring_mem + threadIdx.x, int(blockDim.x)};
+  const RngDev &p = *pp;
   const int kc = p.kc, ch = kc * 24, rawb = (kc + 1) * 24;
   uint64_t x[12];
   int64_t c;
@@ -679,8 +681,9 @@ void rng_plan(unsigned long seed, size_t n_gaussians, RngPlan &plan) {
   // take 3-4 numbers and may repeat); 2.2x plus slack is never exceeded in
   // practice and is checked after the scan.  Chunks of kc blocks: long
   // chunks amortise the jump to each chunk's start, and a small draw takes
-  // shorter ones so that it still has about 64k chunks (one wavefront per
-  // SIMD) to run on.
+  // shorter ones (down to four blocks) so that it still has about 64k
+  // chunks (a wavefront per SIMD) to run on; shorter chunks than that spend
+  // more on the jumps than they gain in parallelism.
   double need = 2.2 * double(n_gaussians) + 4096.0;
   plan.kc = 64;
   while (plan.kc > 4 && need / (plan.kc * 24.0) < 65536.0) plan.kc /= 2;
@@ -711,10 +714,11 @@ size_t rng_workspace_bytes(const RngPlan &plan) {
   long nc = plan.nchunks;
   long nseg = (nc + SEG - 1) / SEG;
   return (size_t(nc) * 5 * 4 + 255) / 256 * 256 + (size_t(nseg) * 5 * 4 + 255) / 256 * 256 +
-         (size_t(nseg) * 8 + 255) / 256 * 256 + raw_bytes(nc, plan.kc) + 256;
+         (size_t(nseg) * 8 + 255) / 256 * 256 + raw_bytes(nc, plan.kc) + (sizeof(RngDev) + 255) / 256 * 256 + 256;
 }
 
 struct RngBufs {
+  RngDev *dev;   // the plan, read by the summary pass
   uint32_t *summary, *segmaps, *raw;
   uint64_t *segstart;
   long nseg;
@@ -733,6 +737,7 @@ static RngBufs carve(const RngPlan &plan, void *ws, size_t bytes) {
   r.segmaps = (uint32_t *)take(size_t(nseg) * 5 * 4);
   r.segstart = (uint64_t *)take(size_t(nseg) * 8);
   r.raw = (uint32_t *)take(raw_bytes(nc, plan.kc));
+  r.dev = (RngDev *)take(sizeof(RngDev));
   r.nseg = nseg;
   if ((size_t)(p - (char *)ws) > bytes) throw std::runtime_error("RNG workspace too small");
   return r;
@@ -767,18 +772,37 @@ __global__ void k_rng_cover(const uint64_t *__restrict__ last_start, const uint3
   if (produced < total) *flag = 1u;
 }
 
+// the plan's device copy, written by a kernel from its arguments
+__global__ void k_rng_put_plan(RngDev d, RngDev *__restrict__ dst) {
+  const uint64_t *s = reinterpret_cast<const uint64_t *>(&d);
+  uint64_t *o = reinterpret_cast<uint64_t *>(dst);
+  for (int i = threadIdx.x; i < int(sizeof(RngDev) / 8); i += blockDim.x) o[i] = s[i];
+}
+static_assert(sizeof(RngDev) % 8 == 0, "RngDev is copied as 64-bit words");
+
+size_t rng_dev_bytes() { return sizeof(RngDev); }
+void rng_dev_pack(const RngPlan &plan, void *host_dst) {
+  RngDev d = to_dev(plan);
+  memcpy(host_dst, &d, sizeof d);
+}
+
 template <typename T>
 void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, void *ws, size_t ws_bytes,
-                   hipStream_t st, TimingHook *hook, double scale, uint32_t *cover_flag) {
+                   hipStream_t st, TimingHook *hook, double scale, uint32_t *cover_flag, const void *dev_plan) {
   if (levels.empty() || levels.size() > 4) throw std::runtime_error("real-space draw supports 1 to 4 levels");
   upload_tables();
   RngBufs b = carve(plan, ws, ws_bytes);
-  RngDev d = to_dev(plan);
   long nc = plan.nchunks;
   const int TB = 256;
   size_t ring_bytes = size_t(RING) * TB * 4;
+  const RngDev *pd = static_cast<const RngDev *>(dev_plan);
+  if (!pd) {
+    hipLaunchKernelGGL(k_rng_put_plan, dim3(1), dim3(64), 0, st, to_dev(plan), b.dev);
+    GIC_HIP(hipGetLastError());
+    pd = b.dev;
+  }
   if (hook) hook->begin("rng_summary", st);
-  hipLaunchKernelGGL(k_rng_summary, dim3((nc + TB - 1) / TB), dim3(TB), ring_bytes, st, d, nc, b.summary, b.raw);
+  hipLaunchKernelGGL(k_rng_summary, dim3((nc + TB - 1) / TB), dim3(TB), ring_bytes, st, pd, nc, b.summary, b.raw);
   GIC_HIP(hipGetLastError());
   if (hook) hook->end("rng_summary", st);
   if (hook) hook->begin("rng_scan", st);
@@ -820,9 +844,9 @@ void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, 
 }
 
 template void rng_draw_real<float>(const RngPlan &, const std::vector<RngLevelOut> &, void *, size_t, hipStream_t,
-                                   TimingHook *, double, uint32_t *);
+                                   TimingHook *, double, uint32_t *, const void *);
 template void rng_draw_real<double>(const RngPlan &, const std::vector<RngLevelOut> &, void *, size_t, hipStream_t,
-                                    TimingHook *, double, uint32_t *);
+                                    TimingHook *, double, uint32_t *, const void *);
 
 void rng_raw(const RngPlan &plan, uint32_t *d_out, size_t n, void *, size_t, hipStream_t st) {
   if (n > size_t(plan.nchunks) * plan.ch()) throw std::runtime_error("rng_raw: plan too small");

@@ -91,6 +91,14 @@ def test_batch_graph_and_chains_are_bit_identical():
     c = _batch(ic, n, seeds[:2], cons[:2], torch.float32, lanes=2, graph=True)
     np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(c, a[:2])
+    # the recorded batch replayed for other seeds and constraints of the same shape
+    seeds2, cons2 = [7, 8, 9, 10, 11], _constraints(len(seeds), 8, 12)
+    d = _batch(ic, n, seeds2, cons2, torch.float32, lanes=3, graph=True)
+    e = _batch(ic, n, seeds2, cons2, torch.float32, lanes=3, graph=True)
+    f = _batch(ic, n, seeds2, cons2, torch.float32, lanes=1, graph=False)
+    np.testing.assert_array_equal(d, f)
+    np.testing.assert_array_equal(e, f)
+    assert not np.array_equal(d, a)
     ic.close()
 
 
@@ -113,4 +121,22 @@ def test_batch_rejects_bad_input():
     ic.init_zoom_grid_with_origin_at(8, 8, 8, 2, 32)
     with pytest.raises(gic.GICError, match="one grid level"):
         ic.batch_constrained([1], ok, out.data_ptr())
+    ic.close()
+
+
+def test_batch_large_spheres_match_one_at_a_time():
+    """Spheres wider than a box of the grid take the whole-field covectors;
+    small and large ones mixed in one batch."""
+    import torch
+    n, seeds = 32, [21, 22]
+    print("batch large spheres", n, seeds, flush=True)
+    cons = _constraints(len(seeds), 3, 5)
+    cons[:, 0, 3] = 22.0   # wider than the 32-cell grid's box can hold
+    ic = _ctx(gic.FLOAT64, n)
+    got = _batch(ic, n, seeds, cons, torch.float64, lanes=2, graph=True)
+    for r, s in enumerate(seeds):
+        want = _one_at_a_time(ic, n, s, cons[r])
+        for f in range(4):
+            np.testing.assert_allclose(got[r, f], want[f], atol=1e-9 * np.abs(want[f]).max(),
+                                       err_msg="realisation %d field %d" % (r, f))
     ic.close()
