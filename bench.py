@@ -335,7 +335,12 @@ def main():
                     help="the CPU baseline runs the workload with every grid 1/scale as many cells a side")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device copy (reference reading for the FETCH_SIZE/WRITE_SIZE counters)")
+    ap.add_argument("--batch-only", action="store_true",
+                    help="run only the batching workload (64 constrained 128^3 realisations per HIP graph)")
     args = ap.parse_args()
+    if args.batch_only:
+        print(json.dumps(batch_throughput()))
+        return
     if args.copy_calibration:
         import torch
         x = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
