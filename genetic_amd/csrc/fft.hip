@@ -204,6 +204,15 @@ __device__ __forceinline__ void run_stages(cx<T> *lds, const cx<T> *tw, int twmu
 enum { PRE_NONE = FFT_PRE_NONE, PRE_TABLE = FFT_PRE_TABLE, PRE_ZA = FFT_PRE_ZA, PRE_KK = FFT_PRE_KK };
 enum { POST_NONE = 0, POST_TABLE = 1, POST_ZA_ADJ = 2, POST_AUX = 3 };
 
+// FilterInline prepared for single precision on the host: per filter the
+// cut kc and 1 / (0.1 kc), the scale, kMin and 1 / kMin^2
+struct FilterF {
+  int nf, zero, over_k2;
+  int kind[8];
+  float kc[8], inv_w[8];
+  float scale, kmin, inv_kmin2;
+};
+
 template <typename T> struct StridedArgs {
   const cx<T> *in;
   cx<T> *out;
@@ -221,6 +230,8 @@ template <typename T> struct StridedArgs {
   FilterInline fil;        // FP32: the table's filter product formed in the pass (fil.nf >= 0)
   int in_win0, in_wlen;   // if 0 < in_wlen < n: the input holds only line points (pt - in_win0) mod n < in_wlen,
                           // stored at index (pt - in_win0) mod n
+  FilterF ff;             // FP32: fil in single precision with its divisions turned into products (set by strided())
+  float inv_k2_f;         // FP32: inv_k2 / kMin^2 (set by strided())
   int nbatch, nouter;     // nbatch > 1: that many fields, in_bs / out_bs elements apart, nouter outer lines each
   long in_bs, out_bs;
   const int *in_win0_b;   // if given: in_win0 of each field of the batch (device array)
@@ -228,24 +239,25 @@ template <typename T> struct StridedArgs {
 
 __device__ __forceinline__ int signed_k(int i, int n) { return i > n / 2 ? i - n : i; }
 
-// The factor of FilterInline at |k|^2 = ksq in single precision.
-__device__ __forceinline__ float filter_inline_f(const FilterInline &F, int ksq, double kmin_d) {
+// The factor of FilterInline at |k|^2 = ksq (filter_table's product) in
+// single precision, with hardware reciprocal and square root (1 ulp each)
+// and the host's reciprocals: no division per mode.
+__device__ __forceinline__ float filter_fast_f(const FilterF &F, int ksq) {
   if (F.zero) return 0.f;
-  float kmin = float(kmin_d);
-  float k = sqrtf(float(ksq)) * kmin;
-  float v = float(F.scale);
+  float fk = float(ksq);
+  float k = __builtin_amdgcn_sqrtf(fk) * F.kmin;
+  float v = F.scale;
   for (int i = 0; i < F.nf; ++i) {
-    float kc = float(F.kc[i]);
-    float lp = 1.f / (1.f + __expf((k - kc) / (kc * 0.1f)));
-    v *= F.kind[i] == 0 ? lp : F.kind[i] == 1 ? sqrtf(fmaxf(1.f - lp * lp, 0.f)) : 1.f - lp;
+    float lp = __builtin_amdgcn_rcpf(1.f + __expf((k - F.kc[i]) * F.inv_w[i]));
+    v *= F.kind[i] == 0 ? lp : F.kind[i] == 1 ? __builtin_amdgcn_sqrtf(fmaxf(1.f - lp * lp, 0.f)) : 1.f - lp;
   }
-  if (F.over_k2) v = ksq == 0 ? 0.f : v / (float(ksq) * kmin * kmin);
+  if (F.over_k2) v = ksq == 0 ? 0.f : v * F.inv_kmin2 * __builtin_amdgcn_rcpf(fk);
   return v;
 }
-// the |k|^2 factor of a pass: inline filter (FP32), table, or inv_k2 / k^2
 template <typename T, class A> __device__ __forceinline__ T kfactor(const A &a, int ksq) {
   if constexpr (sizeof(T) == 4) {
-    if (a.fil.nf >= 0) return filter_inline_f(a.fil, ksq, a.kmin_d);
+    if (a.ff.nf >= 0) return filter_fast_f(a.ff, ksq);
+    if (!a.tab) return ksq == 0 ? 0.f : a.inv_k2_f * __builtin_amdgcn_rcpf(float(ksq));
   }
   return a.tab ? a.tab[ksq] : T(ksq == 0 ? 0.0 : (1.0 / (double(ksq) * a.kmin_d * a.kmin_d)) * a.inv_k2);
 }
@@ -656,6 +668,24 @@ template <typename T, int N> struct FftN {
   template <int DIR, int PRE, int POST, int PASS = 0>
   static void strided(const StridedArgs<T> &a0, int nouter, hipStream_t st, TimingHook *hook, const char *name) {
     StridedArgs<T> a = a0;
+    a.ff.nf = -1;
+    if (sizeof(T) == 4) {
+      if (a.fil.nf >= 0) {
+        if (a.fil.nf > 8) throw std::runtime_error("FilterInline: at most 8 factors");
+        a.ff.nf = a.fil.nf;
+        a.ff.zero = a.fil.zero;
+        a.ff.over_k2 = a.fil.over_k2;
+        for (int i = 0; i < a.fil.nf; ++i) {
+          a.ff.kind[i] = a.fil.kind[i];
+          a.ff.kc[i] = float(a.fil.kc[i]);
+          a.ff.inv_w[i] = 1.f / (float(a.fil.kc[i]) * 0.1f);
+        }
+        a.ff.scale = float(a.fil.scale);
+        a.ff.kmin = float(a.kmin_d);
+        a.ff.inv_kmin2 = a.kmin_d > 0 ? float(1.0 / (a.kmin_d * a.kmin_d)) : 0.f;
+      }
+      a.inv_k2_f = a.kmin_d > 0 ? float(a.inv_k2 / (a.kmin_d * a.kmin_d)) : 0.f;
+    }
     a.ncol = NZ;
     a.nkzb = (NZ + C - 1) / C;
     a.n = N;
