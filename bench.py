@@ -156,6 +156,24 @@ def cpu_baseline(wl, scale, reps=3):
                           s["mods"]["radius"] if s.get("mods") else 0.0, ncells, total))
 
 
+def single_level_throughput(name="single256", steps=20, warmup=3, precision=32):
+    """The metric's 256^3 point: seed -> noise -> delta -> Zel'dovich psi on
+    one 256^3 level, resident in HBM, steps after warmup, wall clock."""
+    wl = WORKLOADS[name]
+    ic = make_context(wl, precision)
+    for w in range(warmup):
+        one_step(ic, 5000 + w, None)
+    ic.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        one_step(ic, 1 + k, None)
+    ic.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    ic.close()
+    return dict(value=round(wl["n"] ** 3 / dt / 1e6, 2), unit="Mparticles/s", ms_per_step=round(dt * 1e3, 4),
+                steps=steps, config=wl["desc"] + ", FP32, seeds 1..%d real space" % steps)
+
+
 def adjoint_ratio(n=256, box=50.0, reps=20):
     """Forward w -> psi and its transpose psi_bar -> w_bar on one n^3 FP32
     level (dense device arrays in and out): median wall time of reps
@@ -410,6 +428,7 @@ def main():
     )
     ic.close()
     if not args.no_extras:
+        out["throughput_256"] = single_level_throughput()
         out["adjoint_over_forward"] = adjoint_ratio()
         out["batch"] = batch_throughput()
         out["psi_fp32_error"] = psi_precision(wl)

@@ -409,7 +409,10 @@ void do_batch(gic_context *c, int nreal, const uint64_t *seeds, int ncons, const
   // boxes around the spheres: x, y windows of wl cells from
   // floor((c - o - R)/dx - 1/2) (mod n), which hold every selected cell
   int wl = 0;
-  for (int q = 0; q < nreal * ncons; ++q) wl = std::max(wl, 2 * int(std::ceil(cons[5 * q + 3] / g.dx)) + 3);
+  for (int q = 0; q < nreal * ncons; ++q) {
+    double span = 2.0 * std::ceil(cons[5 * q + 3] / g.dx) + 3.0;
+    wl = std::max(wl, span >= double(n) ? n : int(span));
+  }
   // (the row kernel indexes rows x n values in 32 bits)
   const bool boxes = wl < n && (double(ncons) * wl * wl + 1024.0) * n < 2e9;
   const int rows_mult = fft_box_rows_multiple<T>(n);

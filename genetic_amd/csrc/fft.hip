@@ -16,6 +16,7 @@
 // Twiddles come from an N-entry table generated in FP64 on the host.
 #include "gic_common.h"
 #include "fft.h"
+#include "fft_tiles.h"
 #include <vector>
 #include <cmath>
 #include <map>
@@ -648,23 +649,32 @@ template <typename T> TwTable<T> &twtable() {
   return t;
 }
 
-#ifndef GIC_STRIDED_COLS_F32
-#define GIC_STRIDED_COLS_F32 8
+// Tile shape of the strided passes: the tuned table, or one shape for every
+// size when a tuning build names it (GIC_TUNE_THREADS, GIC_TUNE_COLS).
+template <typename T, int N> constexpr int strided_threads() {
+#ifdef GIC_TUNE_THREADS
+  return GIC_TUNE_THREADS;
+#else
+  return TileShape<T, N>::threads;
 #endif
-#ifndef GIC_STRIDED_THREADS
-#define GIC_STRIDED_THREADS 512
+}
+template <typename T, int N> constexpr int strided_cols() {
+#ifdef GIC_TUNE_COLS
+  return sizeof(T) == 4 ? GIC_TUNE_COLS : (GIC_TUNE_COLS > 8 ? 8 : GIC_TUNE_COLS);
+#else
+  return TileShape<T, N>::cols;
 #endif
-template <typename T> constexpr int strided_cols() { return sizeof(T) == 4 ? GIC_STRIDED_COLS_F32 : 4; }
+}
 
 template <typename T, int N> struct FftN {
   static constexpr int THREADS = 256;
-  static constexpr int C = strided_cols<T>();
+  static constexpr int C = strided_cols<T, N>();
   static constexpr int M = N / 2;
   static constexpr int NZ = N / 2 + 1;
   static constexpr int ROWS0 = (8192 / N) < 1 ? 1 : (8192 / N);
   static constexpr int ROWS = ROWS0 > N * N ? N * N : ROWS0;
 
-  static constexpr int STHREADS = GIC_STRIDED_THREADS;
+  static constexpr int STHREADS = strided_threads<T, N>();
   template <int DIR, int PRE, int POST, int PASS = 0>
   static void strided(const StridedArgs<T> &a0, int nouter, hipStream_t st, TimingHook *hook, const char *name) {
     StridedArgs<T> a = a0;
