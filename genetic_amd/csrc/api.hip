@@ -922,6 +922,10 @@ int gic_import_level(gic_context *c, int level, const double *host, size_t n) {
   });
 }
 
+int gic_splice(gic_context *c, unsigned long new_seed, int *cg_iterations) {
+  return guarded(c, [&] { DISPATCH(c, do_splice, c, new_seed, cg_iterations); });
+}
+
 int gic_apply_power_spectrum(gic_context *c) {
   return guarded(c, [&] {
     DISPATCH(c, do_apply_ps, c);

@@ -318,6 +318,7 @@ template <typename T> double mod_variance_gradient(gic_context *c, const T *cons
 // quadratic modification steps: the reference's linearised steps (off) or an
 // exact line search along the same direction (on)
 void mod_set_line_search(gic_context *c, bool on);
+template <typename T> void do_splice(gic_context *c, unsigned long new_seed, int *cg_iterations);
 template <typename T>
 void do_batch(gic_context *c, int nreal, const uint64_t *seeds, int ncons, const double *cons, T *out, int nlanes,
               bool graph, double *ms);

@@ -60,6 +60,7 @@ SIGNATURES = {
     "gic_displacement_adjoint": ([_v, _v, _v], _i),
     "gic_variance_gradient": ([_v, _v, _P(_d), _v], _i),
     "gic_batch_constrained": ([_v, _i, _v, _i, _v, _v, _i, _i, _P(_d)], _i),
+    "gic_splice": ([_v, ctypes.c_ulong, _P(_i)], _i),
     "gic_set_quadratic_line_search": ([_v, _i], _i),
     "gic_particles": ([_v, _i, _v, _s, _s, _v, _v], _i),
     "gic_particle_mass_eps": ([_v, _i, _P(_d), _P(_d)], _i),
@@ -223,6 +224,14 @@ class ICGenerator:
     def import_level(self, level, data):
         d = np.ascontiguousarray(np.asarray(data, dtype=np.float64)).ravel()
         self._check(lib().gic_import_level(self._h, int(level), _ptr(d), d.size))
+
+    def splice(self, new_seed):
+        """Keep the flagged region's overdensity from a new real-space draw of
+        new_seed and splice it into the old noise elsewhere; returns the
+        conjugate-gradient iterations."""
+        it = ctypes.c_int()
+        self._check(lib().gic_splice(self._h, int(new_seed), ctypes.byref(it)))
+        return it.value
 
     def apply_power_spectrum(self):
         self._check(lib().gic_apply_power_spectrum(self._h))

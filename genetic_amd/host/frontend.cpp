@@ -730,6 +730,20 @@ struct Generator {
     ensureDrawn();
     check(gic_reverse(c()));
   }
+  // splice (ic.hpp:1657-1686): the flagged cells take a new seed's field
+  void splice(int newSeed) {
+    ensureDrawn();
+    syncFlags();
+    if (grids.size() > 1)
+      log() << "WARNING: Splicing operations on multiple levels work, but may have edge artefacts. Use "
+               "experimentally/at your own risk!"
+            << std::endl;
+    log() << "Constructing new random field for exterior of splice" << std::endl;
+    int iterations = 0;
+    check(gic_splice(c(), (unsigned long)newSeed, &iterations));
+    log() << "Finished constructing new random field. Beginning splice operation." << std::endl;
+    log() << "Conjugate gradient ended after " << iterations << " iterations" << std::endl;
+  }
   void reverseSmallK(double kmax) {
     ensureDrawn();
     check(gic_reverse_small_k(c(), kmax));
@@ -1815,6 +1829,7 @@ struct Dispatcher {
     add("fix_power", fixp);
     add("exact_power_spectrum_enforcement", fixp);
     add("reverse", std::function<void()>([this]() { ic.reverse(); }));
+    add("splice", std::function<void(int)>([this](int s) { ic.splice(s); }));
     add("reverse_small_k", std::function<void(double)>([this](double k) { ic.reverseSmallK(k); }));
     add("import_level", std::function<void(int, std::string)>([this](int l, std::string f) { ic.importLevel(l, f); }));
     add("dump_ps", std::function<void(int)>([this](int l) { ic.dumpPS(l); }));

@@ -94,6 +94,14 @@ int gic_reverse_small_k(gic_context *ctx, double kmax);
  * real-space dark-matter overdensity of the level (z fastest); it replaces the
  * level's white noise after the transfer ratio P_dm^-1/2. */
 int gic_import_level(gic_context *ctx, int level, const double *host, size_t n);
+/* Splicing (ICGenerator::splice, ic.hpp:1657-1686; spliceOneLevel,
+ * splice.hpp:37-118): a new white-noise field is drawn in real space from
+ * new_seed, and on every level the noise becomes the field equal to the new
+ * one on the flagged cells (in the overdensity basis) that minimises its
+ * chi^2 distance to the old one elsewhere, by conjugate gradients
+ * (cg.hpp:13-59, rtol 1e-6).  Before the output is generated.
+ * cg_iterations (optional) receives the CG iterations, summed over levels. */
+int gic_splice(gic_context *ctx, unsigned long new_seed, int *cg_iterations);
 /* applyPowerSpec (ic.hpp:825-843): delta_k = w_k sqrt(P(k)) on every level. */
 int gic_apply_power_spectrum(gic_context *ctx);
 /* Zel'dovich displacements of every level and the combination of levels

@@ -157,3 +157,42 @@ def test_reversal_paramfile_matches_reference(tmp_path):
     assert np.abs(r.imag).max() < 1e-12 * max(np.abs(r.real).max(), 1e-300)
     info = open(os.path.join(d, "grid-info-0-fourier.txt")).read().split("\n")[0].split()
     assert [float(x) for x in info] == [0.0, 0.0, 0.0, 100.0]
+
+
+@pytest.mark.parametrize("precision,tol", [(gic.FLOAT64, 1e-8), (gic.FLOAT32, 2e-4)])
+def test_splice_keeps_the_new_field_in_the_flagged_region(precision, tol):
+    """splice (splice.hpp:37-118): inside the flagged region the spliced
+    overdensity is the new seed's, up to the constant the k = 0 mode takes
+    away when the power spectrum is applied; outside it follows the old
+    field (closer to it than to the new one)."""
+    from helpers import select_sphere_cells
+    print("splice", precision, flush=True)
+    n, new_seed = 32, SEED + 1
+    cells = select_sphere_cells(n, BOX, (25.0, 25.0, 25.0), 10.0)
+    old, new, sp = (_ic(precision, zoom=False, seed=s) for s in (SEED, new_seed, SEED))
+    sp.set_flags(0, cells)
+    iters = sp.splice(new_seed)
+    assert iters > 0
+    fields = []
+    for ic in (old, new, sp):
+        ic.apply_power_spectrum()
+        fields.append(ic.field(gic.FIELD_OVERDENSITY, 0).ravel())
+        ic.close()
+    d_old, d_new, d_sp = fields
+    scale = np.abs(d_new).max()
+    diff = (d_sp - d_new)[cells]
+    assert np.abs(diff - diff.mean()).max() < tol * scale
+    out = np.ones(n ** 3, bool)
+    out[cells] = False
+    assert np.linalg.norm((d_sp - d_old)[out]) < np.linalg.norm((d_sp - d_new)[out])
+
+
+def test_splicing_paramfile_matches_reference(tmp_path):
+    """test_24_splicing: a 10 Mpc/h sphere spliced from seed 8896132 into the
+    8896131 field; tipsy particles and grid-0 as the reference wrote them."""
+    d, err = _run("test_24_splicing", tmp_path)
+    assert "Conjugate gradient ended after" in err
+    compare_particles(os.path.join(d, "test_22.tipsy"), golden("test_24_splicing", "reference_output"))
+    got = np.load(os.path.join(d, "grid-0.npy"))
+    ref = np.load(golden("test_24_splicing", "reference_grid", "grid-0.npy"))
+    assert_decimal(got, ref, 4, "test_24 grid-0")
