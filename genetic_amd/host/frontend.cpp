@@ -282,6 +282,7 @@ struct Generator {
   bool fp32 = false;
   double Om = 0.3, Ol = 0, Ob = 0, hubble = 0.701, s8 = 0, ns = 0.96, z = 0, TCMB = 2.725;
   double epsNorm = 0.01075;
+  double smoothNorm = 0;   // num_neighbours: SPH smoothing scale per cell size (read by no writer built here)
   double velOffset[3] = {0, 0, 0};
   bool haveCamb = false, drawn = false, generated = false;
   std::vector<Grid> grids;
@@ -838,6 +839,39 @@ struct Generator {
     inf << "It gives the x-offset, y-offset and z-offset of the low-left corner and also the box length" << std::endl;
   }
 
+  // expandFlaggedRegion (ic.hpp:1527-1540, grid.hpp:282-300): n times, each
+  // flagged cell's neighbours along z, then y, then x join the flags
+  // (indices wrapped around the level's grid)
+  void expandFlaggedRegion(int nCells) {
+    if (grids.empty()) throw std::runtime_error("No grid has yet been initialised");
+    if (nCells < 0) throw std::runtime_error("expand_flagged_region takes a number of cells");
+    log() << "Expand flagged region by " << nCells << " cells" << std::endl;
+    for (size_t l = 0; l < grids.size(); ++l) {
+      Grid &g = grids[l];
+      size_t initial = g.flags.size();
+      const long n = g.n;
+      for (int it = 0; it < nCells; ++it)
+        for (int d = 2; d >= 0; --d) {
+          size_t old = g.flags.size();
+          std::vector<size_t> f(old * 3);
+          for (size_t i = 0; i < old; ++i) {
+            size_t id = g.flags[i];
+            long c[3] = {long(id / size_t(n * n)), long((id / size_t(n)) % size_t(n)), long(id % size_t(n))};
+            f[i] = id;
+            for (int k = 0; k < 2; ++k) {
+              long cc[3] = {c[0], c[1], c[2]};
+              cc[d] = ((cc[d] + (k == 0 ? 1 : -1)) % n + n) % n;
+              f[i + old * (k + 1)] = size_t((cc[0] * n + cc[1]) * n + cc[2]);
+            }
+          }
+          std::sort(f.begin(), f.end());
+          f.erase(std::unique(f.begin(), f.end()), f.end());
+          g.flags = f;
+        }
+      log() << "  - level " << l << " increased number of flagged cells by " << g.flags.size() - initial << " (now "
+            << g.flags.size() << ")" << std::endl;
+    }
+  }
   void zeroLevel(int level) {
     log() << "*** Warning: your script calls zeroLevel(" << level << "). This is intended for testing purposes only!"
           << std::endl;
@@ -1792,6 +1826,18 @@ struct Dispatcher {
     std::function<void(int)> zl = [this](int l) { ic.zeroLevel(l); };
     add("zero_level", zl);
     add("zerolevel", zl);
+    // zero_level_field <level> <field>: one output field here (no baryon transfer function)
+    add("zero_level_field", std::function<void(int, int)>([this](int l, int f) {
+          if (f != 0) throw std::runtime_error("Field " + std::to_string(f) + " does not exist");
+          ic.zeroLevel(l);
+        }));
+    add("expand_flagged_region", std::function<void(int)>([this](int nc) { ic.expandFlaggedRegion(nc); }));
+    // setNumNeighbours (ic.hpp:1268-1271): the SPH smoothing scale, cell size x
+    // (3 n / 4 pi)^(1/3); only the HDF5 writer (gadgethdf.hpp:233) reads it, so
+    // it is kept but changes no output here
+    std::function<void(int)> nn = [this](int v) { ic.smoothNorm = std::pow(3.0 * v / (4.0 * M_PI), 1.0 / 3.0); };
+    add("num_neighbours", nn);
+    add("num_neighbors", nn);
     add("eps_norm", D([this](double v) { ic.epsNorm = v; }));
     add("gadget_num_files", std::function<void(int)>([this](int n) {
           if (n < 1) throw std::runtime_error("Number of gadget files must be positive");
@@ -1821,6 +1867,7 @@ struct Dispatcher {
     add("supersample", std::function<void(int)>([this](int f) { ic.supersample = f; }));
     add("pvar", std::function<void(double)>([this](double v) { ic.pvarValue = v; }));
     add("center_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));
+    add("centre_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));
     add("center_grafic_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));
     add("dump_mask", std::function<void()>([this]() { ic.dumpMask(); }));
     add("dump_grid", std::function<void(int)>([this](int l) { ic.dumpGrid(l); }));

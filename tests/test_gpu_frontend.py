@@ -150,3 +150,52 @@ def test_modified_overdensity_array_and_grids(tmp_path):
         g = np.load(os.path.join(d, "grid-%d.npy" % lev))
         r = np.load(golden("test_20_velmodif_zoom", "reference_grid", "grid-%d.npy" % lev))
         assert_decimal(g, r, 4, "grid-%d" % lev)
+
+
+_BASE = """Om 0.279
+Ol 0.721
+s8 0.817
+zin 99
+random_seed_real_space 8896131
+camb ../camb_transfer_kmax40_z0.dat
+outname out
+outdir ./
+outformat tipsy
+basegrid 50.0 16
+"""
+
+
+def test_expand_flagged_region_is_a_one_cell_box_dilation(tmp_path):
+    """expand_flagged_region 1 (grid.hpp:282-300): neighbours along z, then
+    y, then x join the flags, which adds up to every cell within one cell
+    (Chebyshev distance, wrapped) of a flagged one."""
+    text = _BASE + "center 3 3 3\nselect_sphere 4\ndump_id_file before.txt\nexpand_flagged_region 1\n" \
+                   "dump_id_file after.txt\n"
+    d = stage_test("expand_case", tmp_path, text)
+    rc, err = run_frontend(d)
+    assert rc == 0, err
+    assert "Expand flagged region by 1 cells" in err
+    before = np.loadtxt(os.path.join(d, "before.txt"), dtype=np.int64, ndmin=1)
+    after = np.loadtxt(os.path.join(d, "after.txt"), dtype=np.int64, ndmin=1)
+    n = 16
+    m = np.zeros((n, n, n), bool)
+    m.ravel()[before] = True
+    want = np.zeros_like(m)
+    for dx in (-1, 0, 1):
+        for dy in (-1, 0, 1):
+            for dz in (-1, 0, 1):
+                want |= np.roll(m, (dx, dy, dz), axis=(0, 1, 2))
+    np.testing.assert_array_equal(np.sort(after), np.flatnonzero(want.ravel()))
+    assert "increased number of flagged cells by %d (now %d)" % (len(after) - len(before), len(after)) in err
+
+
+def test_num_neighbours_and_centre_output_are_accepted(tmp_path):
+    """num_neighbours (ic.hpp:1268-1271) sets the SPH smoothing scale, which
+    only the HDF5 writer reads: the tipsy softening stays cell size x
+    epsNorm (0.01075, ic.hpp:101; zeldovich.hpp:116-118), in the tipsy length
+    unit of the box size."""
+    d = stage_test("neighbours_case", tmp_path, _BASE + "num_neighbours 32\ncentre_output\ndone\n")
+    rc, err = run_frontend(d)
+    assert rc == 0, err
+    t = read_tipsy(os.path.join(d, "out.tipsy"))
+    np.testing.assert_allclose(t["dark"]["eps"], 0.01075 / 16, rtol=1e-6)
