@@ -727,6 +727,22 @@ int gic_set_cosmology(gic_context *c, double Om, double Ol, double Ob, double hu
   });
 }
 
+int gic_set_power_law(gic_context *c, double amplitude) {
+  return guarded(c, [&] {
+    if (!(amplitude > 0) || !std::isfinite(amplitude)) throw std::runtime_error("powerlaw_amplitude must be positive");
+    c->kpts.clear();
+    c->ns_frozen = c->cosmo.ns;
+    c->amplitude = amplitude;
+    c->spline_dev = SplineHost{};       // n = 0: unit transfer function for every species
+    c->spline_dm_dev = SplineHost{};
+    c->have_camb = true;
+    for (auto &Lp : c->levels) {
+      Lp->sqrtp.release();
+      Lp->sqrtp_dm.release();
+    }
+  });
+}
+
 int gic_set_camb_table(gic_context *c, const double *v, size_t nv, int ncols) {
   return guarded(c, [&] {
     if (ncols != 7 && ncols != 13) throw std::runtime_error("CAMB transfer file doesn't have a sensible number of columns");

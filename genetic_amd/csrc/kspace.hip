@@ -63,8 +63,12 @@ __global__ void k_sqrtp_table(SplineDev sp, double amplitude, double ns, double 
   double k = sqrt(double(s)) * kmin;
   double tf = 0;
   if (k != 0) {
-    double lx = log(k);
-    if (!(lx < sp.minx || lx > sp.maxx)) tf = exp(spline_eval(sp, lx));
+    if (sp.n == 0) {
+      tf = 1;   // a pure power law (PowerLawPowerSpectrum, camb.hpp:145-168)
+    } else {
+      double lx = log(k);
+      if (!(lx < sp.minx || lx > sp.maxx)) tf = exp(spline_eval(sp, lx));
+    }
   }
   double P = amplitude * pow(k, ns) * tf * tf * norm;
   out[s] = T(P != 0.0 ? pow(P, 0.5) : 0.0);

@@ -61,6 +61,7 @@ SIGNATURES = {
     "gic_variance_gradient": ([_v, _v, _P(_d), _v], _i),
     "gic_batch_constrained": ([_v, _i, _v, _i, _v, _v, _i, _i, _P(_d)], _i),
     "gic_splice": ([_v, ctypes.c_ulong, _P(_i)], _i),
+    "gic_set_power_law": ([_v, _d], _i),
     "gic_set_quadratic_line_search": ([_v, _i], _i),
     "gic_particles": ([_v, _i, _v, _s, _s, _v, _v], _i),
     "gic_particle_mass_eps": ([_v, _i, _P(_d), _P(_d)], _i),
@@ -180,6 +181,10 @@ class ICGenerator:
         else:
             table = np.ascontiguousarray(path_or_table, dtype=np.float64)
         self._check(lib().gic_set_camb_table(self._h, _ptr(table), table.size, int(ncols)))
+
+    def set_power_law(self, amplitude):
+        """powerlaw_amplitude: P(k) = amplitude k^ns (ns as set now)."""
+        self._check(lib().gic_set_power_law(self._h, float(amplitude)))
 
     # --- grids ----------------------------------------------------------
     def init_base_grid(self, box, n):

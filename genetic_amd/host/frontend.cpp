@@ -346,6 +346,11 @@ struct Generator {
     check(gic_set_camb_table(c(), v.data(), v.size(), ncols));
     haveCamb = true;
   }
+  // setPowerLawAmplitude (ic.hpp:731-734): P(k) = A k^ns instead of a CAMB table
+  void setPowerLaw(double amplitude) {
+    check(gic_set_power_law(c(), amplitude));
+    haveCamb = true;
+  }
 
   // --- grids ---
   void addLevel(double size, int n, double ox, double oy, double oz) {
@@ -1760,6 +1765,7 @@ struct Dispatcher {
     add("seed_field_fourier_reverse", seed(GIC_SEED_FOURIER_REVERSE));
     using S = std::function<void(std::string)>;
     add("camb", S([this](std::string f) { ic.setCamb(f); }));
+    add("powerlaw_amplitude", D([this](double a) { ic.setPowerLaw(a); }));
     add("outdir", S([this](std::string s) { ic.outdir = s; }));
     add("outname", S([this](std::string s) { ic.outname = s; }));
     add("outformat", S([this](std::string s) {

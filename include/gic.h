@@ -62,6 +62,11 @@ int gic_set_cosmology(gic_context *ctx, double omega_m, double omega_lambda, dou
  * ns, sigma8 and the growth factor at the current cosmology. */
 int gic_set_camb_table(gic_context *ctx, const double *values, size_t nvalues, int ncols);
 
+/* powerlaw_amplitude <A> (setPowerLawAmplitude, ic.hpp:731-734;
+ * PowerLawPowerSpectrum, camb.hpp:145-168): P(k) = A k^ns for every species,
+ * ns as set now, no sigma8 normalisation; replaces a CAMB table. */
+int gic_set_power_law(gic_context *ctx, double amplitude);
+
 /* base_grid <box> <n> (ic.hpp:365-386). */
 int gic_init_base_grid(gic_context *ctx, double box, int n);
 /* zoom_grid_with_origin_at <x0> <y0> <z0> <zoomfac> <n> (ic.hpp:501-601). */

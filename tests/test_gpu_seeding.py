@@ -196,3 +196,29 @@ def test_splicing_paramfile_matches_reference(tmp_path):
     got = np.load(os.path.join(d, "grid-0.npy"))
     ref = np.load(golden("test_24_splicing", "reference_grid", "grid-0.npy"))
     assert_decimal(got, ref, 4, "test_24 grid-0")
+
+
+@pytest.mark.parametrize("precision,tol", [(gic.FLOAT64, 1e-12), (gic.FLOAT32, 2e-6)])
+def test_power_law_spectrum(precision, tol):
+    """powerlaw_amplitude (camb.hpp:145-168): delta_k / w_k = sqrt(A k^ns
+    kw^3 n^3 / (2 pi)^3), kw = 2 pi / box, the CAMB path's covariance
+    normalisation with a unit transfer function; 0 at k = 0."""
+    print("power law", precision, flush=True)
+    A, ns, n = 3.7e4, 0.8, 32
+    ic = gic.ICGenerator(precision=precision)
+    ic.set_cosmology(Om=0.3, Ol=0.7, s8=0.8, ns=ns, z=0.0)
+    ic.set_power_law(A)
+    ic.init_base_grid(BOX, n)
+    ic.set_seed(SEED)
+    ic.draw()
+    w = _noise(ic, 0)
+    ic.apply_power_spectrum()
+    d = _noise(ic, 0)
+    ic.close()
+    ix, iy, iz = _modes(n)
+    kw = 2 * np.pi / BOX
+    k = np.sqrt(ix ** 2 + iy ** 2 + iz ** 2) * kw
+    with np.errstate(divide="ignore"):
+        want = np.where(k > 0, np.sqrt(A * k ** ns * kw ** 3 * n ** 3 / (2 * np.pi) ** 3), 0.0)
+    np.testing.assert_allclose(d, w * want, rtol=tol * 10, atol=tol * np.abs(w * want).max())
+    assert d[0, 0, 0] == 0
