@@ -1627,6 +1627,22 @@ struct Generator {
     inf << "It gives the x-offset, y-offset and z-offset of the low-left corner and also the box length" << std::endl;
   }
 
+  // dumpVelocityX (ic.hpp:920-923): the level's first Zel'dovich offset field
+  // (zeldovich.hpp:151-253) as vx-<level>.npy, generating the output first
+  void dumpVx(int level) {
+    if (level < 0 || level >= int(grids.size())) throw std::runtime_error("Specified level does not exist.");
+    generate();
+    int n = grids[level].n;
+    std::vector<double> d(size_t(n) * n * n);
+    check(gic_copy_field(c(), GIC_FIELD_PSI_X, level, GIC_REAL_SPACE, d.data(), GIC_DTYPE_F64));
+    saveNpy(outdir + "/vx-" + std::to_string(level) + ".npy", n, d);
+    std::ofstream inf(outdir + "/vx-info-" + std::to_string(level) + ".txt");
+    const Grid &g = grids[level];
+    inf << g.ox << " " << g.oy << " " << g.oz << " " << g.size() << std::endl;
+    inf << "The line above contains information about grid level " << level << std::endl;
+    inf << "It gives the x-offset, y-offset and z-offset of the low-left corner and also the box length" << std::endl;
+  }
+
   // dumpPS / dumpPowerSpectrum (ic.hpp:943-963, cosmology/parameters.hpp:112-205):
   // 100 log bins between 2pi/L and pi N/L; the GPU returns per-|k|^2 sums.
   void dumpPS(int level) {
@@ -1878,6 +1894,7 @@ struct Dispatcher {
     add("dump_mask", std::function<void()>([this]() { ic.dumpMask(); }));
     add("dump_grid", std::function<void(int)>([this](int l) { ic.dumpGrid(l); }));
     add("dump_grid_fourier", std::function<void(int)>([this](int l) { ic.dumpGridFourier(l); }));
+    add("dump_vx", std::function<void(int)>([this](int l) { ic.dumpVx(l); }));
     std::function<void()> fixp = [this]() { ic.check(gic_set_exact_power_spectrum(ic.c(), 1)); };
     add("fix_power", fixp);
     add("exact_power_spectrum_enforcement", fixp);

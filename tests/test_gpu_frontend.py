@@ -199,3 +199,25 @@ def test_num_neighbours_and_centre_output_are_accepted(tmp_path):
     assert rc == 0, err
     t = read_tipsy(os.path.join(d, "out.tipsy"))
     np.testing.assert_allclose(t["dark"]["eps"], 0.01075 / 16, rtol=1e-6)
+
+
+def test_dump_vx_is_the_zeldovich_offset_of_the_dumped_grid(tmp_path):
+    """dump_vx 0 (ic.hpp:920-923): the x offset field, whose transform is
+    i k_x delta_k / k^2 of dump_grid's delta (zeldovich.hpp:151-181), zero at
+    k = 0 and at the x Nyquist plane."""
+    d = stage_test("vx_case", tmp_path, _BASE + "done\ndump_grid 0\ndump_vx 0\n")
+    rc, err = run_frontend(d)
+    assert rc == 0, err
+    delta = np.load(os.path.join(d, "grid-0.npy"))
+    vx = np.load(os.path.join(d, "vx-0.npy"))
+    n, box = 16, 50.0
+    f = np.fft.fftfreq(n, 1.0 / n)
+    kx, ky, kz = np.meshgrid(f, f, f, indexing="ij")
+    kw = 2 * np.pi / box
+    k2 = (kx ** 2 + ky ** 2 + kz ** 2) * kw * kw
+    with np.errstate(divide="ignore", invalid="ignore"):
+        op = np.where(k2 > 0, 1j * kx * kw / k2, 0.0)
+    op[n // 2, :, :] = 0
+    want = np.fft.ifftn(op * np.fft.fftn(delta)).real
+    assert_decimal(vx, want, 5, "vx-0")
+    assert open(os.path.join(d, "vx-info-0.txt")).read().split("\n")[0].split() == ["0", "0", "0", "50"]
