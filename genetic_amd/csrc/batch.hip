@@ -100,7 +100,7 @@ __global__ __launch_bounds__(256) void k_batch_box_rows(T *__restrict__ rows, un
   unsigned e = blockIdx.x * blockDim.x + threadIdx.x;
   bool live = e < nrows * unsigned(n);
   unsigned row = live ? e / unsigned(n) : 0u;
-  int z = int(e - row * unsigned(n));
+  int z = live ? int(e - row * unsigned(n)) : 0;
   unsigned ww = unsigned(wl) * unsigned(wl);
   int f = int(row / ww);
   bool inside = false;
