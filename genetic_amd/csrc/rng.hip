@@ -44,6 +44,7 @@ __host__ __device__ inline bool u576_geq_m(const uint64_t *a) {
   // m = 2^576 - 2^240 + 1: limbs 0..2: [1, 0, 0], limb 3: 2^64 - 2^48 (bits 48..63 set),
   // limbs 4..8: all ones.
   const uint64_t m[9] = {1, 0, 0, 0xFFFF000000000000ull, ~0ull, ~0ull, ~0ull, ~0ull, ~0ull};
+  #pragma unroll
   for (int i = 8; i >= 0; --i) {
     if (a[i] != m[i]) return a[i] > m[i];
   }
@@ -52,6 +53,7 @@ __host__ __device__ inline bool u576_geq_m(const uint64_t *a) {
 __host__ __device__ inline void u576_sub_m(uint64_t *a) {
   const uint64_t m[9] = {1, 0, 0, 0xFFFF000000000000ull, ~0ull, ~0ull, ~0ull, ~0ull, ~0ull};
   uint64_t borrow = 0;
+  #pragma unroll
   for (int i = 0; i < 9; ++i) {
     u128 d = (u128)a[i] - m[i] - borrow;
     a[i] = (uint64_t)d;
@@ -61,12 +63,16 @@ __host__ __device__ inline void u576_sub_m(uint64_t *a) {
 
 // s (nlimbs) <- L + H * 2^240 - H  with H given as hn limbs; result limbs
 // written to out (length 14).  Requires H*2^240 >= H, true for any H >= 0.
-__host__ __device__ inline void fold(const uint64_t *L, const uint64_t *H, int hn, uint64_t *out) {
+template <int hn>
+__host__ __device__ inline void fold(const uint64_t *L, const uint64_t *H, uint64_t *out) {
   uint64_t t[14];
+  #pragma unroll
   for (int i = 0; i < 14; ++i) t[i] = 0;
+  #pragma unroll
   for (int i = 0; i < 9; ++i) t[i] = L[i];
   // add H << 240 (= 3 limbs + 48 bits)
   u128 carry = 0;
+  #pragma unroll
   for (int i = 0; i < 11; ++i) {
     uint64_t sh = 0;
     if (i < hn) sh |= H[i] << 48;
@@ -77,20 +83,25 @@ __host__ __device__ inline void fold(const uint64_t *L, const uint64_t *H, int h
   }
   // subtract H
   uint64_t borrow = 0;
+  #pragma unroll
   for (int i = 0; i < 14; ++i) {
     uint64_t hv = i < hn ? H[i] : 0;
     u128 d = (u128)t[i] - hv - borrow;
     t[i] = (uint64_t)d;
     borrow = (uint64_t)(d >> 64) & 1;
   }
+  #pragma unroll
   for (int i = 0; i < 14; ++i) out[i] = t[i];
 }
 
 __host__ __device__ inline void u576_mulmod(const uint64_t *a, const uint64_t *b, uint64_t *r) {
   uint64_t p[18];
+  #pragma unroll
   for (int i = 0; i < 18; ++i) p[i] = 0;
+  #pragma unroll
   for (int i = 0; i < 9; ++i) {
     uint64_t carry = 0;
+    #pragma unroll
     for (int j = 0; j < 9; ++j) {
       u128 cur = (u128)a[i] * b[j] + p[i + j] + carry;
       p[i + j] = (uint64_t)cur;
@@ -99,9 +110,10 @@ __host__ __device__ inline void u576_mulmod(const uint64_t *a, const uint64_t *b
     p[i + 9] = carry;
   }
   uint64_t s[14];
-  fold(p, p + 9, 9, s);          // < 2^576 + 2^816
+  fold<9>(p, p + 9, s);          // < 2^576 + 2^816
   uint64_t s2[14];
-  fold(s, s + 9, 5, s2);         // < 2^576 + 2^561
+  fold<5>(s, s + 9, s2);         // < 2^576 + 2^561
+  #pragma unroll
   for (int i = 0; i < 9; ++i) r[i] = s2[i];
   // s2 fits in 9 limbs plus a possible carry bit in limb 9
   if (s2[9]) {
