@@ -481,7 +481,11 @@ struct LevelTargets {
 // them cooperatively, two lanes' runs of 32 consecutive cells per store
 // instruction, so every store writes whole 128-byte lines.
 constexpr int EMIT_RING = 32;
+// staged Gaussians per lane: a flush leaves fewer than EMIT_FLUSH and one
+// block of 24 numbers (plus up to 3 left over) adds at most 13
+constexpr int EMIT_FLUSH = 32;   // flushing 16 at a time ran slower at 1024^3 (more barriers than occupancy gained)
 constexpr int EMIT_STAGE = 48;
+static_assert(EMIT_FLUSH - 1 + 13 <= EMIT_STAGE, "staging overflow");
 
 template <typename T>
 __device__ __forceinline__ T *cell_ptr(const LevelTargets &lt, uint64_t g) {
@@ -512,9 +516,10 @@ __global__ __launch_bounds__(64) void k_rng_emit(const uint32_t *__restrict__ ra
     g = ss >> 2;
     e = int(ss & 3);
     for (long u = s * SEG; u < t; ++u) {
-      Map4 m = load_map(summary, u);
-      g += m.cnt[e];
-      e = m.ex[e];
+      // the map's entry e only (no per-thread array indexed by e)
+      uint32_t exw = summary[u * 5];
+      g += summary[u * 5 + 1 + e];
+      e = int((exw >> (2 * e)) & 3u);
     }
     if (g >= total) active = false;
   }
@@ -526,7 +531,7 @@ __global__ __launch_bounds__(64) void k_rng_emit(const uint32_t *__restrict__ ra
   uint64_t gstage = g;   // global index of staged slot 0
   auto flush = [&](bool final) {
     __syncthreads();
-    int want = final ? fill : (fill >= 32 ? 32 : 0);
+    int want = final ? fill : (fill >= EMIT_FLUSH ? EMIT_FLUSH : 0);
     unsigned long long mask = __ballot(want > 0);
     while (mask) {
       int s0 = __ffsll((long long)mask) - 1;
