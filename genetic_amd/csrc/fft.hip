@@ -156,7 +156,12 @@ __device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, L
       Dft<R, DIR, T>::run(v[b]);
       int j = (i - k) * R + k;
 #pragma unroll
-      for (int s = 0; s < R; ++s) st(line, j + s * p, v[b][s]);
+      for (int s = 0; s < R; ++s) {
+        // a store functor that takes the butterfly's (b, s) gets them (values
+        // it fetched for this slot before the first stage)
+        if constexpr (std::is_invocable<St, int, int, cx<T>, int, int>::value) st(line, j + s * p, v[b][s], b, s);
+        else st(line, j + s * p, v[b][s]);
+      }
     }
   }
 }
@@ -320,7 +325,39 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
       return cx<T>{v.x * f, v.y * f};
     }
   };
-  auto gst = [&](int line, int pt, cx<T> v) {
+  // POST_AUX: the spectrum it adds is fetched before the first stage, for the
+  // outputs this thread stores in the last one (line q % C, points
+  // q / C + s L / R), so that its latency overlaps the transform
+  using RP = RadixPlan<L>;
+  constexpr int RL = RP::radix(RP::nstages - 1);
+  constexpr int NBL = C * (L / RL);
+  constexpr int PERL = (NBL + THREADS - 1) / THREADS;
+  constexpr bool PREFETCH = POST == POST_AUX;
+  cx<T> pre[PREFETCH ? PERL : 1][PREFETCH ? RL : 1];
+  if constexpr (PREFETCH) {
+    const cx<T> *src = a.aux + (ob - a.out);
+#pragma unroll
+    for (int bb = 0; bb < PERL; ++bb) {
+      int q = int(threadIdx.x) + bb * THREADS;
+#pragma unroll
+      for (int s = 0; s < RL; ++s) {
+        cx<T> v{0, 0};
+        if (NBL % THREADS == 0 || q < NBL) {
+          int line = q % C, pt = q / C + s * (L / RL);
+          int o = pt;
+          bool ok = line < nc;
+          if (a.wlen < L) {
+            o = pt - a.win0;
+            if (o < 0) o += L;
+            ok = ok && o < a.wlen;
+          }
+          if (ok) v = src[long(o) * a.out_ls + line];
+        }
+        pre[bb][s] = v;
+      }
+    }
+  }
+  auto gst = [&](int line, int pt, cx<T> v, auto... slot) {
     if (line >= nc) return;
     int o = pt;
     if (a.wlen < L) {
@@ -344,8 +381,13 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
     if constexpr (POST == POST_AUX) {
       int kx = signed_k(pt, a.n), kz = col0 + line;
       T f = kfactor<T>(a, kx * kx + ky2 + kz * kz);
-      long q = long(o) * a.out_ls + line + (ob - a.out);
-      cx<T> x = a.aux[q];
+      cx<T> x;
+      if constexpr (sizeof...(slot) == 2) {
+        int bs[2] = {slot...};
+        x = pre[bs[0]][bs[1]];
+      } else {
+        x = a.aux[long(o) * a.out_ls + line + (ob - a.out)];
+      }
       ob[o * a.out_ls + line] = cx<T>{x.x * f + v.x * s, x.y * f + v.y * s};
       return;
     }
