@@ -693,22 +693,24 @@ struct Generator {
     ids.erase(std::unique(ids.begin(), ids.end()), ids.end());
     if (!append)
       for (auto &g : grids) g.flags.clear();
-    checkGraficIds();
-    buildMapper();
-    flagParticles(int(grids.size()) - 1, ids);
+    if (fmt == Fmt::grafic) {
+      graficFlagParticles(ids);
+    } else {
+      buildMapper();
+      flagParticles(int(grids.size()) - 1, ids);
+    }
     centreOnFlagged();
   }
-  // With grafic output, particle IDs index the cells of the grafic hierarchy;
-  // only the case where that hierarchy is the base grid alone is supported here.
-  void checkGraficIds() const {
-    if (fmt == Fmt::grafic && (grids.size() > 1 || subsample > 1 || supersample > 1))
-      throw std::runtime_error("id files over a multi-level grafic hierarchy are not supported by this build");
-  }
   void dumpIDFile(const std::string &fn) {
-    checkGraficIds();
-    buildMapper();
+    std::vector<size_t> ids;
+    if (fmt == Fmt::grafic) {
+      ids = graficFlaggedParticles();
+    } else {
+      buildMapper();
+      ids = flaggedParticles();
+    }
     std::ofstream f(fn);
-    for (size_t id : flaggedParticles()) f << id << std::endl;
+    for (size_t id : ids) f << id << std::endl;
   }
   void setGadgetParticleType(unsigned t) {
     if (t >= 6) throw std::runtime_error("Gadget particle type must be between 0 and 5");
@@ -1440,6 +1442,98 @@ struct Generator {
   static float inMask(const std::vector<std::vector<size_t>> &flagged, size_t l, size_t i) {
     if (flagged[l].empty()) return 1.f;
     return std::binary_search(flagged[l].begin(), flagged[l].end(), i) ? 1.f : 0.f;
+  }
+  // Particle IDs of grafic output (GraficMapper, graficmapper.hpp:75-142):
+  // the cells of the hierarchy's levels numbered one level after another.
+  // Flags live on the real grids; a subsampled level sees them coarsened
+  // (SubSampleGrid, virtualgrid.hpp:556-568), a supersampled one refined
+  // (SuperSampleGrid, virtualgrid.hpp:161-175; Grid::upscaleCellFlagVector and
+  // downscaleCellFlagVector, grid.hpp:384-436); centring and offsets move
+  // centroids but keep cell ids.
+  static std::vector<size_t> levelFlagsToReal(const VLevel &v, const Grid &real, const std::vector<size_t> &ids) {
+    std::vector<size_t> out;
+    if (v.factor > 1) {
+      int f = v.factor;
+      for (size_t id : ids) {
+        I3 c = v.g.coord(id);
+        for (int x = c.x * f; x < c.x * f + f && x < real.n; ++x)
+          for (int y = c.y * f; y < c.y * f + f && y < real.n; ++y)
+            for (int z = c.z * f; z < c.z * f + f && z < real.n; ++z) out.push_back(real.index(x, y, z));
+      }
+    } else if (v.factor < -1) {
+      int f = -v.factor;
+      for (size_t id : ids) {
+        I3 c = v.g.coord(id);
+        out.push_back(real.index(c.x / f, c.y / f, c.z / f));
+      }
+    } else {
+      out = ids;
+    }
+    return out;
+  }
+  static std::vector<size_t> realFlagsToLevel(const VLevel &v, const Grid &real) {
+    std::vector<size_t> out;
+    if (v.factor > 1) {
+      int f = v.factor;
+      for (size_t id : real.flags) {
+        I3 c = real.coord(id);
+        out.push_back(v.g.index(c.x / f, c.y / f, c.z / f));
+      }
+      std::sort(out.begin(), out.end());
+      out.erase(std::unique(out.begin(), out.end()), out.end());
+    } else if (v.factor < -1) {
+      // each coarse flag becomes its f^3 sub-cells, x slowest, in flag order
+      int f = -v.factor;
+      for (size_t id : real.flags) {
+        I3 c = real.coord(id);
+        for (int x = c.x * f; x < c.x * f + f; ++x)
+          for (int y = c.y * f; y < c.y * f + f; ++y)
+            for (int z = c.z * f; z < c.z * f + f; ++z) out.push_back(v.g.index(x, y, z));
+      }
+    } else {
+      out = real.flags;
+    }
+    return out;
+  }
+  // GraficMapper::flagParticles + propagateFlagsThroughHierarchy
+  // (graficmapper.hpp:75-108,163-184); ids ascending
+  void graficFlagParticles(const std::vector<size_t> &ids) {
+    auto st = graficStack({x0, y0, z0});
+    size_t start = 0, i = 0;
+    for (const VLevel &v : st) {
+      size_t end = start + v.g.n3();
+      std::vector<size_t> cells;
+      while (i < ids.size() && ids[i] < end) cells.push_back(ids[i++] - start);
+      if (v.factor != 1 && !cells.empty())
+        log() << cells.size() << " input ids reference a GRAFIC intermediate grid - make sure you intended to do this !"
+              << std::endl;
+      grids[v.src].flagCells(levelFlagsToReal(v, grids[v.src], cells));
+      start = end;
+    }
+    if (i != ids.size()) throw std::runtime_error("Ran out of grids when interpreting grafic cell IDs - check IDs?");
+    // flagged cells of each full-resolution level flag the cells under their
+    // centroids on the next coarser full-resolution level, finest first
+    std::vector<size_t> real;
+    for (size_t l = 0; l < st.size(); ++l)
+      if (st[l].factor == 1) real.push_back(l);
+    for (size_t k = real.size() - 1; k > 0; --k) {
+      const VLevel &fine = st[real[k]], &coarse = st[real[k - 1]];
+      std::vector<size_t> down;
+      for (size_t id : grids[fine.src].flags) down.push_back(vIndexFromPoint(coarse, vCentroid(fine, id)));
+      std::sort(down.begin(), down.end());
+      down.erase(std::unique(down.begin(), down.end()), down.end());
+      grids[coarse.src].flagCells(down);
+    }
+  }
+  // GraficMapper::getFlaggedParticles (graficmapper.hpp:127-142)
+  std::vector<size_t> graficFlaggedParticles() const {
+    std::vector<size_t> out;
+    size_t offset = 0;
+    for (const VLevel &v : graficStack({x0, y0, z0})) {
+      for (size_t id : realFlagsToLevel(v, grids[v.src])) out.push_back(id + offset);
+      offset += v.g.n3();
+    }
+    return out;
   }
   std::array<double, 3> graficCentre() const {
     if (centerOnTargetRegion) return {x0, y0, z0};
