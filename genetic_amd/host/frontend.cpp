@@ -435,7 +435,7 @@ struct Generator {
     log() << "  dx                    = " << ng.dx << std::endl;
     log() << "  Zoom factor           = " << zoomfac << std::endl;
     log() << "  Num particles         = " << untrimmed.size() << std::endl;
-    log() << "  Low-left corner in parent grid = " << x0i << ", " << y0i << ", " << z0i << std::endl;
+    log() << "  Low-left corner in parent grid = (" << x0i << ", " << y0i << ", " << z0i << ")" << std::endl;
     log() << "  Low-left corner (h**-1 Mpc)    = " << ng.ox << ", " << ng.oy << ", " << ng.oz << std::endl;
     if (fmt == Fmt::grafic) {
       // grafic output keeps no particle mapper across levels: its "particles"
@@ -496,6 +496,19 @@ struct Generator {
     actual.wrapCorners();
     zw.moveInto(actual);
     zw.shift(aboveLo, -1);
+    // flagged cells within 3 cells of the window's faces (ic.hpp:462-473;
+    // Window::containsWithBorderSafety, window.hpp:296-302)
+    const int borderSafety = 3;
+    for (size_t id : above.flags) {
+      I3 c = above.coord(id);
+      bool in = true;
+      for (int d = 0; d < 3; ++d) in &= Window::within(zw.lo[d] + borderSafety, zw.hi[d] - borderSafety, c[d]);
+      if (!in) {
+        log() << "WARNING: Opening a zoom where flagged particles are within " << borderSafety
+              << " pixels of the edge. This is prone to numerical errors." << std::endl;
+        break;
+      }
+    }
     initZoomGridWithLowLeftCornerAt(zw.lo.x, zw.lo.y, zw.lo.z, zoomfac, n);
   }
 
@@ -583,7 +596,20 @@ struct Generator {
   }
 
   // TwoLevelParticleMapper constructor (twolevelmapper.hpp:270-320), per level
+  // setSubsample / setSupersample (ic.hpp:296-330) rebuild the particle
+  // mapper at once; with particle output that needs the resampled mappers
+  void setResampling(int sub, int super) {
+    if ((sub > 1 || super > 1) && fmt != Fmt::grafic && fmt != Fmt::unknown)
+      throw std::runtime_error("subsample and supersample apply only to grafic output in this build");
+    subsample = sub;
+    supersample = super;
+  }
   void buildMapper() {
+    // resampled particle output (the sub/super-sampled mappers,
+    // twolevelmapper.hpp:560-610) is not built: refuse rather than write
+    // particles at the grids' own resolution
+    if (subsample > 1 || supersample > 1)
+      throw std::runtime_error("subsample and supersample apply only to grafic output in this build");
     maps.assign(grids.size(), LevelMap());
     for (size_t L = 1; L < grids.size(); ++L) {
       LevelMap &m = maps[L];
@@ -1979,8 +2005,8 @@ struct Dispatcher {
           ic.lptOrder = o;
         }));
     // grafic hierarchy and mask (ic.hpp:301-350, main.cpp:59-74)
-    add("subsample", std::function<void(int)>([this](int f) { ic.subsample = f; }));
-    add("supersample", std::function<void(int)>([this](int f) { ic.supersample = f; }));
+    add("subsample", std::function<void(int)>([this](int f) { ic.setResampling(f, ic.supersample); }));
+    add("supersample", std::function<void(int)>([this](int f) { ic.setResampling(ic.subsample, f); }));
     add("pvar", std::function<void(double)>([this](double v) { ic.pvarValue = v; }));
     add("center_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));
     add("centre_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));
