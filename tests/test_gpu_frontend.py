@@ -39,6 +39,7 @@ def particle_file(d):
     "test_25_zoom_factor_1",
     "test_13_parallel_seed",
     "test_05a_seed_old",
+    "test_10h_gadget_zoom_ptype_gas",
 ])
 def test_particle_output_matches_reference(tmp_path, test):
     d, _ = _run(test, tmp_path)
@@ -133,10 +134,30 @@ def test_modification_logs_match_reference(tmp_path, test):
     "test_14_velocity_modif",
     "test_22_angular_momentum_modif",
     "test_20_velmodif_zoom",
+    "test_02e_two_separate_modifs",
 ])
 def test_modified_particle_output_matches_reference(tmp_path, test):
     d, _ = _run(test, tmp_path)
     compare_particles(particle_file(d), golden(test, "reference_output"))
+
+
+@pytest.mark.parametrize("test", [
+    "test_02a_onemodif_zeroed_zoom",
+    "test_02b_onemodif_zoom",
+    "test_02d_modif_double_zoom",
+    "test_02i_multilevel_algorithm",
+])
+def test_modified_zoom_grids_match_reference(tmp_path, test):
+    """Constraints on zoom hierarchies (one and two zooms, zeroed levels, a
+    filtered multi-level variance modification): every level's dumped grid
+    as compare.py compare_grids checks it, 4 decimals."""
+    import glob
+    d, _ = _run(test, tmp_path)
+    refs = sorted(glob.glob(os.path.join(golden(test, "reference_grid"), "grid-?.npy")))
+    assert refs
+    for ref in refs:
+        name = os.path.basename(ref)
+        assert_decimal(np.load(os.path.join(d, name)), np.load(ref), 4, test + " " + name)
 
 
 def test_modified_overdensity_array_and_grids(tmp_path):
