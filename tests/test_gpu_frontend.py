@@ -146,6 +146,7 @@ def test_modified_particle_output_matches_reference(tmp_path, test):
     "test_02b_onemodif_zoom",
     "test_02d_modif_double_zoom",
     "test_02i_multilevel_algorithm",
+    "test_02c_two_modifs",
 ])
 def test_modified_zoom_grids_match_reference(tmp_path, test):
     """Constraints on zoom hierarchies (one and two zooms, zeroed levels, a
@@ -242,3 +243,16 @@ def test_dump_vx_is_the_zeldovich_offset_of_the_dumped_grid(tmp_path):
     want = np.fft.ifftn(op * np.fft.fftn(delta)).real
     assert_decimal(vx, want, 5, "vx-0")
     assert open(os.path.join(d, "vx-info-0.txt")).read().split("\n")[0].split() == ["0", "0", "0", "50"]
+
+
+def test_wrapping_double_zoom_matches_reference(tmp_path):
+    """test_01e_wrapping_zoom: two nested zooms around a point near the box
+    corner, so both windows wrap; particles as compare.py checks them and
+    the dump_tipsy overdensity array to 5 decimals (compare.py:50-51)."""
+    test = "test_01e_wrapping_zoom"
+    d, _ = _run(test, tmp_path)
+    compare_particles(particle_file(d), golden(test, "reference_output"))
+    got = read_tipsy_array(os.path.join(d, "test_1.tipsy.overdensity"))
+    want = read_tipsy_array(golden(test, "reference_output.overdensity"))
+    assert got.shape == want.shape
+    np.testing.assert_almost_equal(got, want, decimal=5)
