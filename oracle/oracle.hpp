@@ -37,6 +37,9 @@ struct Ranlxs {
   int pos = 24;        // next output index in out[]
   uint64_t nblocks = 0;
   explicit Ranlxs(unsigned long seed, int luxury = 397);
+  // the generator at a block boundary with the given 12 words (words[0] the
+  // oldest) and carry: the next get() runs the 397 steps of a new block
+  Ranlxs(const int64_t *words, int carry_in, int luxury = 397);
   void increment();
   uint32_t get() {                 // gsl_rng_get: 24-bit integer
     if (pos >= 24) increment();

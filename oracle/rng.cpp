@@ -47,6 +47,15 @@ Ranlxs::Ranlxs(unsigned long s, int luxury) {
   pos = 24;
 }
 
+Ranlxs::Ranlxs(const int64_t *words, int carry_in, int luxury) {
+  for (int k = 0; k < 12; ++k) x[k] = words[k] & MASK48;
+  carry = carry_in ? 1 : 0;
+  ir = 0;
+  jr = 7;
+  pr = luxury;
+  pos = 24;
+}
+
 void Ranlxs::increment() {
   // pr steps of the subtract-with-borrow recurrence x_n = x_{n-5} - x_{n-12} - c
   // in base 2^48, then emit the 12 newest words (oldest first) as low/high

@@ -217,3 +217,40 @@ def select_ellipse_cells(n, box, centre, a, b, c):
     a2, b2, c2 = (np.float64(np.float32(v) * np.float32(v)) for v in (a, b, c))
     r = d[0][:, None, None] ** 2 / a2 + d[1][None, :, None] ** 2 / b2 + d[2][None, None, :] ** 2 / c2
     return np.flatnonzero((r < 1.0).ravel())
+
+
+def plane_wave(n, m, amplitude):
+    """amplitude * cos(2 pi m.i / n) on the n^3 index grid (float64)."""
+    i = np.arange(n)
+    ph = [2 * np.pi * m[a] * i / n for a in range(3)]
+    return amplitude * np.cos(ph[0][:, None, None] + ph[1][None, :, None] + ph[2][None, None, :])
+
+
+def potential_plane_wave(n, box, m, amplitude, cells, Om, z):
+    """calculate potential for the overdensity amplitude cos(k.x) with k =
+    2 pi m / box, from the Poisson equation in real space: the mean over
+    the flagged cells of -(3/2)(Om/a)(100/c)^2 delta / |k|^2 (c in km/s),
+    and zero for a wave with no y component -- densityToPotential zeroes
+    every mode with k_y = 0 (genetIC/src/cosmology/parameters.hpp:208-225)."""
+    if m[1] == 0:
+        return 0.0
+    a = 1.0 / (1.0 + z)
+    prefac = 1.5 * Om / a * 100.0 * 100.0 / 299792.0 / 299792.0
+    k2 = (2 * np.pi / box) ** 2 * float(m[0] ** 2 + m[1] ** 2 + m[2] ** 2)
+    wave = plane_wave(n, m, 1.0).ravel()[cells]
+    return -prefac * amplitude / k2 * float(wave.mean())
+
+
+# cyclic axis permutation (x, y, z) -> (y, z, x) of a field f(q) -> f(P q):
+# velocity and angular-momentum components map as (x', y', z') = (z, x, y)
+def permute_field(f):
+    return np.transpose(f, (2, 0, 1))
+
+
+def permute_cells(n, cells):
+    mask = np.zeros(n ** 3, dtype=bool)
+    mask[cells] = True
+    return np.flatnonzero(permute_field(mask.reshape(n, n, n)).ravel())
+
+
+PERMUTED_NAME = {"vx": "vz", "vy": "vx", "vz": "vy", "Lx": "Lz", "Ly": "Lx", "Lz": "Ly"}

@@ -76,6 +76,40 @@ def test_raw_stream_properties():
     assert np.array_equal(po.ranlxs_raw(0, 100), po.ranlxs_raw(1, 100))
 
 
+@pytest.mark.parametrize("seed", [8896131, 1, 42, 13842314])
+def test_independent_stream_jump_matches_serial_stream(seed):
+    """The big-integer jump of tests/rng_jump.py, continued by the oracle's
+    serial generator, reproduces the serial stream at block boundaries from
+    0 to 2e5 blocks; a dropped carry, a wrong lag or a wrong modulus fails
+    at the first block past the jump."""
+    import rng_jump as rj
+    nb = 200_000 if seed == 42 else 3000
+    ref = po.ranlxs_raw(seed, 24 * nb)
+    # the seeded words themselves give block 0
+    assert np.array_equal(po.ranlxs_raw_from_state(rj.seed_words(seed), 0, 48), ref[:48])
+    blocks = [0, 1, 2, 7, 100, nb - 6]
+    if seed == 42:
+        blocks += [65535, 65536, 131071, 123457, 199990]
+    for k in blocks:
+        words, carry = rj.state_at_block(seed, k)
+        assert np.array_equal(po.ranlxs_raw_from_state(words, carry, 120), ref[24 * k:24 * k + 120]), k
+    # encode / decode are inverse on the residues
+    y = rj.encode(rj.seed_words(seed), 1)
+    w, c = rj.decode(y)
+    assert rj.encode(w, c) == y
+
+
+def test_gaussian_window_located_by_value():
+    """A parse from a block boundary mid-stream merges with the stream's own
+    parse and is found in it by value."""
+    import rng_jump as rj
+    g, _ = po.gaussians(99, 2_000_000)
+    vals = rj.gaussians_near(po, 99, 1_500_000, 2.05338, 5000)
+    i = rj.locate(g[1_300_000:1_700_000], vals)
+    assert i >= 0
+    np.testing.assert_allclose(g[1_300_000 + i:1_300_000 + i + 5000], vals, rtol=0, atol=1e-13)
+
+
 def test_gaussian_moments():
     g, _ = po.gaussians(42, 400000)
     assert abs(g.mean()) < 5 / np.sqrt(g.size)
@@ -560,3 +594,65 @@ def test_oracle_2lpt_mixed_derivative_closed_form():
     o2.zeldovich()
     for a in range(3):
         assert np.abs(o2.field(4 + a, 0)).max() < 1e-14
+
+
+# ------------------------------------------------- potential, v, L covectors ---
+EH_TABLE = os.path.join(ROOT, "genetic_amd", "data", "eh_nowiggle_512.dat")
+EH_COSMO = dict(Om=0.31, Ol=0.69, s8=0.81, ns=0.967, z=99.0, hubble=0.677)
+
+
+def _eh_oracle(n, box):
+    o = po.OracleIC()
+    o.set_cosmology(**EH_COSMO)
+    tab, nc = po.read_camb(EH_TABLE)
+    o.set_camb(tab, nc)
+    o.init_base_grid(box, n)
+    return o
+
+
+@pytest.mark.parametrize("m", [(2, 1, 3), (0, 3, 0), (1, -2, 1), (3, 0, 2), (4, 0, 0)])
+def test_oracle_potential_plane_wave_closed_form(m):
+    """calculate potential of a plane-wave overdensity equals the mean of
+    the real-space Poisson solution over the flagged cells; waves with no y
+    component give exactly zero (the reference zeroes k_y = 0 modes).  The
+    EH table has equal dark-matter and total columns, so the transfer ratio
+    the covector carries after the power spectrum is 1."""
+    from helpers import plane_wave, potential_plane_wave
+    n, box, A = 32, 50.0, 0.01
+    o = _eh_oracle(n, box)
+    o.seed(5)
+    o.apply_power_spectrum()
+    o.set_field(0, plane_wave(n, m, A))
+    cells = select_sphere_cells(n, box, (20.0, 27.0, 31.0), 9.0)
+    o.set_flags(0, cells)
+    want = potential_plane_wave(n, box, m, A, cells, EH_COSMO["Om"], EH_COSMO["z"])
+    got = o.calculate("potential")
+    if m[1] == 0:
+        assert abs(got) < 1e-22
+    else:
+        assert abs(got - want) <= 1e-10 * abs(want), (got, want)
+    # linear in the field, and the overdensity covector is the plain mean
+    assert abs(o.calculate("overdensity") - A * plane_wave(n, m, 1.0).ravel()[cells].mean()) < 1e-14
+
+
+def test_oracle_velocity_and_angular_momentum_axis_permutation():
+    """Permuting the axes of the noise and of the flagged region cyclically
+    permutes the velocity and angular-momentum components (the stencils,
+    Nyquist rules and wrapped centre treat the axes alike)."""
+    from helpers import permute_field, permute_cells, PERMUTED_NAME
+    n, box = 32, 50.0
+    rng = np.random.default_rng(3)
+    w = rng.standard_normal((n, n, n))
+    cells = select_sphere_cells(n, box, (11.0, 30.0, 44.0), 7.0)
+    vals = {}
+    for perm in (False, True):
+        o = _eh_oracle(n, box)
+        o.seed(1)
+        o.draw()
+        o.set_field(0, permute_field(w) if perm else w)
+        o.set_flags(0, permute_cells(n, cells) if perm else cells)
+        vals[perm] = {name: o.calculate(name) for name in PERMUTED_NAME}
+    for name, src in PERMUTED_NAME.items():
+        a, b = vals[True][name], vals[False][src]
+        assert abs(a - b) <= 1e-10 * max(abs(b), 1e-30), (name, a, b)
+    assert len({round(v, 12) for v in vals[False].values()}) == 6   # six distinct values

@@ -84,9 +84,7 @@ def test_two_zoom_grids_and_power_spectra(tmp_path):
     d, _ = _run("test_01d_two_zooms", tmp_path)
     ref = np.load(golden("test_01d_two_zooms", "reference_grid", "grid-2.npy"))
     got = np.load(os.path.join(d, "grid-2.npy"))
-    # the fixture predates the reference's current build by a uniform ~2e-5
-    # of its maximum on this level (it matches the oracle of the current code)
-    assert np.abs(got - ref).max() < 5e-5 * np.abs(ref).max() + 1.5e-4
+    assert_decimal(got, ref, 4, "grid-2")
     for lev in (0, 1, 2):
         r = np.loadtxt(golden("test_01d_two_zooms", "reference_ps", "test_01d_%d.ps" % lev))
         g = np.loadtxt(os.path.join(d, "test_1_%d.ps" % lev))
