@@ -106,17 +106,63 @@ def cpu_sample_workload(wl, scale):
     return out
 
 
-def cpu_baseline(wl, scale, reps=3):
+def host_cpu():
+    """lscpu's model name, sockets, cores per socket and threads per core."""
+    import subprocess
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for ln in out.splitlines():
+            if ":" in ln:
+                k, v = ln.split(":", 1)
+                info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return dict(model=info.get("Model name"), sockets=info.get("Socket(s)"),
+                cores_per_socket=info.get("Core(s) per socket"), threads_per_core=info.get("Thread(s) per core"),
+                cpus=info.get("CPU(s)"))
+
+
+def gpu_fft_ms(n=256, precision=32, reps=20):
+    """Median wall ms of an in-place r2c + c2r pair of a padded n^3 field on
+    the GPU (synchronous calls), after 3 warm-up pairs."""
+    import numpy as np
+    import torch
+    from genetic_amd import gic
+    g = gic.ICGenerator(precision=precision)
+    dt = torch.float32 if precision == 32 else torch.float64
+    buf = torch.randn(n * n * (n + 2), dtype=dt, device="cuda")
+    torch.cuda.synchronize()
+    ts = []
+    for r in range(reps + 3):
+        t0 = time.perf_counter()
+        g.fft_inplace(buf.data_ptr(), n, -1)
+        g.fft_inplace(buf.data_ptr(), n, +1)
+        g.synchronize()
+        if r >= 3:
+            ts.append(time.perf_counter() - t0)
+    g.close()
+    return float(np.median(ts)) * 1e3
+
+
+def cpu_baseline(wl, scale, reps=5):
     """The oracle (FP64 C++ restatement of the reference's stage order:
     serial real-space stream, threaded FFTs, modifications, sqrt P,
-    Zel'dovich, 2LPT, level combination) timed on one realisation of the
-    workload's CPU sample, on the host's cores."""
+    Zel'dovich, 2LPT, level combination) timed on the workload's CPU sample
+    on all of the host's cores: one untimed realisation, then the median of
+    `reps`.  On AVX-512 hosts the x86-64-v4 build of the oracle runs.  Also
+    an FFT-only comparison: an r2c + c2r pair at 256^3 on the host (the
+    oracle's own FFT; FFTW is not installed) against the GPU's."""
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
+    build = "liboracle_v4.so" if po.host_has_avx512() else "liboracle.so"
+    po.lib(build)
     s = cpu_sample_workload(wl, scale)
     tab, nc = po.read_camb(EH_TABLE)
-    total = 0.0
-    for r in range(reps):
+    times = []
+    ncells = 0
+    for r in range(reps + 1):
         # the oracle, like the reference, seeds once per generator: one
         # generator per realisation, set up outside the timed region
         o = po.OracleIC()
@@ -145,15 +191,28 @@ def cpu_baseline(wl, scale, reps=3):
             o.apply_modifications()
         o.apply_power_spectrum()
         o.zeldovich()
-        total += time.perf_counter() - t0
+        if r > 0:
+            times.append(time.perf_counter() - t0)
         del o
+    med = float(np.median(times))
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     desc = "%d^3 in %.1f Mpc/h + zooms %s" % (s["n"], s["box"], ", ".join("%d^3" % z[4] for z in s["zooms"]))
-    return dict(value=round(reps * ncells / total / 1e6, 3), unit="Mparticles/s", cores=threads, kind="port",
-                sample="oracle FP64, %d realisations of the workload scaled by 1/%d per side (%s, same cell sizes, "
-                       "%s, constraint sphere %.2f Mpc/h): %d cells each, %.2f s in total"
-                       % (reps, scale, desc, "2LPT" if s["lpt"] == 2 else "ZA",
-                          s["mods"]["radius"] if s.get("mods") else 0.0, ncells, total))
+    cpu_fft = po.time_fft(256, 3)
+    gpu_fft32 = gpu_fft_ms(256, 32)
+    gpu_fft64 = gpu_fft_ms(256, 64)
+    return dict(value=round(ncells / med / 1e6, 3), unit="Mparticles/s", cores=threads, kind="port",
+                host=host_cpu(), build=build,
+                sample="oracle FP64 (%s), the workload scaled by 1/%d per side (%s, same cell sizes, %s, "
+                       "constraint sphere %.2f Mpc/h): %d cells, median of %d realisations after 1 warm-up, "
+                       "%.2f s each"
+                       % (build, scale, desc, "2LPT" if s["lpt"] == 2 else "ZA",
+                          s["mods"]["radius"] if s.get("mods") else 0.0, ncells, reps, med),
+                fft_only=dict(n=256, cpu_ms=round(cpu_fft, 2), gpu_ms_f32=round(gpu_fft32, 4),
+                              gpu_ms_f64=round(gpu_fft64, 4), speedup_f32=round(cpu_fft / gpu_fft32, 1),
+                              speedup_f64=round(cpu_fft / gpu_fft64, 1),
+                              note="r2c + c2r pair of a 256^3 field; CPU: the oracle's own FP64 FFT on all host "
+                                   "threads (FFTW is not installed); GPU: the product's transforms, wall time of "
+                                   "synchronous calls"))
 
 
 def single_level_throughput(name="single256", steps=20, warmup=3, precision=32):
@@ -390,23 +449,39 @@ def main():
         one_step(ic, 1 + k, mods)
     ic.synchronize()
     t1 = time.perf_counter()
-    timings = ic.event_timings()
+    timings = ic.event_timings(with_bytes=True)
     dt = (t1 - t0) / args.steps
 
-    # roofline of the dominant HBM-bound kernel: a strided FFT pass on the
-    # base grid reads and writes the padded field once per launch.
-    esize = 4 if args.precision == 32 else 8
-    n0 = wl["n"]
-    field_bytes = n0 * n0 * (n0 + 2) * esize
-    # full-field passes only (the windowed passes write a sub-volume)
-    hbm = {k: v for k, v in timings.items()
-           if k.startswith("fft_") and k.endswith("." + str(n0)) and "mirror" not in k and "_win" not in k}
-    dom = max(hbm, key=lambda k: hbm[k][0])
-    avg_ms = hbm[dom][0] / hbm[dom][1]
-    achieved = 2 * field_bytes / (avg_ms * 1e-3) / 1e9
+    # Roofline of the dominant kernel: of every timed launch that states its
+    # algorithmic HBM bytes (each input element read once, each output
+    # element written once; a windowed pass counts its whole input and the
+    # window of its output), the kernel with the largest total time.  Its
+    # achieved rate is those bytes over its mean launch time, measured with
+    # hipEvents on the context's stream around each launch.
+    kern = {k: v for k, v in timings.items() if v[2] > 0}
+    dom = max(kern, key=lambda k: kern[k][0])
+    ms, cnt, nbytes = kern[dom]
+    avg_ms = ms / cnt
+    per_launch = nbytes / cnt
+    achieved = per_launch / (avg_ms * 1e-3) / 1e9
     roofline = dict(bound="hbm", kernel=dom, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=pmc_traffic(args.workload, dom),
-                    bytes_per_launch=2 * field_bytes, avg_launch_ms=round(avg_ms, 4))
+                    bytes_per_launch=round(per_launch), avg_launch_ms=round(avg_ms, 4))
+    # every byte-stating kernel, largest total time first
+    top = sorted(kern.items(), key=lambda kv: -kv[1][0])
+    kernels = {k: dict(ms_per_step=round(v[0] / args.steps, 4), launches_per_step=round(v[1] / args.steps, 2),
+                       GBps=round(v[2] / (v[0] * 1e-3) / 1e9, 1), frac=round(v[2] / (v[0] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4))
+               for k, v in top}
+    # the whole step: the algorithmic bytes of all its timed kernels over the
+    # step time, and the share of the step those kernels cover
+    step_bytes = sum(v[2] for v in kern.values()) / args.steps
+    step_kernel_ms = sum(v[0] for v in kern.values()) / args.steps
+    step = dict(algorithmic_bytes=round(step_bytes), GBps=round(step_bytes / dt / 1e9, 1),
+                frac=round(step_bytes / dt / 1e9 / HBM_PEAK_GBS, 4),
+                kernels_ms=round(step_kernel_ms, 3), step_ms=round(dt * 1e3, 3))
+    rng_ms = sum(timings[k][0] for k in ("rng_summary", "rng_scan", "rng_emit") if k in timings) / args.steps
+    rng = dict(gaussians_per_s=round(ncells / (rng_ms * 1e-3), 1) if rng_ms > 0 else None, ms_per_step=round(rng_ms, 4),
+               note="integer-ALU bound (ranlxs2 at luxury 397: 16.5 48-bit steps per 24-bit number)")
     stages = {k: round(v[0] / args.steps, 4) for k, v in sorted(timings.items())}
     out = dict(
         metric=METRIC,
@@ -424,8 +499,23 @@ def main():
         config=dict(workload=args.workload, description=wl["desc"], cells=ncells,
                     cosmology=COSMO, seeds="1..%d real space" % args.steps),
         roofline=roofline,
+        step_roofline=step,
+        rng=rng,
+        kernels=kernels,
         stage_ms_per_step=stages,
     )
+    if mods and not args.no_extras:
+        # the same workload without the modifications (SURVEY C5 as stated)
+        nm_steps = max(2, args.steps // 2)
+        one_step(ic, 3000, None)
+        ic.synchronize()
+        t2 = time.perf_counter()
+        for k in range(nm_steps):
+            one_step(ic, 2000 + k, None)
+        ic.synchronize()
+        dtn = (time.perf_counter() - t2) / nm_steps
+        out["without_modifications"] = dict(value=round(ncells / dtn / 1e6, 2), unit="Mparticles/s",
+                                            ms_per_step=round(dtn * 1e3, 3), steps=nm_steps)
     ic.close()
     if not args.no_extras:
         out["throughput_256"] = single_level_throughput()

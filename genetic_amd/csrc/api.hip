@@ -442,9 +442,12 @@ template <typename T> void do_zeldovich(gic_context *c) {
         v = lowpass_full<T>(c, src, C.g, lp);
       }
       if (f == 0) to_real<T>(c, F.noise, F.g.n, F.noise_fourier);
-      if (c->hook()) c->hook()->begin("interp_add", c->st);
-      interp_add<T>((T *)targets[f]->p, F.g.n, v, ig, c->st);
-      if (c->hook()) c->hook()->end("interp_add", c->st);
+      {
+        // the fine field read and written, the coarse window read once
+        double coarse = double(v.len[0]) * v.len[1] * v.len[2];
+        Timed t(c, "interp_add", (2.0 * F.g.n3() + coarse) * sizeof(T));
+        interp_add<T>((T *)targets[f]->p, F.g.n, v, ig, c->st);
+      }
     }
   }
   if (c->lpt_order == 2) {
@@ -1146,6 +1149,7 @@ int gic_reset_event_timing(gic_context *c) {
     if (c->timer) {
       c->timer->collect();
       c->timer->totals.clear();
+      c->timer->algo_bytes.clear();
     }
   });
 }
@@ -1157,7 +1161,9 @@ int gic_list_event_timing(gic_context *c, char *buf, size_t buflen, size_t *need
     std::string s;
     char line[256];
     for (auto &kv : c->timer->totals) {
-      snprintf(line, sizeof line, "%s\t%.6f\t%ld\n", kv.first.c_str(), kv.second.first, kv.second.second);
+      auto ab = c->timer->algo_bytes.find(kv.first);
+      double b = ab == c->timer->algo_bytes.end() ? 0.0 : ab->second;
+      snprintf(line, sizeof line, "%s\t%.6f\t%ld\t%.0f\n", kv.first.c_str(), kv.second.first, kv.second.second, b);
       s += line;
     }
     if (needed) *needed = s.size() + 1;

@@ -117,6 +117,8 @@ struct EventTimer : TimingHook {
   std::vector<Rec> pending;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, long>> totals;
+  std::map<std::string, double> algo_bytes;   // summed over launches
+  void bytes(const char *name, double b) override { algo_bytes[name] += b; }
   hipEvent_t get() {
     if (!pool.empty()) {
       hipEvent_t e = pool.back();
@@ -266,6 +268,22 @@ namespace gic {
 
 // A named region on the context's stream for the event timer (no-op when
 // timing is off); regions nest and overlap the per-kernel records.
+// One kernel launch or device copy timed under `name`, with its
+// algorithmic HBM bytes (when event timing is on).
+struct Timed {
+  gic_context *c;
+  const char *name;
+  Timed(gic_context *cc, const char *n, double bytes) : c(cc), name(n) {
+    if (c->hook()) {
+      c->hook()->begin(name, c->st);
+      c->hook()->bytes(name, bytes);
+    }
+  }
+  ~Timed() {
+    if (c->hook()) c->hook()->end(name, c->st);
+  }
+};
+
 struct Phase {
   gic_context *c;
   const char *name;

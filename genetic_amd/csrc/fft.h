@@ -10,6 +10,10 @@ struct TimingHook {
   virtual ~TimingHook() = default;
   virtual void begin(const char *name, hipStream_t st) = 0;
   virtual void end(const char *name, hipStream_t st) = 0;
+  // the algorithmic HBM bytes of one launch under this name: each input
+  // element read once, each output element written once (reads of an
+  // accumulated output counted too); tables and indices are not counted
+  virtual void bytes(const char *, double) {}
 };
 
 // In-place 3D transform of a padded field (N*N*(N+2) reals of type T).

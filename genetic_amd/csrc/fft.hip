@@ -748,7 +748,15 @@ template <typename T, int N> struct FftN {
     auto kern = k_fft_strided<T, N, C, DIR, STHREADS, PRE, POST, PASS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     std::string tag = std::string(name) + "." + std::to_string(N);
-    if (hook) hook->begin(tag.c_str(), st);
+    if (hook) {
+      hook->begin(tag.c_str(), st);
+      // lines of the pass; a pruned input or output moves only its window
+      double lines = double(nouter) * NZ * (a.nbatch > 1 ? a.nbatch : 1);
+      double rd = lines * ((a.in_wlen > 0 && a.in_wlen < N) ? a.in_wlen : N);
+      double wr = lines * (a.wlen < N ? a.wlen : N);
+      bool reads_out = a.accum || POST == POST_AUX;   // accumulated output, or the spectrum it adds
+      hook->bytes(tag.c_str(), sizeof(cx<T>) * (rd + wr + (reads_out ? wr : 0.0)));
+    }
     hipLaunchKernelGGL(kern, dim3(nblk), dim3(STHREADS), shm, st, a, tw);
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);
@@ -779,6 +787,7 @@ template <typename T, int N> struct FftN {
     size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
     const cx<T> *tw = twtable<T>().get(N);
     std::string tag = std::string(name) + "." + std::to_string(N);
+    if (hook) hook->bytes(tag.c_str(), double(nrows) * (2 * N + 2) * sizeof(T));   // N reals one way, N+2 the other
     if (dir < 0) {
       auto kern = k_fft_z_r2c<T, N, ROWS, THREADS>;
       if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
@@ -799,7 +808,10 @@ template <typename T, int N> struct FftN {
     int nt = 256;
     long nn = long(N) * N;
     std::string tag = "fft_mirror." + std::to_string(N);
-    if (hook) hook->begin(tag.c_str(), st);
+    if (hook) {
+      hook->begin(tag.c_str(), st);
+      hook->bytes(tag.c_str(), 2.0 * nn * sizeof(cx<T>));   // half of two planes read, half written
+    }
     hipLaunchKernelGGL(k_mirror_planes<T>, dim3((nn + nt - 1) / nt), dim3(nt), 0, st, c, N);
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);
@@ -850,7 +862,10 @@ template <typename T, int N> struct FftN {
     auto kern = k_fft_z_r2c<T, N, ROWS, THREADS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     std::string tag = "fft_z_r2c_boxes." + std::to_string(N);
-    if (hook) hook->begin(tag.c_str(), st);
+    if (hook) {
+      hook->begin(tag.c_str(), st);
+      hook->bytes(tag.c_str(), double(nrows) * (2 * N + 2) * sizeof(T));
+    }
     hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, zbuf, nrows, T(1), tw, rows);
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);
@@ -961,7 +976,12 @@ template <typename T, int N> struct FftN {
     auto launch = [&](auto kern) {
       if (shm > 65536)
         GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-      if (hook) hook->begin(tag.c_str(), st);
+      if (hook) {
+        hook->begin(tag.c_str(), st);
+        // the spectrum rows in; S and A: read A, write S and A (1), read A and
+        // S, write S (2), read and write S (3)
+        hook->bytes(tag.c_str(), double(nrows) * ((N + 2) + (epi == 3 ? 2 : 3) * N) * sizeof(T));
+      }
       hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, out, nrows, norm(), tw, (T *)nullptr, S, A);
       GIC_HIP(hipGetLastError());
       if (hook) hook->end(tag.c_str(), st);
@@ -1019,7 +1039,10 @@ template <typename T, int N> struct FftN {
     auto kern = k_fft_z_c2r_window<T, N, ROWS, THREADS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     std::string tag = "fft_z_c2r_win." + std::to_string(N);
-    if (hook) hook->begin(tag.c_str(), st);
+    if (hook) {
+      hook->begin(tag.c_str(), st);
+      hook->bytes(tag.c_str(), double(nrows) * ((N + 2) + wl[2]) * sizeof(T));
+    }
     hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, bufB, out, nrows, w0[2], wl[2], norm(), tw);
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);
@@ -1039,7 +1062,10 @@ template <typename T, int N> struct FftN {
     auto kern = k_fft_z_r2c_window<T, N, ROWS, THREADS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     std::string tag = "fft_z_r2c_win." + std::to_string(N);
-    if (hook) hook->begin(tag.c_str(), st);
+    if (hook) {
+      hook->begin(tag.c_str(), st);
+      hook->bytes(tag.c_str(), double(nrows) * ((N + 2) + wl[2]) * sizeof(T));
+    }
     hipLaunchKernelGGL(kern, dim3(nblk), dim3(THREADS), shm, st, win, bufB, nrows, w0[2], wl[2], T(1), tw);
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);

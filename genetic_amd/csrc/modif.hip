@@ -722,6 +722,7 @@ template <typename T> struct MLF {
   }
   const T *p(size_t l) const { return (const T *)lv[l].p; }
   void copy_level(const MLF &o, size_t l) {
+    Timed t(c, "copy_field", 2.0 * reals(l) * sizeof(T));
     GIC_HIP(hipMemcpyAsync(p(l), o.p(l), reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
     fourier[l] = o.fourier[l];
   }
@@ -772,6 +773,7 @@ template <typename T> MLF<T> copy_output(gic_context *c) {
   for (size_t l = 0; l < m.nl(); ++l) {
     Level &L = *c->levels[l];
     if (L.dc_pending) ensure_noise_fourier<T>(c, L);
+    Timed t(c, "copy_field", 2.0 * m.reals(l) * sizeof(T));
     GIC_HIP(hipMemcpyAsync(m.p(l), L.noise.p, m.reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
     m.fourier[l] = L.noise_fourier;
   }
@@ -781,6 +783,7 @@ template <typename T> void store_output(gic_context *c, MLF<T> &m) {
   m.toF();
   for (size_t l = 0; l < m.nl(); ++l) {
     Level &L = *c->levels[l];
+    Timed t(c, "copy_field", 2.0 * m.reals(l) * sizeof(T));
     GIC_HIP(hipMemcpyAsync(L.noise.p, m.p(l), m.reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
     L.noise_fourier = true;
     L.dc_pending = false;
@@ -860,6 +863,7 @@ template <typename T> double inner(gic_context *c, MLF<T> &a, MLF<T> &b) {
   c->mod->scalar.ensure(sizeof(double) * 8);
   size_t nl = a.nl();
   for (size_t l = 0; l < nl; ++l) {
+    Timed t(c, "inner", 2.0 * a.g(l).nmodes() * sizeof(cx<T>));
     hipLaunchKernelGGL(k_inner_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st,
                        reinterpret_cast<const cx<T> *>(a.p(l)), reinterpret_cast<const cx<T> *>(b.p(l)), a.g(l).n,
                        (double *)c->mod->part.p);
@@ -1327,11 +1331,27 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
     GIC_HIP(hipGetLastError());
   };
   std::vector<bool> done(nl, false);
-  // deinterpolate down the levels, one level at a time, in the contiguous layout
+  // deinterpolate down the levels, one level at a time, in the contiguous
+  // layout.  The scatter accumulates in FP64 for either precision and is
+  // rounded to T once: the 64-tap sums of the velocity and angular-momentum
+  // stencils cancel heavily, and FP32 atomic sums would leave the coarse
+  // levels' covector both imprecise and dependent on the order of the adds,
+  // so a covector rebuilt later (calculate) would differ from the one the
+  // modification solved with.
   if (L > 0) {
     PBuf accs[2];
-    const T *src = m.p(L);
+    const T *src_t = m.p(L);        // the finest level: padded, type T
+    const double *src_d = nullptr;  // coarser levels: contiguous FP64 sums
     int src_padded = 1;
+    auto deinterp = [&](const DeinterpGeom &dg, size_t nf3, double *acc, size_t len, size_t lo, unsigned *drop) {
+      if (src_d)
+        hipLaunchKernelGGL((k_deinterp<double, double>), dim3(blocks(nf3)), dim3(256), 0, c->st, src_d, src_padded,
+                           dg, acc, len, lo, drop);
+      else
+        hipLaunchKernelGGL((k_deinterp<double, T>), dim3(blocks(nf3)), dim3(256), 0, c->st, src_t, src_padded, dg,
+                           acc, len, lo, drop);
+      GIC_HIP(hipGetLastError());
+    };
     PBuf dropped(c->mod->pool, sizeof(unsigned));
     for (long l = long(L) - 1; l >= 0; --l) {
       const LevelGeom &gh = m.g(size_t(l) + 1), &gl = m.g(size_t(l));
@@ -1363,12 +1383,12 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
         if (x_lo >= 0 && x_hi < gl.n - 1 && x_hi - x_lo + 1 < gl.n) {
           int wl0 = int(x_hi - x_lo + 1);
           size_t len = size_t(wl0) * nn;
-          PBuf win(c->mod->pool, len * sizeof(T));
-          fill<T>((T *)win.p, len, T(0), c->st);
+          PBuf wacc(c->mod->pool, len * sizeof(double));
+          fill<double>((double *)wacc.p, len, 0.0, c->st);
           fill<unsigned>((unsigned *)dropped.p, 1, 0u, c->st);
-          hipLaunchKernelGGL((k_deinterp<T, T>), dim3(blocks(nf3)), dim3(256), 0, c->st, src, src_padded, dg,
-                             (T *)win.p, len, size_t(x_lo) * nn, (unsigned *)dropped.p);
-          GIC_HIP(hipGetLastError());
+          deinterp(dg, nf3, (double *)wacc.p, len, size_t(x_lo) * nn, (unsigned *)dropped.p);
+          PBuf win(c->mod->pool, len * sizeof(T));
+          convert<double, T>((const double *)wacc.p, len, (T *)win.p, c->st);
           unsigned nd = 0;
           GIC_HIP(hipMemcpyAsync(&nd, dropped.p, sizeof(unsigned), hipMemcpyDeviceToHost, c->st));
           GIC_HIP(hipStreamSynchronize(c->st));
@@ -1388,13 +1408,11 @@ template <typename T> MLF<T> linear_covector(gic_context *c, const std::string &
       }
       size_t len = gl.n3() + 2 * nn;
       PBuf &acc = accs[l & 1];
-      acc = PBuf(c->mod->pool, len * sizeof(T));
-      fill<T>((T *)acc.p, len, T(0), c->st);
-      hipLaunchKernelGGL((k_deinterp<T, T>), dim3(blocks(nf3)), dim3(256), 0, c->st, src, src_padded, dg, (T *)acc.p,
-                         len, size_t(0), (unsigned *)nullptr);
-      GIC_HIP(hipGetLastError());
-      pad<T, T>((const T *)acc.p, gl.n, m.p(size_t(l)), c->st);
-      src = (const T *)acc.p;   // the next level reads this level's contiguous values
+      acc = PBuf(c->mod->pool, len * sizeof(double));
+      fill<double>((double *)acc.p, len, 0.0, c->st);
+      deinterp(dg, nf3, (double *)acc.p, len, size_t(0), (unsigned *)nullptr);
+      pad<double, T>((const double *)acc.p, gl.n, m.p(size_t(l)), c->st);
+      src_d = (const double *)acc.p;   // the next level reads this level's contiguous FP64 values
       src_padded = 0;
     }
   }
@@ -1521,10 +1539,12 @@ template <typename T> double chi2_of(gic_context *c, MLF<T> &field) {
                           c->st, c->hook());
       }
       const T *F = spec_table<T>(c, g, spec_levels(c, long(l), -1, 1.0));
+      Timed t(c, "chi2_window", 2.0 * g.nmodes() * sizeof(cx<T>));
       hipLaunchKernelGGL(k_chi2_window_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, xs,
                          reinterpret_cast<const cx<T> *>(uspec.p), F, g.n, (double *)c->mod->part.p);
     } else {
       const T *F2 = spec_table<T>(c, g, spec_levels(c, long(l), long(l), 1.0));
+      Timed t(c, "inner", double(g.nmodes()) * sizeof(cx<T>));
       hipLaunchKernelGGL(k_inner_tab_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, xs, xs, F2, g.n,
                          (double *)c->mod->part.p);
     }
@@ -1553,6 +1573,7 @@ template <typename T> double chi2_of(gic_context *c, MLF<T> &field) {
                            nb, (double *)c->mod->part.p);
       } else {
         R.toF(l);
+        Timed t(c, "inner", 2.0 * g.nmodes() * sizeof(cx<T>));
         hipLaunchKernelGGL(k_inner_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, xs,
                            reinterpret_cast<const cx<T> *>(R.p(l)), g.n, (double *)c->mod->part.p);
       }
@@ -1702,6 +1723,9 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
               md.pb[q] = pq.second < 0 ? 8 : 4 + pq.second - int(jb);
             }
             md.np = np;
+            int nfields = 0;
+            for (int q = 0; q < MAXF; ++q) nfields += md.f[q] != nullptr;
+            Timed t(c, "multi_inner", double(nfields) * field.g(l).nmodes() * sizeof(cx<T>));
             if (!launch_multi_inner<T>(md, field.g(l).n, (double *)c->mod->part.p, c->st))
               throw std::runtime_error("internal: no batched inner product for this many pairs");
             hipLaunchKernelGGL(k_reduce_rows, dim3(np), dim3(RED_THREADS), 0, c->st, (const double *)c->mod->part.p,
@@ -1761,6 +1785,7 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
         }
         al.k = int(k);
         size_t n = field.reals(l);
+        Timed t(c, "axpy_multi", double(k + 2) * n * sizeof(T));
         hipLaunchKernelGGL(k_axpy_multi<T>, dim3(blocks(n)), dim3(256), 0, c->st, field.p(l), al, n);
       }
     }

@@ -400,16 +400,17 @@ class ICGenerator:
         self._check(lib().gic_get_event_timing(self._h, kernel.encode(), ctypes.byref(t), ctypes.byref(c)))
         return t.value, c.value
 
-    def event_timings(self):
-        """{kernel: (total_ms, launches)} for every timed kernel."""
+    def event_timings(self, with_bytes=False):
+        """{kernel: (total_ms, launches)} for every timed kernel; with_bytes
+        adds the launches' summed algorithmic HBM bytes as a third entry."""
         need = ctypes.c_size_t()
         self._check(lib().gic_list_event_timing(self._h, None, 0, ctypes.byref(need)))
         buf = ctypes.create_string_buffer(need.value + 16)
         self._check(lib().gic_list_event_timing(self._h, buf, len(buf), ctypes.byref(need)))
         out = {}
         for ln in buf.value.decode().splitlines():
-            name, ms, cnt = ln.split("\t")
-            out[name] = (float(ms), int(cnt))
+            name, ms, cnt, nbytes = ln.split("\t")
+            out[name] = (float(ms), int(cnt), float(nbytes)) if with_bytes else (float(ms), int(cnt))
         return out
 
     def reset_event_timing(self):

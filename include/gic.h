@@ -232,8 +232,11 @@ int gic_synchronize(gic_context *ctx);
 int gic_set_event_timing(gic_context *ctx, int enable);
 int gic_get_event_timing(gic_context *ctx, const char *kernel, double *total_ms, long *count);
 int gic_reset_event_timing(gic_context *ctx);
-/* All timed kernels as text lines "name<TAB>total_ms<TAB>count"; writes at
- * most buflen bytes (NUL-terminated) and the full length to *needed. */
+/* All timed kernels as text lines "name<TAB>total_ms<TAB>count<TAB>bytes",
+ * bytes being the launches' summed algorithmic HBM traffic (each input
+ * element read once, each output element written once; 0 for regions that
+ * group several kernels); writes at most buflen bytes (NUL-terminated) and
+ * the full length to *needed. */
 int gic_list_event_timing(gic_context *ctx, char *buf, size_t buflen, size_t *needed);
 /* Device bytes currently held by the context. */
 int gic_device_bytes(gic_context *ctx, size_t *out);

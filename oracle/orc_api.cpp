@@ -39,6 +39,27 @@ int orc_gaussians(unsigned long seed, size_t n, double *out, size_t *raw_used) {
   })
 }
 
+// The serial stream continued from a state at a block boundary (12 words,
+// words[0] the oldest, and the carry), for windows far into a stream whose
+// state a test has computed by other means.
+int orc_ranlxs_raw_from_state(const int64_t *words, int carry, size_t n, uint32_t *out) {
+  ORC_TRY({
+    Ranlxs r(words, carry);
+    for (size_t i = 0; i < n; ++i) out[i] = r.get();
+  })
+}
+
+// Gaussians parsed from that state after discarding `skip` raw numbers (the
+// parse then starts mid-block, possibly inside a ziggurat iteration of the
+// true stream; parses from different offsets merge after a few rejections).
+int orc_gaussians_from_state(const int64_t *words, int carry, size_t skip, size_t n, double *out) {
+  ORC_TRY({
+    Ranlxs r(words, carry);
+    for (size_t i = 0; i < skip; ++i) r.get();
+    for (size_t i = 0; i < n; ++i) out[i] = gaussian_ziggurat(r);
+  })
+}
+
 int orc_zig_tables(double *ytab, double *wtab, uint32_t *ktab) {
   ORC_TRY({
     const ZigTables &t = ZigTables::get();
@@ -236,6 +257,33 @@ int orc_time_pipeline(double box, int n, unsigned long seed, const double *table
     *seconds_total = std::chrono::duration<double>(t3 - t1).count();
     *seconds_rng = std::chrono::duration<double>(t2 - t1).count();
     (void)t0;
+  })
+}
+
+// Median wall time (ms) of reps pairs of an in-place r2c and c2r of a
+// random n^3 field on the host's threads, after one untimed pair: the FFT
+// stage alone, for the CPU baseline's FFT-only comparison.
+int orc_time_fft(int n, int reps, double *ms_median) {
+  ORC_TRY({
+    std::vector<double> d(2 * size_t(n) * n * (n / 2 + 1));
+    uint64_t x = 88172645463325252ull;
+    for (size_t i = 0; i < size_t(n) * n * n; ++i) {
+      x ^= x << 13;
+      x ^= x >> 7;
+      x ^= x << 17;
+      d[i] = double(x >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    fft_r2c_inplace(d, n);
+    fft_c2r_inplace(d, n);
+    std::vector<double> t;
+    for (int r = 0; r < reps; ++r) {
+      auto a = std::chrono::steady_clock::now();
+      fft_r2c_inplace(d, n);
+      fft_c2r_inplace(d, n);
+      t.push_back(std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - a).count());
+    }
+    std::sort(t.begin(), t.end());
+    *ms_median = t[t.size() / 2];
   })
 }
 

@@ -19,10 +19,26 @@ def build():
     subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
-def lib():
+def host_has_avx512():
+    """The x86-64-v4 feature set (AVX-512 F, BW, CD, DQ, VL) on this host."""
+    try:
+        flags = set()
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("flags"):
+                flags = set(ln.split(":", 1)[1].split())
+                break
+        return {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= flags
+    except OSError:
+        return False
+
+
+def lib(name=None):
+    """Load the oracle library once: liboracle.so, or the build named by
+    `name` (or GIC_ORACLE_LIB), e.g. liboracle_v4.so on AVX-512 hosts."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        name = name or os.environ.get("GIC_ORACLE_LIB", "liboracle.so")
+        path = os.path.join(_HERE, name)
         if not os.path.exists(path):
             build()
         L = ctypes.CDLL(path)
@@ -35,6 +51,8 @@ def lib():
             "orc_ranlxs_raw": [u, s, v],
             "orc_gaussians": [u, s, v, v],
             "orc_zig_tables": [v, v, v],
+            "orc_ranlxs_raw_from_state": [v, i, s, v],
+            "orc_gaussians_from_state": [v, i, s, s, v],
             "orc_fft_r2c": [i, v],
             "orc_fft_c2r": [i, v],
             "orc_growth_factor": [d, d, d, P(d)],
@@ -62,6 +80,7 @@ def lib():
             "orc_ic_clear_modifications": [v],
             "orc_ic_chi2": [v, P(d), P(s)],
             "orc_time_pipeline": [d, i, u, v, s, i, d, d, d, d, d, P(d), P(d)],
+            "orc_time_fft": [i, i, P(d)],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -97,6 +116,13 @@ def read_camb(path):
     return np.array(vals, dtype=np.float64), ncols
 
 
+def time_fft(n, reps=3):
+    """Median ms of an r2c + c2r pair of a random n^3 field (host threads)."""
+    ms = ctypes.c_double()
+    _check(lib().orc_time_fft(n, reps, ctypes.byref(ms)))
+    return ms.value
+
+
 def ranlxs_raw(seed, n):
     out = np.empty(n, dtype=np.uint32)
     _check(lib().orc_ranlxs_raw(seed, n, _ptr(out)))
@@ -108,6 +134,23 @@ def gaussians(seed, n):
     used = ctypes.c_size_t(0)
     _check(lib().orc_gaussians(seed, n, _ptr(out), ctypes.byref(used)))
     return out, used.value
+
+
+def ranlxs_raw_from_state(words, carry, n):
+    """n raw numbers of the serial generator continued from a block boundary
+    with the 12 given 48-bit words (oldest first) and carry."""
+    w = np.ascontiguousarray(words, dtype=np.int64)
+    out = np.empty(n, dtype=np.uint32)
+    _check(lib().orc_ranlxs_raw_from_state(_ptr(w), int(carry), n, _ptr(out)))
+    return out
+
+
+def gaussians_from_state(words, carry, skip, n):
+    """n ziggurat Gaussians parsed from that state after skipping `skip` raw numbers."""
+    w = np.ascontiguousarray(words, dtype=np.int64)
+    out = np.empty(n, dtype=np.float64)
+    _check(lib().orc_gaussians_from_state(_ptr(w), int(carry), skip, n, _ptr(out)))
+    return out
 
 
 def zig_tables():
