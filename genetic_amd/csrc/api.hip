@@ -697,6 +697,7 @@ int gic_create(gic_context **out, int device, int precision) {
     if (device < 0 || device >= ndev) throw std::runtime_error("no such HIP device");
     GIC_HIP(hipSetDevice(device));
     GIC_HIP(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+    rng_scratch_setup(device);   // the fixed workspace of gic_gaussian_stream_noalloc on this device
   } catch (std::exception &e) {
     delete c;
     return 4;
@@ -1119,6 +1120,15 @@ int gic_gaussian_stream(gic_context *c, unsigned long seed, double *host_out, si
     std::vector<RngLevelOut> outs{{dense.p, 0, n}};
     rng_draw_real<double>(plan, outs, c->work.p, c->work.bytes, c->st, c->hook());
     GIC_HIP(hipMemcpyAsync(host_out, dense.p, n * 8, hipMemcpyDeviceToHost, c->st));
+    GIC_HIP(hipStreamSynchronize(c->st));
+  });
+}
+
+int gic_gaussian_stream_noalloc(gic_context *c, unsigned long seed, double *device_out, size_t n) {
+  return guarded(c, [&] {
+    if (n == 0) return;
+    if (!device_out) throw std::runtime_error("gic_gaussian_stream_noalloc: no output array");
+    rng_gaussians_noalloc(c->device, seed, device_out, n, c->st);
     GIC_HIP(hipStreamSynchronize(c->st));
   });
 }

@@ -51,6 +51,14 @@ size_t rng_workspace_bytes(const RngPlan &plan);
 // graph records
 void rng_prepare();
 
+// Draws into a caller's device array with the fixed per-device workspace:
+// rng_scratch_setup(device) once per device with that device current, then
+// rng_gaussians_noalloc (no device allocation; at most
+// rng_noalloc_max_gaussians() values; overlapping calls on one GPU race).
+void rng_scratch_setup(int device);
+size_t rng_noalloc_max_gaussians();
+void rng_gaussians_noalloc(int device, unsigned long seed, double *dev_out, size_t n, hipStream_t st);
+
 // Raw 24-bit stream (testing / bit-exactness checks): the first n numbers.
 void rng_raw(const RngPlan &plan, uint32_t *d_out, size_t n, void *workspace, size_t ws_bytes, hipStream_t st);
 

@@ -26,6 +26,8 @@
 #include "fft.h"
 #include <cstring>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 namespace gic {
 
@@ -760,13 +762,22 @@ static RngBufs carve(const RngPlan &plan, void *ws, size_t bytes) {
   return r;
 }
 
+constexpr int RNG_MAX_DEVICES = 64;
+static std::mutex g_tables_mu;
+static bool g_tables_done[RNG_MAX_DEVICES];
+
+// the ziggurat tables into the constant memory of the current device (each
+// device has its own copy), once per device
 static void upload_tables() {
-  static bool done = false;
-  if (done) return;
+  int dev = 0;
+  GIC_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= RNG_MAX_DEVICES) throw std::runtime_error("RNG: device index out of range");
+  std::lock_guard<std::mutex> g(g_tables_mu);
+  if (g_tables_done[dev]) return;
   GIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_ytab), ZIG_YTAB, sizeof ZIG_YTAB));
   GIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_wtab), ZIG_WTAB, sizeof ZIG_WTAB));
   GIC_HIP(hipMemcpyToSymbol(HIP_SYMBOL(c_ktab), ZIG_KTAB, sizeof ZIG_KTAB));
-  done = true;
+  g_tables_done[dev] = true;
 }
 
 void rng_prepare() { upload_tables(); }
@@ -804,9 +815,9 @@ void rng_dev_pack(const RngPlan &plan, void *host_dst) {
 }
 
 template <typename T>
-void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, void *ws, size_t ws_bytes,
-                   hipStream_t st, TimingHook *hook, double scale, uint32_t *cover_flag, const void *dev_plan) {
-  if (levels.empty() || levels.size() > 4) throw std::runtime_error("real-space draw supports 1 to 4 levels");
+static void draw_real_list(const RngPlan &plan, const RngLevelOut *levels, size_t nlev, void *ws, size_t ws_bytes,
+                           hipStream_t st, TimingHook *hook, double scale, uint32_t *cover_flag, const void *dev_plan) {
+  if (nlev < 1 || nlev > 4) throw std::runtime_error("real-space draw supports 1 to 4 levels");
   upload_tables();
   RngBufs b = carve(plan, ws, ws_bytes);
   long nc = plan.nchunks;
@@ -832,9 +843,9 @@ void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, 
   LevelTargets lt;
   size_t acc = 0;
   for (size_t i = 0; i < 4; ++i) {
-    lt.f[i] = i < levels.size() ? levels[i].field : nullptr;
-    lt.n[i] = i < levels.size() ? levels[i].n : 0;
-    if (i < levels.size()) acc += levels[i].count;
+    lt.f[i] = i < nlev ? levels[i].field : nullptr;
+    lt.n[i] = i < nlev ? levels[i].n : 0;
+    if (i < nlev) acc += levels[i].count;
     lt.cum[i] = acc;
   }
   lt.scale = scale;
@@ -860,10 +871,63 @@ void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, 
   if (hook) hook->end("rng_emit", st);
 }
 
+template <typename T>
+void rng_draw_real(const RngPlan &plan, const std::vector<RngLevelOut> &levels, void *ws, size_t ws_bytes,
+                   hipStream_t st, TimingHook *hook, double scale, uint32_t *cover_flag, const void *dev_plan) {
+  draw_real_list<T>(plan, levels.data(), levels.size(), ws, ws_bytes, st, hook, scale, cover_flag, dev_plan);
+}
+
 template void rng_draw_real<float>(const RngPlan &, const std::vector<RngLevelOut> &, void *, size_t, hipStream_t,
                                    TimingHook *, double, uint32_t *, const void *);
 template void rng_draw_real<double>(const RngPlan &, const std::vector<RngLevelOut> &, void *, size_t, hipStream_t,
                                     TimingHook *, double, uint32_t *, const void *);
+
+// ---- draws with a fixed workspace ------------------------------------------------
+// For callers that cannot allocate device memory: the draw's workspace (chunk
+// summaries, segment maps and starts, raw numbers, the plan) lives in this
+// file-scope device array, one copy per GPU, held until the process ends.
+// Its address on each device is looked up once (rng_scratch_setup, called by
+// gic_create with the device current) and only that address is passed to
+// kernels: the array's name in host code is a host-side stand-in.  The
+// workspace needs no clearing (every pass writes what the next one reads).
+// Calls that overlap on one GPU share the array and race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(13) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[RNG_MAX_DEVICES];
+
+void rng_scratch_setup(int device) {
+  if (device < 0 || device >= RNG_MAX_DEVICES) throw std::runtime_error("RNG: device index out of range");
+  upload_tables();
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void *p = nullptr;
+  GIC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw std::runtime_error("RNG: no device address for the fixed workspace");
+  g_scratch_ptr[device] = p;
+}
+
+size_t rng_noalloc_max_gaussians() { return size_t(1) << 20; }
+
+void rng_gaussians_noalloc(int device, unsigned long seed, double *dev_out, size_t n, hipStream_t st) {
+  if (n > rng_noalloc_max_gaussians())
+    throw std::runtime_error("gic_gaussian_stream_noalloc: at most " + std::to_string(rng_noalloc_max_gaussians()) +
+                             " Gaussians per call (the fixed workspace holds " +
+                             std::to_string(NOALLOC_SCRATCH_BYTES >> 20) +
+                             " MiB); gic_gaussian_stream draws longer streams");
+  void *ws = nullptr;
+  {
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (device >= 0 && device < RNG_MAX_DEVICES) ws = g_scratch_ptr[device];
+  }
+  if (!ws) throw std::runtime_error("gic_gaussian_stream_noalloc: the fixed workspace of this device is not set up");
+  RngPlan plan;
+  rng_plan(seed, n, plan);
+  if (rng_workspace_bytes(plan) > NOALLOC_SCRATCH_BYTES)
+    throw std::runtime_error("internal error: the fixed RNG workspace is too small for this draw");
+  RngLevelOut out{dev_out, 0, n};
+  draw_real_list<double>(plan, &out, 1, ws, NOALLOC_SCRATCH_BYTES, st, nullptr, 1.0, nullptr, nullptr);
+}
 
 void rng_raw(const RngPlan &plan, uint32_t *d_out, size_t n, void *, size_t, hipStream_t st) {
   if (n > size_t(plan.nchunks) * plan.ch()) throw std::runtime_error("rng_raw: plan too small");

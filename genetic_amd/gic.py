@@ -69,6 +69,7 @@ SIGNATURES = {
     "gic_fft_inplace": ([_v, _v, _i, _i], _i),
     "gic_raw_stream": ([_v, _u, _v, _s], _i),
     "gic_gaussian_stream": ([_v, _u, _v, _s], _i),
+    "gic_gaussian_stream_noalloc": ([_v, _u, _v, _s], _i),
     "gic_synchronize": ([_v], _i),
     "gic_set_event_timing": ([_v, _i], _i),
     "gic_get_event_timing": ([_v, ctypes.c_char_p, _P(_d), _P(ctypes.c_long)], _i),
@@ -383,6 +384,12 @@ class ICGenerator:
         out = np.empty(n, dtype=np.float64)
         self._check(lib().gic_gaussian_stream(self._h, int(seed), _ptr(out), n))
         return out
+
+    def gaussian_stream_noalloc(self, seed, device_ptr, n):
+        """The first n Gaussians of the stream into a device array of n
+        doubles, with the library's fixed workspace (no device allocation;
+        n <= 1048576)."""
+        self._check(lib().gic_gaussian_stream_noalloc(self._h, int(seed), ctypes.c_void_p(int(device_ptr)), int(n)))
 
     def fft_inplace(self, device_ptr, n, direction):
         self._check(lib().gic_fft_inplace(self._h, ctypes.c_void_p(device_ptr), int(n), int(direction)))

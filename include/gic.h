@@ -226,6 +226,15 @@ int gic_raw_stream(gic_context *ctx, unsigned long seed, uint32_t *host_out, siz
 /* First n gsl_ran_gaussian_ziggurat(ranlxs2, 1) draws (randomfieldgenerator.hpp:191). */
 int gic_gaussian_stream(gic_context *ctx, unsigned long seed, double *host_out, size_t n);
 
+/* The first n Gaussians of gic_gaussian_stream, written to the caller's
+ * device array device_out (n doubles) without any device allocation.  No
+ * counterpart in the reference: for callers that cannot allocate (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The workspace is a fixed 13 MiB device array per GPU, held until the
+ * process ends, so n is at most 1048576 (larger n: error, nothing drawn);
+ * calls that overlap on one GPU share it and race.  Synchronous on return. */
+int gic_gaussian_stream_noalloc(gic_context *ctx, unsigned long seed, double *device_out, size_t n);
+
 /* Stream control and per-kernel event timing (hipEvents on the context's
  * stream around each launch). */
 int gic_synchronize(gic_context *ctx);
