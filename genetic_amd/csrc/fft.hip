@@ -17,6 +17,7 @@
 #include "gic_common.h"
 #include "fft.h"
 #include "fft_tiles.h"
+#include <algorithm>
 #include <vector>
 #include <cmath>
 #include <map>
@@ -100,9 +101,10 @@ __device__ __forceinline__ int xcd_tile(int b, int nblk) {
 
 // One Stockham stage.  Butterfly q is mapped to (line, i): LINE_FAST puts
 // adjacent q on adjacent lines (strided passes), otherwise on adjacent points.
-struct NoOp {
-  template <typename C> __device__ __forceinline__ C operator()(int, int, C v) const { return v; }
-};
+// An operator on the first stage's loads has a real factor per (line, point)
+// that depends only on the mode (factor) and combines it with the loaded
+// value (apply).
+struct NoOp {};
 
 template <typename T, int L, int NL, int THREADS, int S, int DIR, bool LINE_FAST, class Ld, class St, class Op = NoOp>
 __device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, Ld ld, St st, bool sync_between,
@@ -124,16 +126,28 @@ __device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, L
       for (int r = 0; r < R; ++r) v[b][r] = ld(line, i + r * (L / R));
     }
   }
-  // operators on the loaded values run after every load has been issued
+  // operators: every factor is formed while the loads are in flight (they
+  // depend only on the mode), and only then applied to the loaded values
   if constexpr (!std::is_same<Op, NoOp>::value) {
+    T fac[PER][R];
 #pragma unroll
     for (int b = 0; b < PER; ++b) {
       int q = tid + b * THREADS;
+      // only the slots that hold a butterfly: an idle slot's (line, point)
+      // lies outside the tile and would index the factor table out of range
       if (NB % THREADS == 0 || q < NB) {
         int line = LINE_FAST ? q % NL : q / (L / R);
         int i = LINE_FAST ? q / NL : q % (L / R);
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[b][r] = op(line, i + r * (L / R), v[b][r]);
+        for (int r = 0; r < R; ++r) fac[b][r] = op.factor(line, i + r * (L / R));
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < PER; ++b) {
+      int q = tid + b * THREADS;
+      if (NB % THREADS == 0 || q < NB) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[b][r] = op.apply(v[b][r], fac[b][r]);
       }
     }
   }
@@ -272,14 +286,18 @@ template <typename T, class A> __device__ __forceinline__ T kfactor(const A &a, 
 // filter(k) / k^2 (0 at k = 0); the component at k_c = +Nyquist is zero.
 // PASS only names the pass (0 x, 1 y, 2 windowed x, 3 windowed y) so that
 // profiles tell the launches apart.
-template <typename T, int L, int C, int DIR, int THREADS, int PRE, int POST, int PASS>
-__global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const cx<T> *__restrict__ tw) {
-  extern __shared__ char smem_raw[];
-  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
-  int tile = xcd_tile(blockIdx.x, gridDim.x);
+// The part of the field one tile of a strided pass works on: C adjacent kz
+// columns of one outer line (of one field of a batch).
+template <typename T> struct TileG {
+  int col0, nc, ky, ky2, iw0;
+  const cx<T> *ib;
+  cx<T> *ob;
+};
+template <typename T, int C> __device__ __forceinline__ TileG<T> tile_geom(const StridedArgs<T> &a, int tile) {
+  TileG<T> g;
   int outer = tile / a.nkzb, kzb = tile - outer * a.nkzb;
-  int col0 = kzb * C;
-  int nc = min(C, a.ncol - col0);
+  g.col0 = kzb * C;
+  g.nc = min(C, a.ncol - g.col0);
   long boff_in = 0, boff_out = 0;
   int b = 0;
   if (a.nbatch > 1) {
@@ -288,43 +306,64 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
     boff_in = b * a.in_bs;
     boff_out = b * a.out_bs;
   }
-  const int iw0 = a.in_win0_b ? a.in_win0_b[b] : a.in_win0;
-  const cx<T> *ib = a.in + boff_in + outer * a.in_os + col0;
-  cx<T> *ob = a.out + boff_out + outer * a.out_os + col0;
-  int ky = signed_k(outer, a.n);
-  int ky2 = ky * ky;
-  auto gld = [&](int line, int pt) -> cx<T> {
-    if (line >= nc) return cx<T>{0, 0};
-    if (a.in_wlen > 0 && a.in_wlen < L) {   // pruned input: zero outside the window
-      int o = pt - iw0;
-      if (o < 0) o += L;
-      if (o >= a.in_wlen) return cx<T>{0, 0};
-      return ib[o * a.in_ls + line];
-    }
-    return ib[pt * a.in_ls + line];
-  };
-  auto op = [&](int line, int pt, cx<T> v) -> cx<T> {
-    int kx = signed_k(pt, a.n), kz = col0 + line;
-    int ksq = kx * kx + ky2 + kz * kz;
+  g.iw0 = a.in_win0_b ? a.in_win0_b[b] : a.in_win0;
+  g.ib = a.in + boff_in + outer * a.in_os + g.col0;
+  g.ob = a.out + boff_out + outer * a.out_os + g.col0;
+  g.ky = signed_k(outer, a.n);
+  g.ky2 = g.ky * g.ky;
+  return g;
+}
+// input value at (line, point) of a tile: zero past the last column and, for
+// a pruned input, outside its window
+template <typename T, int L>
+__device__ __forceinline__ cx<T> tile_in(const StridedArgs<T> &a, const TileG<T> &g, int line, int pt) {
+  if (line >= g.nc) return cx<T>{0, 0};
+  if (a.in_wlen > 0 && a.in_wlen < L) {
+    int o = pt - g.iw0;
+    if (o < 0) o += L;
+    if (o >= a.in_wlen) return cx<T>{0, 0};
+    return g.ib[o * a.in_ls + line];
+  }
+  return g.ib[pt * a.in_ls + line];
+}
+// One tile of a strided pass; ld supplies the first stage's inputs.
+template <typename T, int L, int C, int DIR, int THREADS, int PRE, int POST, class Ld>
+__device__ __forceinline__ void strided_tile(const StridedArgs<T> &a, const cx<T> *__restrict__ tw, cx<T> *lds,
+                                             const TileG<T> &g, Ld gld) {
+  const int col0 = g.col0, nc = g.nc, ky = g.ky, ky2 = g.ky2;
+  cx<T> *ob = g.ob;
+  struct PreOp {
+    const StridedArgs<T> &a;
+    int col0, ky, ky2;
     // the factor of |k|^2: the caller's table, its filter formed here (FP32),
-    // or inv_k2 / (|k|^2 kMin^2) formed exactly as filter_table forms it
-    T f = kfactor<T>(a, ksq);
-    if constexpr (PRE == PRE_ZA) {
-      int kc = a.comp == 0 ? kx : a.comp == 1 ? ky : kz;
-      if (kc == a.n / 2) return cx<T>{0, 0};
-      f *= T(kc) * a.kmin;
-      return cx<T>{-v.y * f, v.x * f};
-    } else if constexpr (PRE == PRE_KK) {
-      int ca = a.comp % 3, cb = a.comp / 3;
-      int k1 = ca == 0 ? kx : ca == 1 ? ky : kz;
-      int k2 = cb == 0 ? kx : cb == 1 ? ky : kz;
-      if (ca != cb && (k1 == a.n / 2 || k2 == a.n / 2)) return cx<T>{0, 0};
-      f *= (T(k1) * a.kmin) * (T(k2) * a.kmin);
-      return cx<T>{v.x * f, v.y * f};
-    } else {
-      return cx<T>{v.x * f, v.y * f};
+    // or inv_k2 / (|k|^2 kMin^2) formed exactly as filter_table forms it;
+    // times k_c (Zel'dovich, zero at k_c = +Nyquist) or k_a k_b (second
+    // derivatives, zero at a +Nyquist component when a != b)
+    __device__ __forceinline__ T factor(int line, int pt) const {
+      int kx = signed_k(pt, a.n), kz = col0 + line;
+      if (kz >= a.ncol) return T(0);   // past the last column (its loads are zero): no table read
+      int ksq = kx * kx + ky2 + kz * kz;
+      T f = kfactor<T>(a, ksq);
+      if constexpr (PRE == PRE_ZA) {
+        int kc = a.comp == 0 ? kx : a.comp == 1 ? ky : kz;
+        return kc == a.n / 2 ? T(0) : f * (T(kc) * a.kmin);
+      } else if constexpr (PRE == PRE_KK) {
+        int ca = a.comp % 3, cb = a.comp / 3;
+        int k1 = ca == 0 ? kx : ca == 1 ? ky : kz;
+        int k2 = cb == 0 ? kx : cb == 1 ? ky : kz;
+        if (ca != cb && (k1 == a.n / 2 || k2 == a.n / 2)) return T(0);
+        return f * ((T(k1) * a.kmin) * (T(k2) * a.kmin));
+      } else {
+        return f;
+      }
+    }
+    // Zel'dovich: i f v; otherwise f v
+    __device__ __forceinline__ cx<T> apply(cx<T> v, T f) const {
+      if constexpr (PRE == PRE_ZA) return cx<T>{-v.y * f, v.x * f};
+      else return cx<T>{v.x * f, v.y * f};
     }
   };
+  PreOp op{a, col0, ky, ky2};
   // POST_AUX: the spectrum it adds is fetched before the first stage, for the
   // outputs this thread stores in the last one (line q % C, points
   // q / C + s L / R), so that its latency overlaps the transform
@@ -415,6 +454,16 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
   } else {
     run_stages<T, L, C, THREADS, DIR, true, false, false, 0>(lds, tw, 1, gld, gst, lidx, op);
   }
+}
+
+// One tile per block, tiles in XCD order.
+template <typename T, int L, int C, int DIR, int THREADS, int PRE, int POST, int PASS>
+__global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const cx<T> *__restrict__ tw) {
+  extern __shared__ char smem_raw[];
+  cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  TileG<T> g = tile_geom<T, C>(a, xcd_tile(blockIdx.x, gridDim.x));
+  strided_tile<T, L, C, DIR, THREADS, PRE, POST>(a, tw, lds, g,
+                                                 [&](int line, int pt) { return tile_in<T, L>(a, g, line, pt); });
 }
 
 // ---- z pass, r2c --------------------------------------------------------------
