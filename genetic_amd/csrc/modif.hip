@@ -767,28 +767,29 @@ template <typename T> struct MLF {
   }
 };
 
-// the output field (noise buffers) as a multi-level field
-template <typename T> MLF<T> copy_output(gic_context *c) {
-  MLF<T> m(c);
-  for (size_t l = 0; l < m.nl(); ++l) {
-    Level &L = *c->levels[l];
-    if (L.dc_pending) ensure_noise_fourier<T>(c, L);
-    Timed t(c, "copy_field", 2.0 * m.reals(l) * sizeof(T));
-    GIC_HIP(hipMemcpyAsync(m.p(l), L.noise.p, m.reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
-    m.fourier[l] = L.noise_fourier;
+// The output field's own buffers (the noise of every level) as a multi-level
+// field, for work that reads it or updates it in place (chi^2, calculate, the
+// modifications themselves) without the copies in and out of the pool.  The
+// levels' Fourier flags are written back when the view goes out of scope.
+template <typename T> struct OutputView {
+  gic_context *c;
+  MLF<T> m;
+  explicit OutputView(gic_context *cc) : c(cc), m(cc) {
+    for (size_t l = 0; l < m.nl(); ++l) {
+      Level &L = *c->levels[l];
+      if (L.dc_pending) ensure_noise_fourier<T>(c, L);
+      m.lv[l].p = L.noise.p;   // no pool: the buffer stays the level's
+      m.fourier[l] = L.noise_fourier;
+    }
   }
-  return m;
-}
-template <typename T> void store_output(gic_context *c, MLF<T> &m) {
-  m.toF();
-  for (size_t l = 0; l < m.nl(); ++l) {
-    Level &L = *c->levels[l];
-    Timed t(c, "copy_field", 2.0 * m.reals(l) * sizeof(T));
-    GIC_HIP(hipMemcpyAsync(L.noise.p, m.p(l), m.reals(l) * sizeof(T), hipMemcpyDeviceToDevice, c->st));
-    L.noise_fourier = true;
-    L.dc_pending = false;
+  OutputView(const OutputView &) = delete;
+  ~OutputView() {
+    for (size_t l = 0; l < m.nl(); ++l) {
+      m.lv[l].p = nullptr;
+      c->levels[l]->noise_fourier = m.fourier[l];
+    }
   }
-}
+};
 
 bool combined(gic_context *c) { return c->zeldovich_done && c->levels.size() >= 2; }
 
@@ -1595,7 +1596,8 @@ template <typename T> void ensure_ready(gic_context *c) {
 // ============================================================== entry points ==
 template <typename T> double mod_calculate(gic_context *c, const std::string &name) {
   ensure_ready<T>(c);
-  MLF<T> field = copy_output<T>(c);
+  OutputView<T> out(c);
+  MLF<T> &field = out.m;
   field.toF();
   if (!strcasecmp(name.c_str(), "variance")) {
     check_variance_scale(c, c->variance_scale);
@@ -1632,9 +1634,9 @@ void mod_clear(gic_context *c) {
 template <typename T> double mod_chi2(gic_context *c) {
   ensure_ready<T>(c);
   if (c->power_applied) throw std::runtime_error("Cannot calculate the chi^2 after the output field has been generated");
-  MLF<T> field = copy_output<T>(c);
-  field.toF();
-  return chi2_of(c, field);
+  OutputView<T> out(c);
+  out.m.toF();
+  return chi2_of(c, out.m);
 }
 
 size_t num_dof(gic_context *c) {
@@ -1657,7 +1659,9 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
     Phase ph(c, "phase:chi2_pre");
     chi_pre = mod_chi2<T>(c);
   }
-  MLF<T> field = copy_output<T>(c);
+  // the modifications update the noise in place
+  std::unique_ptr<OutputView<T>> out(new OutputView<T>(c));
+  MLF<T> &field = out->m;
   field.toF();
   std::vector<MLF<T>> alphas;
   std::vector<double> targets;
@@ -1865,7 +1869,9 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
     int nsteps = init * (int)std::ceil(std::sqrt(achieved / want));
     iterate(field, nsteps > init ? nsteps : init);
   }
-  store_output<T>(c, field);
+  field.toF();
+  for (size_t l = 0; l < field.nl(); ++l) c->levels[l]->dc_pending = false;
+  out.reset();   // the levels' Fourier flags written back
   double chi_post;
   {
     Phase ph(c, "phase:chi2_post");
