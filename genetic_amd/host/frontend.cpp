@@ -294,6 +294,11 @@ struct Generator {
   int nGadgetFiles = 1;
   unsigned mruGadgetType = 1;
   bool usedGadgetType[6] = {false, false, false, false, false, false};
+  // gadget_flagged_particle_type: the finest level's flagged cells as a
+  // separate block of this type, the cells taken when the mapper is rebuilt
+  bool flaggedTypeOn = false;
+  unsigned flaggedType = 0;
+  std::vector<size_t> flaggedTypeCells;
   double varianceScale = -1;
   size_t nModifications = 0;
   int initialNumberSteps = 10;   // ic.hpp:155-161
@@ -471,6 +476,10 @@ struct Generator {
           }
     }
     fine.flagCells(fineFlags);
+    if (flaggedTypeOn) {   // the mapper rebuild takes the new finest level's flags
+      flaggedTypeCells = fine.flags;
+      std::sort(flaggedTypeCells.begin(), flaggedTypeCells.end());
+    }
   }
 
   // initZoomGrid (ic.hpp:398-478): smallest window around the flags,
@@ -738,6 +747,23 @@ struct Generator {
     std::ofstream f(fn);
     for (size_t id : ids) f << id << std::endl;
   }
+  // setFlaggedGadgetParticleType (ic.hpp:706-714): the flagged particles of
+  // the finest level are split off into their own gadget type, behind every
+  // other particle (a two-level mapper whose fine grid is the finest grid
+  // itself, ic.hpp:1180-1190)
+  void setFlaggedGadgetParticleType(unsigned t) {
+    if (t >= 6) throw std::runtime_error("Gadget particle type must be between 0 and 5");
+    if (usedGadgetType[t])
+      throw std::runtime_error(
+          "Cannot reuse a gadget particle type; this type was already used in a different section of the ICs");
+    if (grids.empty()) throw std::runtime_error("No grid has yet been initialised");
+    flaggedTypeOn = true;
+    flaggedType = t;
+    usedGadgetType[t] = true;
+    flaggedTypeCells = grids.back().flags;
+    std::sort(flaggedTypeCells.begin(), flaggedTypeCells.end());
+  }
+
   void setGadgetParticleType(unsigned t) {
     if (t >= 6) throw std::runtime_error("Gadget particle type must be between 0 and 5");
     if (t != mruGadgetType) {
@@ -945,6 +971,22 @@ struct Generator {
       check(gic_particle_mass_eps(c(), int(l), &b.mass, nullptr));
       b.eps = grids[l].dx * epsNorm;
       if (Ob > 0 && l == nl - 1) b.mass *= 1.0 - massRatio;
+    }
+    if (flaggedTypeOn) {
+      if (Ob > 0) throw std::runtime_error("gadget_flagged_particle_type with baryons is not built here");
+      // the finest block keeps its unflagged cells in order; the flagged ones
+      // follow as their own block
+      Block f = dm[nl - 1];
+      f.type = flaggedType;
+      std::vector<uint64_t> keep, moved;
+      for (uint64_t cell : dm[nl - 1].cells) {
+        if (std::binary_search(flaggedTypeCells.begin(), flaggedTypeCells.end(), size_t(cell))) moved.push_back(cell);
+        else keep.push_back(cell);
+      }
+      std::sort(moved.begin(), moved.end());
+      dm[nl - 1].cells = std::move(keep);
+      f.cells = std::move(moved);
+      dm.push_back(std::move(f));
     }
     std::vector<Block> out;
     if (Ob > 0) {
@@ -1986,6 +2028,8 @@ struct Dispatcher {
           ic.nGadgetFiles = n;
         }));
     add("gadget_particle_type", std::function<void(int)>([this](int t) { ic.setGadgetParticleType(unsigned(t)); }));
+    add("gadget_flagged_particle_type",
+        std::function<void(int)>([this](int t) { ic.setFlaggedGadgetParticleType(unsigned(t)); }));
     add("velocity_offset", std::function<void(double, double, double)>([this](double vx, double vy, double vz) {
           double c = std::pow(ic.a(), -0.5);   // km/s -> gadget units (ic.hpp:285-289)
           ic.velOffset[0] = vx * c;

@@ -17,6 +17,7 @@ from helpers import run_frontend, stage_test
     ("outformat 9\n", 1, "Unknown output format"),
     ("select_sphere 3\n", 1, "No grid has yet been initialised"),
     ("gadget_particle_type 7\n", 1, "Gadget particle type must be between 0 and 5"),
+    ("gadget_flagged_particle_type 6\n", 1, "Gadget particle type must be between 0 and 5"),
     ("camb does_not_exist.dat\n", 1, "File does_not_exist.dat not found"),
 ])
 def test_parser_errors(tmp_path, text, line, what):

@@ -7,7 +7,7 @@ import os
 import numpy as np
 import pytest
 
-from helpers import (assert_decimal, compare_particles, golden, read_tipsy, read_tipsy_array, run_frontend,
+from helpers import (assert_decimal, compare_particles, golden, read_gadget, read_tipsy, read_tipsy_array, run_frontend,
                      stage_test)
 
 pytestmark = pytest.mark.gpu
@@ -254,3 +254,47 @@ def test_wrapping_double_zoom_matches_reference(tmp_path):
     want = read_tipsy_array(golden(test, "reference_output.overdensity"))
     assert got.shape == want.shape
     np.testing.assert_almost_equal(got, want, decimal=5)
+
+
+def test_gadget_flagged_particle_type(tmp_path):
+    """test_10d (run-only in the reference): the particles an id file flags
+    on the finest level leave their block and follow every other particle as
+    gadget type 0, with the same positions and velocities as the same run
+    without the command, new ids at the end of the mapper and the finest
+    level's mass (ic.hpp:706-714,1180-1190)."""
+    test = "test_10d_gadget_flagged_particles"
+    d, _ = _run(test, tmp_path / "flagged")
+    got = read_gadget(particle_file(d))
+    plain = open(golden(test, "paramfile.txt")).read().replace("gadget_flagged_particle_type 0", "")
+    d2 = stage_test(test, tmp_path / "plain", paramfile_text=plain)
+    rc, err = run_frontend(d2)
+    assert rc == 0, err
+    ref = read_gadget(particle_file(d2))
+    n = 64 ** 3
+    flagged = np.unique(np.loadtxt(golden(test, "zoomtest.txt"), dtype=np.int64))
+    nf = flagged.size
+    assert got["npart"][:2] == (nf, n - nf) and sum(got["npart"][2:]) == 0
+    assert ref["npart"][1] == n
+    rest = np.setdiff1d(np.arange(n), flagged)
+    # file order is by type: the flagged block (type 0) first, then type 1
+    np.testing.assert_array_equal(got["ids"][:nf], np.arange(n - nf, n))
+    np.testing.assert_array_equal(got["ids"][nf:], np.arange(n - nf))
+    np.testing.assert_array_equal(got["pos"][:nf], ref["pos"][flagged])
+    np.testing.assert_array_equal(got["vel"][:nf], ref["vel"][flagged])
+    np.testing.assert_array_equal(got["pos"][nf:], ref["pos"][rest])
+    np.testing.assert_array_equal(got["vel"][nf:], ref["vel"][rest])
+    assert got["mass"][0] == got["mass"][1] == ref["mass"][1]
+
+
+@pytest.mark.parametrize("test", ["test_01h_zoom_grid_pos_offset", "test_01i_zoom_pos_offset_wrap"])
+def test_centred_grafic_zoom_runs(tmp_path, test):
+    """test_01h / 01i (run-only in the reference's compare.py): a zoom near a
+    box corner, wrapping in 01i, written as centred grafic output -- the run
+    succeeds and writes one complete directory per level."""
+    import glob
+    d, _ = _run(test, tmp_path)
+    dirs = sorted(glob.glob(os.path.join(d, "*.grafic_*")))
+    assert len(dirs) >= 2
+    for g in dirs:
+        for name in ("ic_velcx", "ic_poscx", "ic_deltab", "ic_refmap", "ic_particle_ids"):
+            assert os.path.getsize(os.path.join(g, name)) > 44, (g, name)
