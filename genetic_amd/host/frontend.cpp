@@ -268,6 +268,7 @@ struct LevelMap {
 // mapper order
 struct Block {
   int level = 0;
+  int factor = 1;   // the grid it is laid on: > 1 subsampled, < -1 supersampled by -factor
   bool gas = false;
   unsigned type = 1;
   std::vector<uint64_t> cells;
@@ -275,6 +276,17 @@ struct Block {
   double mass = 0, eps = 0;
   std::vector<float> pos, vel;
   size_t count() const { return cells.size(); }
+};
+
+// A level as written out: a real level, or a sub/supersampled copy of one
+// (the virtual grids of virtualgrid.hpp), possibly re-centred for grafic
+struct VLevel {
+  int src = 0;         // real level holding the data
+  int factor = 1;      // > 1: subsampled by factor; < -1: supersampled by -factor
+  Grid g;              // geometry of this level (after any offset)
+  Grid gu;             // the same before the offset (where fields are evaluated)
+  bool centred = false;
+  I3 shift;            // CenteredGrid offset, in this level's cells
 };
 
 struct Generator {
@@ -288,6 +300,7 @@ struct Generator {
   std::vector<Grid> grids;
   std::vector<std::vector<size_t>> zoomParticles;   // zoomParticleArray: per level >= 1, cells of level-1
   std::vector<LevelMap> maps;
+  std::vector<VLevel> outLv;   // each level's particle grid (its own, or sub/supersampled)
   double x0 = 0, y0 = 0, z0 = 0;
   Fmt fmt = Fmt::unknown;
   std::string outdir = "./", outname;
@@ -522,9 +535,10 @@ struct Generator {
   }
 
   // --- particle mapper ---------------------------------------------------
-  size_t mapperSize(int L) const { return L == 0 ? grids[0].n3() : maps[L].total; }
+  size_t mapperSize(int L) const { return L == 0 ? outLv[0].g.n3() : maps[L].total; }
 
-  // getFlaggedParticles with an explicit per-level flag set (twolevelmapper.hpp:473-513)
+  // getFlaggedParticles with an explicit per-level flag set, in the indexing
+  // of each level's output grid (twolevelmapper.hpp:473-513)
   std::vector<size_t> flaggedParticles(int L, const std::vector<std::vector<size_t>> &f) const {
     std::vector<size_t> out;
     if (L == 0) {
@@ -553,7 +567,7 @@ struct Generator {
   }
   std::vector<size_t> flaggedParticles() const {
     std::vector<std::vector<size_t>> f;
-    for (auto &g : grids) f.push_back(g.flags);
+    for (size_t l = 0; l < grids.size(); ++l) f.push_back(toOutputCells(outLv[l], grids[l].flags));
     return flaggedParticles(int(grids.size()) - 1, f);
   }
 
@@ -561,8 +575,8 @@ struct Generator {
   void flagParticles(int L, const std::vector<size_t> &ids) {
     if (L == 0) {
       for (size_t id : ids)
-        if (id >= grids[0].n3()) throw std::runtime_error("Particle ID out of range");
-      grids[0].flagCells(ids);
+        if (id >= outLv[0].g.n3()) throw std::runtime_error("Particle ID out of range");
+      grids[0].flagCells(toRealCells(outLv[0], ids));
       return;
     }
     const LevelMap &m = maps[L];
@@ -582,8 +596,8 @@ struct Generator {
         break;
       }
     }
-    const Grid &gc = grids[L - 1];
-    const Grid &gf = grids[L];
+    const Grid &gc = outLv[L - 1].g;
+    const Grid &gf = outLv[L].g;
     size_t lastLr = SIZE_MAX;
     std::vector<size_t> cube;
     for (size_t i = firstHr; i < ids.size(); ++i) {
@@ -601,42 +615,169 @@ struct Generator {
     std::sort(lr.begin(), lr.end());
     std::sort(hr.begin(), hr.end());
     flagParticles(L - 1, lr);
-    grids[L].flagCells(hr);
+    grids[L].flagCells(toRealCells(outLv[L], hr));
   }
 
   // TwoLevelParticleMapper constructor (twolevelmapper.hpp:270-320), per level
   // setSubsample / setSupersample (ic.hpp:296-330) rebuild the particle
   // mapper at once; with particle output that needs the resampled mappers
-  void setResampling(int sub, int super) {
-    if ((sub > 1 || super > 1) && fmt != Fmt::grafic && fmt != Fmt::unknown)
-      throw std::runtime_error("subsample and supersample apply only to grafic output in this build");
-    subsample = sub;
-    supersample = super;
+  // setSubsample / setSupersample / setSupersampleGas (ic.hpp:305-336):
+  // the particle mapper is rebuilt at once, so a zoom that no longer fits
+  // its grids after resampling is reported here
+  void setSubsample(int f) {
+    if (f <= 0) throw std::runtime_error("Subsampling factor must be greater then zero");
+    subsample = f;
+    refreshMapper();
   }
+  void setSupersample(int f) {
+    if (f <= 1) throw std::runtime_error("Supersampling factor must be greater then one");
+    supersample = f;
+    refreshMapper();
+  }
+  void setSupersampleGas(int f) {
+    if (f <= 1) throw std::runtime_error("Supersampling factor must be greater then one");
+    supersampleGas = f;
+    refreshMapper();
+  }
+  void refreshMapper() {
+    if (grids.empty() || fmt == Fmt::grafic) return;
+    buildMapper();
+    if (Ob > 0 && supersampleGas > 1) gasCells(resampledLevel(int(grids.size()) - 1, -supersampleGas));
+  }
+
+  // The grid the particles of level l are laid on (updateParticleMapper and
+  // superOrSubSample, ic.hpp:1129-1236): the finest level supersampled by
+  // `super` (SuperSampleGrid, virtualgrid.hpp:126-208: cells 1/super the
+  // size, mass / super^3, the level's own softening), then the base level
+  // subsampled by `sub` (SubSampleGrid, virtualgrid.hpp:526-647: n / sub
+  // cells, mass * sub^3, softening of the larger cell).  On one level the two
+  // combine: a supersampled grid subsampled by the same factor is the level
+  // itself, by a larger one a subsampled level (virtualgrid.hpp:180-196).
+  VLevel outputLevel(int l, int super, int sub) const {
+    int sup = l == int(grids.size()) - 1 ? super : 1;
+    int sb = l == 0 ? sub : 1;
+    int f = 1;
+    if (sup > 1 && sb > 1) {
+      if ((sb > sup ? sb % sup : sup % sb) != 0) throw std::runtime_error("Ratio is not an integer");
+      f = sb == sup ? 1 : sb > sup ? sb / sup : -(sup / sb);
+    } else if (sb > 1) {
+      f = sb;
+    } else if (sup > 1) {
+      f = -sup;
+    }
+    return resampledLevel(l, f);
+  }
+  // Real level l subsampled (f > 1) or supersampled (f < -1) by |f|
+  VLevel resampledLevel(int l, int f) const {
+    VLevel v;
+    v.src = l;
+    v.factor = f;
+    v.g = grids[l];
+    v.g.flags.clear();
+    if (f > 1) {
+      v.g.n = grids[l].n / f;
+      v.g.dx = grids[l].dx * f;
+    } else if (f < -1) {
+      v.g.n = grids[l].n * -f;
+      v.g.dx = grids[l].dx / -f;
+    }
+    v.gu = v.g;
+    return v;
+  }
+  // Flags of real level v.src as cells of its output grid (the virtual
+  // grids' getFlaggedCells, virtualgrid.hpp:155-160,556-561): a subsampled
+  // cell is flagged when any real cell under it is, a supersampled level
+  // flags every subcell of a flagged cell.
+  std::vector<size_t> toOutputCells(const VLevel &v, const std::vector<size_t> &cells) const {
+    std::vector<size_t> out;
+    const Grid &rg = grids[v.src];
+    if (v.factor == 1) {
+      out = cells;
+    } else if (v.factor > 1) {
+      int f = v.factor;
+      for (size_t c : cells) {
+        I3 q = rg.coord(c);
+        if (q.x / f < v.g.n && q.y / f < v.g.n && q.z / f < v.g.n) out.push_back(v.g.index(q.x / f, q.y / f, q.z / f));
+      }
+    } else {
+      int f = -v.factor;
+      for (size_t c : cells) {
+        I3 q = rg.coord(c);
+        for (int i = 0; i < f; ++i)
+          for (int j = 0; j < f; ++j)
+            for (int k = 0; k < f; ++k) out.push_back(v.g.index(q.x * f + i, q.y * f + j, q.z * f + k));
+      }
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
+  }
+  // ...and output cells as the real cells they flag (the virtual grids'
+  // flagCells): every real cell under a subsampled cell, the real cell of a
+  // supersampled one.
+  std::vector<size_t> toRealCells(const VLevel &v, const std::vector<size_t> &cells) const {
+    if (v.factor == 1) return cells;
+    std::vector<size_t> out;
+    const Grid &rg = grids[v.src];
+    if (v.factor > 1) {
+      int f = v.factor;
+      for (size_t c : cells) {
+        I3 q = v.g.coord(c);
+        for (int x = q.x * f; x < std::min(q.x * f + f, rg.n); ++x)
+          for (int y = q.y * f; y < std::min(q.y * f + f, rg.n); ++y)
+            for (int z = q.z * f; z < std::min(q.z * f + f, rg.n); ++z) out.push_back(rg.index(x, y, z));
+      }
+    } else {
+      int f = -v.factor;
+      for (size_t c : cells) {
+        I3 q = v.g.coord(c);
+        out.push_back(rg.index(q.x / f, q.y / f, q.z / f));
+      }
+    }
+    std::sort(out.begin(), out.end());
+    out.erase(std::unique(out.begin(), out.end()), out.end());
+    return out;
+  }
+
+  // The cells of a zoom level's output grid that replace each zoomed coarse
+  // cell, in mapper order: the cube under every replaced cell of the coarse
+  // output grid (zoomParticleArrayHiresUnsorted, twolevelmapper.hpp:270-320);
+  // after resampling the cubes must still fit the fine grid
+  // (superOrSubSample, twolevelmapper.hpp:582-610).
+  void zoomCubes(const Grid &gc, const std::vector<size_t> &coarse, const Grid &gf, std::vector<size_t> &out) const {
+    try {
+      for (size_t cid : coarse) gf.cubeIds(gc.centroid(cid), gc.dx, out);
+    } catch (std::runtime_error &) {
+      if (subsample > 1 || supersample > 1 || supersampleGas > 1)
+        throw std::runtime_error(
+            "Following sub/super-sampling, the zoom particles no longer fit fully on their grids. Try rerunning with "
+            "strays_on, or (better) increase the size of the zoom region.");
+      throw;
+    }
+  }
+
   void buildMapper() {
-    // resampled particle output (the sub/super-sampled mappers,
-    // twolevelmapper.hpp:560-610) is not built: refuse rather than write
-    // particles at the grids' own resolution
-    if (subsample > 1 || supersample > 1)
-      throw std::runtime_error("subsample and supersample apply only to grafic output in this build");
+    outLv.clear();
+    for (size_t l = 0; l < grids.size(); ++l) outLv.push_back(outputLevel(int(l), supersample, subsample));
     maps.assign(grids.size(), LevelMap());
     for (size_t L = 1; L < grids.size(); ++L) {
       LevelMap &m = maps[L];
-      m.cells = zoomParticles[L - 1];
-      std::sort(m.cells.begin(), m.cells.end());
+      const Grid &gc = outLv[L - 1].g, &gf = outLv[L].g;
+      // the zoomed cells of the level above, read through its output grid
+      m.cells = toOutputCells(outLv[L - 1], zoomParticles[L - 1]);
       std::vector<std::vector<size_t>> f(L);
       f[L - 1] = m.cells;
       m.particles = flaggedParticles(int(L) - 1, f);
       if (m.particles.size() != m.cells.size())
         throw std::runtime_error("Consistency error when relating cells to particles");
-      int r = ratioInt(grids[L - 1].dx, grids[L].dx);
+      int r = ratioInt(gc.dx, gf.dx);
       m.nhr = size_t(r) * r * r;
       m.size1 = mapperSize(int(L) - 1);
       m.total = m.size1 + (m.nhr - 1) * m.particles.size();
       m.first2 = m.size1 - m.particles.size();
       m.hires.clear();
       m.hires.reserve(m.cells.size() * m.nhr);
-      for (size_t cid : m.cells) grids[L].cubeIds(grids[L - 1].centroid(cid), grids[L - 1].dx, m.hires);
+      zoomCubes(gc, m.cells, gf, m.hires);
     }
   }
 
@@ -646,7 +787,7 @@ struct Generator {
   std::vector<std::vector<uint64_t>> levelOrder() const {
     size_t nl = grids.size();
     std::vector<std::vector<uint64_t>> lists(1);
-    lists[0].resize(grids[0].n3());
+    lists[0].resize(outLv[0].g.n3());
     std::iota(lists[0].begin(), lists[0].end(), uint64_t(0));
     for (size_t L = 1; L < nl; ++L) {
       const LevelMap &m = maps[L];
@@ -953,11 +1094,44 @@ struct Generator {
     return s.str();
   }
 
-  // Particle blocks in mapper order: the DM levels, then (gadget) or before
-  // them (tipsy) the gas copy of the finest level when Ob > 0 (ic.hpp:1190-1214,
+  // Mass and softening of a block's particles (ZeldovichParticleEvaluator::
+  // getMass/getEps, zeldovich.hpp:110-119): a subsampled grid carries
+  // factor^3 of the level's mass per cell and the larger cell's softening, a
+  // supersampled one 1/factor^3 of it and the level's own softening
+  // (virtualgrid.hpp:139-147,539-545).
+  void setMassEps(Block &b) {
+    check(gic_particle_mass_eps(c(), b.level, &b.mass, nullptr));
+    double f = std::abs(b.factor);
+    if (b.factor > 1) b.mass *= f * f * f;
+    if (b.factor < -1) b.mass /= f * f * f;
+    b.eps = grids[b.level].dx * epsNorm * (b.factor > 1 ? f : 1.0);
+  }
+  // The cells of the gas copy of the finest level, in mapper order, on the
+  // finest grid supersampled by supersample_gas: the gas mapper keeps the
+  // levels above without resampling and lists only the finest level's
+  // particles (TwoLevelParticleMapper::splitMass with skipLevel1,
+  // twolevelmapper.hpp:542-575; the gas resampled by itself, ic.hpp:1200-1201).
+  std::vector<uint64_t> gasCells(const VLevel &gv) const {
+    size_t nl = grids.size();
+    std::vector<uint64_t> out;
+    if (nl == 1) {
+      out.resize(gv.g.n3());
+      std::iota(out.begin(), out.end(), uint64_t(0));
+      return out;
+    }
+    std::vector<size_t> coarse(zoomParticles[nl - 2]), hr;
+    std::sort(coarse.begin(), coarse.end());
+    zoomCubes(grids[nl - 2], coarse, gv.g, hr);
+    out.assign(hr.begin(), hr.end());
+    return out;
+  }
+
+  // The blocks of the particle list in mapper order, without their particles:
+  // the DM levels on their output grids, then (gadget) or before them (tipsy)
+  // the gas copy of the finest level when Ob > 0 (ic.hpp:1129-1222,
   // AddGasMapper in particles/mapper/gasmapper.hpp; the split masses follow
   // OneLevelParticleMapper::splitMass, onelevelmapper.hpp:155-173).
-  std::vector<Block> particleBlocks(bool gasFirst) {
+  std::vector<Block> layoutBlocks(bool gasFirst) {
     buildMapper();
     auto order = levelOrder();
     size_t nl = grids.size();
@@ -966,21 +1140,22 @@ struct Generator {
     for (size_t l = 0; l < nl; ++l) {
       Block &b = dm[l];
       b.level = int(l);
+      b.factor = outLv[l].factor;
       b.type = grids[l].gadgetType;
       b.cells = std::move(order[l]);
-      check(gic_particle_mass_eps(c(), int(l), &b.mass, nullptr));
-      b.eps = grids[l].dx * epsNorm;
+      setMassEps(b);
       if (Ob > 0 && l == nl - 1) b.mass *= 1.0 - massRatio;
     }
     if (flaggedTypeOn) {
       if (Ob > 0) throw std::runtime_error("gadget_flagged_particle_type with baryons is not built here");
       // the finest block keeps its unflagged cells in order; the flagged ones
-      // follow as their own block
+      // follow as their own block, on the same (possibly resampled) grid
+      auto flaggedOut = toOutputCells(outLv[nl - 1], flaggedTypeCells);
       Block f = dm[nl - 1];
       f.type = flaggedType;
       std::vector<uint64_t> keep, moved;
       for (uint64_t cell : dm[nl - 1].cells) {
-        if (std::binary_search(flaggedTypeCells.begin(), flaggedTypeCells.end(), size_t(cell))) moved.push_back(cell);
+        if (std::binary_search(flaggedOut.begin(), flaggedOut.end(), size_t(cell))) moved.push_back(cell);
         else keep.push_back(cell);
       }
       std::sort(moved.begin(), moved.end());
@@ -990,14 +1165,18 @@ struct Generator {
     }
     std::vector<Block> out;
     if (Ob > 0) {
-      Block g = dm[nl - 1];
+      VLevel gv = resampledLevel(int(nl - 1), supersampleGas > 1 ? -supersampleGas : 1);
+      Block g;
+      g.level = int(nl - 1);
+      g.factor = gv.factor;
       g.gas = true;
       g.type = 0;
-      check(gic_particle_mass_eps(c(), int(nl - 1), &g.mass, nullptr));
+      g.cells = gasCells(gv);
+      setMassEps(g);
       g.mass *= massRatio;
-      if (gasFirst) out.push_back(g);
+      if (gasFirst) out.push_back(std::move(g));
       for (auto &b : dm) out.push_back(std::move(b));
-      if (!gasFirst) out.push_back(g);
+      if (!gasFirst) out.push_back(std::move(g));
     } else {
       out = std::move(dm);
     }
@@ -1005,13 +1184,56 @@ struct Generator {
     for (auto &b : out) {
       b.first = first;
       first += b.count();
+    }
+    return out;
+  }
+
+  // The F64 real-space values of field k (GIC_FIELD_*) on real level l
+  std::vector<double> levelField(int k, int l) {
+    std::vector<double> f(grids[l].n3());
+    check(gic_copy_field(c(), k, l, GIC_REAL_SPACE, f.data(), GIC_DTYPE_F64));
+    return f;
+  }
+
+  // Particle blocks in mapper order with positions and velocities: real
+  // levels from the device, resampled levels evaluated here from the
+  // level's displacement fields through the virtual grid
+  // (ZeldovichParticleEvaluator::getParticleNoWrap, zeldovich.hpp:87-108;
+  // ParticleEvaluator::getParticle wraps, generator.hpp:49-53).
+  std::vector<Block> particleBlocks(bool gasFirst) {
+    auto out = layoutBlocks(gasFirst);
+    double velRatio = 0;
+    bool second = lptOrder == 2;
+    for (auto &b : out) {
       size_t m = b.count();
       b.pos.resize(3 * m);
       b.vel.resize(3 * m);
       if (!m) continue;
-      bool identity = m == grids[b.level].n3();
-      for (size_t i = 0; identity && i < m; ++i) identity = b.cells[i] == i;
-      check(gic_particles(c(), b.level, identity ? nullptr : b.cells.data(), 0, m, b.pos.data(), b.vel.data()));
+      if (b.factor == 1) {
+        bool identity = m == grids[b.level].n3();
+        for (size_t i = 0; identity && i < m; ++i) identity = b.cells[i] == i;
+        check(gic_particles(c(), b.level, identity ? nullptr : b.cells.data(), 0, m, b.pos.data(), b.vel.data()));
+      } else {
+        if (velRatio == 0) check(gic_velocity_ratio(c(), &velRatio));
+        VLevel v = resampledLevel(b.level, b.factor);
+        std::vector<std::vector<double>> psi;
+        for (int k = 1; k <= (second ? 6 : 3); ++k) psi.push_back(levelField(k, b.level));
+#pragma omp parallel for schedule(static)
+        for (size_t i = 0; i < m; ++i) {
+          size_t id = b.cells[i];
+          auto cen = v.gu.centroid(id);
+          for (int a3 = 0; a3 < 3; ++a3) {
+            double p = evalVirtual(v, psi[a3], id), vel = p * velRatio;
+            if (second) {
+              double p2 = evalVirtual(v, psi[3 + a3], id);
+              p += p2;
+              vel += 2 * velRatio * p2;
+            }
+            b.pos[3 * i + a3] = float(v.g.wrap(p + cen[a3]));
+            b.vel[3 * i + a3] = float(vel);
+          }
+        }
+      }
       if (velOffset[0] != 0 || velOffset[1] != 0 || velOffset[2] != 0)
         for (size_t i = 0; i < m; ++i)
           for (int d = 0; d < 3; ++d) b.vel[3 * i + d] = float(b.vel[3 * i + d] + velOffset[d]);
@@ -1361,15 +1583,7 @@ struct Generator {
   // then every level covering the box is re-centred on a point by an integer
   // cell shift (CenteredGrid, virtualgrid.hpp:688-800) and the others are
   // offset by the same distance (OffsetGrid, virtualgrid.hpp:806-840).
-  struct VLevel {
-    int src = 0;         // real level holding the data
-    int factor = 1;      // > 1: subsampled by factor; < -1: supersampled by -factor
-    Grid g;              // geometry of this level (after any offset)
-    Grid gu;             // the same before the offset (where fields are evaluated)
-    bool centred = false;
-    I3 shift;            // CenteredGrid offset, in this level's cells
-  };
-  int subsample = 1, supersample = 1, lptOrder = 1;
+  int subsample = 1, supersample = 1, supersampleGas = 1, lptOrder = 1;
   double pvarValue = 1.0;
   bool centerOnTargetRegion = false;
 
@@ -1394,6 +1608,63 @@ struct Generator {
     v.g.dx = grids[l].dx * f;
     return v;
   }
+  // Value at cell i of a virtual level (indexing of v.g) of a field f of its
+  // real level (n^3 values, index order): the level itself; a subsampled
+  // level averaging the real cells it covers (SubSampleGrid::forEachSubcell,
+  // virtualgrid.hpp:576-607, which clips at the grid's far edge); a
+  // supersampled level interpolating the real level at the cell's centroid
+  // (SuperSampleEvaluator, field/evaluator.hpp:96-126: evaluateInterpolated,
+  // field.hpp:483-508, with the tricubic of field.hpp:598-617), wrapping on a
+  // level that covers the box and repeating edge values otherwise.
+  double evalVirtual(const VLevel &v, const std::vector<double> &f, size_t i) const {
+    const Grid &rg = grids[v.src];
+    if (v.factor < -1) {
+      auto p = v.gu.centroid(i);
+      double o[3] = {rg.ox, rg.oy, rg.oz};
+      int p0[3];
+      double w[3][4];
+      for (int a3 = 0; a3 < 3; ++a3) {
+        double u = rg.wrap(p[a3] - o[a3]) / rg.dx - 0.5;
+        p0[a3] = int(std::floor(u));
+        double t = u - p0[a3], t2 = t * t, t3 = t2 * t;
+        w[a3][0] = -0.5 * t3 + t2 - 0.5 * t;
+        w[a3][1] = 1.5 * t3 - 2.5 * t2 + 1.0;
+        w[a3][2] = -1.5 * t3 + 2.0 * t2 + 0.5 * t;
+        w[a3][3] = 0.5 * t3 - 0.5 * t2;
+      }
+      bool wrapc = rg.n == rg.simEquivalent();
+      int sim = rg.simEquivalent();
+      auto fix = [&](int c) {
+        if (wrapc) {
+          if (c > sim - 1) c -= sim;
+          if (c < 0) c += sim;
+        } else {
+          c = std::min(std::max(c, 0), rg.n - 1);
+        }
+        return c;
+      };
+      double sum = 0;
+      for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b)
+          for (int cc = 0; cc < 4; ++cc)
+            sum += w[0][a] * w[1][b] * w[2][cc] *
+                   f[rg.index(fix(p0[0] + a - 1), fix(p0[1] + b - 1), fix(p0[2] + cc - 1))];
+      return sum;
+    }
+    if (v.factor <= 1) return f[i];
+    I3 c0 = v.g.coord(i);
+    double sum = 0;
+    int cnt = 0;
+    int fct = v.factor;
+    for (int x = c0.x * fct; x < std::min(c0.x * fct + fct, rg.n); ++x)
+      for (int y = c0.y * fct; y < std::min(c0.y * fct + fct, rg.n); ++y)
+        for (int z = c0.z * fct; z < std::min(c0.z * fct + fct, rg.n); ++z) {
+          sum += f[rg.index(x, y, z)];
+          ++cnt;
+        }
+    return sum / cnt;
+  }
+
   std::vector<VLevel> graficStack(std::array<double, 3> centre) const {
     int extraLow = powerOf2(subsample), extraHigh = powerOf2(supersample);
     std::vector<VLevel> st;
@@ -1659,7 +1930,6 @@ struct Generator {
       const VLevel &v = st[l];
       const Grid &g = v.g;
       int n = g.n;
-      const Grid &rg = grids[v.src];
       auto &f = fieldsOf(v.src);
       size_t effective = size_t(std::llround(base.dx * base.n / g.dx));
       std::string dir = name + "_" + std::to_string(effective);
@@ -1687,57 +1957,7 @@ struct Generator {
       // value of real-level field k at stack-level cell i: a subsampled level
       // averages the real cells it covers (SubSampleGrid::forEachSubcell,
       // virtualgrid.hpp:576-607, which clips at the grid's far edge)
-      auto eval = [&](int k, size_t i) -> double {
-        if (v.factor < -1) {
-          // SuperSampleEvaluator (field/evaluator.hpp:96-126): the real
-          // level interpolated at this cell's centroid (evaluateInterpolated,
-          // field.hpp:483-508, tricubic of field.hpp:598-617), wrapping on a
-          // level that covers the box and repeating edge values otherwise
-          auto p = v.gu.centroid(i);
-          double o[3] = {rg.ox, rg.oy, rg.oz};
-          int p0[3];
-          double w[3][4];
-          for (int a3 = 0; a3 < 3; ++a3) {
-            double u = rg.wrap(p[a3] - o[a3]) / rg.dx - 0.5;
-            p0[a3] = int(std::floor(u));
-            double t = u - p0[a3], t2 = t * t, t3 = t2 * t;
-            w[a3][0] = -0.5 * t3 + t2 - 0.5 * t;
-            w[a3][1] = 1.5 * t3 - 2.5 * t2 + 1.0;
-            w[a3][2] = -1.5 * t3 + 2.0 * t2 + 0.5 * t;
-            w[a3][3] = 0.5 * t3 - 0.5 * t2;
-          }
-          bool wrapc = rg.n == rg.simEquivalent();
-          int sim = rg.simEquivalent();
-          auto fix = [&](int c) {
-            if (wrapc) {
-              if (c > sim - 1) c -= sim;
-              if (c < 0) c += sim;
-            } else {
-              c = std::min(std::max(c, 0), rg.n - 1);
-            }
-            return c;
-          };
-          double sum = 0;
-          for (int a = 0; a < 4; ++a)
-            for (int b = 0; b < 4; ++b)
-              for (int cc = 0; cc < 4; ++cc)
-                sum += w[0][a] * w[1][b] * w[2][cc] *
-                       f[k][rg.index(fix(p0[0] + a - 1), fix(p0[1] + b - 1), fix(p0[2] + cc - 1))];
-          return sum;
-        }
-        if (v.factor <= 1) return f[k][i];
-        I3 c0 = g.coord(i);
-        double sum = 0;
-        int cnt = 0;
-        int fct = v.factor;
-        for (int x = c0.x * fct; x < std::min(c0.x * fct + fct, rg.n); ++x)
-          for (int y = c0.y * fct; y < std::min(c0.y * fct + fct, rg.n); ++y)
-            for (int z = c0.z * fct; z < std::min(c0.z * fct + fct, rg.n); ++z) {
-              sum += f[k][rg.index(x, y, z)];
-              ++cnt;
-            }
-        return sum / cnt;
-      };
+      auto eval = [&](int k, size_t i) -> double { return evalVirtual(v, f[k], i); };
       for (int iz = 0; iz < n; ++iz) {
         for (int iy = 0; iy < n; ++iy)
           for (int ix = 0; ix < n; ++ix) {
@@ -1848,26 +2068,22 @@ struct Generator {
   // particle in mapper order, as float, after a 4-byte count
   void dumpTipsyArray(const std::string &fn) {
     generate();
-    buildMapper();
-    auto order = levelOrder();
+    auto blocks = layoutBlocks(fmt == Fmt::tipsy);
     size_t total = 0;
-    for (auto &o : order) total += o.size();
-    size_t ngas = Ob > 0 ? order.back().size() : 0;
+    for (auto &b : blocks) total += b.count();
     std::ofstream f(fn, std::ios::binary);
-    uint32_t len = uint32_t(total + ngas);
+    uint32_t len = uint32_t(total);
     f.write((const char *)&len, 4);
-    auto emit = [&](size_t l) {
-      int n = grids[l].n;
-      std::vector<double> d(size_t(n) * n * n);
-      check(gic_copy_field(c(), GIC_FIELD_OVERDENSITY, int(l), GIC_REAL_SPACE, d.data(), GIC_DTYPE_F64));
-      std::vector<float> v(order[l].size());
-      for (size_t i = 0; i < v.size(); ++i) v[i] = float(d[order[l][i]]);
-      f.write((const char *)v.data(), v.size() * 4);
-    };
-    bool gasFirst = fmt == Fmt::tipsy;
-    if (ngas && gasFirst) emit(order.size() - 1);
-    for (size_t l = 0; l < order.size(); ++l) emit(l);
-    if (ngas && !gasFirst) emit(order.size() - 1);
+    std::vector<std::vector<double>> delta(grids.size());
+    for (auto &b : blocks) {
+      auto &d = delta[b.level];
+      if (d.empty()) d = levelField(GIC_FIELD_OVERDENSITY, b.level);
+      VLevel v = resampledLevel(b.level, b.factor);
+      std::vector<float> vals(b.count());
+#pragma omp parallel for schedule(static)
+      for (size_t i = 0; i < vals.size(); ++i) vals[i] = float(b.factor == 1 ? d[b.cells[i]] : evalVirtual(v, d, b.cells[i]));
+      f.write((const char *)vals.data(), vals.size() * 4);
+    }
   }
 };
 
@@ -2049,8 +2265,9 @@ struct Dispatcher {
           ic.lptOrder = o;
         }));
     // grafic hierarchy and mask (ic.hpp:301-350, main.cpp:59-74)
-    add("subsample", std::function<void(int)>([this](int f) { ic.setResampling(f, ic.supersample); }));
-    add("supersample", std::function<void(int)>([this](int f) { ic.setResampling(ic.subsample, f); }));
+    add("subsample", std::function<void(int)>([this](int f) { ic.setSubsample(f); }));
+    add("supersample", std::function<void(int)>([this](int f) { ic.setSupersample(f); }));
+    add("supersample_gas", std::function<void(int)>([this](int f) { ic.setSupersampleGas(f); }));
     add("pvar", std::function<void(double)>([this](double v) { ic.pvarValue = v; }));
     add("center_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));
     add("centre_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));

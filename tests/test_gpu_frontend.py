@@ -298,3 +298,56 @@ def test_centred_grafic_zoom_runs(tmp_path, test):
     for g in dirs:
         for name in ("ic_velcx", "ic_poscx", "ic_deltab", "ic_refmap", "ic_particle_ids"):
             assert os.path.getsize(os.path.join(g, name)) > 44, (g, name)
+
+
+@pytest.mark.parametrize("test", [
+    "test_04a_supersampling",
+    "test_04b_subsampling",
+    "test_04d_gas_supersampling",
+    "test_10e_gadget_zoom_ptype_resample",
+])
+def test_resampled_particle_output_matches_reference(tmp_path, test):
+    """Particles laid on sub/supersampled grids (04a: DM supersampled beside
+    unsampled gas; 04b: base subsampled and zoom supersampled; 04d: gas and
+    DM supersampled by different factors; 10e: gadget types kept through
+    both) at compare.py's tolerances; 04b's dump_tipsy overdensity array,
+    evaluated on the same virtual grids, to 5 decimals (compare.py:50-51)."""
+    d, _ = _run(test, tmp_path)
+    compare_particles(particle_file(d), golden(test, "reference_output"))
+    if test == "test_04b_subsampling":
+        got = read_tipsy_array(os.path.join(d, "test_4.tipsy.overdensity"))
+        want = read_tipsy_array(golden(test, "reference_output.overdensity"))
+        assert got.shape == want.shape
+        np.testing.assert_almost_equal(got, want, decimal=5)
+
+
+def test_double_zoom_supersampling_keeps_the_coarse_levels(tmp_path):
+    """test_04c (run-only in the reference): supersampling the finest of two
+    zoom levels by 2 leaves the particles of the levels above unchanged and
+    gives each finest cell 8 particles of an eighth of its mass with its
+    softening, whose mean displacement follows the cell's own."""
+    test = "test_04c_doublezoom_supersampling"
+    d, _ = _run(test, tmp_path)
+    plain = os.path.join(str(tmp_path), "plain")
+    text = open(os.path.join(d, "paramfile.txt")).read()
+    assert "supersample 2" in text
+    dp = stage_test(test, plain, text.replace("supersample 2", ""))
+    rc, err = run_frontend(dp)
+    assert rc == 0, err
+    a = read_tipsy(os.path.join(d, "test_1.tipsy"))["dark"]
+    b = read_tipsy(os.path.join(dp, "test_1.tipsy"))["dark"]
+    mfine = b["mass"].min()
+    nfine = int(np.sum(b["mass"] == mfine))
+    ncoarse = len(b) - nfine
+    assert len(a) == ncoarse + 8 * nfine
+    for k in ("pos", "vel", "eps"):
+        np.testing.assert_array_equal(a[k][:ncoarse], b[k][:ncoarse])
+    # tipsy masses are normalised to sum to Om: compare ratios
+    np.testing.assert_allclose(a["mass"][ncoarse:] / a["mass"][0], b["mass"][-1] / b["mass"][0] / 8, rtol=1e-6)
+    np.testing.assert_array_equal(a["eps"][ncoarse:], b["eps"][-1])
+    # velocities are proportional to displacements: the supersampled level
+    # interpolates the same field, so its mean velocity over the level and
+    # its rms stay within a few per cent of the unsampled level's
+    fa, fb = a["vel"][ncoarse:], b["vel"][ncoarse:]
+    np.testing.assert_allclose(fa.mean(axis=0), fb.mean(axis=0), atol=0.02 * fb.std())
+    np.testing.assert_allclose(fa.std(axis=0), fb.std(axis=0), rtol=0.05)

@@ -3,10 +3,10 @@
 compare_outputlogfile does: every reference line appears in the log, time
 stamps stripped): the zoom corner of a second zoom opened next to the first
 one's edge (ic.hpp:580-588), the border-safety warning (ic.hpp:462-473) and
-the gadget particle-type reuse error with its paramfile line.
-test_15_edge_subsampling_error expects the resampled two-level mapper's
-error (twolevelmapper.hpp:600-605); resampled particle output is not built
-here, so the front-end must refuse the run instead of writing particles."""
+the gadget particle-type reuse error with its paramfile line, and the
+resampled two-level mapper's error when a subsampled base grid's zoomed cells
+reach past the zoom grid (twolevelmapper.hpp:600-605), reported at the
+subsample command."""
 import pytest
 
 from helpers import golden, run_frontend, stage_test
@@ -18,6 +18,7 @@ pytestmark = pytest.mark.gpu
     ("test_01j_two_zooms_edge", True),
     ("test_09a_border_safety", True),
     ("test_10j_impossible_gadget_ptype_request_error", False),
+    ("test_15_edge_subsampling_error", False),
 ])
 def test_log_contains_reference_lines(test, must_succeed, tmp_path):
     print("log case", test, flush=True)
@@ -29,13 +30,3 @@ def test_log_contains_reference_lines(test, must_succeed, tmp_path):
     for want in open(golden(test, "reference.txt")).read().splitlines():
         assert want in lines, "line %r missing from the log:\n%s" % (want, err)
 
-
-def test_resampled_particle_output_is_refused(tmp_path):
-    test = "test_15_edge_subsampling_error"
-    print("log case", test, flush=True)
-    d = stage_test(test, tmp_path)
-    rc, err = run_frontend(d)
-    assert rc != 0
-    assert "subsample and supersample apply only to grafic output" in err
-    # refused at the command, as the reference's mapper rebuild there fails
-    assert 'on line 29 ("subsample 2")' in err
