@@ -19,9 +19,11 @@
 // 3 or 4 on the rare edge/tail tests, so the Gaussian index of a chunk is
 // data dependent.  Pass 1 generates every chunk and parses it from the 4
 // possible entry offsets (the parses merge after a few iterations), pass 2
-// scans the entry->(exit, count) maps, pass 3 regenerates each chunk, parses
-// it from its true entry and writes the Gaussians to their cells.  No raw
-// numbers go to HBM: the parse reads them from a per-thread LDS ring.
+// scans the entry->(exit, count) maps, pass 3 parses each chunk again from
+// its true entry and writes the Gaussians to their cells.  Pass 1 stores the
+// chunk's raw 24-bit numbers (4 bytes each, about 2.05 per Gaussian) for
+// pass 3 to read back, so the generator runs once; within a pass the parse
+// reads them from a per-thread LDS ring.
 #include "rng.h"
 #include "fft.h"
 #include <cstring>
@@ -258,18 +260,36 @@ __device__ void chunk_start_state(const RngDev &p, long t, uint64_t *x, int64_t 
 }
 
 // ---- ziggurat iteration ---------------------------------------------------------
+// The ziggurat tables in LDS: the lookups are indexed by random bytes, so
+// each lane reads a different entry (constant memory serves one address per
+// wavefront)
+struct ZigTabs {
+  double y[128];
+  double w[128];
+  uint32_t k[128];
+};
+// all threads of the block call this before any of them returns
+__device__ __forceinline__ void zig_tabs_load(ZigTabs &z) {
+  for (int i = int(threadIdx.x); i < 128; i += int(blockDim.x)) {
+    z.y[i] = c_ytab[i];
+    z.w[i] = c_wtab[i];
+    z.k[i] = c_ktab[i];
+  }
+  __syncthreads();
+}
+
 // One loop iteration of gsl_ran_gaussian_ziggurat starting at stream position
 // pos: returns the number of numbers consumed (2, 3 or 4) and whether it
 // accepted; *val receives the Gaussian when accepted.
 template <class Get>
-__device__ __forceinline__ int zig_iter(Get get, long pos, bool &accept, double &val) {
+__device__ __forceinline__ int zig_iter(Get get, long pos, bool &accept, double &val, const ZigTabs &zt) {
   const double PARAM_R = 3.44428647676;
   uint32_t k1 = get(pos), k2 = get(pos + 1);
   uint32_t i = k1 & 0xFF, j = k2 & 0xFFFFFF;
   double sign = (i & 0x80) ? 1.0 : -1.0;
   i &= 0x7f;
-  double x = j * c_wtab[i];
-  if (j < c_ktab[i]) {
+  double x = j * zt.w[i];
+  if (j < zt.k[i]) {
     accept = true;
     val = sign * x;
     return 2;
@@ -277,7 +297,7 @@ __device__ __forceinline__ int zig_iter(Get get, long pos, bool &accept, double 
   double y;
   int used;
   if (i < 127) {
-    double y0 = c_ytab[i], y1 = c_ytab[i + 1];
+    double y0 = zt.y[i], y1 = zt.y[i + 1];
     double U1 = get(pos + 2) / 16777216.0;
     y = y1 + (y0 - y1) * U1;
     used = 3;
@@ -302,8 +322,9 @@ constexpr int RING = 48;
 template <int SLOTS> struct RingT {
   uint32_t *base;
   int stride;
-  __device__ __forceinline__ uint32_t get(long pos) const { return base[int(pos % SLOTS) * stride]; }
-  __device__ __forceinline__ void put(long pos, uint32_t v) const { base[int(pos % SLOTS) * stride] = v; }
+  // positions within a chunk are non-negative and below 2^31
+  __device__ __forceinline__ uint32_t get(long pos) const { return base[int(uint32_t(pos) % SLOTS) * stride]; }
+  __device__ __forceinline__ void put(long pos, uint32_t v) const { base[int(uint32_t(pos) % SLOTS) * stride] = v; }
 };
 using Ring = RingT<RING>;
 
@@ -331,6 +352,8 @@ __global__ __launch_bounds__(256) void k_rng_summary(const RngDev *__restrict__ 
                                                     uint32_t *__restrict__ summary,
                                                     uint32_t *__restrict__ raw) {
   extern __shared__ uint32_t ring_mem[];
+  __shared__ ZigTabs zt;
+  zig_tabs_load(zt);
   long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (t >= nchunks) return;
   Ring ring{ring_mem + threadIdx.x, int(blockDim.x)};
@@ -359,7 +382,7 @@ __global__ __launch_bounds__(256) void k_rng_summary(const RngDev *__restrict__ 
       while (pos < ch && pos + 4 <= avail) {
         bool acc;
         double v;
-        pos += zig_iter(get, pos, acc, v);
+        pos += zig_iter(get, pos, acc, v, zt);
         shared_cnt += acc;
       }
       cur[0] = pos;
@@ -371,7 +394,7 @@ __global__ __launch_bounds__(256) void k_rng_summary(const RngDev *__restrict__ 
         while (pos < ch && pos + 4 <= avail) {
           bool acc;
           double v;
-          pos += zig_iter(get, pos, acc, v);
+          pos += zig_iter(get, pos, acc, v, zt);
           cnt[e] += acc;
         }
         cur[e] = pos;
@@ -507,6 +530,8 @@ __global__ __launch_bounds__(64) void k_rng_emit(const uint32_t *__restrict__ ra
                                                 size_t total, int kc) {
   __shared__ uint32_t ring_mem[EMIT_RING * 64];
   __shared__ T stage_mem[EMIT_STAGE * 64];
+  __shared__ ZigTabs zt;
+  zig_tabs_load(zt);
   const int lane = threadIdx.x;
   long t = long(blockIdx.x) * 64 + lane;
   bool active = t < nchunks;
@@ -568,7 +593,7 @@ __global__ __launch_bounds__(64) void k_rng_emit(const uint32_t *__restrict__ ra
       while (pos < ch && pos + 4 <= avail && g < total) {
         bool acc;
         double v;
-        pos += zig_iter(get, pos, acc, v);
+        pos += zig_iter(get, pos, acc, v, zt);
         if (acc) {
           stage_mem[fill * 64 + lane] = T(v * lt.scale);
           ++fill;
