@@ -523,6 +523,22 @@ __device__ __forceinline__ T *cell_ptr(const LevelTargets &lt, uint64_t g) {
   return reinterpret_cast<T *>(lt.f[L]) + (xx * n + y) * (n + 2) + z;
 }
 
+// Level and (x, y, z) of global Gaussian index g; n[L] == 0 (a dense target)
+// leaves the coordinates unused
+__device__ __forceinline__ void cell_coords(const LevelTargets &lt, uint64_t g, int &L, uint32_t &x, uint32_t &y,
+                                            uint32_t &z) {
+  L = g < lt.cum[0] ? 0 : g < lt.cum[1] ? 1 : g < lt.cum[2] ? 2 : 3;
+  size_t i = g - (L ? lt.cum[L - 1] : 0);
+  int n = lt.n[L];
+  x = y = z = 0;
+  if (n == 0) return;
+  size_t nn = size_t(n) * n;
+  size_t xx = i / nn, r = i - xx * nn, yy = r / n;
+  x = uint32_t(xx);
+  y = uint32_t(yy);
+  z = uint32_t(r - yy * n);
+}
+
 template <typename T>
 __global__ __launch_bounds__(64) void k_rng_emit(const uint32_t *__restrict__ raw, long nchunks,
                                                 const uint32_t *__restrict__ summary,
@@ -560,6 +576,11 @@ __global__ __launch_bounds__(64) void k_rng_emit(const uint32_t *__restrict__ ra
     __syncthreads();
     int want = final ? fill : (fill >= EMIT_FLUSH ? EMIT_FLUSH : 0);
     unsigned long long mask = __ballot(want > 0);
+    // each lane locates its own run's first cell once; the 32 lanes storing
+    // a run step from there (no divisions per stored value)
+    int myL = 0;
+    uint32_t mx = 0, my = 0, mz = 0;
+    if (want > 0) cell_coords(lt, gstage, myL, mx, my, mz);
     while (mask) {
       int s0 = __ffsll((long long)mask) - 1;
       mask &= mask - 1;
@@ -570,9 +591,29 @@ __global__ __launch_bounds__(64) void k_rng_emit(const uint32_t *__restrict__ ra
       }
       int src = lane < 32 ? s0 : s1;
       int j = lane & 31;
-      int w = __shfl(want, src < 0 ? 0 : src);
-      uint64_t gs = __shfl(gstage, src < 0 ? 0 : src);
-      if (src >= 0 && j < w) *cell_ptr<T>(lt, gs + j) = stage_mem[j * 64 + src];
+      int sl = src < 0 ? 0 : src;
+      int w = __shfl(want, sl);
+      uint64_t gs = __shfl(gstage, sl);
+      int L = __shfl(myL, sl);
+      uint32_t x = __shfl(mx, sl), y = __shfl(my, sl), z = __shfl(mz, sl);
+      if (src >= 0 && j < w) {
+        T *ptr;
+        int n = lt.n[L];
+        if (n != 0 && gs + j < lt.cum[L]) {
+          uint32_t zz = z + j, yy = y, xx = x;
+          while (zz >= uint32_t(n)) {   // at most once when n >= 32
+            zz -= n;
+            if (++yy == uint32_t(n)) {
+              yy = 0;
+              ++xx;
+            }
+          }
+          ptr = reinterpret_cast<T *>(lt.f[L]) + (size_t(xx) * n + yy) * (n + 2) + zz;
+        } else {
+          ptr = cell_ptr<T>(lt, gs + j);   // a dense target, or a run that crosses into the next level
+        }
+        *ptr = stage_mem[j * 64 + src];
+      }
     }
     __syncthreads();
     if (want > 0) {
