@@ -279,33 +279,44 @@ template <typename T> void do_second_order(gic_context *c) {
     c->tab_a.ensure(nent * sizeof(T));
     c->tab_c.ensure(nent * sizeof(T));
     filter_table<T>(2, 0.0, g.kmin(), nent, (T *)c->tab_a.p, c->st, true);
-    const T *inv_k2 = (const T *)c->tab_a.p;
     T *A = (T *)L.psi2[0].p, *B = (T *)L.psi2[1].p;
-    (void)inv_k2;   // 1/k^2 is formed inside the transforms
-    auto phi = [&](T *out, int a, int b) {
-      fft_c2r_from<T>(spec, out, n, FFT_PRE_KK, nullptr, a + 3 * b, g.kmin(), c->st, c->hook(), nullptr, 1.0);
-    };
+    // the x passes of (1/k^2) delta, i k_x (1/k^2) delta and k_x^2 (1/k^2)
+    // delta serve all six phi_ab (their y passes multiply by the k_y, k_z
+    // factors); the base level's first two are left by the Zel'dovich pass
+    c->xhalf.ensure(3 * bytes);
+    T *X0 = (T *)c->xhalf.p, *X1 = X0 + reals, *X2 = X1 + reals;
     if (c->hook()) c->hook()->begin("lpt_source", c->st);
-    // each phi_ab is folded into S by its transform's last pass
-    auto fold = [&](int a, int b, int epi) {
-      fft_c2r_lpt<T>(spec, B, n, nullptr, a + 3 * b, g.kmin(), epi, S, A, c->st, c->hook(), 1.0);
+    if (!(l == 0 && c->xhalf_base && spec == (const T *)L.noise.p)) {
+      fft_c2r_xpass<T>(spec, X0, n, FFT_PRE_TABLE, nullptr, 0, g.kmin(), c->st, c->hook(), 1.0);
+      fft_c2r_xpass<T>(spec, X1, n, FFT_PRE_ZA, nullptr, 0, g.kmin(), c->st, c->hook(), 1.0);
+    }
+    c->xhalf_base = false;
+    fft_c2r_xpass<T>(spec, X2, n, FFT_PRE_KK, nullptr, 0, g.kmin(), c->st, c->hook(), 1.0);
+    // each phi_ab but xx is folded into S by its transform's last pass
+    auto fold = [&](const T *X, int yop, int epi) {
+      fft_c2r_yz_lpt<T>(X, B, n, yop, g.kmin(), epi, S, A, c->st, c->hook());
     };
-    phi(A, 0, 0);
-    fold(1, 1, 1);   // S = xx yy, A = xx + yy
-    fold(2, 2, 2);   // S += (xx + yy) zz
-    fold(0, 1, 3);   // S -= xy^2
-    fold(0, 2, 3);   // S -= xz^2
-    fold(1, 2, 3);   // S -= yz^2
+    fft_c2r_yz<T>(X2, A, n, FFT_YZ_ONE, g.kmin(), c->st, c->hook());   // A = xx
+    fold(X0, FFT_YZ_KYKY, 1);   // S = xx yy, A = xx + yy
+    fold(X0, FFT_YZ_KZKZ, 2);   // S += (xx + yy) zz
+    fold(X1, FFT_YZ_MIKY, 3);   // S -= xy^2
+    fold(X1, FFT_YZ_MIKZ, 3);   // S -= xz^2
+    fold(X0, FFT_YZ_KYKZ, 3);   // S -= yz^2
     if (c->hook()) c->hook()->end("lpt_source", c->st);
     fft3d<T>(S, n, -1, c->st, c->hook());
     double kc = l >= 1 ? kcut_for_level(c, int(l) - 1) : 0.0;
     filter_table<T>(l >= 1 ? 1 : 2, kc, g.kmin(), nent, (T *)c->tab_c.p, c->st, true, 3.0 / 7.0);
-    for (int a = 0; a < 3; ++a) {
-      if (l >= 1)
-        fft_c2r_from<T>(S, (T *)L.psi2[a].p, n, FFT_PRE_ZA, (const T *)c->tab_c.p, a, g.kmin(), c->st, c->hook());
-      else   // unit filter: (3/7) / k^2 formed inside the transform
-        fft_c2r_from<T>(S, (T *)L.psi2[a].p, n, FFT_PRE_ZA, nullptr, a, g.kmin(), c->st, c->hook(), nullptr,
-                        3.0 / 7.0);
+    {
+      // psi2_x from the x pass of i k_x f S, psi2_y and psi2_z from that of
+      // f S, f = the level's filter (3/7)/k^2 (unit filter on the base level,
+      // formed inside the transform)
+      const T *tab = l >= 1 ? (const T *)c->tab_c.p : nullptr;
+      double ik2 = l >= 1 ? 0.0 : 3.0 / 7.0;
+      fft_c2r_xpass<T>(S, X1, n, FFT_PRE_ZA, tab, 0, g.kmin(), c->st, c->hook(), ik2);
+      fft_c2r_yz<T>(X1, (T *)L.psi2[0].p, n, FFT_YZ_ONE, g.kmin(), c->st, c->hook());
+      fft_c2r_xpass<T>(S, X0, n, FFT_PRE_TABLE, tab, 0, g.kmin(), c->st, c->hook(), ik2);
+      fft_c2r_yz<T>(X0, (T *)L.psi2[1].p, n, FFT_YZ_IKY, g.kmin(), c->st, c->hook());
+      fft_c2r_yz<T>(X0, (T *)L.psi2[2].p, n, FFT_YZ_IKZ, g.kmin(), c->st, c->hook());
     }
     if (l >= 1) {
       Level &C = *c->levels[l - 1];
@@ -366,8 +377,11 @@ template <typename T> void do_zeldovich(gic_context *c) {
   size_t nl = c->levels.size();
   // Zel'dovich per level, formed inside the first inverse pass from the
   // overdensity spectrum; zoom levels carry their high-pass filter (the cut
-  // of the level above) on delta and psi.
-  for (size_t l = 0; l < nl; ++l) {
+  // of the level above) on delta and psi.  The levels are independent here;
+  // the base level goes last so that its x passes stay in xhalf for 2LPT.
+  c->xhalf_base = false;
+  for (size_t li = nl; li-- > 0;) {
+    size_t l = li;
     Level &L = *c->levels[l];
     ensure_noise_fourier<T>(c, L);
     for (auto &p : L.psi) alloc_field(c, p, L.g);
@@ -376,13 +390,26 @@ template <typename T> void do_zeldovich(gic_context *c) {
     c->tab_c.ensure(nent * sizeof(T));
     double kc = l >= 1 ? kcut_for_level(c, l - 1) : 0.0;
     filter_table<T>(l >= 1 ? 1 : 2, kc, L.g.kmin(), nent, (T *)c->tab_c.p, c->st, true);
-    for (int a = 0; a < 3; ++a) {
-      if (l >= 1)
-        fft_c2r_from<T>((const T *)L.noise.p, (T *)L.psi[a].p, L.g.n, FFT_PRE_ZA, (const T *)c->tab_c.p, a,
-                        L.g.kmin(), c->st, c->hook());
-      else   // unit filter: 1/k^2 formed inside the transform
-        fft_c2r_from<T>((const T *)L.noise.p, (T *)L.psi[a].p, L.g.n, FFT_PRE_ZA, nullptr, a, L.g.kmin(), c->st,
-                        c->hook(), nullptr, 1.0);
+    {
+      // psi_x from the x pass of i k_x f delta; psi_y and psi_z share the x
+      // pass of f delta, their y passes multiplying by i k_y and i k_z
+      // (f: the high-pass filter over k^2 on a zoom level, 1/k^2 formed
+      // inside the transform on the base level)
+      size_t reals = L.g.padded_reals();
+      c->xhalf.ensure(3 * reals * sizeof(T));
+      T *X0 = (T *)c->xhalf.p, *X1 = X0 + reals;
+      const T *tab = l >= 1 ? (const T *)c->tab_c.p : nullptr;
+      double ik2 = l >= 1 ? 0.0 : 1.0;
+      const T *spec = (const T *)L.noise.p;
+      int n = L.g.n;
+      fft_c2r_xpass<T>(spec, X1, n, FFT_PRE_ZA, tab, 0, L.g.kmin(), c->st, c->hook(), ik2);
+      fft_c2r_yz<T>(X1, (T *)L.psi[0].p, n, FFT_YZ_ONE, L.g.kmin(), c->st, c->hook());
+      fft_c2r_xpass<T>(spec, X0, n, FFT_PRE_TABLE, tab, 0, L.g.kmin(), c->st, c->hook(), ik2);
+      fft_c2r_yz<T>(X0, (T *)L.psi[1].p, n, FFT_YZ_IKY, L.g.kmin(), c->st, c->hook());
+      fft_c2r_yz<T>(X0, (T *)L.psi[2].p, n, FFT_YZ_IKZ, L.g.kmin(), c->st, c->hook());
+      // the base level's X0 (1/k^2 delta) and X1 (i k_x / k^2 delta) are the
+      // first two x passes of the second derivatives
+      c->xhalf_base = l == 0 && c->lpt_order == 2;
     }
     if (l >= 1) {
       filter_table<T>(1, kc, L.g.kmin(), nent, (T *)c->tab_a.p, c->st);

@@ -258,6 +258,11 @@ struct gic_context {
   int lpt_order = 1;   // 1: Zel'dovich, 2: second-order LPT on top
   bool fix_power = false;   // fix_power: unit-modulus white noise after the draw
   DevBuf work, stage, tab_a, tab_b, tab_c, lpt_win, adj_scratch;
+  // three padded fields of data transformed along x only (fft_c2r_xpass), so
+  // that one x pass serves several displacement or second-derivative fields;
+  // after the Zel'dovich pass it holds the base level's X0, X1 for 2LPT
+  DevBuf xhalf;
+  bool xhalf_base = false;
   std::unique_ptr<EventTimer> timer;
   TimingHook *hook() { return timer.get(); }
   size_t tsize() const { return prec == 32 ? 4 : 8; }

@@ -89,6 +89,40 @@ void fft_c2r_from(const T *spec, T *out, int n, int pre, const T *tab, int comp,
 template <typename T>
 void fft_c2r_lpt(const T *spec, T *scratch, int n, const T *tab, int comp, double kmin, int epi, T *S, T *A,
                  hipStream_t st, TimingHook *hook, double inv_k2 = 0);
+// The same transforms split after their first pass, so that one x pass serves
+// several outputs (c2r is linear and a factor of k_y, k_z commutes with the
+// transform along x):
+//   half <- the x pass of c2r(op(spec)) alone (op, comp, tab, inv_k2 as for
+//           fft_c2r_from; half is a padded field holding complex x-k_y-k_z data);
+//   out  <- the y and z passes of c2r from half, the y pass multiplying each
+//           mode by m(k_y, k_z) as it reads it (FFT_YZ_*; dst as for
+//           fft_c2r_from);
+//   fft_c2r_yz_lpt: the same, its last pass folding the result into the 2LPT
+//           source as fft_c2r_lpt does (out is scratch).
+// With X0 = x pass of tab spec, X1 = x pass of i k_x tab spec (FFT_PRE_ZA comp 0)
+// and X2 = x pass of k_x^2 tab spec (FFT_PRE_KK comp 0): the Zel'dovich
+// components are yz(X1, ONE), yz(X0, IKY), yz(X0, IKZ) and the second
+// derivatives yz(X2, ONE), yz(X1, MIKY), yz(X1, MIKZ), yz(X0, KYKY),
+// yz(X0, KYKZ), yz(X0, KZKZ), with the +Nyquist rules of FFT_PRE_ZA/KK.
+enum {
+  FFT_YZ_ONE = 0,    // 1
+  FFT_YZ_IKY = 1,    // i k_y (0 at k_y = +Nyquist)
+  FFT_YZ_IKZ = 2,    // i k_z (0 at k_z = +Nyquist)
+  FFT_YZ_MIKY = 3,   // -i k_y (0 at k_y = +Nyquist)
+  FFT_YZ_MIKZ = 4,   // -i k_z (0 at k_z = +Nyquist)
+  FFT_YZ_KYKY = 5,   // k_y^2
+  FFT_YZ_KYKZ = 6,   // k_y k_z (0 at either +Nyquist)
+  FFT_YZ_KZKZ = 7    // k_z^2
+};
+template <typename T>
+void fft_c2r_xpass(const T *spec, T *half, int n, int pre, const T *tab, int comp, double kmin, hipStream_t st,
+                   TimingHook *hook, double inv_k2 = 0);
+template <typename T>
+void fft_c2r_yz(const T *half, T *out, int n, int yop, double kmin, hipStream_t st, TimingHook *hook,
+                T *dst = nullptr);
+template <typename T>
+void fft_c2r_yz_lpt(const T *half, T *scratch, int n, int yop, double kmin, int epi, T *S, T *A, hipStream_t st,
+                    TimingHook *hook);
 // Spectrum (r2c, N^-1.5) of a field that is zero outside the real-space window
 // [w0, w0+wl) (per axis, wrapping mod n), given as the dense window win
 // (wl[0]*wl[1]*wl[2] reals, z fastest): pruned z and y passes and a full x

@@ -221,7 +221,9 @@ __device__ __forceinline__ void run_stages(cx<T> *lds, const cx<T> *tw, int twmu
 //        POST_ZA_ADJ: out (+)= -i k_c tab[|k|^2] v, zero at k_c = +Nyquist: the
 //                     transpose of PRE_ZA, accumulated over components in out
 //        POST_AUX:   out = tab[|k|^2] aux + v, aux a spectrum laid out as out
-enum { PRE_NONE = FFT_PRE_NONE, PRE_TABLE = FFT_PRE_TABLE, PRE_ZA = FFT_PRE_ZA, PRE_KK = FFT_PRE_KK };
+// PRE_YZ: the y pass of a field already transformed along x multiplies each
+// mode by the FFT_YZ_* factor a.comp of (k_y, k_z)
+enum { PRE_NONE = FFT_PRE_NONE, PRE_TABLE = FFT_PRE_TABLE, PRE_ZA = FFT_PRE_ZA, PRE_KK = FFT_PRE_KK, PRE_YZ = 4 };
 enum { POST_NONE = 0, POST_TABLE = 1, POST_ZA_ADJ = 2, POST_AUX = 3 };
 
 // FilterInline prepared for single precision on the host: per filter the
@@ -340,6 +342,21 @@ __device__ __forceinline__ void strided_tile(const StridedArgs<T> &a, const cx<T
     // times k_c (Zel'dovich, zero at k_c = +Nyquist) or k_a k_b (second
     // derivatives, zero at a +Nyquist component when a != b)
     __device__ __forceinline__ T factor(int line, int pt) const {
+      if constexpr (PRE == PRE_YZ) {   // a y pass: points run over k_y, columns over k_z
+        int kyy = signed_k(pt, a.n), kz = col0 + line, nyq = a.n / 2;
+        if (kz >= a.ncol) return T(0);
+        T fy = T(kyy) * a.kmin, fz = T(kz) * a.kmin;
+        switch (a.comp) {
+          case FFT_YZ_IKY: return kyy == nyq ? T(0) : fy;
+          case FFT_YZ_IKZ: return kz == nyq ? T(0) : fz;
+          case FFT_YZ_MIKY: return kyy == nyq ? T(0) : -fy;
+          case FFT_YZ_MIKZ: return kz == nyq ? T(0) : -fz;
+          case FFT_YZ_KYKY: return fy * fy;
+          case FFT_YZ_KYKZ: return (kyy == nyq || kz == nyq) ? T(0) : fy * fz;
+          case FFT_YZ_KZKZ: return fz * fz;
+          default: return T(1);
+        }
+      }
       int kx = signed_k(pt, a.n), kz = col0 + line;
       if (kz >= a.ncol) return T(0);   // past the last column (its loads are zero): no table read
       int ksq = kx * kx + ky2 + kz * kz;
@@ -357,10 +374,16 @@ __device__ __forceinline__ void strided_tile(const StridedArgs<T> &a, const cx<T
         return f;
       }
     }
-    // Zel'dovich: i f v; otherwise f v
+    // Zel'dovich and the first-derivative y factors: i f v; otherwise f v
     __device__ __forceinline__ cx<T> apply(cx<T> v, T f) const {
-      if constexpr (PRE == PRE_ZA) return cx<T>{-v.y * f, v.x * f};
-      else return cx<T>{v.x * f, v.y * f};
+      if constexpr (PRE == PRE_ZA) {
+        return cx<T>{-v.y * f, v.x * f};
+      } else if constexpr (PRE == PRE_YZ) {
+        if (a.comp >= FFT_YZ_IKY && a.comp <= FFT_YZ_MIKZ) return cx<T>{-v.y * f, v.x * f};
+        return cx<T>{v.x * f, v.y * f};
+      } else {
+        return cx<T>{v.x * f, v.y * f};
+      }
     }
   };
   PreOp op{a, col0, ky, ky2};
@@ -1000,6 +1023,41 @@ template <typename T, int N> struct FftN {
     strided<+1, PRE_NONE, POST_NONE, 1>(plain(co, false, T(1)), N, st, hook, "fft_y");
     zpass(out, norm(), +1, st, hook, "fft_z_c2r", dst);
   }
+  // the x pass of c2r_from alone, into half
+  static void xpass_from(const T *spec, T *half, int pre, const T *tab, int comp, double kmin, double inv_k2,
+                         hipStream_t st, TimingHook *hook) {
+    StridedArgs<T> ax = plain(reinterpret_cast<cx<T> *>(half), true, T(1));
+    ax.in = reinterpret_cast<const cx<T> *>(spec);
+    ax.tab = tab;
+    ax.comp = comp;
+    ax.kmin = T(kmin);
+    ax.kmin_d = kmin;
+    ax.inv_k2 = inv_k2;
+    if (!tab && pre != PRE_NONE && inv_k2 == 0) throw std::runtime_error("c2r_xpass: operator without a table");
+    if (pre == PRE_ZA) strided<+1, PRE_ZA, POST_NONE>(ax, N, st, hook, "fft_x_za");
+    else if (pre == PRE_KK) strided<+1, PRE_KK, POST_NONE>(ax, N, st, hook, "fft_x_kk");
+    else if (pre == PRE_TABLE) strided<+1, PRE_TABLE, POST_NONE>(ax, N, st, hook, "fft_x_pre");
+    else strided<+1, PRE_NONE, POST_NONE>(ax, N, st, hook, "fft_x");
+  }
+  // the y pass from half into out, times the FFT_YZ_* factor yop
+  static void ypass_from(const T *half, T *out, int yop, double kmin, hipStream_t st, TimingHook *hook) {
+    if (yop < FFT_YZ_ONE || yop > FFT_YZ_KZKZ) throw std::runtime_error("c2r_yz: bad y factor");
+    StridedArgs<T> ay = plain(reinterpret_cast<cx<T> *>(out), false, T(1));
+    ay.in = reinterpret_cast<const cx<T> *>(half);
+    ay.comp = yop;
+    ay.kmin = T(kmin);
+    if (yop == FFT_YZ_ONE) strided<+1, PRE_NONE, POST_NONE, 1>(ay, N, st, hook, "fft_y");
+    else strided<+1, PRE_YZ, POST_NONE, 1>(ay, N, st, hook, "fft_y_op");
+  }
+  static void yz_from(const T *half, T *out, int yop, double kmin, hipStream_t st, TimingHook *hook, T *dst) {
+    ypass_from(half, out, yop, kmin, st, hook);
+    zpass(out, norm(), +1, st, hook, "fft_z_c2r", dst);
+  }
+  static void yz_from_epi(const T *half, T *out, int yop, double kmin, int epi, T *S, T *A, hipStream_t st,
+                          TimingHook *hook) {
+    ypass_from(half, out, yop, kmin, st, hook);
+    zpass_epi(out, epi, S, A, st, hook);
+  }
   // c2r(op(spec)) whose last pass folds its output into the 2LPT source
   // accumulators S, A (EPI 1..3 of k_fft_z_c2r); out is scratch
   static void c2r_from_epi(const T *spec, T *out, int pre, const T *tab, int comp, double kmin, int epi, T *S, T *A,
@@ -1017,6 +1075,10 @@ template <typename T, int N> struct FftN {
     if (pre == PRE_KK) strided<+1, PRE_KK, POST_NONE>(ax, N, st, hook, "fft_x_kk");
     else throw std::runtime_error("c2r_from_epi: only the k_a k_b operator");
     strided<+1, PRE_NONE, POST_NONE, 1>(plain(co, false, T(1)), N, st, hook, "fft_y");
+    zpass_epi(out, epi, S, A, st, hook);
+  }
+  // the z pass (c2r) of out folded into S, A (EPI 1..3 of k_fft_z_c2r)
+  static void zpass_epi(T *out, int epi, T *S, T *A, hipStream_t st, TimingHook *hook) {
     long nrows = long(N) * N;
     int nblk = int(nrows / ROWS);
     size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
@@ -1220,6 +1282,20 @@ void fft_c2r_lpt(const T *spec, T *scratch, int n, const T *tab, int comp, doubl
   GIC_FFT_SWITCH(n, F::c2r_from_epi(spec, scratch, FFT_PRE_KK, tab, comp, kmin, epi, S, A, st, hook, inv_k2))
 }
 template <typename T>
+void fft_c2r_xpass(const T *spec, T *half, int n, int pre, const T *tab, int comp, double kmin, hipStream_t st,
+                   TimingHook *hook, double inv_k2) {
+  GIC_FFT_SWITCH(n, F::xpass_from(spec, half, pre, tab, comp, kmin, inv_k2, st, hook))
+}
+template <typename T>
+void fft_c2r_yz(const T *half, T *out, int n, int yop, double kmin, hipStream_t st, TimingHook *hook, T *dst) {
+  GIC_FFT_SWITCH(n, F::yz_from(half, out, yop, kmin, st, hook, dst))
+}
+template <typename T>
+void fft_c2r_yz_lpt(const T *half, T *scratch, int n, int yop, double kmin, int epi, T *S, T *A, hipStream_t st,
+                    TimingHook *hook) {
+  GIC_FFT_SWITCH(n, F::yz_from_epi(half, scratch, yop, kmin, epi, S, A, st, hook))
+}
+template <typename T>
 void fft_r2c_window(const T *win, int n, const int *w0, const int *wl, T *bufB, T *bufA, T *spec, const T *post,
                     bool accumulate, hipStream_t st, TimingHook *hook, const T *aux, const FilterInline *fil,
                     double kmin) {
@@ -1250,6 +1326,10 @@ void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, doubl
                                 double, const FilterInline *);                                               \
   template void fft_c2r_lpt<T>(const T *, T *, int, const T *, int, double, int, T *, T *, hipStream_t,       \
                                TimingHook *, double);                                                        \
+  template void fft_c2r_xpass<T>(const T *, T *, int, int, const T *, int, double, hipStream_t, TimingHook *,  \
+                                 double);                                                                    \
+  template void fft_c2r_yz<T>(const T *, T *, int, int, double, hipStream_t, TimingHook *, T *);              \
+  template void fft_c2r_yz_lpt<T>(const T *, T *, int, int, double, int, T *, T *, hipStream_t, TimingHook *); \
   template void fft_r2c_window<T>(const T *, int, const int *, const int *, T *, T *, T *, const T *, bool,      \
                                   hipStream_t, TimingHook *, const T *, const FilterInline *, double);         \
   template void fft_c2r_window<T>(const T *, int, int, const T *, int, double, const int *, const int *, T *, T *, \
