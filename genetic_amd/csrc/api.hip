@@ -353,9 +353,12 @@ template <typename T> void do_second_order(gic_context *c) {
         fil.scale = 3.0 / 7.0;
         fil.over_k2 = 1;
         for (int a = 0; a < 3; ++a) {
+          // x: the x pass of i k_x f S; y, z: one x pass of f S, the y pass
+          // multiplying by i k_y or i k_z
           T *out = (T *)((char *)c->lpt_win.p + a_b + b_b + a * o_b);
-          fft_c2r_window<T>(S, n, FFT_PRE_ZA, (const T *)c->tab_c.p, a, g.kmin(), w0, wl, bufA, bufB, out, c->st,
-                            c->hook(), &fil);
+          fft_c2r_window<T>(S, n, a == 0 ? FFT_PRE_ZA : FFT_PRE_TABLE, (const T *)c->tab_c.p, 0, g.kmin(), w0, wl,
+                            bufA, bufB, out, c->st, c->hook(), &fil,
+                            a == 0 ? FFT_YZ_ONE : a == 1 ? FFT_YZ_IKY : FFT_YZ_IKZ, a != 2);
           win_view[a].p = out;
           for (int d = 0; d < 3; ++d) {
             win_view[a].org[d] = w0[d];
@@ -454,8 +457,12 @@ template <typename T> void do_zeldovich(gic_context *c) {
         fil.kind[0] = 0;
         fil.kc[0] = kc;
         fil.over_k2 = f == 0 ? 0 : 1;
-        fft_c2r_window<T>((const T *)C.noise.p, C.g.n, f == 0 ? FFT_PRE_TABLE : FFT_PRE_ZA, f == 0 ? lp : lp_k2,
-                          f - 1, C.g.kmin(), w0, wl, bufA, bufB, out, c->st, c->hook(), &fil);
+        // delta: the low pass; psi_x: the x pass of i k_x LP/k^2 delta;
+        // psi_y, psi_z: one x pass of LP/k^2 delta, the y pass multiplying
+        // by i k_y or i k_z
+        fft_c2r_window<T>((const T *)C.noise.p, C.g.n, f == 1 ? FFT_PRE_ZA : FFT_PRE_TABLE, f == 0 ? lp : lp_k2, 0,
+                          C.g.kmin(), w0, wl, bufA, bufB, out, c->st, c->hook(), &fil,
+                          f <= 1 ? FFT_YZ_ONE : f == 2 ? FFT_YZ_IKY : FFT_YZ_IKZ, f != 3);
         v.p = out;
         for (int a = 0; a < 3; ++a) {
           v.org[a] = w0[a];

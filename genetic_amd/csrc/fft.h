@@ -137,9 +137,13 @@ void fft_r2c_window(const T *win, int n, const int *w0, const int *wl, T *bufB, 
                     const FilterInline *fil = nullptr, double kmin = 0);
 // Only the real-space window [w0, w0+wl) (per axis, wrapping mod n) of
 // c2r(op(spec)), via pruned passes: bufA holds wl[0]*n*(n/2+1) complex,
-// bufB wl[0]*wl[1]*(n/2+1) complex, out wl[0]*wl[1]*wl[2] reals.
+// bufB wl[0]*wl[1]*(n/2+1) complex, out wl[0]*wl[1]*wl[2] reals.  yop
+// (FFT_YZ_*) multiplies the y pass's input as fft_c2r_yz does; with xpass
+// false bufA already holds the x pass of an earlier call on the same
+// spectrum, operator and window, and only the y and z passes run.
 template <typename T>
 void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, double kmin, const int *w0, const int *wl,
-                    T *bufA, T *bufB, T *out, hipStream_t st, TimingHook *hook, const FilterInline *fil = nullptr);
+                    T *bufA, T *bufB, T *out, hipStream_t st, TimingHook *hook, const FilterInline *fil = nullptr,
+                    int yop = FFT_YZ_ONE, bool xpass = true);
 
 }  // namespace gic

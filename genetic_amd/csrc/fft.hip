@@ -1106,7 +1106,8 @@ template <typename T, int N> struct FftN {
   // bufA holds wl[0] x-planes after the x pass, bufB wl[0]*wl[1] rows after
   // the y pass; out receives wl[0]*wl[1]*wl[2] reals.
   static void c2r_window(const T *spec, int pre, const T *tab, int comp, T kmin, const int *w0, const int *wl, T *bufA,
-                         T *bufB, T *out, hipStream_t st, TimingHook *hook, const FilterInline *fil = nullptr) {
+                         T *bufB, T *out, hipStream_t st, TimingHook *hook, const FilterInline *fil = nullptr,
+                         int yop = FFT_YZ_ONE, bool xpass = true) {
     const cx<T> *ci = reinterpret_cast<const cx<T> *>(spec);
     cx<T> *ca = reinterpret_cast<cx<T> *>(bufA);
     cx<T> *cb = reinterpret_cast<cx<T> *>(bufB);
@@ -1126,9 +1127,16 @@ template <typename T, int N> struct FftN {
     ax.win0 = w0[0];
     ax.wlen = wl[0];
     ax.scale = T(1);
-    if (pre == PRE_ZA) strided<+1, PRE_ZA, POST_NONE, 2>(ax, N, st, hook, "fft_x_za_win");
-    else if (pre == PRE_TABLE) strided<+1, PRE_TABLE, POST_NONE, 2>(ax, N, st, hook, "fft_x_pre_win");
-    else strided<+1, PRE_NONE, POST_NONE, 2>(ax, N, st, hook, "fft_x_win");
+    if (!xpass) {
+      // bufA already holds this x pass (an earlier call on the same spectrum,
+      // operator and window)
+    } else if (pre == PRE_ZA) {
+      strided<+1, PRE_ZA, POST_NONE, 2>(ax, N, st, hook, "fft_x_za_win");
+    } else if (pre == PRE_TABLE) {
+      strided<+1, PRE_TABLE, POST_NONE, 2>(ax, N, st, hook, "fft_x_pre_win");
+    } else {
+      strided<+1, PRE_NONE, POST_NONE, 2>(ax, N, st, hook, "fft_x_win");
+    }
     // y pass: tiles (x_local, kz block), lines over ky, keep y in window; B = [wl0][wl1][NZ]
     StridedArgs<T> ay{};
     ay.in = ca;
@@ -1141,7 +1149,11 @@ template <typename T, int N> struct FftN {
     ay.win0 = w0[1];
     ay.wlen = wl[1];
     ay.scale = T(1);
-    strided<+1, PRE_NONE, POST_NONE, 3>(ay, wl[0], st, hook, "fft_y_win");
+    ay.comp = yop;
+    ay.kmin = kmin;
+    if (yop < FFT_YZ_ONE || yop > FFT_YZ_KZKZ) throw std::runtime_error("c2r_window: bad y factor");
+    if (yop == FFT_YZ_ONE) strided<+1, PRE_NONE, POST_NONE, 3>(ay, wl[0], st, hook, "fft_y_win");
+    else strided<+1, PRE_YZ, POST_NONE, 3>(ay, wl[0], st, hook, "fft_y_op_win");
     // z pass: rows (x_local, y_local), keep z in window
     long nrows = long(wl[0]) * wl[1];
     int nblk = int((nrows + ROWS - 1) / ROWS);
@@ -1306,10 +1318,11 @@ void fft_r2c_window(const T *win, int n, const int *w0, const int *wl, T *bufB, 
 }
 template <typename T>
 void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, double kmin, const int *w0, const int *wl,
-                    T *bufA, T *bufB, T *out, hipStream_t st, TimingHook *hook, const FilterInline *fil) {
+                    T *bufA, T *bufB, T *out, hipStream_t st, TimingHook *hook, const FilterInline *fil, int yop,
+                    bool xpass) {
   for (int a = 0; a < 3; ++a)
     if (wl[a] < 1 || wl[a] > n || w0[a] < 0 || w0[a] >= n) throw std::runtime_error("fft_c2r_window: bad window");
-  GIC_FFT_SWITCH(n, F::c2r_window(spec, pre, tab, comp, T(kmin), w0, wl, bufA, bufB, out, st, hook, fil))
+  GIC_FFT_SWITCH(n, F::c2r_window(spec, pre, tab, comp, T(kmin), w0, wl, bufA, bufB, out, st, hook, fil, yop, xpass))
 }
 
 #define GIC_FFT_INST(T)                                                                                     \
@@ -1333,7 +1346,7 @@ void fft_c2r_window(const T *spec, int n, int pre, const T *tab, int comp, doubl
   template void fft_r2c_window<T>(const T *, int, const int *, const int *, T *, T *, T *, const T *, bool,      \
                                   hipStream_t, TimingHook *, const T *, const FilterInline *, double);         \
   template void fft_c2r_window<T>(const T *, int, int, const T *, int, double, const int *, const int *, T *, T *, \
-                                  T *, hipStream_t, TimingHook *, const FilterInline *);
+                                  T *, hipStream_t, TimingHook *, const FilterInline *, int, bool);
 GIC_FFT_INST(float)
 GIC_FFT_INST(double)
 
