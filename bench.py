@@ -77,6 +77,11 @@ def sphere_cells_geom(n, dx, off, box0, centre, radius):
     return np.flatnonzero(inside.ravel())
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON result goes to stdout)."""
+    print("[bench] %s" % msg, file=sys.stderr, flush=True)
+
+
 def one_step(ic, seed, mods):
     ic.reseed(seed)
     ic.draw()
@@ -440,8 +445,10 @@ def main():
     if mods:
         finest = len(wl["zooms"])
         ic.set_flags(finest, sphere_cells(ic, finest, mods["centre"], mods["radius"]))
+    progress("context ready: %s, %d cells" % (args.workload, ncells))
     for w in range(args.warmup):
         one_step(ic, 1000 + w, mods)
+    progress("%d warm-up steps done" % args.warmup)
     ic.set_event_timing(True)
     ic.synchronize()
     t0 = time.perf_counter()
@@ -451,6 +458,7 @@ def main():
     t1 = time.perf_counter()
     timings = ic.event_timings(with_bytes=True)
     dt = (t1 - t0) / args.steps
+    progress("%d timed steps: %.3f ms per step" % (args.steps, dt * 1e3))
 
     # Roofline of the dominant kernel: of every timed launch that states its
     # algorithmic HBM bytes (each input element read once, each output
@@ -516,14 +524,20 @@ def main():
         dtn = (time.perf_counter() - t2) / nm_steps
         out["without_modifications"] = dict(value=round(ncells / dtn / 1e6, 2), unit="Mparticles/s",
                                             ms_per_step=round(dtn * 1e3, 3), steps=nm_steps)
+        progress("without modifications: %.3f ms per step" % (dtn * 1e3))
     ic.close()
     if not args.no_extras:
         out["throughput_256"] = single_level_throughput()
+        progress("single-level 256^3 done")
         out["adjoint_over_forward"] = adjoint_ratio()
+        progress("adjoint ratio done")
         out["batch"] = batch_throughput()
+        progress("batch done")
         out["psi_fp32_error"] = psi_precision(wl)
+        progress("FP32 displacement error done")
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_scale)
+        progress("CPU baseline done")
     print(json.dumps(out))
 
 
