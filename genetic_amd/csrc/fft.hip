@@ -492,6 +492,18 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
 // ---- z pass, r2c --------------------------------------------------------------
 // ROWS rows of N reals (row stride N+2) per block.  Complex FFT of length
 // M=N/2 on z_j = x_{2j} + i x_{2j+1}, then X_k = F_k + W^k G_k.
+// LDS row of one z-pass line (M + 1 complex): one pad element after every 16
+// points breaks the power-of-two strides of the Stockham stores, which
+// otherwise put up to 8 lanes of a wavefront on the same bank
+template <int M> struct ZRow {
+  static constexpr int PAD = M >= 64 ? 16 : 0;
+  static constexpr int len = M + 1 + (PAD ? M / PAD : 0);
+  __device__ __forceinline__ static int idx(int line, int pt) {
+    if constexpr (PAD) return line * len + pt + pt / PAD;
+    else return line * len + pt;
+  }
+};
+
 template <typename T, int N, int ROWS, int THREADS>
 __global__ __launch_bounds__(THREADS) void k_fft_z_r2c(T *__restrict__ data, long nrows, T scale,
                                                        const cx<T> *__restrict__ twN, const T *__restrict__ src) {
@@ -504,7 +516,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_r2c(T *__restrict__ data, lon
   const cx<T> *srows = src ? reinterpret_cast<const cx<T> *>(src) + row0 * M : rows;
   const long srs = src ? M : M + 1;
   auto gld = [&](int line, int pt) -> cx<T> { return srows[long(line) * srs + pt]; };
-  auto lidx = [](int line, int pt) { return line * (M + 1) + pt; };
+  auto lidx = [](int line, int pt) { return ZRow<M>::idx(line, pt); };
   auto lst = [&](int line, int pt, cx<T> v) { lds[lidx(line, pt)] = v; };
   if constexpr (M == 1) {
     if (threadIdx.x < ROWS) lds[lidx(threadIdx.x, 0)] = gld(threadIdx.x, 0);
@@ -545,7 +557,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_c2r(T *__restrict__ data, lon
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   long row0 = long(blockIdx.x) * ROWS;
   cx<T> *rows = reinterpret_cast<cx<T> *>(data) + row0 * (M + 1);
-  auto lidx = [](int line, int pt) { return line * (M + 1) + pt; };
+  auto lidx = [](int line, int pt) { return ZRow<M>::idx(line, pt); };
   constexpr int H = M / 2 + 1;
   for (int q = threadIdx.x; q < ROWS * H; q += THREADS) {
     int line = q / H, k = q - line * H;
@@ -615,7 +627,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_c2r_window(const T *__restric
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   long row0 = long(blockIdx.x) * ROWS;
   const cx<T> *rows = reinterpret_cast<const cx<T> *>(in) + row0 * (M + 1);
-  auto lidx = [](int line, int pt) { return line * (M + 1) + pt; };
+  auto lidx = [](int line, int pt) { return ZRow<M>::idx(line, pt); };
   constexpr int H = M / 2 + 1;
   for (int q = threadIdx.x; q < ROWS * H; q += THREADS) {
     int line = q / H, k = q - line * H;
@@ -682,7 +694,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_r2c_window(const T *__restric
     }
     return cx<T>{v[0], v[1]};
   };
-  auto lidx = [](int line, int pt) { return line * (M + 1) + pt; };
+  auto lidx = [](int line, int pt) { return ZRow<M>::idx(line, pt); };
   auto lst = [&](int line, int pt, cx<T> v) { lds[lidx(line, pt)] = v; };
   if constexpr (M == 1) {
     if (threadIdx.x < ROWS) lds[lidx(threadIdx.x, 0)] = gld(threadIdx.x, 0);
@@ -856,7 +868,7 @@ template <typename T, int N> struct FftN {
                     int nfields = 1) {
     long nrows = long(N) * N * nfields;
     int nblk = int(nrows / ROWS);
-    size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
+    size_t shm = sizeof(cx<T>) * ROWS * ZRow<M>::len;
     const cx<T> *tw = twtable<T>().get(N);
     std::string tag = std::string(name) + "." + std::to_string(N);
     if (hook) hook->bytes(tag.c_str(), double(nrows) * (2 * N + 2) * sizeof(T));   // N reals one way, N+2 the other
@@ -929,7 +941,7 @@ template <typename T, int N> struct FftN {
   static void r2c_boxes(const T *rows, long nrows, int nb, int wl, const int *winx, const int *winy, T *zbuf,
                         T *ybuf, T *spec, long spec_stride, const T *post, hipStream_t st, TimingHook *hook) {
     int nblk = int(nrows / ROWS);
-    size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
+    size_t shm = sizeof(cx<T>) * ROWS * ZRow<M>::len;
     const cx<T> *tw = twtable<T>().get(N);
     auto kern = k_fft_z_r2c<T, N, ROWS, THREADS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
@@ -1081,7 +1093,7 @@ template <typename T, int N> struct FftN {
   static void zpass_epi(T *out, int epi, T *S, T *A, hipStream_t st, TimingHook *hook) {
     long nrows = long(N) * N;
     int nblk = int(nrows / ROWS);
-    size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
+    size_t shm = sizeof(cx<T>) * ROWS * ZRow<M>::len;
     const cx<T> *tw = twtable<T>().get(N);
     std::string tag = "fft_z_c2r_lpt." + std::to_string(N);
     auto launch = [&](auto kern) {
@@ -1157,7 +1169,7 @@ template <typename T, int N> struct FftN {
     // z pass: rows (x_local, y_local), keep z in window
     long nrows = long(wl[0]) * wl[1];
     int nblk = int((nrows + ROWS - 1) / ROWS);
-    size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
+    size_t shm = sizeof(cx<T>) * ROWS * ZRow<M>::len;
     const cx<T> *tw = twtable<T>().get(N);
     auto kern = k_fft_z_c2r_window<T, N, ROWS, THREADS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
@@ -1180,7 +1192,7 @@ template <typename T, int N> struct FftN {
                          double kmin = 0) {
     long nrows = long(wl[0]) * wl[1];
     int nblk = int((nrows + ROWS - 1) / ROWS);
-    size_t shm = sizeof(cx<T>) * ROWS * (M + 1);
+    size_t shm = sizeof(cx<T>) * ROWS * ZRow<M>::len;
     const cx<T> *tw = twtable<T>().get(N);
     auto kern = k_fft_z_r2c_window<T, N, ROWS, THREADS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
