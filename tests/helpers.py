@@ -12,6 +12,18 @@ def golden(*parts):
     return os.path.join(GOLDEN, *parts)
 
 
+def golden_array(path):
+    """A fixture array: the .npy at path, or the same array kept compressed
+    as key "a" of the .npz beside it (the masks); None if neither exists."""
+    if os.path.exists(path):
+        return np.load(path, allow_pickle=False)
+    z = path[:-4] + ".npz" if path.endswith(".npy") else None
+    if z and os.path.exists(z):
+        with np.load(z, allow_pickle=False) as f:
+            return f["a"]
+    return None
+
+
 def parse_paramfile(path):
     """Commands of a paramfile as (name, [args]) with comments stripped."""
     cmds = []
