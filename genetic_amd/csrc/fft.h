@@ -30,6 +30,8 @@ enum { FFT_PRE_NONE = 0, FFT_PRE_TABLE = 1, FFT_PRE_ZA = 2, FFT_PRE_KK = 3 };
 // scattered): scale * prod_i f_i(k), f = LP, sqrt(1 - LP^2) or 1 - LP for
 // kind 0, 1, 2 with LP = 1 / (1 + exp((k - kc) / (0.1 kc))); over_k2 divides
 // by k^2 (0 at k = 0); zero makes every factor 0.  nf < 0: not given.
+// the most factors a FilterInline may hold (more: use a table)
+constexpr int FILTER_INLINE_MAX = 4;
 struct FilterInline {
   int nf = -1;
   int kind[8];

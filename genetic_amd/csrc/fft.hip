@@ -269,7 +269,11 @@ __device__ __forceinline__ float filter_fast_f(const FilterF &F, int ksq) {
   float fk = float(ksq);
   float k = __builtin_amdgcn_sqrtf(fk) * F.kmin;
   float v = F.scale;
-  for (int i = 0; i < F.nf; ++i) {
+  // a fixed bound with a uniform exit: the parameters are indexed by
+  // constants and stay in scalar registers
+#pragma unroll
+  for (int i = 0; i < FILTER_INLINE_MAX; ++i) {
+    if (i >= F.nf) break;
     float lp = __builtin_amdgcn_rcpf(1.f + __expf((k - F.kc[i]) * F.inv_w[i]));
     v *= F.kind[i] == 0 ? lp : F.kind[i] == 1 ? __builtin_amdgcn_sqrtf(fmaxf(1.f - lp * lp, 0.f)) : 1.f - lp;
   }
@@ -334,8 +338,18 @@ __device__ __forceinline__ void strided_tile(const StridedArgs<T> &a, const cx<T
                                              const TileG<T> &g, Ld gld) {
   const int col0 = g.col0, nc = g.nc, ky = g.ky, ky2 = g.ky2;
   cx<T> *ob = g.ob;
+  // the operator's parameters, copied out of the kernel arguments once per
+  // tile (read through a reference they were reloaded for every mode)
+  struct PreArgs {
+    const T *tab;
+    int comp, ncol, n;
+    T kmin;
+    double inv_k2, kmin_d;
+    FilterF ff;
+    float inv_k2_f;
+  };
   struct PreOp {
-    const StridedArgs<T> &a;
+    PreArgs a;
     int col0, ky, ky2;
     // the factor of |k|^2: the caller's table, its filter formed here (FP32),
     // or inv_k2 / (|k|^2 kMin^2) formed exactly as filter_table forms it;
@@ -386,7 +400,7 @@ __device__ __forceinline__ void strided_tile(const StridedArgs<T> &a, const cx<T
       }
     }
   };
-  PreOp op{a, col0, ky, ky2};
+  PreOp op{PreArgs{a.tab, a.comp, a.ncol, a.n, a.kmin, a.inv_k2, a.kmin_d, a.ff, a.inv_k2_f}, col0, ky, ky2};
   // POST_AUX: the spectrum it adds is fetched before the first stage, for the
   // outputs this thread stores in the last one (line q % C, points
   // q / C + s L / R), so that its latency overlaps the transform
@@ -807,7 +821,7 @@ template <typename T, int N> struct FftN {
     a.ff.nf = -1;
     if (sizeof(T) == 4) {
       if (a.fil.nf >= 0) {
-        if (a.fil.nf > 8) throw std::runtime_error("FilterInline: at most 8 factors");
+        if (a.fil.nf > FILTER_INLINE_MAX) throw std::runtime_error("FilterInline: at most 4 factors");
         a.ff.nf = a.fil.nf;
         a.ff.zero = a.fil.zero;
         a.ff.over_k2 = a.fil.over_k2;

@@ -834,7 +834,7 @@ template <typename T> const T *spec_table(gic_context *c, const LevelGeom &g, co
 // the same product for the transforms to form in their passes (FP32)
 FilterInline inline_of(const FilterSpec &s) {
   FilterInline f;
-  if (s.na + s.nb > 8) return f;   // nf = -1: use the table
+  if (s.na + s.nb > FILTER_INLINE_MAX) return f;   // nf = -1: use the table
   f.nf = 0;
   for (int i = 0; i < s.na; ++i) {
     f.kind[f.nf] = s.ka[i];
