@@ -322,15 +322,15 @@ template <typename T, int C> __device__ __forceinline__ TileG<T> tile_geom(const
 // input value at (line, point) of a tile: zero past the last column and, for
 // a pruned input, outside its window
 template <typename T, int L>
-__device__ __forceinline__ cx<T> tile_in(const StridedArgs<T> &a, const TileG<T> &g, int line, int pt) {
+__device__ __forceinline__ cx<T> tile_in(int in_wlen, long in_ls, const TileG<T> &g, int line, int pt) {
   if (line >= g.nc) return cx<T>{0, 0};
-  if (a.in_wlen > 0 && a.in_wlen < L) {
+  if (in_wlen > 0 && in_wlen < L) {
     int o = pt - g.iw0;
     if (o < 0) o += L;
-    if (o >= a.in_wlen) return cx<T>{0, 0};
-    return g.ib[o * a.in_ls + line];
+    if (o >= in_wlen) return cx<T>{0, 0};
+    return g.ib[o * in_ls + line];
   }
-  return g.ib[pt * a.in_ls + line];
+  return g.ib[pt * in_ls + line];
 }
 // One tile of a strided pass; ld supplies the first stage's inputs.
 template <typename T, int L, int C, int DIR, int THREADS, int PRE, int POST, class Ld>
@@ -433,57 +433,62 @@ __device__ __forceinline__ void strided_tile(const StridedArgs<T> &a, const cx<T
       }
     }
   }
+  // the store path's parameters, read once per tile
+  const int wlen = a.wlen, win0 = a.win0, accum = a.accum;
+  const long out_ls = a.out_ls;
+  const T scale = a.scale;
+  const cx<T> *aux = a.aux, *out0 = a.out;
   auto gst = [&](int line, int pt, cx<T> v, auto... slot) {
     if (line >= nc) return;
     int o = pt;
-    if (a.wlen < L) {
-      o = pt - a.win0;
+    if (wlen < L) {
+      o = pt - win0;
       if (o < 0) o += L;
-      if (o >= a.wlen) return;
+      if (o >= wlen) return;
     }
-    T s = a.scale;
+    T s = scale;
     if constexpr (POST == POST_TABLE) {
-      int kx = signed_k(pt, a.n), kz = col0 + line;
-      s *= kfactor<T>(a, kx * kx + ky2 + kz * kz);
+      int kx = signed_k(pt, op.a.n), kz = col0 + line;
+      s *= kfactor<T>(op.a, kx * kx + ky2 + kz * kz);
     }
     if constexpr (POST == POST_NONE || POST == POST_TABLE) {
-      if (a.accum) {
-        cx<T> &dst = ob[o * a.out_ls + line];
+      if (accum) {
+        cx<T> &dst = ob[o * out_ls + line];
         cx<T> prev = dst;
         dst = cx<T>{prev.x + v.x * s, prev.y + v.y * s};
         return;
       }
     }
     if constexpr (POST == POST_AUX) {
-      int kx = signed_k(pt, a.n), kz = col0 + line;
-      T f = kfactor<T>(a, kx * kx + ky2 + kz * kz);
+      int kx = signed_k(pt, op.a.n), kz = col0 + line;
+      T f = kfactor<T>(op.a, kx * kx + ky2 + kz * kz);
       cx<T> x;
       if constexpr (sizeof...(slot) == 2) {
         int bs[2] = {slot...};
         x = pre[bs[0]][bs[1]];
       } else {
-        x = a.aux[long(o) * a.out_ls + line + (ob - a.out)];
+        x = aux[long(o) * out_ls + line + (ob - out0)];
       }
-      ob[o * a.out_ls + line] = cx<T>{x.x * f + v.x * s, x.y * f + v.y * s};
+      ob[o * out_ls + line] = cx<T>{x.x * f + v.x * s, x.y * f + v.y * s};
       return;
     }
     if constexpr (POST == POST_ZA_ADJ) {
-      int kx = signed_k(pt, a.n), kz = col0 + line;
-      int kc = a.comp == 0 ? kx : a.comp == 1 ? ky : kz;
+      int kx = signed_k(pt, op.a.n), kz = col0 + line;
+      int kc = op.a.comp == 0 ? kx : op.a.comp == 1 ? ky : kz;
       cx<T> r{0, 0};
-      if (kc != a.n / 2) {
-        T f = s * a.tab[kx * kx + ky2 + kz * kz] * (T(kc) * a.kmin);
+      if (kc != op.a.n / 2) {
+        T f = s * op.a.tab[kx * kx + ky2 + kz * kz] * (T(kc) * op.a.kmin);
         r = cx<T>{v.y * f, -v.x * f};
       }
-      cx<T> &dst = ob[o * a.out_ls + line];
-      if (a.accum) {
+      cx<T> &dst = ob[o * out_ls + line];
+      if (accum) {
         cx<T> prev = dst;
         r = cx<T>{prev.x + r.x, prev.y + r.y};
       }
       dst = r;
       return;
     }
-    ob[o * a.out_ls + line] = cx<T>{v.x * s, v.y * s};
+    ob[o * out_ls + line] = cx<T>{v.x * s, v.y * s};
   };
   auto lidx = [](int line, int pt) { return pt * C + line; };
   if constexpr (PRE == PRE_NONE) {
@@ -499,8 +504,10 @@ __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const
   extern __shared__ char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
   TileG<T> g = tile_geom<T, C>(a, xcd_tile(blockIdx.x, gridDim.x));
-  strided_tile<T, L, C, DIR, THREADS, PRE, POST>(a, tw, lds, g,
-                                                 [&](int line, int pt) { return tile_in<T, L>(a, g, line, pt); });
+  const int in_wlen = a.in_wlen;   // read once, not per load
+  const long in_ls = a.in_ls;
+  strided_tile<T, L, C, DIR, THREADS, PRE, POST>(
+      a, tw, lds, g, [&](int line, int pt) { return tile_in<T, L>(in_wlen, in_ls, g, line, pt); });
 }
 
 // ---- z pass, r2c --------------------------------------------------------------
