@@ -269,6 +269,7 @@ struct LevelMap {
 struct Block {
   int level = 0;
   int factor = 1;   // the grid it is laid on: > 1 subsampled, < -1 supersampled by -factor
+  bool strays = false;   // on the level's box-covering grid (strays_on)
   bool gas = false;
   unsigned type = 1;
   std::vector<uint64_t> cells;
@@ -287,6 +288,12 @@ struct VLevel {
   Grid gu;             // the same before the offset (where fields are evaluated)
   bool centred = false;
   I3 shift;            // CenteredGrid offset, in this level's cells
+  // stray particles (ResolutionMatchingGrid): g covers the box at this
+  // level's resolution; cells in the window [wlo, wlo + wn) (wrapped) are the
+  // level's own, the others interpolated from the levels above
+  bool strays = false;
+  I3 wlo;
+  int wn = 0;
 };
 
 struct Generator {
@@ -398,6 +405,8 @@ struct Generator {
     if (drawn) throw std::runtime_error("Trying to initialize a grid after the random field was already drawn");
     if (grids.empty()) throw std::runtime_error("Cannot initialise a zoom grid before initialising the base grid");
     if (zoomfac < 1) throw std::runtime_error("Zoom factor must be positive");
+    if (allowStrays && grids.size() >= 2)
+      throw std::runtime_error("strays_on with more than one zoom level is not built here");
     Grid above = grids.back();
     std::vector<size_t> untrimmed = above.flags;
     int nAbove = above.n;
@@ -434,12 +443,17 @@ struct Generator {
         hi = std::nextafter(hin, INFINITY);
         in &= hi >= lo ? (p[d] < hi && p[d] >= lo) : (p[d] < hi || p[d] >= lo);
       }
-      if (in) trimmed.push_back(id);
-      else ++missed;
+      // strays_on keeps the particles outside the window too (ic.hpp:541-564)
+      if (in || allowStrays) trimmed.push_back(id);
+      if (!in) ++missed;
     }
     if (missed > 0) {
       log() << "WARNING: the requested zoom particles do not all fit in the requested zoom window" << std::endl;
-      log() << "         of " << untrimmed.size() << " particles, " << missed << " have been omitted" << std::endl;
+      if (allowStrays)
+        log() << "         of " << untrimmed.size() << " particles, " << missed
+              << " will be interpolated from LR grid (stray particle mode)" << std::endl;
+      else
+        log() << "         of " << untrimmed.size() << " particles, " << missed << " have been omitted" << std::endl;
       log() << "         to make a new zoom flag list of " << trimmed.size() << std::endl;
     }
     check(gic_init_zoom_grid_with_origin_at(c(), x0i, y0i, z0i, zoomfac, n));
@@ -507,7 +521,7 @@ struct Generator {
     for (size_t id : above.flags) zw.expandToInclude(above.coord(id));
     int nreq = zw.maxDim();
     int nuser = above.n / zoomfac;
-    if (nreq > nuser)
+    if (nreq > nuser && !allowStrays)
       throw std::runtime_error("Zoom particles do not fit in specified sub-box. Decrease zoom, or choose different particles");
     if (nreq < nuser) zw.expandSymmetricallyToSize(nuser);
     I3 aboveLo{int(std::llround(above.ox / above.dx)), int(std::llround(above.oy / above.dx)),
@@ -665,7 +679,50 @@ struct Generator {
     } else if (sup > 1) {
       f = -sup;
     }
-    return resampledLevel(l, f);
+    VLevel hi = resampledLevel(l, f);
+    if (!allowStrays || l == 0) return hi;
+    // strays_on: the zoom level's particles lie on a grid covering the box
+    // at its resolution (ResolutionMatchingGrid, virtualgrid.hpp:217-400,
+    // made by MultiLevelGrid::addLevel, multilevelgrid.hpp:130-134, over the
+    // base level; sub/supersampling applies to both of its grids): the
+    // level's own cells inside its window, interpolated cells outside
+    const Grid &g0 = grids[0];
+    VLevel r = hi;
+    r.strays = true;
+    r.g.n = int(std::llround(g0.periodic / hi.g.dx));
+    r.g.ox = g0.ox;
+    r.g.oy = g0.oy;
+    r.g.oz = g0.oz;
+    r.wlo = I3{int(std::llround((hi.g.ox - g0.ox) / hi.g.dx)), int(std::llround((hi.g.oy - g0.oy) / hi.g.dx)),
+               int(std::llround((hi.g.oz - g0.oz) / hi.g.dx))};
+    r.wn = hi.g.n;
+    r.gu = r.g;
+    return r;
+  }
+  // Cell i of a stray-covering level inside the level's window: its index on
+  // the level's own (resampled) grid, or SIZE_MAX outside the window
+  size_t strayInside(const VLevel &v, size_t i) const {
+    I3 c = v.g.coord(i);
+    int n = v.g.n;
+    int lx = ((c.x - v.wlo.x) % n + n) % n, ly = ((c.y - v.wlo.y) % n + n) % n, lz = ((c.z - v.wlo.z) % n + n) % n;
+    if (lx >= v.wn || ly >= v.wn || lz >= v.wn) return SIZE_MAX;
+    return (size_t(lx) * v.wn + ly) * v.wn + lz;
+  }
+  // Field value at cell i of an output level, f(l) giving real level l's
+  // field: the level's own value (through its sub/supersampling), or for a
+  // stray cell the tricubic interpolation of the level above at the cell's
+  // centroid (ResolutionMatchingEvaluator, evaluator.hpp:216-256, over the
+  // SuperSampleEvaluator of the base level)
+  template <class F> double evalOut(const VLevel &v, const VLevel &hi, F f, size_t i) const {
+    if (!v.strays) return v.factor == 1 ? f(v.src)[i] : evalVirtual(v, f(v.src), i);
+    size_t h = strayInside(v, i);
+    if (h != SIZE_MAX) return hi.factor == 1 ? f(v.src)[h] : evalVirtual(hi, f(v.src), h);
+    return interpolate(v.src - 1, v.g.centroid(i), f(v.src - 1));
+  }
+  // The output level a block of particles is laid on
+  VLevel blockLevel(const Block &b) const {
+    if (!b.strays) return resampledLevel(b.level, b.factor);
+    return outLv[b.level];
   }
   // Real level l subsampled (f > 1) or supersampled (f < -1) by |f|
   VLevel resampledLevel(int l, int f) const {
@@ -690,6 +747,16 @@ struct Generator {
   // flags every subcell of a flagged cell.
   std::vector<size_t> toOutputCells(const VLevel &v, const std::vector<size_t> &cells) const {
     std::vector<size_t> out;
+    if (v.strays) {   // the level's cells, shifted to their place on the box-covering grid
+      VLevel hi = resampledLevel(v.src, v.factor);
+      int n = v.g.n;
+      for (size_t c : toOutputCells(hi, cells)) {
+        I3 q = hi.g.coord(c);
+        out.push_back(v.g.index((q.x + v.wlo.x) % n, (q.y + v.wlo.y) % n, (q.z + v.wlo.z) % n));
+      }
+      std::sort(out.begin(), out.end());
+      return out;
+    }
     const Grid &rg = grids[v.src];
     if (v.factor == 1) {
       out = cells;
@@ -716,6 +783,16 @@ struct Generator {
   // flagCells): every real cell under a subsampled cell, the real cell of a
   // supersampled one.
   std::vector<size_t> toRealCells(const VLevel &v, const std::vector<size_t> &cells) const {
+    if (v.strays) {   // cells outside the window flag nothing on the level
+      VLevel hi = resampledLevel(v.src, v.factor);
+      std::vector<size_t> in;
+      for (size_t c : cells) {
+        size_t h = strayInside(v, c);
+        if (h != SIZE_MAX) in.push_back(h);
+      }
+      std::sort(in.begin(), in.end());
+      return toRealCells(hi, in);
+    }
     if (v.factor == 1) return cells;
     std::vector<size_t> out;
     const Grid &rg = grids[v.src];
@@ -1141,6 +1218,7 @@ struct Generator {
       Block &b = dm[l];
       b.level = int(l);
       b.factor = outLv[l].factor;
+      b.strays = outLv[l].strays;
       b.type = grids[l].gadgetType;
       b.cells = std::move(order[l]);
       setMassEps(b);
@@ -1165,6 +1243,7 @@ struct Generator {
     }
     std::vector<Block> out;
     if (Ob > 0) {
+      if (allowStrays && nl > 1) throw std::runtime_error("strays_on with baryons is not built here");
       VLevel gv = resampledLevel(int(nl - 1), supersampleGas > 1 ? -supersampleGas : 1);
       Block g;
       g.level = int(nl - 1);
@@ -1209,23 +1288,29 @@ struct Generator {
       b.pos.resize(3 * m);
       b.vel.resize(3 * m);
       if (!m) continue;
-      if (b.factor == 1) {
+      if (b.factor == 1 && !b.strays) {
         bool identity = m == grids[b.level].n3();
         for (size_t i = 0; identity && i < m; ++i) identity = b.cells[i] == i;
         check(gic_particles(c(), b.level, identity ? nullptr : b.cells.data(), 0, m, b.pos.data(), b.vel.data()));
       } else {
         if (velRatio == 0) check(gic_velocity_ratio(c(), &velRatio));
-        VLevel v = resampledLevel(b.level, b.factor);
-        std::vector<std::vector<double>> psi;
-        for (int k = 1; k <= (second ? 6 : 3); ++k) psi.push_back(levelField(k, b.level));
+        VLevel v = blockLevel(b), hi = resampledLevel(b.level, b.factor);
+        // the displacement fields of the level (and of the base level for
+        // stray cells), fetched before the parallel loop
+        std::map<int, std::vector<std::vector<double>>> psi;
+        for (int l : {b.level, b.strays ? b.level - 1 : b.level})
+          if (!psi.count(l))
+            for (int k = 1; k <= (second ? 6 : 3); ++k) psi[l].push_back(levelField(k, l));
 #pragma omp parallel for schedule(static)
         for (size_t i = 0; i < m; ++i) {
           size_t id = b.cells[i];
           auto cen = v.gu.centroid(id);
           for (int a3 = 0; a3 < 3; ++a3) {
-            double p = evalVirtual(v, psi[a3], id), vel = p * velRatio;
+            double p = evalOut(v, hi, [&](int l) -> const std::vector<double> & { return psi.at(l)[a3]; }, id);
+            double vel = p * velRatio;
             if (second) {
-              double p2 = evalVirtual(v, psi[3 + a3], id);
+              double p2 =
+                  evalOut(v, hi, [&](int l) -> const std::vector<double> & { return psi.at(l)[3 + a3]; }, id);
               p += p2;
               vel += 2 * velRatio * p2;
             }
@@ -1584,6 +1669,7 @@ struct Generator {
   // cell shift (CenteredGrid, virtualgrid.hpp:688-800) and the others are
   // offset by the same distance (OffsetGrid, virtualgrid.hpp:806-840).
   int subsample = 1, supersample = 1, supersampleGas = 1, lptOrder = 1;
+  bool allowStrays = false;   // strays_on (ic.hpp:257-260)
   double pvarValue = 1.0;
   bool centerOnTargetRegion = false;
 
@@ -1618,8 +1704,26 @@ struct Generator {
   // level that covers the box and repeating edge values otherwise.
   double evalVirtual(const VLevel &v, const std::vector<double> &f, size_t i) const {
     const Grid &rg = grids[v.src];
-    if (v.factor < -1) {
-      auto p = v.gu.centroid(i);
+    if (v.factor < -1) return interpolate(v.src, v.gu.centroid(i), f);
+    if (v.factor <= 1) return f[i];
+    I3 c0 = v.g.coord(i);
+    double sum = 0;
+    int cnt = 0;
+    int fct = v.factor;
+    for (int x = c0.x * fct; x < std::min(c0.x * fct + fct, rg.n); ++x)
+      for (int y = c0.y * fct; y < std::min(c0.y * fct + fct, rg.n); ++y)
+        for (int z = c0.z * fct; z < std::min(c0.z * fct + fct, rg.n); ++z) {
+          sum += f[rg.index(x, y, z)];
+          ++cnt;
+        }
+    return sum / cnt;
+  }
+  // Field f of real level l at point p: Field::evaluateInterpolated
+  // (field.hpp:483-508) with the tricubic of field.hpp:598-617, wrapping on a
+  // level that covers the box and repeating edge values otherwise.
+  double interpolate(int l, const std::array<double, 3> &p, const std::vector<double> &f) const {
+    const Grid &rg = grids[l];
+    {
       double o[3] = {rg.ox, rg.oy, rg.oz};
       int p0[3];
       double w[3][4];
@@ -1651,18 +1755,6 @@ struct Generator {
                    f[rg.index(fix(p0[0] + a - 1), fix(p0[1] + b - 1), fix(p0[2] + cc - 1))];
       return sum;
     }
-    if (v.factor <= 1) return f[i];
-    I3 c0 = v.g.coord(i);
-    double sum = 0;
-    int cnt = 0;
-    int fct = v.factor;
-    for (int x = c0.x * fct; x < std::min(c0.x * fct + fct, rg.n); ++x)
-      for (int y = c0.y * fct; y < std::min(c0.y * fct + fct, rg.n); ++y)
-        for (int z = c0.z * fct; z < std::min(c0.z * fct + fct, rg.n); ++z) {
-          sum += f[rg.index(x, y, z)];
-          ++cnt;
-        }
-    return sum / cnt;
   }
 
   std::vector<VLevel> graficStack(std::array<double, 3> centre) const {
@@ -2076,12 +2168,13 @@ struct Generator {
     f.write((const char *)&len, 4);
     std::vector<std::vector<double>> delta(grids.size());
     for (auto &b : blocks) {
-      auto &d = delta[b.level];
-      if (d.empty()) d = levelField(GIC_FIELD_OVERDENSITY, b.level);
-      VLevel v = resampledLevel(b.level, b.factor);
+      for (int l : {b.level, b.strays ? b.level - 1 : b.level})
+        if (delta[l].empty()) delta[l] = levelField(GIC_FIELD_OVERDENSITY, l);
+      VLevel v = blockLevel(b), hi = resampledLevel(b.level, b.factor);
+      auto fl = [&](int l) -> const std::vector<double> & { return delta[l]; };
       std::vector<float> vals(b.count());
 #pragma omp parallel for schedule(static)
-      for (size_t i = 0; i < vals.size(); ++i) vals[i] = float(b.factor == 1 ? d[b.cells[i]] : evalVirtual(v, d, b.cells[i]));
+      for (size_t i = 0; i < vals.size(); ++i) vals[i] = float(evalOut(v, hi, fl, b.cells[i]));
       f.write((const char *)vals.data(), vals.size() * 4);
     }
   }
@@ -2268,6 +2361,7 @@ struct Dispatcher {
     add("subsample", std::function<void(int)>([this](int f) { ic.setSubsample(f); }));
     add("supersample", std::function<void(int)>([this](int f) { ic.setSupersample(f); }));
     add("supersample_gas", std::function<void(int)>([this](int f) { ic.setSupersampleGas(f); }));
+    add("strays_on", std::function<void()>([this]() { ic.allowStrays = true; }));
     add("pvar", std::function<void(double)>([this](double v) { ic.pvarValue = v; }));
     add("center_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));
     add("centre_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));

@@ -351,3 +351,26 @@ def test_double_zoom_supersampling_keeps_the_coarse_levels(tmp_path):
     fa, fb = a["vel"][ncoarse:], b["vel"][ncoarse:]
     np.testing.assert_allclose(fa.mean(axis=0), fb.mean(axis=0), atol=0.02 * fb.std())
     np.testing.assert_allclose(fa.std(axis=0), fb.std(axis=0), rtol=0.05)
+
+
+@pytest.mark.parametrize("test", [
+    "test_06a_stray_particles",
+    "test_06b_stray_supersample",
+    "test_06c_stray_subsampling",
+    "test_06d_stray_wrapping_zoom",
+])
+def test_stray_particles_match_reference(tmp_path, test):
+    """strays_on: zoom particles outside the zoom window are kept, laid on
+    the zoom level's box-covering grid and interpolated from the base level
+    (06a; 06b supersampled; 06c with a subsampled base, gadget; 06d a zoom
+    window that wraps), at compare.py's tolerances, and the dump_tipsy
+    overdensity arrays of 06a and 06b to 5 decimals."""
+    d, err = _run(test, tmp_path)
+    # 06c's flagged region sits far from the zoom window's edges: no strays
+    assert ("will be interpolated from LR grid (stray particle mode)" in err) == (test != "test_06c_stray_subsampling")
+    compare_particles(particle_file(d), golden(test, "reference_output"))
+    if os.path.exists(golden(test, "reference_output.overdensity")):
+        got = read_tipsy_array(os.path.join(d, "test_6.tipsy.overdensity"))
+        want = read_tipsy_array(golden(test, "reference_output.overdensity"))
+        assert got.shape == want.shape
+        np.testing.assert_almost_equal(got, want, decimal=5)
