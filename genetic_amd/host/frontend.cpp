@@ -270,6 +270,7 @@ struct Block {
   int level = 0;
   int factor = 1;   // the grid it is laid on: > 1 subsampled, < -1 supersampled by -factor
   bool strays = false;   // on the level's box-covering grid (strays_on)
+  int chain = -1;        // the mapper chain entry it is laid on (-1: the gas copy)
   bool gas = false;
   unsigned type = 1;
   std::vector<uint64_t> cells;
@@ -307,7 +308,10 @@ struct Generator {
   std::vector<Grid> grids;
   std::vector<std::vector<size_t>> zoomParticles;   // zoomParticleArray: per level >= 1, cells of level-1
   std::vector<LevelMap> maps;
-  std::vector<VLevel> outLv;   // each level's particle grid (its own, or sub/supersampled)
+  // the mapper's chain of particle grids, coarse to fine: each level's own
+  // (or sub/supersampled, or stray-covering) grid, with autopad's padding
+  // grids between levels; maps[k] (k >= 1) relates entry k to entries < k
+  std::vector<VLevel> outLv;
   double x0 = 0, y0 = 0, z0 = 0;
   Fmt fmt = Fmt::unknown;
   std::string outdir = "./", outname;
@@ -477,7 +481,7 @@ struct Generator {
       log() << "  Total particles       = " << total << std::endl;
     } else {
       buildMapper();
-      log() << "  Total particles       = " << mapperSize(int(grids.size()) - 1) << std::endl;
+      log() << "  Total particles       = " << mapperSize(int(outLv.size()) - 1) << std::endl;
     }
     // flag the fine cells under the flagged coarse cells (makeProxyGridToMatch:
     // supersample, then the section at the fine grid's offset, wrapped)
@@ -581,8 +585,8 @@ struct Generator {
   }
   std::vector<size_t> flaggedParticles() const {
     std::vector<std::vector<size_t>> f;
-    for (size_t l = 0; l < grids.size(); ++l) f.push_back(toOutputCells(outLv[l], grids[l].flags));
-    return flaggedParticles(int(grids.size()) - 1, f);
+    for (auto &v : outLv) f.push_back(toOutputCells(v, grids[v.src].flags));
+    return flaggedParticles(int(outLv.size()) - 1, f);
   }
 
   // flagParticles (twolevelmapper.hpp:357-466); ids ascending
@@ -590,7 +594,7 @@ struct Generator {
     if (L == 0) {
       for (size_t id : ids)
         if (id >= outLv[0].g.n3()) throw std::runtime_error("Particle ID out of range");
-      grids[0].flagCells(toRealCells(outLv[0], ids));
+      grids[outLv[0].src].flagCells(toRealCells(outLv[0], ids));
       return;
     }
     const LevelMap &m = maps[L];
@@ -629,7 +633,7 @@ struct Generator {
     std::sort(lr.begin(), lr.end());
     std::sort(hr.begin(), hr.end());
     flagParticles(L - 1, lr);
-    grids[L].flagCells(toRealCells(outLv[L], hr));
+    grids[outLv[L].src].flagCells(toRealCells(outLv[L], hr));
   }
 
   // TwoLevelParticleMapper constructor (twolevelmapper.hpp:270-320), per level
@@ -721,8 +725,7 @@ struct Generator {
   }
   // The output level a block of particles is laid on
   VLevel blockLevel(const Block &b) const {
-    if (!b.strays) return resampledLevel(b.level, b.factor);
-    return outLv[b.level];
+    return b.chain >= 0 ? outLv[b.chain] : resampledLevel(b.level, b.factor);
   }
   // Real level l subsampled (f > 1) or supersampled (f < -1) by |f|
   VLevel resampledLevel(int l, int f) const {
@@ -834,14 +837,21 @@ struct Generator {
   }
 
   void buildMapper() {
-    outLv.clear();
-    for (size_t l = 0; l < grids.size(); ++l) outLv.push_back(outputLevel(int(l), supersample, subsample));
-    maps.assign(grids.size(), LevelMap());
-    for (size_t L = 1; L < grids.size(); ++L) {
+    // the chain: one entry per level, each level's zoomed cells read through
+    // the output grid of the level above
+    outLv.assign(1, outputLevel(0, supersample, subsample));
+    std::vector<std::vector<size_t>> zc(1);   // zc[k]: cells of entry k-1 that entry k replaces
+    for (size_t l = 1; l < grids.size(); ++l) {
+      zc.push_back(toOutputCells(outLv.back(), zoomParticles[l - 1]));
+      outLv.push_back(outputLevel(int(l), supersample, subsample));
+    }
+    if (autopad > 0 && grids.size() > 1) insertPadding(zc);
+    size_t K = outLv.size();
+    maps.assign(K, LevelMap());
+    for (size_t L = 1; L < K; ++L) {
       LevelMap &m = maps[L];
       const Grid &gc = outLv[L - 1].g, &gf = outLv[L].g;
-      // the zoomed cells of the level above, read through its output grid
-      m.cells = toOutputCells(outLv[L - 1], zoomParticles[L - 1]);
+      m.cells = zc[L];
       std::vector<std::vector<size_t>> f(L);
       f[L - 1] = m.cells;
       m.particles = flaggedParticles(int(L) - 1, f);
@@ -858,11 +868,88 @@ struct Generator {
     }
   }
 
+  // The supersampled-by-2 version of an output level (makeSupersampled,
+  // virtualgrid.hpp:176-200,610-630: a subsampled level is un-subsampled
+  // first)
+  VLevel superBy2(const VLevel &c) const {
+    if (c.strays) throw std::runtime_error("autopad with strays_on is not built here");
+    int f = c.factor;
+    int nf = f == 1 ? -2 : f < -1 ? 2 * f : (f == 2 ? 1 : f / 2);
+    if (f > 2 && f % 2) throw std::runtime_error("Ratio is not an integer");
+    return resampledLevel(c.src, nf);
+  }
+  // The 8 subcells on p (c supersampled by 2) of cells of c
+  static std::vector<size_t> subcells2(const Grid &c, const Grid &p, const std::vector<size_t> &cells) {
+    std::vector<size_t> out;
+    out.reserve(cells.size() * 8);
+    for (size_t id : cells) {
+      I3 q = c.coord(id);
+      for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j)
+          for (int k = 0; k < 2; ++k) out.push_back(p.index(2 * q.x + i, 2 * q.y + j, 2 * q.z + k));
+    }
+    std::sort(out.begin(), out.end());
+    return out;
+  }
+  // Grid::getFlaggedCellsSize (grid.hpp:318-329) of a cell list
+  static int cellsSize(const Grid &g, const std::vector<size_t> &cells) {
+    if (cells.empty()) return 0;
+    Window w(g.simEquivalent(), g.coord(cells[0]));
+    for (size_t id : cells) w.expandToInclude(g.coord(id));
+    return w.maxDim();
+  }
+  // autopad (TwoLevelParticleMapper::insertIntermediateResolutionPadding,
+  // twolevelmapper.hpp:620-735, with ratio 2): where a level refines the one
+  // above by more than 2, the coarse cells it replaces, grown by autopad
+  // cells (and kept to cells that are particles), are replaced by the coarse
+  // level supersampled by 2; its cells under the zoom are replaced by the
+  // fine level, padding again until the steps are 2.
+  void insertPadding(std::vector<std::vector<size_t>> &zc) {
+    std::vector<VLevel> ch{outLv[0]};
+    std::vector<std::vector<size_t>> z(1);
+    for (size_t k = 1; k < outLv.size(); ++k) {
+      const VLevel &F = outLv[k];
+      std::vector<size_t> Z = zc[k];
+      // the coarse entry's cells that are particles: all of the base, the
+      // zoom cubes of a finer entry
+      bool all = ch.size() == 1;
+      std::vector<size_t> access;
+      if (!all) {
+        zoomCubes(ch[ch.size() - 2].g, z.back(), ch.back().g, access);
+        std::sort(access.begin(), access.end());
+      }
+      while (ratioInt(ch.back().g.dx, F.g.dx) > 2) {
+        VLevel C = ch.back();
+        VLevel P = superBy2(C);
+        std::vector<size_t> S = subcells2(C.g, P.g, Z);
+        std::vector<size_t> E = Z;
+        dilate(E, C.g.n, int(autopad));
+        if (!all) {
+          std::vector<size_t> keep;
+          std::set_intersection(E.begin(), E.end(), access.begin(), access.end(), std::back_inserter(keep));
+          E.swap(keep);
+        }
+        log() << "Added a padding region of effective resolution " << P.g.simEquivalent() << " of physical size ~"
+              << cellsSize(C.g, E) * C.g.dx << "Mpc/h (inner resolution " << F.g.simEquivalent() << ", size ~"
+              << cellsSize(P.g, S) * P.g.dx << "Mpc/h)" << std::endl;
+        z.push_back(E);
+        ch.push_back(P);
+        access = subcells2(C.g, P.g, E);
+        all = false;
+        Z = S;
+      }
+      z.push_back(Z);
+      ch.push_back(F);
+    }
+    outLv = ch;
+    zc = z;
+  }
+
   // Cells of each level in output order: each two-level mapper emits the
   // coarse mapper's particles minus the replaced ones, then the zoom cubes
   // (twolevelmapper.hpp:252-319,869-892).
   std::vector<std::vector<uint64_t>> levelOrder() const {
-    size_t nl = grids.size();
+    size_t nl = outLv.size();
     std::vector<std::vector<uint64_t>> lists(1);
     lists[0].resize(outLv[0].g.n3());
     std::iota(lists[0].begin(), lists[0].end(), uint64_t(0));
@@ -950,7 +1037,7 @@ struct Generator {
       graficFlagParticles(ids);
     } else {
       buildMapper();
-      flagParticles(int(grids.size()) - 1, ids);
+      flagParticles(int(outLv.size()) - 1, ids);
     }
     centreOnFlagged();
   }
@@ -1119,6 +1206,28 @@ struct Generator {
   // expandFlaggedRegion (ic.hpp:1527-1540, grid.hpp:282-300): n times, each
   // flagged cell's neighbours along z, then y, then x join the flags
   // (indices wrapped around the level's grid)
+  // Grid::expandFlaggedRegion (grid.hpp:282-300): nCells times a step of
+  // one cell both ways along z, then y, then x, wrapping within the grid
+  static void dilate(std::vector<size_t> &flags, long n, int nCells) {
+    for (int it = 0; it < nCells; ++it)
+      for (int d = 2; d >= 0; --d) {
+        size_t old = flags.size();
+        std::vector<size_t> f(old * 3);
+        for (size_t i = 0; i < old; ++i) {
+          size_t id = flags[i];
+          long c[3] = {long(id / size_t(n * n)), long((id / size_t(n)) % size_t(n)), long(id % size_t(n))};
+          f[i] = id;
+          for (int k = 0; k < 2; ++k) {
+            long cc[3] = {c[0], c[1], c[2]};
+            cc[d] = ((cc[d] + (k == 0 ? 1 : -1)) % n + n) % n;
+            f[i + old * (k + 1)] = size_t((cc[0] * n + cc[1]) * n + cc[2]);
+          }
+        }
+        std::sort(f.begin(), f.end());
+        f.erase(std::unique(f.begin(), f.end()), f.end());
+        flags = f;
+      }
+  }
   void expandFlaggedRegion(int nCells) {
     if (grids.empty()) throw std::runtime_error("No grid has yet been initialised");
     if (nCells < 0) throw std::runtime_error("expand_flagged_region takes a number of cells");
@@ -1126,28 +1235,42 @@ struct Generator {
     for (size_t l = 0; l < grids.size(); ++l) {
       Grid &g = grids[l];
       size_t initial = g.flags.size();
-      const long n = g.n;
-      for (int it = 0; it < nCells; ++it)
-        for (int d = 2; d >= 0; --d) {
-          size_t old = g.flags.size();
-          std::vector<size_t> f(old * 3);
-          for (size_t i = 0; i < old; ++i) {
-            size_t id = g.flags[i];
-            long c[3] = {long(id / size_t(n * n)), long((id / size_t(n)) % size_t(n)), long(id % size_t(n))};
-            f[i] = id;
-            for (int k = 0; k < 2; ++k) {
-              long cc[3] = {c[0], c[1], c[2]};
-              cc[d] = ((cc[d] + (k == 0 ? 1 : -1)) % n + n) % n;
-              f[i + old * (k + 1)] = size_t((cc[0] * n + cc[1]) * n + cc[2]);
-            }
-          }
-          std::sort(f.begin(), f.end());
-          f.erase(std::unique(f.begin(), f.end()), f.end());
-          g.flags = f;
-        }
+      dilate(g.flags, g.n, nCells);
       log() << "  - level " << l << " increased number of flagged cells by " << g.flags.size() - initial << " (now "
             << g.flags.size() << ")" << std::endl;
     }
+  }
+  // adaptMask (ic.hpp:1548-1576): from the second-finest level up, the cells
+  // each level replaces on the level above become the parents of the cells
+  // it has zoomed, padded by nPad cells (the flags themselves are untouched)
+  void adaptMask(int nPad) {
+    if (grids.size() < 3) throw std::runtime_error("Adapting the mask requires there to be more than one zoom level");
+    if (nPad < 0) throw std::runtime_error("adapt_mask takes a number of cells");
+    for (size_t level = grids.size() - 2; level > 0; --level) {
+      const Grid &g = grids[level], &ga = grids[level - 1];
+      int r = ratioInt(ga.dx, g.dx);
+      long sim = g.simEquivalent();
+      long off[3] = {std::lround((g.ox - ga.ox) / g.dx), std::lround((g.oy - ga.oy) / g.dx),
+                     std::lround((g.oz - ga.oz) / g.dx)};
+      // makeProxyGridToMatch (grid.hpp:195-228): the section of this level
+      // over the level above, then subsampled to its cells
+      std::vector<size_t> parents;
+      for (size_t id : zoomParticles[level]) {
+        I3 q = g.coord(id);
+        long s3[3] = {q.x + off[0], q.y + off[1], q.z + off[2]};
+        bool in = true;
+        for (long &v : s3) {
+          v = (v % sim + sim) % sim;
+          in &= v < long(ga.n) * r;
+        }
+        if (in) parents.push_back(ga.index(int(s3[0] / r), int(s3[1] / r), int(s3[2] / r)));
+      }
+      std::sort(parents.begin(), parents.end());
+      parents.erase(std::unique(parents.begin(), parents.end()), parents.end());
+      dilate(parents, ga.n, nPad);
+      zoomParticles[level - 1] = parents;
+    }
+    refreshMapper();
   }
   void zeroLevel(int level) {
     log() << "*** Warning: your script calls zeroLevel(" << level << "). This is intended for testing purposes only!"
@@ -1211,33 +1334,37 @@ struct Generator {
   std::vector<Block> layoutBlocks(bool gasFirst) {
     buildMapper();
     auto order = levelOrder();
-    size_t nl = grids.size();
-    std::vector<Block> dm(nl);
+    size_t nl = grids.size(), K = outLv.size();
+    std::vector<Block> dm(K);
     double massRatio = Ob / Om;
-    for (size_t l = 0; l < nl; ++l) {
-      Block &b = dm[l];
-      b.level = int(l);
-      b.factor = outLv[l].factor;
-      b.strays = outLv[l].strays;
-      b.type = grids[l].gadgetType;
-      b.cells = std::move(order[l]);
+    for (size_t k = 0; k < K; ++k) {
+      // a padding entry takes the type of the level it supersamples
+      // (getGadgetParticleTypeForFinestGrid, twolevelmapper.hpp:700-702)
+      const VLevel &v = outLv[k];
+      Block &b = dm[k];
+      b.level = v.src;
+      b.chain = int(k);
+      b.factor = v.factor;
+      b.strays = v.strays;
+      b.type = grids[v.src].gadgetType;
+      b.cells = std::move(order[k]);
       setMassEps(b);
-      if (Ob > 0 && l == nl - 1) b.mass *= 1.0 - massRatio;
+      if (Ob > 0 && v.src == int(nl) - 1) b.mass *= 1.0 - massRatio;
     }
     if (flaggedTypeOn) {
       if (Ob > 0) throw std::runtime_error("gadget_flagged_particle_type with baryons is not built here");
       // the finest block keeps its unflagged cells in order; the flagged ones
       // follow as their own block, on the same (possibly resampled) grid
-      auto flaggedOut = toOutputCells(outLv[nl - 1], flaggedTypeCells);
-      Block f = dm[nl - 1];
+      auto flaggedOut = toOutputCells(outLv[K - 1], flaggedTypeCells);
+      Block f = dm[K - 1];
       f.type = flaggedType;
       std::vector<uint64_t> keep, moved;
-      for (uint64_t cell : dm[nl - 1].cells) {
+      for (uint64_t cell : dm[K - 1].cells) {
         if (std::binary_search(flaggedOut.begin(), flaggedOut.end(), size_t(cell))) moved.push_back(cell);
         else keep.push_back(cell);
       }
       std::sort(moved.begin(), moved.end());
-      dm[nl - 1].cells = std::move(keep);
+      dm[K - 1].cells = std::move(keep);
       f.cells = std::move(moved);
       dm.push_back(std::move(f));
     }
@@ -1670,6 +1797,7 @@ struct Generator {
   // offset by the same distance (OffsetGrid, virtualgrid.hpp:806-840).
   int subsample = 1, supersample = 1, supersampleGas = 1, lptOrder = 1;
   bool allowStrays = false;   // strays_on (ic.hpp:257-260)
+  size_t autopad = 0;         // autopad (ic.hpp:273-277): padding cells around each zoom
   double pvarValue = 1.0;
   bool centerOnTargetRegion = false;
 
@@ -2362,6 +2490,12 @@ struct Dispatcher {
     add("supersample", std::function<void(int)>([this](int f) { ic.setSupersample(f); }));
     add("supersample_gas", std::function<void(int)>([this](int f) { ic.setSupersampleGas(f); }));
     add("strays_on", std::function<void()>([this]() { ic.allowStrays = true; }));
+    add("adapt_mask", std::function<void(int)>([this](int n) { ic.adaptMask(n); }));
+    add("autopad", std::function<void(int)>([this](int n) {
+      if (n < 0) throw std::runtime_error("autopad takes a number of cells");
+      ic.autopad = size_t(n);
+      ic.refreshMapper();
+    }));
     add("pvar", std::function<void(double)>([this](double v) { ic.pvarValue = v; }));
     add("center_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));
     add("centre_output", std::function<void()>([this]() { ic.centerOnTargetRegion = true; }));

@@ -374,3 +374,42 @@ def test_stray_particles_match_reference(tmp_path, test):
         want = read_tipsy_array(golden(test, "reference_output.overdensity"))
         assert got.shape == want.shape
         np.testing.assert_almost_equal(got, want, decimal=5)
+
+
+@pytest.mark.parametrize("test", [
+    "test_10i_gadget_zoom_ptype_autopad",
+    "test_18_intermediate_levels",
+])
+def test_autopad_matches_reference(tmp_path, test):
+    """autopad: a layer of the coarse level supersampled by 2 around each zoom
+    that refines by more than 2 (10i: gadget types and gas; 18: two zooms by 4
+    under a subsampled base, with adapt_mask), at compare.py's tolerances."""
+    d, err = _run(test, tmp_path)
+    assert "Added a padding region of effective resolution" in err
+    compare_particles(particle_file(d), golden(test, "reference_output"))
+
+
+def test_adapt_mask_two_zooms(tmp_path):
+    """test_01f (run-only in the reference): adapt_mask 1 redraws the base
+    level's zoom cells as the parents of the second level's zoomed cells plus
+    one cell; the finest level's particles are those of the same run without
+    it, the two coarser levels trade particles, and the masses keep their
+    ratios 8 and 27."""
+    test = "test_01f_two_zooms_adapt_mask"
+    d, _ = _run(test, tmp_path)
+    text = open(os.path.join(d, "paramfile.txt")).read()
+    assert "adapt_mask 1" in text
+    dp = stage_test(test, os.path.join(str(tmp_path), "plain"), text.replace("adapt_mask 1", ""))
+    rc, err = run_frontend(dp)
+    assert rc == 0, err
+    a = read_tipsy(os.path.join(d, "test_1.tipsy"))["dark"]
+    b = read_tipsy(os.path.join(dp, "test_1.tipsy"))["dark"]
+    ma, mb = np.unique(a["mass"]), np.unique(b["mass"])
+    assert len(ma) == 3 and len(mb) == 3
+    np.testing.assert_allclose(ma[2] / ma[1], 27, rtol=1e-5)
+    np.testing.assert_allclose(ma[1] / ma[0], 8, rtol=1e-5)
+    fa, fb = a[a["mass"] == ma[0]], b[b["mass"] == mb[0]]
+    assert len(fa) == len(fb)
+    for k in ("pos", "vel", "eps"):
+        np.testing.assert_array_equal(fa[k], fb[k])
+    assert np.sum(a["mass"] == ma[2]) != np.sum(b["mass"] == mb[2])
