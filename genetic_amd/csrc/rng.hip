@@ -624,13 +624,24 @@ __global__ __launch_bounds__(64) void k_rng_emit(const uint32_t *__restrict__ ra
     __syncthreads();
   };
   const int ch = kc * 24, rawb = (kc + 1) * 24;
+  // the next block's numbers are loaded while this block is parsed (every
+  // block q <= kc of a chunk lies inside the raw buffer)
+  uint32_t nxt[24];
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < 24; ++k) nxt[k] = raw[raw_at(t, 0, k, rawb)];
+  }
   for (int q = 0; q <= kc; ++q) {
     bool busy = active && pos < ch && g < total;
     if (!__any(busy)) break;
     if (busy) {
 #pragma unroll
-      for (int k = 0; k < 24; ++k) ring.put(avail + k, raw[raw_at(t, q, k, rawb)]);
+      for (int k = 0; k < 24; ++k) ring.put(avail + k, nxt[k]);
       avail += 24;
+      if (q < kc) {
+#pragma unroll
+        for (int k = 0; k < 24; ++k) nxt[k] = raw[raw_at(t, q + 1, k, rawb)];
+      }
       while (pos < ch && pos + 4 <= avail && g < total) {
         bool acc;
         double v;
