@@ -503,11 +503,16 @@ template <typename T, int L, int C, int DIR, int THREADS, int PRE, int POST, int
 __global__ __launch_bounds__(THREADS) void k_fft_strided(StridedArgs<T> a, const cx<T> *__restrict__ tw) {
   extern __shared__ char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  // the L twiddles in LDS after the tile: the stages after the first read
+  // them there instead of through the vector memory path (the first stage
+  // needs none, and the barrier after it publishes the copy)
+  cx<T> *ltw = lds + C * L;
+  for (int i = int(threadIdx.x); i < L; i += THREADS) ltw[i] = tw[i];
   TileG<T> g = tile_geom<T, C>(a, xcd_tile(blockIdx.x, gridDim.x));
   const int in_wlen = a.in_wlen;   // read once, not per load
   const long in_ls = a.in_ls;
   strided_tile<T, L, C, DIR, THREADS, PRE, POST>(
-      a, tw, lds, g, [&](int line, int pt) { return tile_in<T, L>(in_wlen, in_ls, g, line, pt); });
+      a, ltw, lds, g, [&](int line, int pt) { return tile_in<T, L>(in_wlen, in_ls, g, line, pt); });
 }
 
 // ---- z pass, r2c --------------------------------------------------------------
@@ -848,7 +853,7 @@ template <typename T, int N> struct FftN {
     a.n = N;
     a.nouter = nouter;
     int nblk = nouter * a.nkzb * (a.nbatch > 1 ? a.nbatch : 1);
-    size_t shm = sizeof(cx<T>) * C * N;
+    size_t shm = sizeof(cx<T>) * (C + 1) * N;   // the tile, then the line's twiddles
     const cx<T> *tw = twtable<T>().get(N);
     auto kern = k_fft_strided<T, N, C, DIR, STHREADS, PRE, POST, PASS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
