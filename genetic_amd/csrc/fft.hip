@@ -536,6 +536,11 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_r2c(T *__restrict__ data, lon
   constexpr int M = N / 2;
   extern __shared__ char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  // the M-point transform's twiddles (every other entry of the N-point
+  // table) in LDS after the rows, for the stages after the first (which
+  // needs none; the barriers before the second stage publish them)
+  cx<T> *ltw = lds + ROWS * ZRow<M>::len;
+  for (int i = int(threadIdx.x); i < M; i += THREADS) ltw[i] = twN[2 * i];
   long row0 = long(blockIdx.x) * ROWS;
   cx<T> *rows = reinterpret_cast<cx<T> *>(data) + row0 * (M + 1);
   // rows come from the padded field itself, or from a dense array (row stride N reals)
@@ -547,7 +552,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_r2c(T *__restrict__ data, lon
   if constexpr (M == 1) {
     if (threadIdx.x < ROWS) lds[lidx(threadIdx.x, 0)] = gld(threadIdx.x, 0);
   } else {
-    run_stages<T, M, ROWS, THREADS, -1, false, false, true, 0>(lds, twN, 2, gld, lst, lidx);
+    run_stages<T, M, ROWS, THREADS, -1, false, false, true, 0>(lds, ltw, 1, gld, lst, lidx);
   }
   __syncthreads();
   constexpr int H = M / 2 + 1;
@@ -581,6 +586,11 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_c2r(T *__restrict__ data, lon
   constexpr int M = N / 2;
   extern __shared__ char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  // the M-point transform's twiddles (every other entry of the N-point
+  // table) in LDS after the rows, for the stages after the first (which
+  // needs none; the barriers before the second stage publish them)
+  cx<T> *ltw = lds + ROWS * ZRow<M>::len;
+  for (int i = int(threadIdx.x); i < M; i += THREADS) ltw[i] = twN[2 * i];
   long row0 = long(blockIdx.x) * ROWS;
   cx<T> *rows = reinterpret_cast<cx<T> *>(data) + row0 * (M + 1);
   auto lidx = [](int line, int pt) { return ZRow<M>::idx(line, pt); };
@@ -636,7 +646,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_c2r(T *__restrict__ data, lon
   if constexpr (M == 1) {
     if (threadIdx.x < ROWS) gst(threadIdx.x, 0, lld(threadIdx.x, 0));
   } else {
-    run_stages<T, M, ROWS, THREADS, +1, false, true, false, 0>(lds, twN, 2, lld, gst, lidx);
+    run_stages<T, M, ROWS, THREADS, +1, false, true, false, 0>(lds, ltw, 1, lld, gst, lidx);
   }
 }
 
@@ -651,6 +661,11 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_c2r_window(const T *__restric
   constexpr int M = N / 2;
   extern __shared__ char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  // the M-point transform's twiddles (every other entry of the N-point
+  // table) in LDS after the rows, for the stages after the first (which
+  // needs none; the barriers before the second stage publish them)
+  cx<T> *ltw = lds + ROWS * ZRow<M>::len;
+  for (int i = int(threadIdx.x); i < M; i += THREADS) ltw[i] = twN[2 * i];
   long row0 = long(blockIdx.x) * ROWS;
   const cx<T> *rows = reinterpret_cast<const cx<T> *>(in) + row0 * (M + 1);
   auto lidx = [](int line, int pt) { return ZRow<M>::idx(line, pt); };
@@ -693,7 +708,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_c2r_window(const T *__restric
   if constexpr (M == 1) {
     if (threadIdx.x < ROWS) gst(threadIdx.x, 0, lld(threadIdx.x, 0));
   } else {
-    run_stages<T, M, ROWS, THREADS, +1, false, true, false, 0>(lds, twN, 2, lld, gst, lidx);
+    run_stages<T, M, ROWS, THREADS, +1, false, true, false, 0>(lds, ltw, 1, lld, gst, lidx);
   }
 }
 
@@ -708,6 +723,11 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_r2c_window(const T *__restric
   constexpr int M = N / 2;
   extern __shared__ char smem_raw[];
   cx<T> *lds = reinterpret_cast<cx<T> *>(smem_raw);
+  // the M-point transform's twiddles (every other entry of the N-point
+  // table) in LDS after the rows, for the stages after the first (which
+  // needs none; the barriers before the second stage publish them)
+  cx<T> *ltw = lds + ROWS * ZRow<M>::len;
+  for (int i = int(threadIdx.x); i < M; i += THREADS) ltw[i] = twN[2 * i];
   long row0 = long(blockIdx.x) * ROWS;
   auto gld = [&](int line, int pt) -> cx<T> {
     if (row0 + line >= nrows) return cx<T>{0, 0};
@@ -725,7 +745,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_r2c_window(const T *__restric
   if constexpr (M == 1) {
     if (threadIdx.x < ROWS) lds[lidx(threadIdx.x, 0)] = gld(threadIdx.x, 0);
   } else {
-    run_stages<T, M, ROWS, THREADS, -1, false, false, true, 0>(lds, twN, 2, gld, lst, lidx);
+    run_stages<T, M, ROWS, THREADS, -1, false, false, true, 0>(lds, ltw, 1, gld, lst, lidx);
   }
   __syncthreads();
   constexpr int H = M / 2 + 1;
@@ -894,7 +914,7 @@ template <typename T, int N> struct FftN {
                     int nfields = 1) {
     long nrows = long(N) * N * nfields;
     int nblk = int(nrows / ROWS);
-    size_t shm = sizeof(cx<T>) * ROWS * ZRow<M>::len;
+    size_t shm = sizeof(cx<T>) * (ROWS * ZRow<M>::len + M);   // rows, then the M-point twiddles
     const cx<T> *tw = twtable<T>().get(N);
     std::string tag = std::string(name) + "." + std::to_string(N);
     if (hook) hook->bytes(tag.c_str(), double(nrows) * (2 * N + 2) * sizeof(T));   // N reals one way, N+2 the other
@@ -967,7 +987,7 @@ template <typename T, int N> struct FftN {
   static void r2c_boxes(const T *rows, long nrows, int nb, int wl, const int *winx, const int *winy, T *zbuf,
                         T *ybuf, T *spec, long spec_stride, const T *post, hipStream_t st, TimingHook *hook) {
     int nblk = int(nrows / ROWS);
-    size_t shm = sizeof(cx<T>) * ROWS * ZRow<M>::len;
+    size_t shm = sizeof(cx<T>) * (ROWS * ZRow<M>::len + M);   // rows, then the M-point twiddles
     const cx<T> *tw = twtable<T>().get(N);
     auto kern = k_fft_z_r2c<T, N, ROWS, THREADS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
@@ -1119,7 +1139,7 @@ template <typename T, int N> struct FftN {
   static void zpass_epi(T *out, int epi, T *S, T *A, hipStream_t st, TimingHook *hook) {
     long nrows = long(N) * N;
     int nblk = int(nrows / ROWS);
-    size_t shm = sizeof(cx<T>) * ROWS * ZRow<M>::len;
+    size_t shm = sizeof(cx<T>) * (ROWS * ZRow<M>::len + M);   // rows, then the M-point twiddles
     const cx<T> *tw = twtable<T>().get(N);
     std::string tag = "fft_z_c2r_lpt." + std::to_string(N);
     auto launch = [&](auto kern) {
@@ -1195,7 +1215,7 @@ template <typename T, int N> struct FftN {
     // z pass: rows (x_local, y_local), keep z in window
     long nrows = long(wl[0]) * wl[1];
     int nblk = int((nrows + ROWS - 1) / ROWS);
-    size_t shm = sizeof(cx<T>) * ROWS * ZRow<M>::len;
+    size_t shm = sizeof(cx<T>) * (ROWS * ZRow<M>::len + M);   // rows, then the M-point twiddles
     const cx<T> *tw = twtable<T>().get(N);
     auto kern = k_fft_z_c2r_window<T, N, ROWS, THREADS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
@@ -1218,7 +1238,7 @@ template <typename T, int N> struct FftN {
                          double kmin = 0) {
     long nrows = long(wl[0]) * wl[1];
     int nblk = int((nrows + ROWS - 1) / ROWS);
-    size_t shm = sizeof(cx<T>) * ROWS * ZRow<M>::len;
+    size_t shm = sizeof(cx<T>) * (ROWS * ZRow<M>::len + M);   // rows, then the M-point twiddles
     const cx<T> *tw = twtable<T>().get(N);
     auto kern = k_fft_z_r2c_window<T, N, ROWS, THREADS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
