@@ -10,42 +10,47 @@ template <typename T, int N> struct TileShape {
   static constexpr int threads = 512;
   static constexpr int cols = sizeof(T) == 4 ? 8 : 4;
 };
-// float n=128: 0.1184 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
+// float n=128: 0.1124 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
 template <> struct TileShape<float, 128> {
   static constexpr int threads = 256;
   static constexpr int cols = 8;
 };
-// float n=256: 0.3489 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
+// float n=256: 0.3011 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
 template <> struct TileShape<float, 256> {
   static constexpr int threads = 512;
   static constexpr int cols = 16;
 };
-// float n=512: 3.2984 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
+// float n=512: 2.9122 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
 template <> struct TileShape<float, 512> {
   static constexpr int threads = 1024;
   static constexpr int cols = 16;
 };
+// float n=1024: 23.2576 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
+template <> struct TileShape<float, 1024> {
+  static constexpr int threads = 1024;
+  static constexpr int cols = 8;
+};
 // double n=64: 0.0874 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
 template <> struct TileShape<double, 64> {
-  static constexpr int threads = 256;
-  static constexpr int cols = 8;
-};
-// double n=128: 0.1539 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
-template <> struct TileShape<double, 128> {
-  static constexpr int threads = 256;
-  static constexpr int cols = 8;
-};
-// double n=256: 0.6483 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
-template <> struct TileShape<double, 256> {
   static constexpr int threads = 512;
   static constexpr int cols = 8;
 };
-// double n=512: 5.8421 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
+// double n=128: 0.1455 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
+template <> struct TileShape<double, 128> {
+  static constexpr int threads = 256;
+  static constexpr int cols = 4;
+};
+// double n=256: 0.6072 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
+template <> struct TileShape<double, 256> {
+  static constexpr int threads = 256;
+  static constexpr int cols = 8;
+};
+// double n=512: 5.5496 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
 template <> struct TileShape<double, 512> {
   static constexpr int threads = 256;
   static constexpr int cols = 8;
 };
-// double n=1024: 51.2000 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
+// double n=1024: 47.7723 ms for the four timed transforms (round trip, ZA, windowed c2r, windowed r2c)
 template <> struct TileShape<double, 1024> {
   static constexpr int threads = 1024;
   static constexpr int cols = 8;

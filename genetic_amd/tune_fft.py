@@ -48,9 +48,10 @@ def lib_path(t, c):
 
 
 def fits(t, c, n, prec):
-    """The tile (cols x n complex) fits the LDS, and FP64 tiles take at most 8 columns."""
+    """The tile (cols x n complex) and the line's n twiddles fit the LDS, and
+    FP64 tiles take at most 8 columns."""
     cols = c if prec == 32 else min(c, 8)
-    return cols * n * (8 if prec == 32 else 16) <= LDS_BYTES
+    return (cols + 1) * n * (8 if prec == 32 else 16) <= LDS_BYTES
 
 
 def build(jobs=6):
