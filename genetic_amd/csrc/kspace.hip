@@ -22,10 +22,19 @@ namespace gic {
 // mode index -> signed integer wavenumbers, with the stored N/2 read as +N/2
 __device__ __forceinline__ void mode_ints(size_t idx, int n, int &ix, int &iy, int &iz) {
   int nz = n / 2 + 1;
-  iz = int(idx % nz);
-  size_t r = idx / nz;
-  iy = int(r % n);
-  ix = int(r / n);
+  if ((n & (n - 1)) == 0 && idx < (size_t(1) << 32)) {
+    // the transform sizes (powers of two up to 1024): one 32-bit division,
+    // then a shift and a mask
+    uint32_t i32 = uint32_t(idx), r = i32 / uint32_t(nz);
+    iz = int(i32 - r * uint32_t(nz));
+    iy = int(r & uint32_t(n - 1));
+    ix = int(r >> (31 - __clz(n)));
+  } else {
+    iz = int(idx % nz);
+    size_t r = idx / nz;
+    iy = int(r % n);
+    ix = int(r / n);
+  }
   int h = n / 2;
   if (ix > h) ix -= n;
   if (iy > h) iy -= n;

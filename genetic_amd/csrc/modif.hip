@@ -37,6 +37,47 @@ __device__ __forceinline__ double block_sum(double v) {
   return sh[0];
 }
 
+// The stored modes of an n^3 half spectrum, i = (x n + y)(n/2 + 1) + z, in a
+// grid-stride loop: each thread locates its first mode with one division
+// and steps (row, z) by the stride after that (n is a power of two, so the
+// row splits into x and y by a shift and a mask).
+// (x, y, z) of stored mode i of an n^3 half spectrum (unsigned indices)
+__device__ __forceinline__ void mode_xyz(size_t i, int n, int &x, int &y, int &z) {
+  int nz = n / 2 + 1;
+  if ((n & (n - 1)) == 0 && i < (size_t(1) << 32)) {
+    uint32_t i32 = uint32_t(i), r = i32 / uint32_t(nz);
+    z = int(i32 - r * uint32_t(nz));
+    y = int(r & uint32_t(n - 1));
+    x = int(r >> (31 - __clz(n)));
+  } else {
+    z = int(i % nz);
+    size_t r = i / nz;
+    y = int(r % n);
+    x = int(r / n);
+  }
+}
+template <class F> __device__ __forceinline__ void for_each_mode(int n, F f) {
+  const int nz = n / 2 + 1;
+  const long nm = long(n) * n * nz;
+  const long stride = long(gridDim.x) * blockDim.x;
+  long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= nm) return;
+  const int lg = 31 - __clz(n);
+  long r = i / nz;
+  int z = int(i - r * nz);
+  const long sr = stride / nz;
+  const int sz = int(stride - sr * nz);
+  for (; i < nm; i += stride) {
+    f(i, int(r >> lg), int(r & (n - 1)), z);
+    z += sz;
+    r += sr;
+    if (z >= nz) {
+      z -= nz;
+      ++r;
+    }
+  }
+}
+
 // Sum over stored modes of w * Re(conj(a) b).  The weights reproduce the
 // reference's half-space loop: 2 for 0 < kz < N/2; on the kz = 0 and N/2
 // planes 2 for 0 < kx < N/2, 0 for kx > N/2, and on the kx = 0, N/2 lines
@@ -45,12 +86,8 @@ template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_inner_partial(const cx<T> *__restrict__ a, const cx<T> *__restrict__ b,
                                                                int n, double *__restrict__ part) {
   int h = n / 2;
-  size_t nm = size_t(n) * n * (h + 1);
   double acc = 0;
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nm; i += size_t(gridDim.x) * blockDim.x) {
-    int z = int(i % (h + 1));
-    size_t r = i / (h + 1);
-    int y = int(r % n), x = int(r / n);
+  for_each_mode(n, [&](long i, int x, int y, int z) {
     double w;
     if (z > 0 && z < h) w = 2;
     else if (x > 0 && x < h) w = 2;
@@ -62,7 +99,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_inner_partial(const cx<T> *__re
       cx<T> u = a[i], v = b[i];
       acc += w * (double(u.x) * double(v.x) + double(u.y) * double(v.y));
     }
-  }
+  });
   double s = block_sum(acc);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -101,23 +138,19 @@ struct MultiDot {
 template <typename T, int NP>
 __global__ __launch_bounds__(RED_THREADS) void k_multi_inner_partial(MultiDot md, int n, double *__restrict__ part) {
   int h = n / 2;
-  size_t nm = size_t(n) * n * (h + 1);
   double acc[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) acc[p] = 0;
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nm; i += size_t(gridDim.x) * blockDim.x) {
-    int z = int(i % (h + 1));
-    size_t r = i / (h + 1);
-    int y = int(r % n), x = int(r / n);
+  for_each_mode(n, [&](long i, int x, int y, int z) {
     double w = mode_weight(x, y, z, h);
-    if (w == 0) continue;
+    if (w == 0) return;
 #pragma unroll
     for (int p = 0; p < NP; ++p) {
       cx<T> u = reinterpret_cast<const cx<T> *>(md.f[md.pa[p]])[i];
       cx<T> v = reinterpret_cast<const cx<T> *>(md.f[md.pb[p]])[i];
       acc[p] += w * (double(u.x) * double(v.x) + double(u.y) * double(v.y));
     }
-  }
+  });
 #pragma unroll
   for (int p = 0; p < NP; ++p) {
     double s = block_sum(acc[p]);
@@ -158,21 +191,17 @@ __global__ __launch_bounds__(RED_THREADS) void k_chi2_window_partial(const cx<T>
                                                                      const T *__restrict__ F, int n,
                                                                      double *__restrict__ part) {
   int h = n / 2;
-  size_t nm = size_t(n) * n * (h + 1);
   double acc = 0;
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nm; i += size_t(gridDim.x) * blockDim.x) {
-    int z = int(i % (h + 1));
-    size_t r = i / (h + 1);
-    int y = int(r % n), x = int(r / n);
+  for_each_mode(n, [&](long i, int x, int y, int z) {
     double w = mode_weight(x, y, z, h);
-    if (w == 0) continue;
+    if (w == 0) return;
     double f = double(F[mode_ksq(x, y, z, n)]);
     cx<T> a = xs[i], b = us[i];
     double xx = double(a.x) * a.x + double(a.y) * a.y;
     double uu = double(b.x) * b.x + double(b.y) * b.y;
     double ux = double(b.x) * a.x + double(b.y) * a.y;
     acc += w * (xx - uu + 2 * f * ux - 2 * f * uu + f * f * uu);
-  }
+  });
   double s = block_sum(acc);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -183,17 +212,13 @@ __global__ __launch_bounds__(RED_THREADS) void k_inner_tab_partial(const cx<T> *
                                                                    const T *__restrict__ tab, int n,
                                                                    double *__restrict__ part) {
   int h = n / 2;
-  size_t nm = size_t(n) * n * (h + 1);
   double acc = 0;
-  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < nm; i += size_t(gridDim.x) * blockDim.x) {
-    int z = int(i % (h + 1));
-    size_t r = i / (h + 1);
-    int y = int(r % n), x = int(r / n);
+  for_each_mode(n, [&](long i, int x, int y, int z) {
     double w = mode_weight(x, y, z, h);
-    if (w == 0) continue;
+    if (w == 0) return;
     cx<T> u = a[i], v = b[i];
     acc += w * double(tab[mode_ksq(x, y, z, n)]) * (double(u.x) * v.x + double(u.y) * v.y);
-  }
+  });
   double s = block_sum(acc);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -215,9 +240,8 @@ __global__ void k_copy_mul_table(cx<T> *__restrict__ out, const cx<T> *__restric
   int h = n / 2;
   size_t nm = size_t(n) * n * (h + 1);
   if (i >= nm) return;
-  int z = int(i % (h + 1));
-  size_t r = i / (h + 1);
-  int y = int(r % n), x = int(r / n);
+  int x, y, z;
+  mode_xyz(i, n, x, y, z);
   T v = tab[mode_ksq(x, y, z, n)];
   cx<T> a = in[i];
   out[i] = cx<T>{a.x * v, a.y * v};
@@ -303,9 +327,8 @@ template <typename T> __global__ void k_mode_op(cx<T> *__restrict__ f, int n, do
   size_t nm = size_t(n) * n * (h + 1);
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= nm) return;
-  int iz = int(i % (h + 1));
-  size_t r = i / (h + 1);
-  int iy = int(r % n), ix = int(r / n);
+  int ix, iy, iz;
+  mode_xyz(i, n, ix, iy, iz);
   if (ix > h) ix -= n;
   if (iy > h) iy -= n;
   T k1 = T(ix * kmin), k2 = T(iy * kmin), k3 = T(iz * kmin);
