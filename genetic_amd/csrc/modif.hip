@@ -179,6 +179,32 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_rows(const double *__res
   if (threadIdx.x == 0) out[blockIdx.x] = s;
 }
 
+struct FilterSpec {
+  int na = 0, nb = 0;
+  int ka[6], kb[6];   // 0 low-pass Fermi, 1 sqrt(1-LP^2), 2 1-LP
+  double ca[6], cb[6];
+  double scale = 1.0;
+  bool zero = false;
+};
+__device__ __forceinline__ double filter_factor(int kind, double kc, double k) {
+  double lp = 1. / (1. + exp((k - kc) / (kc * 0.1)));
+  if (kind == 0) return lp;
+  if (kind == 1) return sqrt(1. - lp * lp);
+  return 1. - lp;
+}
+// The filter spec at |k|^2 = ksq, rounded to T exactly as k_filter_spec
+// tabulates it (the sums form it per mode rather than gather it from the
+// table: a gather of 64 scattered entries per wavefront costs more than the
+// arithmetic)
+template <typename T> __device__ __forceinline__ T spec_value(const FilterSpec &s, double kmin, long ksq) {
+  if (s.zero) return T(0);
+  double k = sqrt(double(ksq)) * kmin;
+  double fa = 1.0, fb = 1.0;
+  for (int i = 0; i < s.na; ++i) fa *= filter_factor(s.ka[i], s.ca[i], k);
+  for (int i = 0; i < s.nb; ++i) fb *= filter_factor(s.kb[i], s.cb[i], k);
+  return T(fa * fb * s.scale);
+}
+
 // The chi^2 contribution of a level whose metric filters inside the window
 // of the next level: with M the window mask, u = M x and F the level's
 // filter, <x, B(A(x))> for A = 1 - M + F M, B = 1 - M + M F (two applications
@@ -187,15 +213,14 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_rows(const double *__res
 // summed over modes, given the spectra of x and u.
 template <typename T>
 __global__ __launch_bounds__(RED_THREADS) void k_chi2_window_partial(const cx<T> *__restrict__ xs,
-                                                                     const cx<T> *__restrict__ us,
-                                                                     const T *__restrict__ F, int n,
-                                                                     double *__restrict__ part) {
+                                                                     const cx<T> *__restrict__ us, FilterSpec F,
+                                                                     double kmin, int n, double *__restrict__ part) {
   int h = n / 2;
   double acc = 0;
   for_each_mode(n, [&](long i, int x, int y, int z) {
     double w = mode_weight(x, y, z, h);
     if (w == 0) return;
-    double f = double(F[mode_ksq(x, y, z, n)]);
+    double f = double(spec_value<T>(F, kmin, mode_ksq(x, y, z, n)));
     cx<T> a = xs[i], b = us[i];
     double xx = double(a.x) * a.x + double(a.y) * a.y;
     double uu = double(b.x) * b.x + double(b.y) * b.y;
@@ -283,31 +308,10 @@ template <typename T> __global__ void k_scale(T *__restrict__ x, T s, size_t n) 
 }
 
 // product filters as tables over |k|^2 = m kMin^2: A(k) * B(k) * scale
-struct FilterSpec {
-  int na = 0, nb = 0;
-  int ka[6], kb[6];   // 0 low-pass Fermi, 1 sqrt(1-LP^2), 2 1-LP
-  double ca[6], cb[6];
-  double scale = 1.0;
-  bool zero = false;
-};
-__device__ __forceinline__ double filter_factor(int kind, double kc, double k) {
-  double lp = 1. / (1. + exp((k - kc) / (kc * 0.1)));
-  if (kind == 0) return lp;
-  if (kind == 1) return sqrt(1. - lp * lp);
-  return 1. - lp;
-}
 template <typename T> __global__ void k_filter_spec(FilterSpec s, double kmin, long nent, T *__restrict__ out) {
   long m = long(blockIdx.x) * blockDim.x + threadIdx.x;
   if (m >= nent) return;
-  if (s.zero) {
-    out[m] = T(0);
-    return;
-  }
-  double k = sqrt(double(m)) * kmin;
-  double fa = 1.0, fb = 1.0;
-  for (int i = 0; i < s.na; ++i) fa *= filter_factor(s.ka[i], s.ca[i], k);
-  for (int i = 0; i < s.nb; ++i) fb *= filter_factor(s.kb[i], s.cb[i], k);
-  out[m] = T(fa * fb * s.scale);
+  out[m] = spec_value<T>(s, kmin, m);
 }
 
 // pow(P_all, -1/2) * pow(P_dm, 1/2) from the two sqrt tables (0 where P = 0)
@@ -1562,10 +1566,10 @@ template <typename T> double chi2_of(gic_context *c, MLF<T> &field) {
         fft_r2c_window<T>((const T *)ubox.p, g.n, M.b0, M.bl, (T *)bb.b.p, (T *)bb.a.p, (T *)uspec.p, nullptr, false,
                           c->st, c->hook());
       }
-      const T *F = spec_table<T>(c, g, spec_levels(c, long(l), -1, 1.0));
+      FilterSpec F = spec_levels(c, long(l), -1, 1.0);
       Timed t(c, "chi2_window", 2.0 * g.nmodes() * sizeof(cx<T>));
       hipLaunchKernelGGL(k_chi2_window_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, xs,
-                         reinterpret_cast<const cx<T> *>(uspec.p), F, g.n, (double *)c->mod->part.p);
+                         reinterpret_cast<const cx<T> *>(uspec.p), F, g.kmin(), g.n, (double *)c->mod->part.p);
     } else {
       const T *F2 = spec_table<T>(c, g, spec_levels(c, long(l), long(l), 1.0));
       Timed t(c, "inner", double(g.nmodes()) * sizeof(cx<T>));
