@@ -78,6 +78,37 @@ template <class F> __device__ __forceinline__ void for_each_mode(int n, F f) {
   }
 }
 
+// for_each_mode over pairs of adjacent stored modes (2p, 2p + 1), for FP32
+// spectra read 16 bytes (two modes) per lane: f(i, x0, y0, z0, x1, y1, z1)
+// with i = 2p (n * n is even, so the modes pair up exactly).
+template <class F> __device__ __forceinline__ void for_each_mode_pair(int n, F f) {
+  const int nz = n / 2 + 1;
+  const long np = long(n) * n * nz / 2;
+  const long stride = long(gridDim.x) * blockDim.x;
+  long p = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const int lg = 31 - __clz(n);
+  long r = (2 * p) / nz;
+  int z = int(2 * p - r * nz);
+  const long sr = (2 * stride) / nz;
+  const int sz = int(2 * stride - sr * nz);
+  for (; p < np; p += stride) {
+    int z1 = z + 1;
+    long r1 = r;
+    if (z1 == nz) {
+      z1 = 0;
+      ++r1;
+    }
+    f(2 * p, int(r >> lg), int(r & (n - 1)), z, int(r1 >> lg), int(r1 & (n - 1)), z1);
+    z += sz;
+    r += sr;
+    if (z >= nz) {
+      z -= nz;
+      ++r;
+    }
+  }
+}
+
 // Sum over stored modes of w * Re(conj(a) b).  The weights reproduce the
 // reference's half-space loop: 2 for 0 < kz < N/2; on the kz = 0 and N/2
 // planes 2 for 0 < kx < N/2, 0 for kx > N/2, and on the kx = 0, N/2 lines
@@ -126,47 +157,79 @@ __device__ __forceinline__ long mode_ksq(int x, int y, int z, int n) {
   return kx * kx + ky * ky + long(z) * z;
 }
 
-// Several weighted inner products in one pass: sum over stored modes of
-// w Re(conj(f[pa[p]]) f[pb[p]]) for each pair p (the fields are read once
-// from HBM; repeated loads of a field hit the cache)
-constexpr int MAXF = 12, MAXP = 40;
-struct MultiDot {
-  const void *f[MAXF];
-  int pa[MAXP], pb[MAXP];
-  int np;
+// The Gram entries of one block of up to 4 covectors a_p against up to 4
+// vectors v_q (only q <= p when diag) and, with w given, <a_p, w>: sums over
+// stored modes of weight * Re(conj(a) b), with every field read once per
+// mode.  Row 4 p + q of part holds <a_p, v_q>, row 16 + p holds <a_p, w>.
+constexpr int GRAM_ROWS = 20;
+struct GramBlock {
+  const void *a[4], *v[4], *w;
+  int na, nv, diag;
 };
-template <typename T, int NP>
-__global__ __launch_bounds__(RED_THREADS) void k_multi_inner_partial(MultiDot md, int n, double *__restrict__ part) {
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_gram_block_partial(
+    GramBlock gb, int n, double *__restrict__ part) {
   int h = n / 2;
-  double acc[NP];
+  double acc[GRAM_ROWS];
 #pragma unroll
-  for (int p = 0; p < NP; ++p) acc[p] = 0;
-  for_each_mode(n, [&](long i, int x, int y, int z) {
-    double w = mode_weight(x, y, z, h);
-    if (w == 0) return;
+  for (int r = 0; r < GRAM_ROWS; ++r) acc[r] = 0;
+  const int na = gb.na, nv = gb.nv, diag = gb.diag;
+  const cx<T> *wf = reinterpret_cast<const cx<T> *>(gb.w);
+  // the products of one mode: a, v, f its values in the fields
+  auto mode = [&](double w, const cx<T> *a, const cx<T> *v, cx<T> f) {
 #pragma unroll
-    for (int p = 0; p < NP; ++p) {
-      cx<T> u = reinterpret_cast<const cx<T> *>(md.f[md.pa[p]])[i];
-      cx<T> v = reinterpret_cast<const cx<T> *>(md.f[md.pb[p]])[i];
-      acc[p] += w * (double(u.x) * double(v.x) + double(u.y) * double(v.y));
+    for (int p = 0; p < 4; ++p) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (p < na && q < nv && (!diag || q <= p))
+          acc[4 * p + q] += w * (double(a[p].x) * double(v[q].x) + double(a[p].y) * double(v[q].y));
+      if (wf && p < na) acc[16 + p] += w * (double(a[p].x) * double(f.x) + double(a[p].y) * double(f.y));
     }
-  });
+  };
+  if constexpr (sizeof(T) == 4) {
+    // two modes per lane: one 16-byte load per field
+    for_each_mode_pair(n, [&](long i, int x0, int y0, int z0, int x1, int y1, int z1) {
+      float4 a2[4], v2[4], f2{0, 0, 0, 0};
 #pragma unroll
-  for (int p = 0; p < NP; ++p) {
-    double s = block_sum(acc[p]);
-    if (threadIdx.x == 0) part[size_t(p) * gridDim.x + blockIdx.x] = s;
-    __syncthreads();
+      for (int p = 0; p < 4; ++p) {
+        a2[p] = p < na ? reinterpret_cast<const float4 *>(gb.a[p])[i / 2] : float4{0, 0, 0, 0};
+        v2[p] = p < nv ? reinterpret_cast<const float4 *>(gb.v[p])[i / 2] : float4{0, 0, 0, 0};
+      }
+      if (wf) f2 = reinterpret_cast<const float4 *>(wf)[i / 2];
+      cx<T> a[4], v[4];
+      double w0 = mode_weight(x0, y0, z0, h), w1 = mode_weight(x1, y1, z1, h);
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        a[p] = cx<T>{a2[p].x, a2[p].y};
+        v[p] = cx<T>{v2[p].x, v2[p].y};
+      }
+      if (w0 != 0) mode(w0, a, v, cx<T>{f2.x, f2.y});
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        a[p] = cx<T>{a2[p].z, a2[p].w};
+        v[p] = cx<T>{v2[p].z, v2[p].w};
+      }
+      if (w1 != 0) mode(w1, a, v, cx<T>{f2.z, f2.w});
+    });
+  } else {
+    for_each_mode(n, [&](long i, int x, int y, int z) {
+      double w = mode_weight(x, y, z, h);
+      if (w == 0) return;
+      cx<T> a[4], v[4], f{0, 0};
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        a[p] = p < na ? reinterpret_cast<const cx<T> *>(gb.a[p])[i] : cx<T>{0, 0};
+        v[p] = p < nv ? reinterpret_cast<const cx<T> *>(gb.v[p])[i] : cx<T>{0, 0};
+      }
+      if (wf) f = wf[i];
+      mode(w, a, v, f);
+    });
   }
-}
-// pair counts of nm covectors: nm (nm + 1) / 2 Gram entries plus nm right-hand sides
-template <typename T> bool launch_multi_inner(const MultiDot &md, int n, double *part, hipStream_t st) {
-  switch (md.np) {
-#define GIC_MI(NP) \
-  case NP: hipLaunchKernelGGL((k_multi_inner_partial<T, NP>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, st, md, n, part); return true;
-    GIC_MI(2) GIC_MI(5) GIC_MI(9) GIC_MI(14) GIC_MI(20) GIC_MI(27) GIC_MI(35)
-#undef GIC_MI
-    default:
-      return false;
+#pragma unroll
+  for (int r = 0; r < GRAM_ROWS; ++r) {
+    double s = block_sum(acc[r]);
+    if (threadIdx.x == 0) part[size_t(r) * gridDim.x + blockIdx.x] = s;
+    __syncthreads();
   }
 }
 // out[p] = sum of part[p*np_blocks ...], one block per pair
@@ -217,16 +280,28 @@ __global__ __launch_bounds__(RED_THREADS) void k_chi2_window_partial(const cx<T>
                                                                      double kmin, int n, double *__restrict__ part) {
   int h = n / 2;
   double acc = 0;
-  for_each_mode(n, [&](long i, int x, int y, int z) {
-    double w = mode_weight(x, y, z, h);
-    if (w == 0) return;
-    double f = double(spec_value<T>(F, kmin, mode_ksq(x, y, z, n)));
-    cx<T> a = xs[i], b = us[i];
-    double xx = double(a.x) * a.x + double(a.y) * a.y;
-    double uu = double(b.x) * b.x + double(b.y) * b.y;
-    double ux = double(b.x) * a.x + double(b.y) * a.y;
-    acc += w * (xx - uu + 2 * f * ux - 2 * f * uu + f * f * uu);
-  });
+  auto term = [&](double w, double f, double ax, double ay, double bx, double by) {
+    double xx = ax * ax + ay * ay;
+    double uu = bx * bx + by * by;
+    double ux = bx * ax + by * ay;
+    return w * (xx - uu + 2 * f * ux - 2 * f * uu + f * f * uu);
+  };
+  if constexpr (sizeof(T) == 4) {
+    // two modes per lane: one 16-byte load per spectrum
+    for_each_mode_pair(n, [&](long i, int x0, int y0, int z0, int x1, int y1, int z1) {
+      float4 a = reinterpret_cast<const float4 *>(xs)[i / 2], b = reinterpret_cast<const float4 *>(us)[i / 2];
+      double w0 = mode_weight(x0, y0, z0, h), w1 = mode_weight(x1, y1, z1, h);
+      if (w0 != 0) acc += term(w0, double(spec_value<T>(F, kmin, mode_ksq(x0, y0, z0, n))), a.x, a.y, b.x, b.y);
+      if (w1 != 0) acc += term(w1, double(spec_value<T>(F, kmin, mode_ksq(x1, y1, z1, n))), a.z, a.w, b.z, b.w);
+    });
+  } else {
+    for_each_mode(n, [&](long i, int x, int y, int z) {
+      double w = mode_weight(x, y, z, h);
+      if (w == 0) return;
+      cx<T> a = xs[i], b = us[i];
+      acc += term(w, double(spec_value<T>(F, kmin, mode_ksq(x, y, z, n))), a.x, a.y, b.x, b.y);
+    });
+  }
   double s = block_sum(acc);
   if (threadIdx.x == 0) part[blockIdx.x] = s;
 }
@@ -297,6 +372,23 @@ template <typename T> __global__ void k_axpy_multi(T *__restrict__ y, AxpyList L
   for (int j = 0; j < 8; ++j)
     if (j < L.k) v += T(L.s[j]) * reinterpret_cast<const T *>(L.x[j])[i];
   y[i] = v;
+}
+// the same for FP32 with four values per lane (16-byte accesses); n4 = n / 4
+__global__ void k_axpy_multi4(float *__restrict__ y, AxpyList L, size_t n4) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n4) return;
+  float4 v = reinterpret_cast<float4 *>(y)[i];
+#pragma unroll
+  for (int j = 0; j < 8; ++j)
+    if (j < L.k) {
+      float s = float(L.s[j]);
+      float4 x = reinterpret_cast<const float4 *>(L.x[j])[i];
+      v.x += s * x.x;
+      v.y += s * x.y;
+      v.z += s * x.z;
+      v.w += s * x.w;
+    }
+  reinterpret_cast<float4 *>(y)[i] = v;
 }
 template <typename T> __global__ void k_axpy(T *__restrict__ y, const T *__restrict__ x, T s, size_t n) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -917,8 +1009,8 @@ double finish_reduction(gic_context *c, int slot) {
   return v;
 }
 void ensure_red(gic_context *c) {
-  c->mod->part.ensure(size_t(RED_BLOCKS) * MAXP * sizeof(double));
-  c->mod->scalar.ensure(sizeof(double) * MAXP);
+  c->mod->part.ensure(size_t(RED_BLOCKS) * GRAM_ROWS * sizeof(double));
+  c->mod->scalar.ensure(sizeof(double) * GRAM_ROWS);
 }
 
 // a cyclic run [start, start+len) (mod n) of the indices where member[] is 1;
@@ -1727,46 +1819,31 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
       }
       ensure_red(c);
       std::vector<double> gsum(nm * nm, 0.0), dsum(nm, 0.0);
-      static const int sizes[] = {2, 5, 9, 14, 20, 27, 35};
+      std::vector<double> part(GRAM_ROWS);
       for (size_t ib = 0; ib < nm; ib += 4)
         for (size_t jb = 0; jb <= ib; jb += 4) {
           size_t ie = std::min(nm, ib + 4), je = std::min(nm, jb + 4);
-          std::vector<std::pair<int, int>> pr;   // (i, j); j = -1 for w
-          for (size_t i = ib; i < ie; ++i) {
-            for (size_t j = jb; j < je && j <= i; ++j) pr.push_back({int(i), int(j)});
-            if (jb == 0) pr.push_back({int(i), -1});
-          }
-          int np = 0;
-          for (int sz : sizes)
-            if (sz >= int(pr.size())) {
-              np = sz;
-              break;
-            }
-          std::vector<double> part(np);
           for (size_t l = 0; l < field.nl(); ++l) {
-            MultiDot md{};
-            for (size_t i = ib; i < ie; ++i) md.f[i - ib] = alphas[i].p(l);
-            for (size_t j = jb; j < je; ++j) md.f[4 + j - jb] = vecs[j].p(l);
-            md.f[8] = field.p(l);
-            for (int q = 0; q < np; ++q) {
-              auto pq = pr[q < int(pr.size()) ? q : 0];   // padding repeats the first pair
-              md.pa[q] = pq.first - int(ib);
-              md.pb[q] = pq.second < 0 ? 8 : 4 + pq.second - int(jb);
-            }
-            md.np = np;
-            int nfields = 0;
-            for (int q = 0; q < MAXF; ++q) nfields += md.f[q] != nullptr;
+            GramBlock gb{};
+            for (size_t i = ib; i < ie; ++i) gb.a[i - ib] = alphas[i].p(l);
+            for (size_t j = jb; j < je; ++j) gb.v[j - jb] = vecs[j].p(l);
+            gb.w = jb == 0 ? field.p(l) : nullptr;
+            gb.na = int(ie - ib);
+            gb.nv = int(je - jb);
+            gb.diag = ib == jb;
+            int nfields = gb.na + gb.nv + (gb.w ? 1 : 0);
             Timed t(c, "multi_inner", double(nfields) * field.g(l).nmodes() * sizeof(cx<T>));
-            if (!launch_multi_inner<T>(md, field.g(l).n, (double *)c->mod->part.p, c->st))
-              throw std::runtime_error("internal: no batched inner product for this many pairs");
-            hipLaunchKernelGGL(k_reduce_rows, dim3(np), dim3(RED_THREADS), 0, c->st, (const double *)c->mod->part.p,
-                               RED_BLOCKS, (double *)c->mod->scalar.p);
+            hipLaunchKernelGGL(k_gram_block_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, gb,
+                               field.g(l).n, (double *)c->mod->part.p);
+            hipLaunchKernelGGL(k_reduce_rows, dim3(GRAM_ROWS), dim3(RED_THREADS), 0, c->st,
+                               (const double *)c->mod->part.p, RED_BLOCKS, (double *)c->mod->scalar.p);
             GIC_HIP(hipGetLastError());
-            GIC_HIP(hipMemcpyAsync(part.data(), c->mod->scalar.p, np * sizeof(double), hipMemcpyDeviceToHost, c->st));
+            GIC_HIP(hipMemcpyAsync(part.data(), c->mod->scalar.p, GRAM_ROWS * sizeof(double), hipMemcpyDeviceToHost,
+                                   c->st));
             GIC_HIP(hipStreamSynchronize(c->st));
-            for (size_t q = 0; q < pr.size(); ++q) {
-              if (pr[q].second < 0) dsum[pr[q].first] += part[q];
-              else gsum[pr[q].first * nm + pr[q].second] += part[q];
+            for (size_t i = ib; i < ie; ++i) {
+              for (size_t j = jb; j < je && j <= i; ++j) gsum[i * nm + j] += part[4 * (i - ib) + (j - jb)];
+              if (jb == 0) dsum[i] += part[16 + (i - ib)];
             }
           }
         }
@@ -1817,7 +1894,10 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
         al.k = int(k);
         size_t n = field.reals(l);
         Timed t(c, "axpy_multi", double(k + 2) * n * sizeof(T));
-        hipLaunchKernelGGL(k_axpy_multi<T>, dim3(blocks(n)), dim3(256), 0, c->st, field.p(l), al, n);
+        if (sizeof(T) == 4 && n % 4 == 0)
+          hipLaunchKernelGGL(k_axpy_multi4, dim3(blocks(n / 4)), dim3(256), 0, c->st, (float *)field.p(l), al, n / 4);
+        else
+          hipLaunchKernelGGL(k_axpy_multi<T>, dim3(blocks(n)), dim3(256), 0, c->st, field.p(l), al, n);
       }
     }
     GIC_HIP(hipGetLastError());
