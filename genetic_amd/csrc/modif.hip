@@ -25,6 +25,10 @@ namespace {
 
 constexpr int RED_BLOCKS = 1024;
 constexpr int RED_THREADS = 256;
+// the chi^2 window sum, which holds few registers, runs more blocks than the
+// other reductions to keep more reads in flight (its partials fit the
+// reduction buffer, RED_BLOCKS * GRAM_ROWS doubles)
+constexpr int CHI2_BLOCKS = 4 * RED_BLOCKS;
 
 __device__ __forceinline__ double block_sum(double v) {
   __shared__ double sh[RED_THREADS];
@@ -999,8 +1003,8 @@ template <typename T> double inner(gic_context *c, MLF<T> &a, MLF<T> &b) {
 }
 
 // FP64 reductions: launch a partial kernel into part, finish in scalar[slot]
-double finish_reduction(gic_context *c, int slot) {
-  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, c->st, (const double *)c->mod->part.p, RED_BLOCKS,
+double finish_reduction(gic_context *c, int slot, int nblk = RED_BLOCKS) {
+  hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(RED_THREADS), 0, c->st, (const double *)c->mod->part.p, nblk,
                      (double *)c->mod->scalar.p + slot);
   GIC_HIP(hipGetLastError());
   double v;
@@ -1642,6 +1646,7 @@ template <typename T> double chi2_of(gic_context *c, MLF<T> &field) {
     const LevelGeom &g = c->levels[l]->g;
     const cx<T> *xs = reinterpret_cast<const cx<T> *>(field.p(l));
     PBuf ubox;   // x on the window box of the next level (u = M x there)
+    int nred = RED_BLOCKS;
     if (l + 1 < nl) {
       const Box3 &M = wins[l];
       size_t nb = box_cells(M), nz = size_t(g.n / 2 + 1);
@@ -1660,15 +1665,16 @@ template <typename T> double chi2_of(gic_context *c, MLF<T> &field) {
       }
       FilterSpec F = spec_levels(c, long(l), -1, 1.0);
       Timed t(c, "chi2_window", 2.0 * g.nmodes() * sizeof(cx<T>));
-      hipLaunchKernelGGL(k_chi2_window_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, xs,
+      hipLaunchKernelGGL(k_chi2_window_partial<T>, dim3(CHI2_BLOCKS), dim3(RED_THREADS), 0, c->st, xs,
                          reinterpret_cast<const cx<T> *>(uspec.p), F, g.kmin(), g.n, (double *)c->mod->part.p);
+      nred = CHI2_BLOCKS;
     } else {
       const T *F2 = spec_table<T>(c, g, spec_levels(c, long(l), long(l), 1.0));
       Timed t(c, "inner", double(g.nmodes()) * sizeof(cx<T>));
       hipLaunchKernelGGL(k_inner_tab_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, xs, xs, F2, g.n,
                          (double *)c->mod->part.p);
     }
-    total += finish_reduction(c, 0);
+    total += finish_reduction(c, 0, nred);
     if (nl > 1) {
       PBuf Rbox;
       Box3 B;
