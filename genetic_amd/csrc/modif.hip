@@ -162,22 +162,30 @@ __device__ __forceinline__ long mode_ksq(int x, int y, int z, int n) {
 }
 
 // The Gram entries of one block of up to 4 covectors a_p against up to 4
-// vectors v_q (only q <= p when diag) and, with w given, <a_p, w>: sums over
-// stored modes of weight * Re(conj(a) b), with every field read once per
-// mode.  Row 4 p + q of part holds <a_p, v_q>, row 16 + p holds <a_p, w>.
+// vectors v_q and, with w given, <a_p, w>: sums over stored modes of
+// weight * Re(conj(a) b), with every field read once per mode.  A block off
+// the diagonal has rows 4 p + q for <a_p, v_q> and 16 + p for <a_p, w>; a
+// diagonal block (DIAG) only q <= p, in rows p (p + 1) / 2 + q, and <a_p, w>
+// in rows 10 + p (fewer accumulators, so that it needs no spill registers).
 constexpr int GRAM_ROWS = 20;
 struct GramBlock {
   const void *a[4], *v[4], *w;
-  int na, nv, diag;
+  int na, nv;
 };
-template <typename T>
+template <bool DIAG> __host__ __device__ constexpr int gram_rows() { return DIAG ? 14 : 20; }
+template <bool DIAG> __host__ __device__ constexpr int gram_row(int p, int q) {
+  return DIAG ? p * (p + 1) / 2 + q : 4 * p + q;
+}
+template <bool DIAG> __host__ __device__ constexpr int gram_wrow(int p) { return DIAG ? 10 + p : 16 + p; }
+template <typename T, bool DIAG>
 __global__ __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_eu(4))) void k_gram_block_partial(
     GramBlock gb, int n, double *__restrict__ part) {
+  constexpr int NR = gram_rows<DIAG>();
   int h = n / 2;
-  double acc[GRAM_ROWS];
+  double acc[NR];
 #pragma unroll
-  for (int r = 0; r < GRAM_ROWS; ++r) acc[r] = 0;
-  const int na = gb.na, nv = gb.nv, diag = gb.diag;
+  for (int r = 0; r < NR; ++r) acc[r] = 0;
+  const int na = gb.na, nv = gb.nv;
   const cx<T> *wf = reinterpret_cast<const cx<T> *>(gb.w);
   // the products of one mode: a, v, f its values in the fields
   auto mode = [&](double w, const cx<T> *a, const cx<T> *v, cx<T> f) {
@@ -185,9 +193,9 @@ __global__ __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     for (int p = 0; p < 4; ++p) {
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        if (p < na && q < nv && (!diag || q <= p))
-          acc[4 * p + q] += w * (double(a[p].x) * double(v[q].x) + double(a[p].y) * double(v[q].y));
-      if (wf && p < na) acc[16 + p] += w * (double(a[p].x) * double(f.x) + double(a[p].y) * double(f.y));
+        if (p < na && q < nv && (!DIAG || q <= p))
+          acc[gram_row<DIAG>(p, q)] += w * (double(a[p].x) * double(v[q].x) + double(a[p].y) * double(v[q].y));
+      if (wf && p < na) acc[gram_wrow<DIAG>(p)] += w * (double(a[p].x) * double(f.x) + double(a[p].y) * double(f.y));
     }
   };
   if constexpr (sizeof(T) == 4) {
@@ -230,7 +238,7 @@ __global__ __launch_bounds__(RED_THREADS) __attribute__((amdgpu_waves_per_eu(4))
     });
   }
 #pragma unroll
-  for (int r = 0; r < GRAM_ROWS; ++r) {
+  for (int r = 0; r < NR; ++r) {
     double s = block_sum(acc[r]);
     if (threadIdx.x == 0) part[size_t(r) * gridDim.x + blockIdx.x] = s;
     __syncthreads();
@@ -1836,20 +1844,28 @@ template <typename T> void mod_apply(gic_context *c, double *pre, double *post) 
             gb.w = jb == 0 ? field.p(l) : nullptr;
             gb.na = int(ie - ib);
             gb.nv = int(je - jb);
-            gb.diag = ib == jb;
+            const bool diag = ib == jb;
+            const int nr = diag ? gram_rows<true>() : gram_rows<false>();
             int nfields = gb.na + gb.nv + (gb.w ? 1 : 0);
             Timed t(c, "multi_inner", double(nfields) * field.g(l).nmodes() * sizeof(cx<T>));
-            hipLaunchKernelGGL(k_gram_block_partial<T>, dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, gb,
-                               field.g(l).n, (double *)c->mod->part.p);
-            hipLaunchKernelGGL(k_reduce_rows, dim3(GRAM_ROWS), dim3(RED_THREADS), 0, c->st,
-                               (const double *)c->mod->part.p, RED_BLOCKS, (double *)c->mod->scalar.p);
+            if (diag)
+              hipLaunchKernelGGL((k_gram_block_partial<T, true>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, gb,
+                                 field.g(l).n, (double *)c->mod->part.p);
+            else
+              hipLaunchKernelGGL((k_gram_block_partial<T, false>), dim3(RED_BLOCKS), dim3(RED_THREADS), 0, c->st, gb,
+                                 field.g(l).n, (double *)c->mod->part.p);
+            hipLaunchKernelGGL(k_reduce_rows, dim3(nr), dim3(RED_THREADS), 0, c->st, (const double *)c->mod->part.p,
+                               RED_BLOCKS, (double *)c->mod->scalar.p);
             GIC_HIP(hipGetLastError());
-            GIC_HIP(hipMemcpyAsync(part.data(), c->mod->scalar.p, GRAM_ROWS * sizeof(double), hipMemcpyDeviceToHost,
-                                   c->st));
+            GIC_HIP(hipMemcpyAsync(part.data(), c->mod->scalar.p, nr * sizeof(double), hipMemcpyDeviceToHost, c->st));
             GIC_HIP(hipStreamSynchronize(c->st));
             for (size_t i = ib; i < ie; ++i) {
-              for (size_t j = jb; j < je && j <= i; ++j) gsum[i * nm + j] += part[4 * (i - ib) + (j - jb)];
-              if (jb == 0) dsum[i] += part[16 + (i - ib)];
+              int p = int(i - ib);
+              for (size_t j = jb; j < je && j <= i; ++j) {
+                int q = int(j - jb);
+                gsum[i * nm + j] += part[diag ? gram_row<true>(p, q) : gram_row<false>(p, q)];
+              }
+              if (jb == 0) dsum[i] += part[diag ? gram_wrow<true>(p) : gram_wrow<false>(p)];
             }
           }
         }
