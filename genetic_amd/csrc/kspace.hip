@@ -132,10 +132,29 @@ __global__ void k_mul_table(cx<T> *__restrict__ f, int n, const T *__restrict__ 
   f[idx] = cx<T>{a.x * v, a.y * v};
 }
 
+// the same for FP32 spectra, two adjacent modes per lane (one 16-byte load
+// and store); n * n is even, so the modes pair up exactly
+__global__ void k_mul_table2(float4 *__restrict__ f, int n, const float *__restrict__ tab) {
+  size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  size_t np = size_t(n) * n * (n / 2 + 1) / 2;
+  if (p >= np) return;
+  int x0, y0, z0, x1, y1, z1;
+  mode_ints(2 * p, n, x0, y0, z0);
+  mode_ints(2 * p + 1, n, x1, y1, z1);
+  float v0 = tab[x0 * x0 + y0 * y0 + z0 * z0], v1 = tab[x1 * x1 + y1 * y1 + z1 * z1];
+  float4 a = f[p];
+  f[p] = float4{a.x * v0, a.y * v0, a.z * v1, a.w * v1};
+}
+
 template <typename T> void mul_table(T *field, int n, const T *tab, hipStream_t st) {
   size_t nm = size_t(n) * n * (n / 2 + 1);
-  hipLaunchKernelGGL(k_mul_table<T>, dim3((nm + 255) / 256), dim3(256), 0, st, reinterpret_cast<cx<T> *>(field), n,
-                     tab);
+  if constexpr (sizeof(T) == 4) {
+    hipLaunchKernelGGL(k_mul_table2, dim3((nm / 2 + 255) / 256), dim3(256), 0, st, reinterpret_cast<float4 *>(field),
+                       n, (const float *)tab);
+  } else {
+    hipLaunchKernelGGL(k_mul_table<T>, dim3((nm + 255) / 256), dim3(256), 0, st, reinterpret_cast<cx<T> *>(field), n,
+                       tab);
+  }
   GIC_HIP(hipGetLastError());
 }
 
