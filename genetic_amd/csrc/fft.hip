@@ -11,7 +11,7 @@
 //        (even, odd) samples plus the r2c post-/c2r pre-twiddle;
 //   y,x: strided lines, a tile of C adjacent kz columns is staged in LDS so
 //        every global access is a C*sizeof(complex)-byte contiguous segment.
-// Stages are radix-8 (plus one radix-2/4 stage) Stockham autosort steps;
+// Stages are radix-8 (plus one radix-4 or radix-16 stage) Stockham autosort steps;
 // the first stage reads HBM, the last writes HBM, the rest go through LDS.
 // Twiddles come from an N-entry table generated in FP64 on the host.
 #include "gic_common.h"
@@ -29,11 +29,16 @@ namespace gic {
 
 constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
-template <int L> struct RadixPlan {
+// WIDE: where the radix-8 stages leave a factor 2, the first stage takes 16
+// points instead (one stage, and one pass through LDS, fewer).  It pays
+// where the first stage has no per-mode operator; with one, the operator's
+// work falls on half as many threads and the pass is slower.
+template <int L, bool WIDE = false> struct RadixPlan {
   static constexpr int lg = ilog2(L);
   static constexpr int rem = lg % 3;
-  static constexpr int nstages = lg / 3 + (rem ? 1 : 0);
-  static constexpr int radix(int s) { return (rem && s == 0) ? (1 << rem) : 8; }
+  static constexpr bool r16 = WIDE && rem == 1 && lg >= 4;
+  static constexpr int nstages = r16 ? (lg - 4) / 3 + 1 : lg / 3 + (rem ? 1 : 0);
+  static constexpr int radix(int s) { return s == 0 ? (r16 ? 16 : rem ? (1 << rem) : 8) : 8; }
   static constexpr int pbefore(int s) {
     int p = 1;
     for (int i = 0; i < s; ++i) p *= radix(i);
@@ -91,6 +96,41 @@ template <int DIR, typename T> struct Dft<8, DIR, T> {
   }
 };
 
+// 16 points as 4 x 4: DFT-4 of the four decimated columns, the W16^(jk)
+// twiddles, DFT-4 of the rows; output k + 4m in natural order
+template <int DIR, typename T> struct Dft<16, DIR, T> {
+  __device__ __forceinline__ static cx<T> rot(cx<T> a, T c, T s) {   // a * (c + DIR s i)
+    return cx<T>{a.x * c - T(DIR) * s * a.y, a.y * c + T(DIR) * s * a.x};
+  }
+  __device__ __forceinline__ static void run(cx<T> *x) {
+    const T c1 = T(0.92387953251128675612818318939678828682), s1 = T(0.38268343236508977172845998403039886676);
+    const T h = T(0.70710678118654752440084436210484903928);
+    cx<T> a[4][4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+#pragma unroll
+      for (int m = 0; m < 4; ++m) a[j][m] = x[j + 4 * m];
+      Dft<4, DIR, T>::run(a[j]);
+    }
+    a[1][1] = rot(a[1][1], c1, s1);    // W^1
+    a[1][2] = rot(a[1][2], h, h);      // W^2
+    a[1][3] = rot(a[1][3], s1, c1);    // W^3
+    a[2][1] = rot(a[2][1], h, h);      // W^2
+    a[2][2] = mul_i<DIR>(a[2][2]);     // W^4
+    a[2][3] = rot(a[2][3], -h, h);     // W^6
+    a[3][1] = rot(a[3][1], s1, c1);    // W^3
+    a[3][2] = rot(a[3][2], -h, h);     // W^6
+    a[3][3] = rot(a[3][3], -c1, -s1);  // W^9
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      cx<T> b[4] = {a[0][k], a[1][k], a[2][k], a[3][k]};
+      Dft<4, DIR, T>::run(b);
+#pragma unroll
+      for (int m = 0; m < 4; ++m) x[k + 4 * m] = b[m];
+    }
+  }
+};
+
 // XCD-aware tile order: consecutive tiles land on the same XCD (the
 // dispatcher places workgroup b on XCD b % 8), so tiles that share 128-byte
 // lines are served by one L2.
@@ -106,10 +146,11 @@ __device__ __forceinline__ int xcd_tile(int b, int nblk) {
 // value (apply).
 struct NoOp {};
 
-template <typename T, int L, int NL, int THREADS, int S, int DIR, bool LINE_FAST, class Ld, class St, class Op = NoOp>
+template <typename T, int L, int NL, int THREADS, int S, int DIR, bool LINE_FAST, bool WIDE, class Ld, class St,
+          class Op = NoOp>
 __device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, Ld ld, St st, bool sync_between,
                                       Op op = Op()) {
-  using P = RadixPlan<L>;
+  using P = RadixPlan<L, WIDE>;
   constexpr int R = P::radix(S);
   constexpr int p = P::pbefore(S);
   constexpr int NB = NL * (L / R);
@@ -185,25 +226,25 @@ __device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, L
 // whether gld / gst themselves touch LDS, in which case the stage must put a
 // barrier between its loads and its stores.
 template <typename T, int L, int NL, int THREADS, int DIR, bool LINE_FAST, bool GLD_LDS, bool GST_LDS, int S,
-          class GLd, class GSt, class LIdx, class Op = NoOp>
+          bool WIDE, class GLd, class GSt, class LIdx, class Op = NoOp>
 __device__ __forceinline__ void run_stages(cx<T> *lds, const cx<T> *tw, int twmul, GLd gld, GSt gst, LIdx lidx,
                                            Op op = Op()) {
-  using P = RadixPlan<L>;
+  using P = RadixPlan<L, WIDE>;
   constexpr int NS = P::nstages;
   auto lds_ld = [&](int line, int pt) { return lds[lidx(line, pt)]; };
   auto lds_st = [&](int line, int pt, cx<T> v) { lds[lidx(line, pt)] = v; };
   if constexpr (NS == 1) {
-    stage<T, L, NL, THREADS, 0, DIR, LINE_FAST>(tw, twmul, gld, gst, GLD_LDS && GST_LDS, op);
+    stage<T, L, NL, THREADS, 0, DIR, LINE_FAST, WIDE>(tw, twmul, gld, gst, GLD_LDS && GST_LDS, op);
   } else if constexpr (S == 0) {
-    stage<T, L, NL, THREADS, 0, DIR, LINE_FAST>(tw, twmul, gld, lds_st, GLD_LDS, op);
+    stage<T, L, NL, THREADS, 0, DIR, LINE_FAST, WIDE>(tw, twmul, gld, lds_st, GLD_LDS, op);
     __syncthreads();
-    run_stages<T, L, NL, THREADS, DIR, LINE_FAST, GLD_LDS, GST_LDS, 1>(lds, tw, twmul, gld, gst, lidx);
+    run_stages<T, L, NL, THREADS, DIR, LINE_FAST, GLD_LDS, GST_LDS, 1, WIDE>(lds, tw, twmul, gld, gst, lidx);
   } else if constexpr (S == NS - 1) {
-    stage<T, L, NL, THREADS, S, DIR, LINE_FAST>(tw, twmul, lds_ld, gst, GST_LDS);
+    stage<T, L, NL, THREADS, S, DIR, LINE_FAST, WIDE>(tw, twmul, lds_ld, gst, GST_LDS);
   } else {
-    stage<T, L, NL, THREADS, S, DIR, LINE_FAST>(tw, twmul, lds_ld, lds_st, true);
+    stage<T, L, NL, THREADS, S, DIR, LINE_FAST, WIDE>(tw, twmul, lds_ld, lds_st, true);
     __syncthreads();
-    run_stages<T, L, NL, THREADS, DIR, LINE_FAST, GLD_LDS, GST_LDS, S + 1>(lds, tw, twmul, gld, gst, lidx);
+    run_stages<T, L, NL, THREADS, DIR, LINE_FAST, GLD_LDS, GST_LDS, S + 1, WIDE>(lds, tw, twmul, gld, gst, lidx);
   }
 }
 
@@ -404,7 +445,8 @@ __device__ __forceinline__ void strided_tile(const StridedArgs<T> &a, const cx<T
   // POST_AUX: the spectrum it adds is fetched before the first stage, for the
   // outputs this thread stores in the last one (line q % C, points
   // q / C + s L / R), so that its latency overlaps the transform
-  using RP = RadixPlan<L>;
+  constexpr bool WIDE = PRE == PRE_NONE;
+  using RP = RadixPlan<L, WIDE>;
   constexpr int RL = RP::radix(RP::nstages - 1);
   constexpr int NBL = C * (L / RL);
   constexpr int PERL = (NBL + THREADS - 1) / THREADS;
@@ -492,9 +534,9 @@ __device__ __forceinline__ void strided_tile(const StridedArgs<T> &a, const cx<T
   };
   auto lidx = [](int line, int pt) { return pt * C + line; };
   if constexpr (PRE == PRE_NONE) {
-    run_stages<T, L, C, THREADS, DIR, true, false, false, 0>(lds, tw, 1, gld, gst, lidx);
+    run_stages<T, L, C, THREADS, DIR, true, false, false, 0, WIDE>(lds, tw, 1, gld, gst, lidx);
   } else {
-    run_stages<T, L, C, THREADS, DIR, true, false, false, 0>(lds, tw, 1, gld, gst, lidx, op);
+    run_stages<T, L, C, THREADS, DIR, true, false, false, 0, WIDE>(lds, tw, 1, gld, gst, lidx, op);
   }
 }
 
@@ -552,7 +594,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_r2c(T *__restrict__ data, lon
   if constexpr (M == 1) {
     if (threadIdx.x < ROWS) lds[lidx(threadIdx.x, 0)] = gld(threadIdx.x, 0);
   } else {
-    run_stages<T, M, ROWS, THREADS, -1, false, false, true, 0>(lds, ltw, 1, gld, lst, lidx);
+    run_stages<T, M, ROWS, THREADS, -1, false, false, true, 0, false>(lds, ltw, 1, gld, lst, lidx);
   }
   __syncthreads();
   constexpr int H = M / 2 + 1;
@@ -646,7 +688,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_c2r(T *__restrict__ data, lon
   if constexpr (M == 1) {
     if (threadIdx.x < ROWS) gst(threadIdx.x, 0, lld(threadIdx.x, 0));
   } else {
-    run_stages<T, M, ROWS, THREADS, +1, false, true, false, 0>(lds, ltw, 1, lld, gst, lidx);
+    run_stages<T, M, ROWS, THREADS, +1, false, true, false, 0, false>(lds, ltw, 1, lld, gst, lidx);
   }
 }
 
@@ -708,7 +750,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_c2r_window(const T *__restric
   if constexpr (M == 1) {
     if (threadIdx.x < ROWS) gst(threadIdx.x, 0, lld(threadIdx.x, 0));
   } else {
-    run_stages<T, M, ROWS, THREADS, +1, false, true, false, 0>(lds, ltw, 1, lld, gst, lidx);
+    run_stages<T, M, ROWS, THREADS, +1, false, true, false, 0, false>(lds, ltw, 1, lld, gst, lidx);
   }
 }
 
@@ -745,7 +787,7 @@ __global__ __launch_bounds__(THREADS) void k_fft_z_r2c_window(const T *__restric
   if constexpr (M == 1) {
     if (threadIdx.x < ROWS) lds[lidx(threadIdx.x, 0)] = gld(threadIdx.x, 0);
   } else {
-    run_stages<T, M, ROWS, THREADS, -1, false, false, true, 0>(lds, ltw, 1, gld, lst, lidx);
+    run_stages<T, M, ROWS, THREADS, -1, false, false, true, 0, false>(lds, ltw, 1, gld, lst, lidx);
   }
   __syncthreads();
   constexpr int H = M / 2 + 1;
