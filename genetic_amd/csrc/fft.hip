@@ -170,19 +170,43 @@ __device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, L
   // operators: every factor is formed while the loads are in flight (they
   // depend only on the mode), and only then applied to the loaded values
   if constexpr (!std::is_same<Op, NoOp>::value) {
-    T fac[PER][R];
+    // all slots' factors in one call; an idle slot (no butterfly) takes
+    // line 0 and point r L/R, inside the tile, and its factor is unused
+    int ln[PER * R], pp[PER * R];
 #pragma unroll
     for (int b = 0; b < PER; ++b) {
       int q = tid + b * THREADS;
-      // only the slots that hold a butterfly: an idle slot's (line, point)
-      // lies outside the tile and would index the factor table out of range
-      if (NB % THREADS == 0 || q < NB) {
-        int line = LINE_FAST ? q % NL : q / (L / R);
-        int i = LINE_FAST ? q / NL : q % (L / R);
+      bool has = NB % THREADS == 0 || q < NB;
+      int line = has ? (LINE_FAST ? q % NL : q / (L / R)) : 0;
+      int i = has ? (LINE_FAST ? q / NL : q % (L / R)) : 0;
 #pragma unroll
-        for (int r = 0; r < R; ++r) fac[b][r] = op.factor(line, i + r * (L / R));
+      for (int r = 0; r < R; ++r) {
+        ln[b * R + r] = line;
+        pp[b * R + r] = i + r * (L / R);
       }
     }
+    // in groups of four slots: enough chains to interleave, few enough
+    // registers for two blocks per CU
+    constexpr int NSL = PER * R, CH = NSL % 4 == 0 ? 4 : NSL;
+    T facs[NSL];
+#pragma unroll
+    for (int c0 = 0; c0 < NSL; c0 += CH) {
+      int lc[CH], pc[CH];
+      T fc[CH];
+#pragma unroll
+      for (int j = 0; j < CH; ++j) {
+        lc[j] = ln[c0 + j];
+        pc[j] = pp[c0 + j];
+      }
+      op.factors(lc, pc, fc);
+#pragma unroll
+      for (int j = 0; j < CH; ++j) facs[c0 + j] = fc[j];
+    }
+    T fac[PER][R];
+#pragma unroll
+    for (int b = 0; b < PER; ++b)
+#pragma unroll
+      for (int r = 0; r < R; ++r) fac[b][r] = facs[b * R + r];
 #pragma unroll
     for (int b = 0; b < PER; ++b) {
       int q = tid + b * THREADS;
@@ -321,6 +345,47 @@ __device__ __forceinline__ float filter_fast_f(const FilterF &F, int ksq) {
   if (F.over_k2) v = ksq == 0 ? 0.f : v * F.inv_kmin2 * __builtin_amdgcn_rcpf(fk);
   return v;
 }
+// filter_fast_f for NS values of |k|^2 at once, one factor at a time across
+// them: the parameter tests are taken once for all values and their
+// exp / rcp / sqrt chains interleave.  The same arithmetic per value.
+template <int NS>
+__device__ __forceinline__ void filter_fast_f_n(const FilterF &F, const int (&ksq)[NS], float (&out)[NS]) {
+  if (F.zero) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) out[s] = 0.f;
+    return;
+  }
+  float k[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    k[s] = __builtin_amdgcn_sqrtf(float(ksq[s])) * F.kmin;
+    out[s] = F.scale;
+  }
+#pragma unroll
+  for (int i = 0; i < FILTER_INLINE_MAX; ++i) {
+    if (i >= F.nf) break;
+    const float kc = F.kc[i], iw = F.inv_w[i];
+    const int kind = F.kind[i];
+    float lp[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) lp[s] = __builtin_amdgcn_rcpf(1.f + __expf((k[s] - kc) * iw));
+    if (kind == 0) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) out[s] *= lp[s];
+    } else if (kind == 1) {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) out[s] *= __builtin_amdgcn_sqrtf(fmaxf(1.f - lp[s] * lp[s], 0.f));
+    } else {
+#pragma unroll
+      for (int s = 0; s < NS; ++s) out[s] *= 1.f - lp[s];
+    }
+  }
+  if (F.over_k2) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      out[s] = ksq[s] == 0 ? 0.f : out[s] * F.inv_kmin2 * __builtin_amdgcn_rcpf(float(ksq[s]));
+  }
+}
 template <typename T, class A> __device__ __forceinline__ T kfactor(const A &a, int ksq) {
   if constexpr (sizeof(T) == 4) {
     if (a.ff.nf >= 0) return filter_fast_f(a.ff, ksq);
@@ -373,75 +438,102 @@ __device__ __forceinline__ cx<T> tile_in(int in_wlen, long in_ls, const TileG<T>
   }
   return g.ib[pt * in_ls + line];
 }
+// the operator's parameters, copied out of the kernel arguments once per
+// tile (read through a reference they were reloaded for every mode)
+template <typename T> struct PreArgs {
+  const T *tab;
+  int comp, ncol, n;
+  T kmin;
+  double inv_k2, kmin_d;
+  FilterF ff;
+  float inv_k2_f;
+};
+template <typename T, int PRE> struct PreOp {
+  PreArgs<T> a;
+  int col0, ky, ky2;
+  // the factor of |k|^2: the caller's table, its filter formed here (FP32),
+  // or inv_k2 / (|k|^2 kMin^2) formed exactly as filter_table forms it;
+  // times k_c (Zel'dovich, zero at k_c = +Nyquist) or k_a k_b (second
+  // derivatives, zero at a +Nyquist component when a != b)
+  __device__ __forceinline__ T factor(int line, int pt) const {
+    if constexpr (PRE == PRE_YZ) {   // a y pass: points run over k_y, columns over k_z
+      int kyy = signed_k(pt, a.n), kz = col0 + line, nyq = a.n / 2;
+      if (kz >= a.ncol) return T(0);
+      T fy = T(kyy) * a.kmin, fz = T(kz) * a.kmin;
+      switch (a.comp) {
+        case FFT_YZ_IKY: return kyy == nyq ? T(0) : fy;
+        case FFT_YZ_IKZ: return kz == nyq ? T(0) : fz;
+        case FFT_YZ_MIKY: return kyy == nyq ? T(0) : -fy;
+        case FFT_YZ_MIKZ: return kz == nyq ? T(0) : -fz;
+        case FFT_YZ_KYKY: return fy * fy;
+        case FFT_YZ_KYKZ: return (kyy == nyq || kz == nyq) ? T(0) : fy * fz;
+        case FFT_YZ_KZKZ: return fz * fz;
+        default: return T(1);
+      }
+    }
+    int kx = signed_k(pt, a.n), kz = col0 + line;
+    if (kz >= a.ncol) return T(0);   // past the last column (its loads are zero): no table read
+    int ksq = kx * kx + ky2 + kz * kz;
+    return finish(kx, kz, kfactor<T>(a, ksq));
+  }
+  // factor() for NS slots at once; FP32 filters go through filter_fast_f_n
+  template <int NS>
+  __device__ __forceinline__ void factors(const int (&line)[NS], const int (&pt)[NS], T (&f)[NS]) const {
+    if constexpr (PRE != PRE_YZ && sizeof(T) == 4) {
+      if (a.ff.nf >= 0) {
+        int ksq[NS];
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          int kx = signed_k(pt[s], a.n), kz = col0 + line[s];
+          ksq[s] = kx * kx + ky2 + kz * kz;
+        }
+        float base[NS];
+        filter_fast_f_n<NS>(a.ff, ksq, base);
+#pragma unroll
+        for (int s = 0; s < NS; ++s) {
+          int kx = signed_k(pt[s], a.n), kz = col0 + line[s];
+          f[s] = kz >= a.ncol ? T(0) : finish(kx, kz, base[s]);
+        }
+        return;
+      }
+    }
+#pragma unroll
+    for (int s = 0; s < NS; ++s) f[s] = factor(line[s], pt[s]);
+  }
+  // the |k|^2 factor f times the operator's k-components (PRE_ZA, PRE_KK)
+  __device__ __forceinline__ T finish(int kx, int kz, T f) const {
+    if constexpr (PRE == PRE_ZA) {
+      int kc = a.comp == 0 ? kx : a.comp == 1 ? ky : kz;
+      return kc == a.n / 2 ? T(0) : f * (T(kc) * a.kmin);
+    } else if constexpr (PRE == PRE_KK) {
+      int ca = a.comp % 3, cb = a.comp / 3;
+      int k1 = ca == 0 ? kx : ca == 1 ? ky : kz;
+      int k2 = cb == 0 ? kx : cb == 1 ? ky : kz;
+      if (ca != cb && (k1 == a.n / 2 || k2 == a.n / 2)) return T(0);
+      return f * ((T(k1) * a.kmin) * (T(k2) * a.kmin));
+    } else {
+      return f;
+    }
+  }
+  // Zel'dovich and the first-derivative y factors: i f v; otherwise f v
+  __device__ __forceinline__ cx<T> apply(cx<T> v, T f) const {
+    if constexpr (PRE == PRE_ZA) {
+      return cx<T>{-v.y * f, v.x * f};
+    } else if constexpr (PRE == PRE_YZ) {
+      if (a.comp >= FFT_YZ_IKY && a.comp <= FFT_YZ_MIKZ) return cx<T>{-v.y * f, v.x * f};
+      return cx<T>{v.x * f, v.y * f};
+    } else {
+      return cx<T>{v.x * f, v.y * f};
+    }
+  }
+};
 // One tile of a strided pass; ld supplies the first stage's inputs.
 template <typename T, int L, int C, int DIR, int THREADS, int PRE, int POST, class Ld>
 __device__ __forceinline__ void strided_tile(const StridedArgs<T> &a, const cx<T> *__restrict__ tw, cx<T> *lds,
                                              const TileG<T> &g, Ld gld) {
   const int col0 = g.col0, nc = g.nc, ky = g.ky, ky2 = g.ky2;
   cx<T> *ob = g.ob;
-  // the operator's parameters, copied out of the kernel arguments once per
-  // tile (read through a reference they were reloaded for every mode)
-  struct PreArgs {
-    const T *tab;
-    int comp, ncol, n;
-    T kmin;
-    double inv_k2, kmin_d;
-    FilterF ff;
-    float inv_k2_f;
-  };
-  struct PreOp {
-    PreArgs a;
-    int col0, ky, ky2;
-    // the factor of |k|^2: the caller's table, its filter formed here (FP32),
-    // or inv_k2 / (|k|^2 kMin^2) formed exactly as filter_table forms it;
-    // times k_c (Zel'dovich, zero at k_c = +Nyquist) or k_a k_b (second
-    // derivatives, zero at a +Nyquist component when a != b)
-    __device__ __forceinline__ T factor(int line, int pt) const {
-      if constexpr (PRE == PRE_YZ) {   // a y pass: points run over k_y, columns over k_z
-        int kyy = signed_k(pt, a.n), kz = col0 + line, nyq = a.n / 2;
-        if (kz >= a.ncol) return T(0);
-        T fy = T(kyy) * a.kmin, fz = T(kz) * a.kmin;
-        switch (a.comp) {
-          case FFT_YZ_IKY: return kyy == nyq ? T(0) : fy;
-          case FFT_YZ_IKZ: return kz == nyq ? T(0) : fz;
-          case FFT_YZ_MIKY: return kyy == nyq ? T(0) : -fy;
-          case FFT_YZ_MIKZ: return kz == nyq ? T(0) : -fz;
-          case FFT_YZ_KYKY: return fy * fy;
-          case FFT_YZ_KYKZ: return (kyy == nyq || kz == nyq) ? T(0) : fy * fz;
-          case FFT_YZ_KZKZ: return fz * fz;
-          default: return T(1);
-        }
-      }
-      int kx = signed_k(pt, a.n), kz = col0 + line;
-      if (kz >= a.ncol) return T(0);   // past the last column (its loads are zero): no table read
-      int ksq = kx * kx + ky2 + kz * kz;
-      T f = kfactor<T>(a, ksq);
-      if constexpr (PRE == PRE_ZA) {
-        int kc = a.comp == 0 ? kx : a.comp == 1 ? ky : kz;
-        return kc == a.n / 2 ? T(0) : f * (T(kc) * a.kmin);
-      } else if constexpr (PRE == PRE_KK) {
-        int ca = a.comp % 3, cb = a.comp / 3;
-        int k1 = ca == 0 ? kx : ca == 1 ? ky : kz;
-        int k2 = cb == 0 ? kx : cb == 1 ? ky : kz;
-        if (ca != cb && (k1 == a.n / 2 || k2 == a.n / 2)) return T(0);
-        return f * ((T(k1) * a.kmin) * (T(k2) * a.kmin));
-      } else {
-        return f;
-      }
-    }
-    // Zel'dovich and the first-derivative y factors: i f v; otherwise f v
-    __device__ __forceinline__ cx<T> apply(cx<T> v, T f) const {
-      if constexpr (PRE == PRE_ZA) {
-        return cx<T>{-v.y * f, v.x * f};
-      } else if constexpr (PRE == PRE_YZ) {
-        if (a.comp >= FFT_YZ_IKY && a.comp <= FFT_YZ_MIKZ) return cx<T>{-v.y * f, v.x * f};
-        return cx<T>{v.x * f, v.y * f};
-      } else {
-        return cx<T>{v.x * f, v.y * f};
-      }
-    }
-  };
-  PreOp op{PreArgs{a.tab, a.comp, a.ncol, a.n, a.kmin, a.inv_k2, a.kmin_d, a.ff, a.inv_k2_f}, col0, ky, ky2};
+  PreOp<T, PRE> op{PreArgs<T>{a.tab, a.comp, a.ncol, a.n, a.kmin, a.inv_k2, a.kmin_d, a.ff, a.inv_k2_f}, col0, ky, ky2};
   // POST_AUX: the spectrum it adds is fetched before the first stage, for the
   // outputs this thread stores in the last one (line q % C, points
   // q / C + s L / R), so that its latency overlaps the transform
