@@ -31,8 +31,9 @@ constexpr int ilog2(int n) { return n <= 1 ? 0 : 1 + ilog2(n / 2); }
 
 // WIDE: where the radix-8 stages leave a factor 2, the first stage takes 16
 // points instead (one stage, and one pass through LDS, fewer).  It pays
-// where the first stage has no per-mode operator; with one, the operator's
-// work falls on half as many threads and the pass is slower.
+// where the first stage has no |k|^2 operator (none, or a y pass's k_y / k_z
+// factor); with a filter or table the 16 slots' factors take more registers
+// than two blocks per CU allow.
 template <int L, bool WIDE = false> struct RadixPlan {
   static constexpr int lg = ilog2(L);
   static constexpr int rem = lg % 3;
@@ -537,7 +538,7 @@ __device__ __forceinline__ void strided_tile(const StridedArgs<T> &a, const cx<T
   // POST_AUX: the spectrum it adds is fetched before the first stage, for the
   // outputs this thread stores in the last one (line q % C, points
   // q / C + s L / R), so that its latency overlaps the transform
-  constexpr bool WIDE = PRE == PRE_NONE;
+  constexpr bool WIDE = PRE == PRE_NONE || PRE == PRE_YZ;
   using RP = RadixPlan<L, WIDE>;
   constexpr int RL = RP::radix(RP::nstages - 1);
   constexpr int NBL = C * (L / RL);
