@@ -236,12 +236,7 @@ __device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, L
       Dft<R, DIR, T>::run(v[b]);
       int j = (i - k) * R + k;
 #pragma unroll
-      for (int s = 0; s < R; ++s) {
-        // a store functor that takes the butterfly's (b, s) gets them (values
-        // it fetched for this slot before the first stage)
-        if constexpr (std::is_invocable<St, int, int, cx<T>, int, int>::value) st(line, j + s * p, v[b][s], b, s);
-        else st(line, j + s * p, v[b][s]);
-      }
+      for (int s = 0; s < R; ++s) st(line, j + s * p, v[b][s]);
     }
   }
 }
@@ -535,45 +530,13 @@ __device__ __forceinline__ void strided_tile(const StridedArgs<T> &a, const cx<T
   const int col0 = g.col0, nc = g.nc, ky = g.ky, ky2 = g.ky2;
   cx<T> *ob = g.ob;
   PreOp<T, PRE> op{PreArgs<T>{a.tab, a.comp, a.ncol, a.n, a.kmin, a.inv_k2, a.kmin_d, a.ff, a.inv_k2_f}, col0, ky, ky2};
-  // POST_AUX: the spectrum it adds is fetched before the first stage, for the
-  // outputs this thread stores in the last one (line q % C, points
-  // q / C + s L / R), so that its latency overlaps the transform
   constexpr bool WIDE = PRE == PRE_NONE || PRE == PRE_YZ;
-  using RP = RadixPlan<L, WIDE>;
-  constexpr int RL = RP::radix(RP::nstages - 1);
-  constexpr int NBL = C * (L / RL);
-  constexpr int PERL = (NBL + THREADS - 1) / THREADS;
-  constexpr bool PREFETCH = POST == POST_AUX;
-  cx<T> pre[PREFETCH ? PERL : 1][PREFETCH ? RL : 1];
-  if constexpr (PREFETCH) {
-    const cx<T> *src = a.aux + (ob - a.out);
-#pragma unroll
-    for (int bb = 0; bb < PERL; ++bb) {
-      int q = int(threadIdx.x) + bb * THREADS;
-#pragma unroll
-      for (int s = 0; s < RL; ++s) {
-        cx<T> v{0, 0};
-        if (NBL % THREADS == 0 || q < NBL) {
-          int line = q % C, pt = q / C + s * (L / RL);
-          int o = pt;
-          bool ok = line < nc;
-          if (a.wlen < L) {
-            o = pt - a.win0;
-            if (o < 0) o += L;
-            ok = ok && o < a.wlen;
-          }
-          if (ok) v = src[long(o) * a.out_ls + line];
-        }
-        pre[bb][s] = v;
-      }
-    }
-  }
   // the store path's parameters, read once per tile
   const int wlen = a.wlen, win0 = a.win0, accum = a.accum;
   const long out_ls = a.out_ls;
   const T scale = a.scale;
   const cx<T> *aux = a.aux, *out0 = a.out;
-  auto gst = [&](int line, int pt, cx<T> v, auto... slot) {
+  auto gst = [&](int line, int pt, cx<T> v) {
     if (line >= nc) return;
     int o = pt;
     if (wlen < L) {
@@ -595,15 +558,12 @@ __device__ __forceinline__ void strided_tile(const StridedArgs<T> &a, const cx<T
       }
     }
     if constexpr (POST == POST_AUX) {
+      // the added spectrum is read here, at the store: fetched before the
+      // first stage it held 16 more registers through the transform and left
+      // one block per CU instead of two
       int kx = signed_k(pt, op.a.n), kz = col0 + line;
       T f = kfactor<T>(op.a, kx * kx + ky2 + kz * kz);
-      cx<T> x;
-      if constexpr (sizeof...(slot) == 2) {
-        int bs[2] = {slot...};
-        x = pre[bs[0]][bs[1]];
-      } else {
-        x = aux[long(o) * out_ls + line + (ob - out0)];
-      }
+      cx<T> x = aux[long(o) * out_ls + line + (ob - out0)];
       ob[o * out_ls + line] = cx<T>{x.x * f + v.x * s, x.y * f + v.y * s};
       return;
     }
