@@ -939,7 +939,7 @@ template <typename T, int N> struct FftN {
   static constexpr int M = N / 2;
   static constexpr int NZ = N / 2 + 1;
   // z-pass rows per block: 8192 / N points' worth; at N = 1024 half that, so
-  // that the rows' LDS (about 21 KB) admits six blocks per CU instead of four
+  // that the rows' LDS (about 21 KB) admits seven blocks per CU instead of four
   static constexpr int ROWS0 = N >= 1024 ? 4096 / N : (8192 / N) < 1 ? 1 : (8192 / N);
   static constexpr int ROWS = ROWS0 > N * N ? N * N : ROWS0;
 
