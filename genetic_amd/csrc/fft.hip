@@ -933,6 +933,19 @@ template <typename T, int N> constexpr int strided_cols() {
 #endif
 }
 
+// y passes' tile shape: YTileShape where fft_tiles.h gives one, else the
+// x passes'
+template <typename T, int N, class = void> struct YShape {
+  static constexpr int threads = strided_threads<T, N>();
+  static constexpr int cols = strided_cols<T, N>();
+};
+#ifndef GIC_TUNE_THREADS   // the tuner times one shape for every pass
+template <typename T, int N> struct YShape<T, N, std::void_t<decltype(YTileShape<T, N>::cols)>> {
+  static constexpr int threads = YTileShape<T, N>::threads;
+  static constexpr int cols = YTileShape<T, N>::cols;
+};
+#endif
+
 template <typename T, int N> struct FftN {
   static constexpr int THREADS = 256;
   static constexpr int C = strided_cols<T, N>();
@@ -946,6 +959,9 @@ template <typename T, int N> struct FftN {
   static constexpr int STHREADS = strided_threads<T, N>();
   template <int DIR, int PRE, int POST, int PASS = 0>
   static void strided(const StridedArgs<T> &a0, int nouter, hipStream_t st, TimingHook *hook, const char *name) {
+    constexpr bool YP = PASS == 1 || PASS == 3 || PASS == 5;
+    constexpr int CP = YP ? YShape<T, N>::cols : C;
+    constexpr int TP = YP ? YShape<T, N>::threads : STHREADS;
     StridedArgs<T> a = a0;
     a.ff.nf = -1;
     if (sizeof(T) == 4) {
@@ -966,13 +982,13 @@ template <typename T, int N> struct FftN {
       a.inv_k2_f = a.kmin_d > 0 ? float(a.inv_k2 / (a.kmin_d * a.kmin_d)) : 0.f;
     }
     a.ncol = NZ;
-    a.nkzb = (NZ + C - 1) / C;
+    a.nkzb = (NZ + CP - 1) / CP;
     a.n = N;
     a.nouter = nouter;
     int nblk = nouter * a.nkzb * (a.nbatch > 1 ? a.nbatch : 1);
-    size_t shm = sizeof(cx<T>) * (C + 1) * N;   // the tile, then the line's twiddles
+    size_t shm = sizeof(cx<T>) * (CP + 1) * N;   // the tile, then the line's twiddles
     const cx<T> *tw = twtable<T>().get(N);
-    auto kern = k_fft_strided<T, N, C, DIR, STHREADS, PRE, POST, PASS>;
+    auto kern = k_fft_strided<T, N, CP, DIR, TP, PRE, POST, PASS>;
     if (shm > 65536) GIC_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     std::string tag = std::string(name) + "." + std::to_string(N);
     if (hook) {
@@ -984,7 +1000,7 @@ template <typename T, int N> struct FftN {
       bool reads_out = a.accum || POST == POST_AUX;   // accumulated output, or the spectrum it adds
       hook->bytes(tag.c_str(), sizeof(cx<T>) * (rd + wr + (reads_out ? wr : 0.0)));
     }
-    hipLaunchKernelGGL(kern, dim3(nblk), dim3(STHREADS), shm, st, a, tw);
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(TP), shm, st, a, tw);
     GIC_HIP(hipGetLastError());
     if (hook) hook->end(tag.c_str(), st);
   }

@@ -6,6 +6,16 @@
 
 namespace gic {
 
+// The y passes (lines 4 KB apart) where they take a shape of their own:
+// declared only for the sizes listed after the primary template.
+template <typename T, int N> struct YTileShape;
+// float n=1024: 16-column tiles (128-byte segments, one block per CU) took
+// the step's y passes from 36.8 to 34.8 ms against the x passes' 8 columns
+template <> struct YTileShape<float, 1024> {
+  static constexpr int threads = 1024;
+  static constexpr int cols = 16;
+};
+
 template <typename T, int N> struct TileShape {
   static constexpr int threads = 512;
   static constexpr int cols = sizeof(T) == 4 ? 8 : 4;

@@ -121,6 +121,19 @@ def measure(out_path):
 
 
 DEFAULT = {32: (512, 8), 64: (512, 4)}
+# the y passes' own shapes, chosen from whole-step bench timings rather than
+# by this tuner; apply() writes them back unchanged
+Y_SHAPES = [
+    "// The y passes (lines 4 KB apart) where they take a shape of their own:",
+    "// declared only for the sizes listed after the primary template.",
+    "template <typename T, int N> struct YTileShape;",
+    "// float n=1024: 16-column tiles (128-byte segments, one block per CU) took",
+    "// the step's y passes from 36.8 to 34.8 ms against the x passes' 8 columns",
+    "template <> struct YTileShape<float, 1024> {",
+    "  static constexpr int threads = 1024;",
+    "  static constexpr int cols = 16;",
+    "};",
+]
 MARGIN = 0.02   # a shape replaces the default only if it is this much faster (run-to-run noise is about 1-2 %)
 
 
@@ -144,6 +157,10 @@ def apply(in_path, header=os.path.join(CSRC, "fft_tiles.h")):
         "#pragma once",
         "",
         "namespace gic {",
+        "",
+    ]
+    lines += Y_SHAPES
+    lines += [
         "",
         "template <typename T, int N> struct TileShape {",
         "  static constexpr int threads = 512;",
