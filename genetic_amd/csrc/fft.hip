@@ -187,9 +187,10 @@ __device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, L
       }
     }
     // in groups of four slots: enough chains to interleave, few enough
-    // registers for two blocks per CU
+    // registers for two blocks per CU.  Each group is applied as soon as it
+    // is formed, so that only its four factors are held and it waits only on
+    // its own loads (the later slots' loads stay in flight)
     constexpr int NSL = PER * R, CH = NSL % 4 == 0 ? 4 : NSL;
-    T facs[NSL];
 #pragma unroll
     for (int c0 = 0; c0 < NSL; c0 += CH) {
       int lc[CH], pc[CH];
@@ -201,19 +202,9 @@ __device__ __forceinline__ void stage(const cx<T> *__restrict__ tw, int twmul, L
       }
       op.factors(lc, pc, fc);
 #pragma unroll
-      for (int j = 0; j < CH; ++j) facs[c0 + j] = fc[j];
-    }
-    T fac[PER][R];
-#pragma unroll
-    for (int b = 0; b < PER; ++b)
-#pragma unroll
-      for (int r = 0; r < R; ++r) fac[b][r] = facs[b * R + r];
-#pragma unroll
-    for (int b = 0; b < PER; ++b) {
-      int q = tid + b * THREADS;
-      if (NB % THREADS == 0 || q < NB) {
-#pragma unroll
-        for (int r = 0; r < R; ++r) v[b][r] = op.apply(v[b][r], fac[b][r]);
+      for (int j = 0; j < CH; ++j) {
+        const int b = (c0 + j) / R, r = (c0 + j) % R;
+        if (NB % THREADS == 0 || tid + b * THREADS < NB) v[b][r] = op.apply(v[b][r], fc[j]);
       }
     }
   }
